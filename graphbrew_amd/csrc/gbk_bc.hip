@@ -1,0 +1,262 @@
+// gbk_bc.hip -- Brandes betweenness centrality (bench/src/bc.cc:71-237).
+//
+// Per source, as in the reference:
+//   forward   level-synchronous BFS; a CAS on depth claims a vertex for the
+//             next level; every edge (u,v) with depth[v] == depth[u]+1 sets
+//             its bit in an edge-indexed successor bitmap (bc.cc:109, m bits)
+//             and adds sigma[u] into sigma[v] (double; path counts are
+//             integers < 2^53, so the atomic sum is exact in any order).
+//   backward  levels deepest first; delta[u] = sum over successor edges of
+//             sigma[u]/sigma[v] * (1 + delta[v]); scores[u] += delta[u].
+// Scores are divided by their maximum at the end (bc.cc:224-230).
+// Kernels: the forward uses the coalesced frontier expansion; the backward
+// walks each level's edges wave-wise with per-row LDS accumulators (rows
+// above the heavy threshold are reduced by a whole workgroup).
+#include "gbk_expand.h"
+
+namespace gbk {
+
+struct BcCounters {
+  unsigned long long tail;
+  unsigned long long pad[3];
+};
+
+__global__ void bc_init_source(int64_t n, int32_t source, int32_t *depth, double *sigma,
+                               float *delta, int32_t *queue, BcCounters *c) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < n) {
+    depth[v] = v == source ? 0 : -1;
+    sigma[v] = v == source ? 1.0 : 0.0;
+    delta[v] = 0.0f;
+  }
+  if (v == 0) {
+    queue[0] = source;
+    c->tail = 1;
+  }
+}
+
+struct BcForwardEdge {
+  const int32_t *nbr;
+  int32_t *depth;
+  double *sigma;
+  uint32_t *succ;
+  int32_t *queue;
+  unsigned long long *tail;
+  int64_t edge_base;
+  int32_t d;
+  __device__ void operator()(int64_t u, int64_t k, bool active) {
+    bool claimed = false;
+    int32_t v = 0;
+    if (active) {
+      v = nbr[k];
+      if (ld_agent(&depth[v]) == -1 && atomicCAS(&depth[v], -1, d) == -1) claimed = true;
+      if (ld_agent(&depth[v]) == d) {
+        bit_set_atomic(succ, k - edge_base);
+        unsafeAtomicAdd(&sigma[v], sigma[u]);
+      }
+    }
+    const int64_t at = wave_append(tail, claimed);
+    if (claimed) queue[at] = v;
+  }
+};
+
+__global__ void __launch_bounds__(kBlock)
+bc_forward_level(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigma,
+                 uint32_t *succ, int32_t *queue, int64_t lb, int64_t le, int32_t d,
+                 int64_t heavy_deg, BcCounters *c) {
+  BcForwardEdge f{nbr, depth, sigma, succ, queue, &c->tail, off[0], d};
+  auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
+    s = off[u];
+    l = off[u + 1] - s;
+  };
+  expand_list(queue + lb, le - lb, heavy_deg, rowlen, f);
+}
+
+// One level of the dependency accumulation.
+__global__ void __launch_bounds__(kBlock)
+bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, int64_t count,
+                  const double *sigma, const uint32_t *succ, float *delta, float *scores,
+                  int64_t heavy_deg) {
+  __shared__ float acc[kBlock];
+  __shared__ int32_t heavy_rows[kBlock];
+  __shared__ int heavy_n;
+  __shared__ float red[kWavesPerBlock];
+  const int64_t edge_base = off[0];
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  for (int64_t tile = blockIdx.x; tile * kBlock < count; tile += gridDim.x) {
+    if (threadIdx.x == 0) heavy_n = 0;
+    acc[threadIdx.x] = 0.0f;
+    __syncthreads();
+    const int64_t i = tile * kBlock + threadIdx.x;
+    int32_t u = 0;
+    int64_t start = 0, len = 0;
+    bool light = false;
+    if (i < count) {
+      u = list[i];
+      start = off[u];
+      len = off[u + 1] - start;
+      if (len > heavy_deg) {
+        heavy_rows[atomicAdd(&heavy_n, 1)] = u;
+        len = 0;
+      } else {
+        light = true;
+      }
+    }
+    const double su = sigma[u];
+    const int64_t incl = wave_incl_scan(len);
+    const int64_t excl = incl - len;
+    const int64_t total = __shfl(incl, kWave - 1);
+    for (int64_t c = 0; c < total; c += kWave) {
+      const int64_t e = c + lane;
+      int lo = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {
+        const int probe = lo + step;
+        const int64_t x = __shfl(excl, probe & (kWave - 1));
+        if (probe < kWave && x <= e) lo = probe;
+      }
+      const int64_t ostart = __shfl(start, lo);
+      const int64_t oexcl = __shfl(excl, lo);
+      const double osig = __shfl(su, lo);
+      if (e < total) {
+        const int64_t k = ostart + (e - oexcl);
+        if (bit_get(succ, k - edge_base)) {
+          const int32_t v = nbr[k];
+          atomicAdd(&acc[wv * kWave + lo], static_cast<float>(osig / sigma[v] * (1.0 + delta[v])));
+        }
+      }
+    }
+    __syncthreads();
+    if (light) {
+      const float du = acc[threadIdx.x];
+      delta[u] = du;
+      scores[u] += du;
+    }
+    const int nh = heavy_n;
+    for (int h = 0; h < nh; h++) {
+      const int32_t hu = heavy_rows[h];
+      const int64_t hs = off[hu], he = off[hu + 1];
+      const double hsig = sigma[hu];
+      float part = 0.0f;
+      for (int64_t k = hs + threadIdx.x; k < he; k += blockDim.x) {
+        if (bit_get(succ, k - edge_base)) {
+          const int32_t v = nbr[k];
+          part += static_cast<float>(hsig / sigma[v] * (1.0 + delta[v]));
+        }
+      }
+      const float tot = block_sum(part, red);
+      if (threadIdx.x == 0) {
+        delta[hu] = tot;
+        scores[hu] += tot;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void bc_reach_edges(const int64_t *off, const int32_t *depth, int64_t n,
+                               unsigned long long *acc) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  unsigned long long e = 0;
+  if (v < n && depth[v] >= 0) e = static_cast<unsigned long long>(off[v + 1] - off[v]);
+  e = wave_sum(e);
+  if (lane_id() == 0 && e) atomicAdd(acc, e);
+}
+
+__global__ void bc_max(const float *scores, int64_t n, unsigned int *mx) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  float x = v < n ? scores[v] : 0.0f;
+  x = wave_max(x);
+  // non-negative floats order like their bit patterns
+  if (lane_id() == 0) atomicMax(mx, __float_as_uint(x));
+}
+
+__global__ void bc_normalize(float *scores, int64_t n, const unsigned int *mx) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < n) scores[v] = scores[v] / __uint_as_float(*mx);
+}
+
+}  // namespace gbk
+
+using namespace gbk;
+
+int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int num_sources,
+                  float *scores, gbk_bc_stats *stats) {
+  const int64_t n = g->n;
+  hipStream_t s = ctx->stream;
+  const int64_t words = (g->m + 31) / 32 + 1;
+  auto *depth = static_cast<int32_t *>(scratch(ctx, kS5, sizeof(int32_t) * (n + 1)));
+  auto *sigma = static_cast<double *>(scratch(ctx, kS6, sizeof(double) * (n + 1)));
+  auto *delta = static_cast<float *>(scratch(ctx, kS7, sizeof(float) * (n + 1)));
+  auto *queue = static_cast<int32_t *>(scratch(ctx, kS8, sizeof(int32_t) * (n + 64)));
+  auto *succ = static_cast<uint32_t *>(scratch(ctx, kS9, sizeof(uint32_t) * words));
+  auto *c = static_cast<BcCounters *>(scratch(ctx, kS10, sizeof(BcCounters) + 64));
+  auto *hc = static_cast<BcCounters *>(pinned(ctx, sizeof(BcCounters)));
+  if (!depth || !sigma || !delta || !queue || !succ || !c || !hc)
+    return set_error(ctx, GBK_ENOMEM, "bc scratch");
+  auto *reach = reinterpret_cast<unsigned long long *>(&c->pad[0]);
+  auto *mx = reinterpret_cast<unsigned int *>(&c->pad[1]);
+  const int64_t H = ctx->tune.heavy_degree;
+  EventTimer timer(ctx);
+  fill<float>(ctx, scores, 0.0f, n);
+  fill<unsigned long long>(ctx, reinterpret_cast<unsigned long long *>(&c->pad[0]), 0ull, 2);
+  int max_depth = 0;
+  std::vector<int64_t> level;
+  for (int it = 0; it < num_sources; it++) {
+    const int32_t src = sources[it];
+    fill<uint32_t>(ctx, succ, 0u, words);
+    bc_init_source<<<grid_for(n), kBlock, 0, s>>>(n, src, depth, sigma, delta, queue, c);
+    level.assign({0, 1});
+    int32_t d = 0;
+    for (;;) {
+      d++;
+      const int64_t lb = level[level.size() - 2], le = level.back();
+      bc_forward_level<<<expand_grid(le - lb), kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ,
+                                                              queue, lb, le, d, H, c);
+      GBK_LAUNCH_CHECK(ctx, "bc_forward_level");
+      GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcCounters), hipMemcpyDeviceToHost, s));
+      GBK_HIP(ctx, hipStreamSynchronize(s));
+      const int64_t tail = static_cast<int64_t>(hc->tail);
+      if (tail > n + 1) return set_error(ctx, GBK_EHIP, "bc: queue overflow");
+      if (tail == le) break;
+      level.push_back(tail);
+    }
+    max_depth = std::max<int>(max_depth, static_cast<int>(level.size()) - 2);
+    for (int lv = static_cast<int>(level.size()) - 2; lv >= 0; lv--) {
+      const int64_t lb = level[lv], cnt = level[lv + 1] - lb;
+      bc_backward_level<<<expand_grid(cnt), kBlock, 0, s>>>(g->off, g->nbr, queue + lb, cnt, sigma,
+                                                           succ, delta, scores, H);
+    }
+    bc_reach_edges<<<grid_for(n), kBlock, 0, s>>>(g->off, depth, n, reach);
+    GBK_LAUNCH_CHECK(ctx, "bc_backward_level");
+  }
+  bc_max<<<grid_for(n), kBlock, 0, s>>>(scores, n, mx);
+  bc_normalize<<<grid_for(n), kBlock, 0, s>>>(scores, n, mx);
+  GBK_LAUNCH_CHECK(ctx, "bc_normalize");
+  const double ms = timer.stop_ms();
+  GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcCounters), hipMemcpyDeviceToHost, s));
+  GBK_HIP(ctx, hipStreamSynchronize(s));
+  if (stats) {
+    stats->device_ms = ms;
+    stats->edges_reached = static_cast<int64_t>(hc->pad[0]);
+    stats->max_depth = max_depth;
+  }
+  return GBK_OK;
+}
+
+extern "C" int gbk_bc(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int num_sources,
+                      float *scores_out, gbk_bc_stats *stats) {
+  if (!ctx || !g || !sources || num_sources <= 0) return set_error(ctx, GBK_EINVAL, "bc: bad arguments");
+  for (int i = 0; i < num_sources; i++)
+    if (sources[i] < 0 || sources[i] >= g->n) return set_error(ctx, GBK_EINVAL, "bc: source out of range");
+  auto *scores = static_cast<float *>(scratch(ctx, kS4, sizeof(float) * (g->n + 1)));
+  if (!scores) return set_error(ctx, GBK_ENOMEM, "bc scores");
+  int rc = gbk_bc_device(ctx, g, sources, num_sources, scores, stats);
+  if (rc) return rc;
+  if (scores_out) {
+    GBK_HIP(ctx, hipMemcpyAsync(scores_out, scores, sizeof(float) * g->n, hipMemcpyDeviceToHost, ctx->stream));
+    GBK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GBK_OK;
+}

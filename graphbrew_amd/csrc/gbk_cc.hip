@@ -1,0 +1,208 @@
+// gbk_cc.hip -- connected components: Afforest (bench/src/cc.cc:82-264) and
+// Shiloach-Vishkin (bench/src/cc_sv.cc:64-136).
+//
+// Both keep the reference's union rules (hook the larger label under the
+// smaller one by CAS), so every label only decreases and stays inside its
+// component; at convergence each component is labelled by its smallest
+// vertex id, which makes the output independent of the schedule.
+// Loads of comp[] that decide a CAS use agent-scope atomics, so a spinning
+// Link always sees other workgroups' hooks (no stale per-CU cache line).
+#include <random>
+#include <unordered_map>
+
+#include "gbk_expand.h"
+
+namespace gbk {
+
+__device__ __forceinline__ void link(int32_t u, int32_t v, int32_t *comp) {
+  int32_t p1 = ld_agent(&comp[u]);
+  int32_t p2 = ld_agent(&comp[v]);
+  while (p1 != p2) {
+    const int32_t high = p1 > p2 ? p1 : p2;
+    const int32_t low = p1 + (p2 - high);
+    const int32_t p_high = ld_agent(&comp[high]);
+    if (p_high == low || (p_high == high && atomicCAS(&comp[high], high, low) == high)) break;
+    p1 = ld_agent(&comp[ld_agent(&comp[high])]);
+    p2 = ld_agent(&comp[low]);
+  }
+}
+
+__global__ void cc_init(int32_t *comp, int64_t n) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < n) comp[v] = static_cast<int32_t>(v);
+}
+
+// Compress (cc.cc:100-124): pointer jumping; each step strictly lowers
+// comp[v], so the loop ends.
+__global__ void cc_compress(int32_t *comp, int64_t n) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int32_t p = ld_agent(&comp[v]);
+  int32_t gp = ld_agent(&comp[p]);
+  while (p != gp) {
+    st_agent(&comp[v], gp);
+    p = gp;
+    gp = ld_agent(&comp[p]);
+  }
+}
+
+// Neighbour round r: link u with its r-th neighbour (cc.cc:171-187).
+__global__ void cc_link_round(const int64_t *off, const int32_t *nbr, int64_t n, int r,
+                              int32_t *comp) {
+  const int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const int64_t k = off[u] + r;
+  if (k < off[u + 1]) link(static_cast<int32_t>(u), nbr[k], comp);
+}
+
+__global__ void cc_gather(const int32_t *comp, const int32_t *idx, int k, int32_t *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) out[i] = comp[idx[i]];
+}
+
+struct LinkEdge {
+  const int32_t *nbr;
+  int32_t *comp;
+  __device__ void operator()(int64_t u, int64_t k, bool active) {
+    if (active) link(static_cast<int32_t>(u), nbr[k], comp);
+  }
+};
+
+// Final phase (cc.cc:192-250): every vertex outside the sampled largest
+// component links its remaining edges (skip = rounds for the out-CSR, 0 for
+// the in-CSR of a directed graph).
+__global__ void __launch_bounds__(kBlock)
+cc_final_link(const int64_t *off, const int32_t *nbr, int64_t n, int64_t skip, int32_t c,
+              int32_t *comp, int64_t heavy_deg) {
+  LinkEdge f{nbr, comp};
+  auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
+    s = off[u] + skip;
+    l = off[u + 1] - s;
+    if (l < 0 || ld_agent(&comp[u]) == c) l = 0;
+  };
+  expand_all(n, heavy_deg, rowlen, f);
+}
+
+struct HookEdge {
+  const int32_t *nbr;
+  int32_t *comp;
+  int changed;
+  __device__ void operator()(int64_t u, int64_t k, bool active) {
+    if (!active) return;
+    const int32_t cu = ld_agent(&comp[u]);
+    const int32_t cv = ld_agent(&comp[nbr[k]]);
+    if (cu == cv) return;
+    const int32_t high = cu > cv ? cu : cv;
+    const int32_t low = cu + (cv - high);
+    if (high == ld_agent(&comp[high]) && atomicCAS(&comp[high], high, low) == high) changed = 1;
+  }
+};
+
+// SV hook over every edge (cc_sv.cc:86-106).
+__global__ void __launch_bounds__(kBlock)
+cc_sv_hook(const int64_t *off, const int32_t *nbr, int64_t n, int32_t *comp, int64_t heavy_deg,
+           int *change) {
+  HookEdge f{nbr, comp, 0};
+  auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
+    s = off[u];
+    l = off[u + 1] - s;
+  };
+  expand_all(n, heavy_deg, rowlen, f);
+  if (f.changed) *change = 1;
+}
+
+}  // namespace gbk
+
+using namespace gbk;
+
+// SampleFrequentElement's indices (cc.cc:127-135): default-seeded mt19937
+// through uniform_int_distribution<int32>(0, n-1), 1024 draws.
+static const std::vector<int32_t> &afforest_samples(int64_t n) {
+  static int64_t cached_n = -1;
+  static std::vector<int32_t> idx;
+  if (cached_n != n) {
+    std::mt19937 gen;
+    std::uniform_int_distribution<int32_t> dist(0, static_cast<int32_t>(n - 1));
+    idx.resize(1024);
+    for (auto &x : idx) x = dist(gen);
+    cached_n = n;
+  }
+  return idx;
+}
+
+int gbk_cc_device(gbk_ctx *ctx, const gbk_graph *g, int variant, int32_t *comp,
+                  gbk_cc_stats *stats) {
+  const int64_t n = g->n;
+  hipStream_t s = ctx->stream;
+  const int64_t H = ctx->tune.heavy_degree;
+  EventTimer timer(ctx);
+  cc_init<<<grid_for(n), kBlock, 0, s>>>(comp, n);
+  int iterations = 0;
+  int32_t skipped = -1;
+  if (variant == GBK_CC_AFFOREST) {
+    const int rounds = 2;
+    for (int r = 0; r < rounds; r++) {
+      cc_link_round<<<grid_for(n), kBlock, 0, s>>>(g->off, g->nbr, n, r, comp);
+      cc_compress<<<grid_for(n), kBlock, 0, s>>>(comp, n);
+    }
+    iterations = rounds;
+    const auto &idx = afforest_samples(n);
+    auto *idx_d = static_cast<int32_t *>(scratch(ctx, kS5, sizeof(int32_t) * 2048));
+    auto *hs = static_cast<int32_t *>(pinned(ctx, sizeof(int32_t) * 1024));
+    if (!idx_d || !hs) return set_error(ctx, GBK_ENOMEM, "cc sample");
+    GBK_HIP(ctx, hipMemcpyAsync(idx_d, idx.data(), sizeof(int32_t) * 1024, hipMemcpyHostToDevice, s));
+    cc_gather<<<4, kBlock, 0, s>>>(comp, idx_d, 1024, idx_d + 1024);
+    GBK_HIP(ctx, hipMemcpyAsync(hs, idx_d + 1024, sizeof(int32_t) * 1024, hipMemcpyDeviceToHost, s));
+    GBK_HIP(ctx, hipStreamSynchronize(s));
+    std::unordered_map<int32_t, int> counts(32);
+    for (int i = 0; i < 1024; i++) counts[hs[i]]++;
+    auto best = counts.begin();
+    for (auto it = counts.begin(); it != counts.end(); ++it)
+      if (best->second < it->second) best = it;
+    skipped = best->first;
+    cc_final_link<<<expand_grid(n), kBlock, 0, s>>>(g->off, g->nbr, n, rounds, skipped, comp, H);
+    if (g->directed)
+      cc_final_link<<<expand_grid(n), kBlock, 0, s>>>(g->in_off, g->in_nbr, n, 0, skipped, comp, H);
+    cc_compress<<<grid_for(n), kBlock, 0, s>>>(comp, n);
+    GBK_LAUNCH_CHECK(ctx, "afforest");
+  } else {
+    auto *change = static_cast<int *>(scratch(ctx, kS5, 64));
+    auto *hc = static_cast<int *>(pinned(ctx, sizeof(int)));
+    if (!change || !hc) return set_error(ctx, GBK_ENOMEM, "sv flag");
+    for (;;) {
+      fill<int>(ctx, change, 0, 1);
+      cc_sv_hook<<<expand_grid(n), kBlock, 0, s>>>(g->off, g->nbr, n, comp, H, change);
+      cc_compress<<<grid_for(n), kBlock, 0, s>>>(comp, n);
+      GBK_LAUNCH_CHECK(ctx, "sv");
+      GBK_HIP(ctx, hipMemcpyAsync(hc, change, sizeof(int), hipMemcpyDeviceToHost, s));
+      GBK_HIP(ctx, hipStreamSynchronize(s));
+      iterations++;
+      if (!*hc) break;
+      // every productive round removes at least one root
+      if (iterations > n + 2) return set_error(ctx, GBK_EHIP, "sv: no convergence");
+    }
+  }
+  const double ms = timer.stop_ms();
+  if (stats) {
+    stats->device_ms = ms;
+    stats->iterations = iterations;
+    stats->skipped_label = skipped;
+  }
+  return GBK_OK;
+}
+
+extern "C" int gbk_cc(gbk_ctx *ctx, const gbk_graph *g, int variant, int32_t *comp_out,
+                      gbk_cc_stats *stats) {
+  if (!ctx || !g || (variant != GBK_CC_AFFOREST && variant != GBK_CC_SV))
+    return set_error(ctx, GBK_EINVAL, "cc: bad arguments");
+  if (g->n == 0) return set_error(ctx, GBK_EINVAL, "cc: empty graph");
+  auto *comp = static_cast<int32_t *>(scratch(ctx, kS4, sizeof(int32_t) * (g->n + 1)));
+  if (!comp) return set_error(ctx, GBK_ENOMEM, "cc comp");
+  int rc = gbk_cc_device(ctx, g, variant, comp, stats);
+  if (rc) return rc;
+  if (comp_out) {
+    GBK_HIP(ctx, hipMemcpyAsync(comp_out, comp, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, ctx->stream));
+    GBK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GBK_OK;
+}

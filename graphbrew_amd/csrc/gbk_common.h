@@ -1,0 +1,219 @@
+// gbk_common.h -- shared device/host definitions for the gfx950 graph kernels.
+//
+// Data layout in HBM (one gbk_graph):
+//   off[n+1]  int64   CSR row offsets (SGOffset, gapbs/graph.h:215)
+//   nbr[m]    int32   neighbour ids, rows sorted ascending
+//   wgt[m]    int32   optional edge weights, parallel to nbr (SoA)
+//   in_*              the in-CSR of a directed graph; aliases the out arrays
+//                     for an undirected graph (gapbs/graph.h:293-299)
+//   heavy_out/heavy_in  ids of rows whose degree exceeds kHeavyDegree, sorted,
+//                     built once at upload; those rows get block-wide work.
+// Everything a kernel needs stays resident; no host round trip per launch
+// except where a frontier decision needs a count on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/gbk.h"
+
+namespace gbk {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;                 // 4 waves per workgroup
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+// ----------------------------------------------------------------- tunables
+struct Tuning {
+  int64_t heavy_degree = 2048;  // rows above this get a whole workgroup
+  int bu_group = 8;             // lanes per vertex in the bottom-up BFS step
+  int bfs_alpha = 15;           // bfs.cc:165 defaults
+  int bfs_beta = 18;
+  int xcd_swizzle = 1;          // contiguous vertex ranges per XCD
+};
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int lanemask_lt_count(uint64_t mask) {
+  // popcount of mask bits below this lane (mbcnt).
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0));
+}
+
+// Block index remapped so each of the 8 XCDs (each with its own L2) gets a
+// contiguous range of logical blocks instead of every 8th one.
+__device__ __forceinline__ int64_t swizzled_block(int enable) {
+  const int64_t g = gridDim.x, b = blockIdx.x;
+  if (!enable || g < 16) return b;
+  const int64_t xcd = b % 8, local = b / 8, per = g / 8, rem = g % 8;
+  return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
+// Wave-aggregated append: every lane with pred=true gets a unique slot in
+// [*tail, *tail + count).  One atomic per wave.
+__device__ __forceinline__ int64_t wave_append(unsigned long long *tail, bool pred) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return -1;
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  unsigned long long base = 0;
+  if (lane_id() == leader) base = atomicAdd(tail, static_cast<unsigned long long>(__popcll(mask)));
+  base = __shfl(base, leader);
+  return pred ? static_cast<int64_t>(base) + lanemask_lt_count(mask) : -1;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    T w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Inclusive scan across the wave.
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int l = lane_id();
+  for (int o = 1; o < kWave; o <<= 1) {
+    T w = __shfl_up(v, o);
+    if (l >= o) v += w;
+  }
+  return v;
+}
+
+// Block-wide sum; every thread gets the result.  'smem' holds kWavesPerBlock.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *smem) {
+  v = wave_sum(v);
+  const int w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane_id() == 0) smem[w] = v;
+  __syncthreads();
+  T t = 0;
+  for (int i = 0; i < kWavesPerBlock; i++) t += smem[i];
+  return t;
+}
+
+__device__ __forceinline__ bool bit_get(const uint32_t *bm, int64_t i) {
+  return (bm[i >> 5] >> (i & 31)) & 1u;
+}
+__device__ __forceinline__ void bit_set_atomic(uint32_t *bm, int64_t i) {
+  atomicOr(&bm[i >> 5], 1u << (i & 31));
+}
+
+// Fill kernel used instead of hipMemsetAsync so that every clear is an
+// ordinary kernel node (safe under stream capture).
+template <typename T>
+__global__ void fill_kernel(T *p, T v, int64_t n) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = v;
+}
+
+inline unsigned grid_for(int64_t items, int per_block = kBlock, int64_t cap = 1 << 20) {
+  int64_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+}  // namespace gbk
+
+// ------------------------------------------------------------ host objects
+struct gbk_graph {
+  int64_t n = 0;
+  int64_t m = 0;          // directed edges in the out-CSR
+  int directed = 0;
+  int weighted = 0;
+  int64_t *off = nullptr;
+  int32_t *nbr = nullptr;
+  int32_t *wgt = nullptr;
+  int64_t *in_off = nullptr;
+  int32_t *in_nbr = nullptr;
+  int32_t *in_wgt = nullptr;
+  int32_t *org_ids = nullptr;  // device copy of original ids (new -> original)
+  // rows with degree > heavy_degree, ascending ids
+  int32_t *heavy_out = nullptr;
+  int64_t n_heavy_out = 0;
+  int32_t *heavy_in = nullptr;
+  int64_t n_heavy_in = 0;
+  int64_t max_out_degree = 0;
+  int64_t max_in_degree = 0;
+  std::vector<void *> owned;   // device allocations freed with the graph
+};
+
+struct gbk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  gbk::Tuning tune;
+  // grow-only scratch slots
+  struct Slot {
+    void *p = nullptr;
+    size_t bytes = 0;
+  };
+  Slot slot[16];
+  void *pinned = nullptr;      // small pinned staging for counters
+  size_t pinned_bytes = 0;
+};
+
+namespace gbk {
+
+// Scratch slot ids (each kernel family documents which it uses).
+enum SlotId {
+  kS0 = 0, kS1, kS2, kS3, kS4, kS5, kS6, kS7, kS8, kS9, kS10, kS11, kS12, kS13, kS14, kS15
+};
+
+int set_error(gbk_ctx *ctx, int code, const std::string &msg);
+void *scratch(gbk_ctx *ctx, int slot, size_t bytes);   // nullptr on failure
+void *pinned(gbk_ctx *ctx, size_t bytes);
+int check_hip(gbk_ctx *ctx, hipError_t e, const char *what);
+
+template <typename T>
+inline void fill(gbk_ctx *ctx, T *p, T v, int64_t n) {
+  if (n <= 0) return;
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 4096) g = 4096;
+  fill_kernel<T><<<static_cast<unsigned>(g), kBlock, 0, ctx->stream>>>(p, v, n);
+}
+
+// Device-wide exclusive scan of int64 counts (out[i] = sum in[0..i), out[n] = total).
+int exclusive_scan_i64(gbk_ctx *ctx, const int64_t *in, int64_t *out, int64_t n);
+// Stable compaction of ids [0,n) where flag(i) != 0 (flags given as int32).
+int compact_flags(gbk_ctx *ctx, const int32_t *flags, int64_t n, int32_t *out, int64_t *count);
+// Heavy-row list for a CSR.
+int build_heavy_list(gbk_ctx *ctx, const int64_t *off, int64_t n, int64_t threshold,
+                     int32_t **list, int64_t *count, int64_t *max_degree,
+                     std::vector<void *> &owned);
+// Records ev0/ev1 around a region on the context stream.
+struct EventTimer {
+  gbk_ctx *ctx;
+  explicit EventTimer(gbk_ctx *c);
+  double stop_ms();  // synchronises
+};
+
+}  // namespace gbk
+
+#define GBK_HIP(ctx, expr)                                              \
+  do {                                                                  \
+    hipError_t _e = (expr);                                             \
+    if (_e != hipSuccess) return gbk::check_hip((ctx), _e, #expr);      \
+  } while (0)
+
+#define GBK_LAUNCH_CHECK(ctx, what)                                     \
+  do {                                                                  \
+    hipError_t _e = hipGetLastError();                                  \
+    if (_e != hipSuccess) return gbk::check_hip((ctx), _e, what);       \
+  } while (0)

@@ -1,0 +1,157 @@
+// gbk_expand.h -- edge-expansion building blocks shared by the kernels.
+//
+// Two shapes of work occur on this path:
+//  * "all rows": every vertex's edge list (PR, CC final link, SV hook).  A
+//    wave owns 64 consecutive rows, whose edges are contiguous in the CSR.
+//  * "frontier": the rows named by a vertex list (TD-BFS, SSSP, BC).  A
+//    wave owns 64 consecutive list entries.
+// In both, lanes hold (row, first edge, length); a wave-wide prefix of the
+// lengths lets the wave walk the union of the 64 edge lists in 64-edge
+// chunks, each lane locating the row of its edge with a 6-step shuffle
+// search.  Neighbour reads are therefore coalesced across row boundaries,
+// and a wave never idles on short rows.  Rows longer than the heavy
+// threshold are taken out of the wave walk and processed by the whole
+// workgroup (256 lanes per row) afterwards, so one hub cannot serialise a
+// wave.  Functors are called with an 'active' flag by every lane of the
+// wave, so they may use wave-aggregated appends.
+#pragma once
+
+#include "gbk_common.h"
+
+namespace gbk {
+
+__device__ __forceinline__ int32_t ld_agent(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(int32_t *p, int32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Walks the union of the lanes' edge ranges.  f(row, edge_index, active).
+template <class F>
+__device__ __forceinline__ void wave_walk(int64_t row, int64_t start, int64_t len, F &f) {
+  const int lane = lane_id();
+  const int64_t incl = wave_incl_scan(len);
+  const int64_t excl = incl - len;
+  const int64_t total = __shfl(incl, kWave - 1);
+  for (int64_t c = 0; c < total; c += kWave) {
+    const int64_t e = c + lane;
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+      const int probe = lo + step;
+      const int64_t x = __shfl(excl, probe & (kWave - 1));
+      if (probe < kWave && x <= e) lo = probe;
+    }
+    const int64_t orow = __shfl(row, lo);
+    const int64_t ostart = __shfl(start, lo);
+    const int64_t oexcl = __shfl(excl, lo);
+    const bool active = e < total;
+    f(orow, active ? ostart + (e - oexcl) : 0, active);
+  }
+}
+
+// Block-cooperative walk of one heavy row: uniform trip count per wave.
+template <class F>
+__device__ __forceinline__ void block_walk(int64_t row, int64_t start, int64_t len, F &f) {
+  for (int64_t base = 0; base < len; base += blockDim.x) {
+    const int64_t e = base + threadIdx.x;
+    const bool active = e < len;
+    f(row, active ? start + e : 0, active);
+  }
+}
+
+// Frontier expansion over list[0, count).  Each workgroup handles tiles of
+// 256 entries (grid-stride); heavy rows of a tile are queued in LDS and
+// walked by the whole workgroup after the light rows.
+// rowlen(row, &start, &len) returns the edge range to walk for that row
+// (len may be 0 to skip it).
+template <class RowLen, class F>
+__device__ void expand_list(const int32_t *list, int64_t count, int64_t heavy_deg, RowLen rowlen,
+                            F &f) {
+  __shared__ int32_t heavy_rows[kBlock];
+  __shared__ int64_t heavy_start[kBlock];
+  __shared__ int64_t heavy_len[kBlock];
+  __shared__ int heavy_n;
+  for (int64_t tile = blockIdx.x; tile * kBlock < count; tile += gridDim.x) {
+    if (threadIdx.x == 0) heavy_n = 0;
+    __syncthreads();
+    const int64_t i = tile * kBlock + threadIdx.x;
+    int64_t row = 0, start = 0, len = 0;
+    if (i < count) {
+      row = list[i];
+      rowlen(row, start, len);
+      if (len > heavy_deg) {
+        const int slot = atomicAdd(&heavy_n, 1);
+        heavy_rows[slot] = static_cast<int32_t>(row);
+        heavy_start[slot] = start;
+        heavy_len[slot] = len;
+        len = 0;
+      }
+    }
+    wave_walk(row, start, len, f);
+    __syncthreads();
+    const int nh = heavy_n;
+    for (int h = 0; h < nh; h++) block_walk(heavy_rows[h], heavy_start[h], heavy_len[h], f);
+    __syncthreads();
+  }
+}
+
+// Same over all rows [0, n) (row = list index).
+template <class RowLen, class F>
+__device__ void expand_all(int64_t n, int64_t heavy_deg, RowLen rowlen, F &f) {
+  __shared__ int32_t heavy_rows[kBlock];
+  __shared__ int64_t heavy_start[kBlock];
+  __shared__ int64_t heavy_len[kBlock];
+  __shared__ int heavy_n;
+  for (int64_t tile = blockIdx.x; tile * kBlock < n; tile += gridDim.x) {
+    if (threadIdx.x == 0) heavy_n = 0;
+    __syncthreads();
+    const int64_t row = tile * kBlock + threadIdx.x;
+    int64_t start = 0, len = 0;
+    if (row < n) {
+      rowlen(row, start, len);
+      if (len > heavy_deg) {
+        const int slot = atomicAdd(&heavy_n, 1);
+        heavy_rows[slot] = static_cast<int32_t>(row);
+        heavy_start[slot] = start;
+        heavy_len[slot] = len;
+        len = 0;
+      }
+    }
+    wave_walk(row, start, len, f);
+    __syncthreads();
+    const int nh = heavy_n;
+    for (int h = 0; h < nh; h++) block_walk(heavy_rows[h], heavy_start[h], heavy_len[h], f);
+    __syncthreads();
+  }
+}
+
+// Number of workgroups for a grid-stride expansion over 'items' entries.
+inline unsigned expand_grid(int64_t items) {
+  int64_t g = (items + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > 8192) g = 8192;
+  return static_cast<unsigned>(g);
+}
+
+// "Last workgroup done" helper: returns true in exactly one thread (thread 0
+// of the last workgroup to arrive) after every workgroup's prior writes are
+// visible.  The counter is reset by that thread.
+__device__ __forceinline__ bool last_block_done(unsigned int *counter) {
+  __shared__ bool is_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    is_last = (prev == gridDim.x - 1);
+    if (is_last) {
+      __threadfence();
+      *counter = 0;
+    }
+  }
+  __syncthreads();
+  return is_last && threadIdx.x == 0;
+}
+
+}  // namespace gbk

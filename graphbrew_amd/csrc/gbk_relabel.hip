@@ -1,0 +1,454 @@
+// gbk_relabel.hip -- RelabelByMapping on the GPU (gapbs/builder.h:1046-1206).
+//
+// Reference: for every old row u, degrees[new_ids[u]] = deg(u); prefix sum;
+// the old row is copied to row new_ids[u] with every neighbour remapped
+// through new_ids; each new row is sorted by (neighbour, weight).
+//
+// Here: (1) one pass inverts the mapping (inv[new] = old, which also proves
+// it is a permutation), writes the new degrees and composes org_ids;
+// (2) device exclusive scan -> new offsets; (3) rows are binned by degree
+// and each bin is gathered, remapped and sorted in one fused kernel, so the
+// neighbour array is read once and the relabelled array written once:
+//   deg <= 16    windows of 128 rows sorted together in LDS
+//   deg <= 128   windows of 16 rows
+//   deg <= 2048  one row per workgroup
+//   larger       2048-element chunks sorted in LDS, then merge-path passes
+//                over a compact staging buffer, then copied into place.
+// Sorting keys: the window kernels sort (row-in-window, neighbour[, weight]);
+// since rows of a window are sorted by row first, each element's output slot
+// is new_off[row] + its rank inside the row.
+#include <algorithm>
+
+#include "gbk_common.h"
+
+int gbk_graph_alloc(gbk_ctx *ctx, int64_t n, int64_t m, int64_t m_in, int directed, int weighted,
+                    gbk_graph **out);
+int gbk_graph_finalize(gbk_ctx *ctx, gbk_graph *g);
+
+namespace gbk {
+
+constexpr int kSortCap = 2048;  // elements per LDS sort (keys 16 KiB + weights 8 KiB)
+
+__global__ void relabel_invert(const int64_t *off, const int32_t *new_ids, const int32_t *org_old,
+                               int64_t n, int32_t *inv, int64_t *deg_new, int32_t *org_new,
+                               int32_t *err) {
+  const int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const int32_t r = new_ids[u];
+  if (r < 0 || r >= n) {
+    atomicOr(err, 1);
+    return;
+  }
+  if (atomicCAS(&inv[r], -1, static_cast<int32_t>(u)) != -1) {
+    atomicOr(err, 2);
+    return;
+  }
+  deg_new[r] = off[u + 1] - off[u];
+  org_new[r] = org_old[u];
+}
+
+// Degree classes 0..3 (see file comment); empty rows need no work.
+__global__ void relabel_bin_rows(const int64_t *deg_new, int64_t n, int32_t *l0, int32_t *l1,
+                                 int32_t *l2, int32_t *l3, unsigned long long *counts) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t d = r < n ? deg_new[r] : 0;
+  const int cls = d == 0 ? -1 : d <= 16 ? 0 : d <= 128 ? 1 : d <= kSortCap ? 2 : 3;
+  int32_t *lists[4] = {l0, l1, l2, l3};
+  for (int c = 0; c < 4; c++) {
+    const int64_t at = wave_append(&counts[c], cls == c);
+    if (cls == c) lists[c][at] = static_cast<int32_t>(r);
+  }
+}
+
+template <bool W>
+struct SortItem {
+  uint64_t key;   // (row-in-window << 32) | neighbour
+  int32_t w;
+};
+
+template <bool W>
+__device__ __forceinline__ bool item_greater(uint64_t ka, int32_t wa, uint64_t kb, int32_t wb) {
+  if (ka != kb) return ka > kb;
+  return W ? (wa > wb) : false;
+}
+
+// Bitonic sort of P (power of two) items in LDS by the whole workgroup.
+template <bool W>
+__device__ void lds_bitonic(uint64_t *key, int32_t *wt, int P) {
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          const uint64_t ka = key[i], kb = key[ixj];
+          const int32_t wa = W ? wt[i] : 0, wb = W ? wt[ixj] : 0;
+          if (item_greater<W>(ka, wa, kb, wb) == up) {
+            key[i] = kb;
+            key[ixj] = ka;
+            if (W) {
+              wt[i] = wb;
+              wt[ixj] = wa;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// One workgroup per window of up to R rows taken from 'rows' (total elements
+// of the window <= kSortCap by construction of the degree class).
+template <bool W>
+__global__ void __launch_bounds__(kBlock)
+relabel_window_sort(const int32_t *rows, int64_t nrows, int R, const int32_t *inv,
+                    const int64_t *off, const int32_t *nbr, const int32_t *wgt,
+                    const int32_t *new_ids, const int64_t *new_off, int32_t *nbr_out,
+                    int32_t *wgt_out) {
+  __shared__ uint64_t key[kSortCap];
+  __shared__ int32_t wt[W ? kSortCap : 1];
+  __shared__ int64_t pre[129];   // prefix of row sizes inside the window
+  __shared__ int32_t row_id[128];
+  __shared__ int64_t src_off[128];
+  const int64_t first = static_cast<int64_t>(blockIdx.x) * R;
+  const int cnt = static_cast<int>(min(static_cast<int64_t>(R), nrows - first));
+  if (threadIdx.x < cnt) {
+    const int32_t r = rows[first + threadIdx.x];
+    row_id[threadIdx.x] = r;
+    const int32_t u = inv[r];
+    src_off[threadIdx.x] = off[u];
+    pre[threadIdx.x + 1] = off[u + 1] - off[u];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    pre[0] = 0;
+    for (int i = 0; i < cnt; i++) pre[i + 1] += pre[i];
+  }
+  __syncthreads();
+  const int total = static_cast<int>(pre[cnt]);
+  int P = 1;
+  while (P < total) P <<= 1;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    if (i < total) {
+      int lo = 0, hi = cnt;  // last row with pre[row] <= i
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= i) lo = mid;
+        else hi = mid;
+      }
+      const int64_t e = src_off[lo] + (i - pre[lo]);
+      const uint32_t v = static_cast<uint32_t>(new_ids[nbr[e]]);
+      key[i] = (static_cast<uint64_t>(lo) << 32) | v;
+      if (W) wt[i] = wgt[e];
+    } else {
+      key[i] = ~0ull;
+      if (W) wt[i] = INT32_MAX;
+    }
+  }
+  __syncthreads();
+  lds_bitonic<W>(key, wt, P);
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int lr = static_cast<int>(key[i] >> 32);
+    const int64_t dst = new_off[row_id[lr]] + (i - pre[lr]);
+    nbr_out[dst] = static_cast<int32_t>(key[i] & 0xffffffffu);
+    if (W) wgt_out[dst] = wt[i];
+  }
+}
+
+// Big rows: composite key so a plain integer compare gives (v, w) order.
+template <bool W>
+struct BigKey {
+  using T = typename std::conditional<W, uint64_t, uint32_t>::type;
+  __device__ static T make(int32_t v, int32_t w) {
+    if constexpr (W)
+      return (static_cast<uint64_t>(static_cast<uint32_t>(v)) << 32) |
+             (static_cast<uint32_t>(w) ^ 0x80000000u);
+    else
+      return static_cast<uint32_t>(v);
+  }
+  __device__ static int32_t v(T k) {
+    if constexpr (W) return static_cast<int32_t>(k >> 32);
+    else return static_cast<int32_t>(k);
+  }
+  __device__ static int32_t w(T k) {
+    if constexpr (W) return static_cast<int32_t>(static_cast<uint32_t>(k) ^ 0x80000000u);
+    else return 1;
+  }
+};
+
+// Chunk sort: chunk c covers elements [cstart, cstart+len) of big row 'row'
+// (staging coordinates: row i of the big list starts at stage_off[i]).
+template <bool W>
+__global__ void __launch_bounds__(kBlock)
+relabel_big_chunks(const int32_t *big_rows, const int64_t *stage_off, const int32_t *chunk_row,
+                   const int64_t *chunk_start, const int32_t *inv, const int64_t *off,
+                   const int32_t *nbr, const int32_t *wgt, const int32_t *new_ids,
+                   typename BigKey<W>::T *stage) {
+  using K = typename BigKey<W>::T;
+  __shared__ uint64_t key[kSortCap];
+  __shared__ int32_t wt[1];
+  const int bi = chunk_row[blockIdx.x];
+  const int64_t cs = chunk_start[blockIdx.x];
+  const int32_t u = inv[big_rows[bi]];
+  const int64_t deg = off[u + 1] - off[u];
+  const int len = static_cast<int>(min(static_cast<int64_t>(kSortCap), deg - cs));
+  int P = 1;
+  while (P < len) P <<= 1;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    if (i < len) {
+      const int64_t e = off[u] + cs + i;
+      key[i] = static_cast<uint64_t>(BigKey<W>::make(new_ids[nbr[e]], W ? wgt[e] : 1));
+    } else {
+      key[i] = ~0ull;
+    }
+  }
+  __syncthreads();
+  lds_bitonic<false>(key, wt, P);
+  for (int i = threadIdx.x; i < len; i += blockDim.x)
+    stage[stage_off[bi] + cs + i] = static_cast<K>(key[i]);
+}
+
+// One merge pass of run width 'width': element t of the staging layout is
+// produced by merging its pair of runs inside its row.  Each thread makes
+// kMergeItems consecutive outputs, re-searching the merge path only when it
+// starts or crosses a pair boundary.
+constexpr int kMergeItems = 8;
+
+template <typename K>
+__global__ void __launch_bounds__(kBlock)
+relabel_merge_pass(const K *src, K *dst, const int64_t *stage_off, int nbig, int64_t total,
+                   int64_t width) {
+  const int64_t t0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kMergeItems;
+  if (t0 >= total) return;
+  // row containing t0
+  int lo = 0, hi = nbig;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (stage_off[mid] <= t0) lo = mid;
+    else hi = mid;
+  }
+  int row = lo;
+  int64_t t = t0;
+  const int64_t tend = min(t0 + kMergeItems, total);
+  while (t < tend) {
+    while (stage_off[row + 1] <= t) row++;
+    const int64_t rs = stage_off[row], rlen = stage_off[row + 1] - rs;
+    const int64_t o = t - rs;
+    const int64_t pb = (o / (2 * width)) * (2 * width);
+    const int64_t la = min(width, rlen - pb);
+    const int64_t lb = max<int64_t>(0, min(width, rlen - pb - width));
+    const K *A = src + rs + pb;
+    const K *Bp = A + la;
+    const int64_t diag = o - pb;
+    int64_t b = max<int64_t>(0, diag - lb), e = min(diag, la);
+    while (b < e) {
+      const int64_t mid = (b + e) >> 1;
+      if (!(Bp[diag - 1 - mid] < A[mid])) b = mid + 1;
+      else e = mid;
+    }
+    int64_t ia = b, ib = diag - b;
+    const int64_t pend = rs + pb + la + lb;  // end of this pair (staging coords)
+    while (t < tend && t < pend) {
+      K x;
+      if (ia < la && (ib >= lb || !(Bp[ib] < A[ia]))) x = A[ia++];
+      else x = Bp[ib++];
+      dst[t++] = x;
+    }
+  }
+}
+
+template <bool W>
+__global__ void relabel_big_scatter(const typename BigKey<W>::T *stage, const int64_t *stage_off,
+                                    const int32_t *big_rows, int nbig, int64_t total,
+                                    const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int lo = 0, hi = nbig;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (stage_off[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  const int64_t dst = new_off[big_rows[lo]] + (t - stage_off[lo]);
+  nbr_out[dst] = BigKey<W>::v(stage[t]);
+  if (W) wgt_out[dst] = BigKey<W>::w(stage[t]);
+}
+
+__global__ void gather_i64(const int64_t *src, const int32_t *idx, int64_t n, int64_t *out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = src[idx[i]];
+}
+
+// Relabels one CSR (out or in) of 'g' into (off_out, nbr_out, wgt_out).
+template <bool W>
+static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_t *nbr,
+                       const int32_t *wgt, const int32_t *new_ids_d, const int32_t *inv,
+                       int64_t *deg_new, int64_t *off_out, int32_t *nbr_out, int32_t *wgt_out,
+                       bool degrees_ready) {
+  hipStream_t s = ctx->stream;
+  if (!degrees_ready) return set_error(ctx, GBK_EINVAL, "relabel: degrees not computed");
+  int rc = exclusive_scan_i64(ctx, deg_new, off_out, n);
+  if (rc) return rc;
+  int32_t *lists[4];
+  for (int c = 0; c < 4; c++) {
+    lists[c] = static_cast<int32_t *>(scratch(ctx, kS4 + c, sizeof(int32_t) * (n + 1)));
+    if (!lists[c]) return set_error(ctx, GBK_ENOMEM, "relabel bins");
+  }
+  auto *counts_d = static_cast<unsigned long long *>(scratch(ctx, kS8, 64));
+  if (!counts_d) return set_error(ctx, GBK_ENOMEM, "relabel counts");
+  fill<unsigned long long>(ctx, counts_d, 0ull, 4);
+  if (n > 0)
+    relabel_bin_rows<<<grid_for(n), kBlock, 0, s>>>(deg_new, n, lists[0], lists[1], lists[2],
+                                                    lists[3], counts_d);
+  GBK_LAUNCH_CHECK(ctx, "relabel_bin_rows");
+  unsigned long long counts[4];
+  GBK_HIP(ctx, hipMemcpyAsync(counts, counts_d, sizeof(counts), hipMemcpyDeviceToHost, s));
+  GBK_HIP(ctx, hipStreamSynchronize(s));
+  const int R[3] = {128, 16, 1};
+  for (int c = 0; c < 3; c++) {
+    const int64_t cnt = static_cast<int64_t>(counts[c]);
+    if (cnt == 0) continue;
+    const int64_t blocks = (cnt + R[c] - 1) / R[c];
+    relabel_window_sort<W><<<blocks, kBlock, 0, s>>>(lists[c], cnt, R[c], inv, off, nbr, wgt,
+                                                      new_ids_d, off_out, nbr_out, wgt_out);
+    GBK_LAUNCH_CHECK(ctx, "relabel_window_sort");
+  }
+  const int nbig = static_cast<int>(counts[3]);
+  if (nbig == 0) return GBK_OK;
+  // Big rows: host builds the staging layout and chunk table (few rows).
+  std::vector<int32_t> big(nbig);
+  std::vector<int64_t> bdeg(nbig);
+  GBK_HIP(ctx, hipMemcpyAsync(big.data(), lists[3], sizeof(int32_t) * nbig, hipMemcpyDeviceToHost, s));
+  GBK_HIP(ctx, hipStreamSynchronize(s));
+  std::sort(big.begin(), big.end());
+  {
+    // degrees of the big rows: deg_new[row], gathered on the device
+    auto *tmp = static_cast<int64_t *>(scratch(ctx, kS9, sizeof(int64_t) * nbig));
+    if (!tmp) return set_error(ctx, GBK_ENOMEM, "relabel big degrees");
+    GBK_HIP(ctx, hipMemcpyAsync(lists[3], big.data(), sizeof(int32_t) * nbig, hipMemcpyHostToDevice, s));
+    gather_i64<<<grid_for(nbig), kBlock, 0, s>>>(deg_new, lists[3], nbig, tmp);
+    GBK_LAUNCH_CHECK(ctx, "gather_i64");
+    GBK_HIP(ctx, hipMemcpyAsync(bdeg.data(), tmp, sizeof(int64_t) * nbig, hipMemcpyDeviceToHost, s));
+    GBK_HIP(ctx, hipStreamSynchronize(s));
+  }
+  std::vector<int64_t> soff(nbig + 1, 0);
+  std::vector<int32_t> crow;
+  std::vector<int64_t> cstart;
+  int64_t maxdeg = 0;
+  for (int i = 0; i < nbig; i++) {
+    soff[i + 1] = soff[i] + bdeg[i];
+    maxdeg = std::max(maxdeg, bdeg[i]);
+    for (int64_t c = 0; c < bdeg[i]; c += kSortCap) {
+      crow.push_back(i);
+      cstart.push_back(c);
+    }
+  }
+  const int64_t total = soff[nbig];
+  using K = typename BigKey<W>::T;
+  const size_t meta_bytes = sizeof(int32_t) * nbig + sizeof(int64_t) * (nbig + 1) +
+                            sizeof(int32_t) * crow.size() + sizeof(int64_t) * cstart.size() + 64;
+  char *meta = static_cast<char *>(scratch(ctx, kS9, meta_bytes));
+  K *st0 = static_cast<K *>(scratch(ctx, kS10, sizeof(K) * total));
+  K *st1 = static_cast<K *>(scratch(ctx, kS11, sizeof(K) * total));
+  if (!meta || !st0 || !st1) return set_error(ctx, GBK_ENOMEM, "relabel staging");
+  auto *big_d = reinterpret_cast<int32_t *>(meta);
+  auto *soff_d = reinterpret_cast<int64_t *>(meta + ((sizeof(int32_t) * nbig + 7) / 8) * 8);
+  auto *crow_d = reinterpret_cast<int32_t *>(soff_d + nbig + 1);
+  auto *cstart_d = reinterpret_cast<int64_t *>(
+      reinterpret_cast<char *>(crow_d) + ((sizeof(int32_t) * crow.size() + 7) / 8) * 8);
+  GBK_HIP(ctx, hipMemcpyAsync(big_d, big.data(), sizeof(int32_t) * nbig, hipMemcpyHostToDevice, s));
+  GBK_HIP(ctx, hipMemcpyAsync(soff_d, soff.data(), sizeof(int64_t) * (nbig + 1), hipMemcpyHostToDevice, s));
+  GBK_HIP(ctx, hipMemcpyAsync(crow_d, crow.data(), sizeof(int32_t) * crow.size(), hipMemcpyHostToDevice, s));
+  GBK_HIP(ctx, hipMemcpyAsync(cstart_d, cstart.data(), sizeof(int64_t) * cstart.size(), hipMemcpyHostToDevice, s));
+  relabel_big_chunks<W><<<static_cast<unsigned>(crow.size()), kBlock, 0, s>>>(
+      big_d, soff_d, crow_d, cstart_d, inv, off, nbr, wgt, new_ids_d, st0);
+  GBK_LAUNCH_CHECK(ctx, "relabel_big_chunks");
+  K *src = st0, *dst = st1;
+  for (int64_t width = kSortCap; width < maxdeg; width *= 2) {
+    const int64_t threads = (total + kMergeItems - 1) / kMergeItems;
+    relabel_merge_pass<K><<<grid_for(threads), kBlock, 0, s>>>(src, dst, soff_d, nbig, total, width);
+    GBK_LAUNCH_CHECK(ctx, "relabel_merge_pass");
+    std::swap(src, dst);
+  }
+  relabel_big_scatter<W><<<grid_for(total), kBlock, 0, s>>>(src, soff_d, big_d, nbig, total, off_out,
+                                                            nbr_out, wgt_out);
+  GBK_LAUNCH_CHECK(ctx, "relabel_big_scatter");
+  GBK_HIP(ctx, hipStreamSynchronize(s));  // staging buffers are reused by the next CSR
+  return GBK_OK;
+}
+
+__global__ void degrees_through_inverse(const int64_t *off, const int32_t *inv, int64_t n,
+                                        int64_t *deg_new) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r < n) {
+    const int32_t u = inv[r];
+    deg_new[r] = off[u + 1] - off[u];
+  }
+}
+
+}  // namespace gbk
+
+using namespace gbk;
+
+extern "C" int gbk_relabel(gbk_ctx *ctx, const gbk_graph *in, const int32_t *new_ids,
+                           gbk_graph **out, gbk_relabel_stats *stats) {
+  if (!ctx || !in || !new_ids || !out) return set_error(ctx, GBK_EINVAL, "relabel: null argument");
+  *out = nullptr;
+  const int64_t n = in->n;
+  hipStream_t s = ctx->stream;
+  auto *ids_d = static_cast<int32_t *>(scratch(ctx, kS0, sizeof(int32_t) * (n + 1)));
+  auto *inv = static_cast<int32_t *>(scratch(ctx, kS1, sizeof(int32_t) * (n + 1)));
+  auto *deg_new = static_cast<int64_t *>(scratch(ctx, kS2, sizeof(int64_t) * (n + 1)));
+  auto *err = static_cast<int32_t *>(scratch(ctx, kS3, 64));
+  if (!ids_d || !inv || !deg_new || !err) return set_error(ctx, GBK_ENOMEM, "relabel scratch");
+  GBK_HIP(ctx, hipMemcpyAsync(ids_d, new_ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  int64_t m_in = in->m;
+  if (in->directed)
+    GBK_HIP(ctx, hipMemcpyAsync(&m_in, in->in_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  GBK_HIP(ctx, hipStreamSynchronize(s));
+  gbk_graph *g = nullptr;
+  int rc = gbk_graph_alloc(ctx, n, in->m, m_in, in->directed, in->weighted, &g);
+  if (rc) return rc;
+  EventTimer timer(ctx);
+  fill<int32_t>(ctx, inv, -1, n);
+  fill<int32_t>(ctx, err, 0, 1);
+  if (n > 0)
+    relabel_invert<<<grid_for(n), kBlock, 0, s>>>(in->off, ids_d, in->org_ids, n, inv, deg_new,
+                                                  g->org_ids, err);
+  int32_t herr = 0;
+  hipError_t e = hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    gbk_graph_free(ctx, g);
+    return check_hip(ctx, e, "relabel_invert");
+  }
+  if (herr) {
+    gbk_graph_free(ctx, g);
+    return set_error(ctx, GBK_EPERM_MAP,
+                     herr & 1 ? "relabel: mapping contains an out-of-range ID"
+                              : "relabel: mapping contains a duplicate ID");
+  }
+  auto run = [&](const int64_t *off, const int32_t *nbr, const int32_t *wgt, int64_t *off_o,
+                 int32_t *nbr_o, int32_t *wgt_o, bool first) -> int {
+    if (!first && n > 0) degrees_through_inverse<<<grid_for(n), kBlock, 0, s>>>(off, inv, n, deg_new);
+    if (in->weighted)
+      return relabel_csr<true>(ctx, n, off, nbr, wgt, ids_d, inv, deg_new, off_o, nbr_o, wgt_o, true);
+    return relabel_csr<false>(ctx, n, off, nbr, wgt, ids_d, inv, deg_new, off_o, nbr_o, wgt_o, true);
+  };
+  rc = run(in->off, in->nbr, in->wgt, g->off, g->nbr, g->wgt, true);
+  if (!rc && in->directed) rc = run(in->in_off, in->in_nbr, in->in_wgt, g->in_off, g->in_nbr, g->in_wgt, false);
+  const double ms = timer.stop_ms();
+  if (!rc) rc = gbk_graph_finalize(ctx, g);
+  if (rc) {
+    gbk_graph_free(ctx, g);
+    return rc;
+  }
+  if (stats) {
+    stats->device_ms = ms;
+    stats->n = n;
+    stats->m = in->m;
+  }
+  *out = g;
+  return GBK_OK;
+}

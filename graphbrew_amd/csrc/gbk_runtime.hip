@@ -1,0 +1,440 @@
+// gbk_runtime.hip -- context, scratch memory, device scan/compaction and
+// graph upload/download for libgbk.so.
+#include <cstring>
+
+#include "gbk_common.h"
+
+namespace gbk {
+
+int set_error(gbk_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int check_hip(gbk_ctx *ctx, hipError_t e, const char *what) {
+  if (e == hipSuccess) return GBK_OK;
+  const int code = (e == hipErrorOutOfMemory) ? GBK_ENOMEM : GBK_EHIP;
+  return set_error(ctx, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void *scratch(gbk_ctx *ctx, int slot, size_t bytes) {
+  auto &s = ctx->slot[slot];
+  if (bytes == 0) bytes = 1;
+  if (s.bytes >= bytes) return s.p;
+  if (s.p) {
+    hipStreamSynchronize(ctx->stream);
+    hipFree(s.p);
+    s.p = nullptr;
+    s.bytes = 0;
+  }
+  // Round up so small growth steps do not reallocate every call.
+  size_t want = bytes + bytes / 8 + 4096;
+  if (hipMalloc(&s.p, want) != hipSuccess) {
+    s.p = nullptr;
+    return nullptr;
+  }
+  s.bytes = want;
+  return s.p;
+}
+
+void *pinned(gbk_ctx *ctx, size_t bytes) {
+  if (ctx->pinned_bytes >= bytes) return ctx->pinned;
+  if (ctx->pinned) hipHostFree(ctx->pinned);
+  ctx->pinned = nullptr;
+  ctx->pinned_bytes = 0;
+  if (hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+  ctx->pinned_bytes = bytes;
+  return ctx->pinned;
+}
+
+EventTimer::EventTimer(gbk_ctx *c) : ctx(c) { hipEventRecord(ctx->ev0, ctx->stream); }
+
+double EventTimer::stop_ms() {
+  hipEventRecord(ctx->ev1, ctx->stream);
+  hipEventSynchronize(ctx->ev1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  return ms;
+}
+
+// ------------------------------------------------------------------- scan
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+template <typename In>
+__global__ void scan_reduce_tiles(const In *in, int64_t n, int64_t *tile_sums) {
+  __shared__ int64_t red[kWavesPerBlock];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  int64_t s = 0;
+  for (int i = 0; i < kScanItems; i++) {
+    const int64_t idx = base + static_cast<int64_t>(i) * kBlock + threadIdx.x;
+    if (idx < n) s += static_cast<int64_t>(in[idx]);
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = s;
+}
+
+// One workgroup scans the tile sums in place (exclusive) and writes the total.
+__global__ void scan_tile_sums(int64_t *tile_sums, int64_t tiles, int64_t *total_out) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (tiles + blockDim.x - 1) / blockDim.x;
+  const int64_t b = t * per, e = (b + per < tiles) ? b + per : tiles;
+  int64_t s = 0;
+  for (int64_t i = b; i < e; i++) s += tile_sums[i];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int64_t run = 0;
+    for (int i = 0; i < static_cast<int>(blockDim.x); i++) {
+      const int64_t x = part[i];
+      part[i] = run;
+      run += x;
+    }
+    *total_out = run;
+  }
+  __syncthreads();
+  int64_t run = part[t];
+  for (int64_t i = b; i < e; i++) {
+    const int64_t x = tile_sums[i];
+    tile_sums[i] = run;
+    run += x;
+  }
+}
+
+template <typename In>
+__global__ void scan_tiles(const In *in, int64_t n, const int64_t *tile_prefix, int64_t *out) {
+  __shared__ int64_t wsum[kWavesPerBlock];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  // each thread owns kScanItems consecutive elements
+  const int64_t my = base + static_cast<int64_t>(threadIdx.x) * kScanItems;
+  int64_t v[kScanItems];
+  int64_t s = 0;
+  for (int i = 0; i < kScanItems; i++) {
+    v[i] = (my + i < n) ? static_cast<int64_t>(in[my + i]) : 0;
+    s += v[i];
+  }
+  int64_t incl = wave_incl_scan(s);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) wsum[w] = incl;
+  __syncthreads();
+  int64_t wprefix = 0;
+  for (int i = 0; i < w; i++) wprefix += wsum[i];
+  int64_t run = tile_prefix[blockIdx.x] + wprefix + incl - s;
+  for (int i = 0; i < kScanItems; i++) {
+    if (my + i < n) out[my + i] = run;
+    run += v[i];
+  }
+}
+
+template <typename In>
+static int exclusive_scan_impl(gbk_ctx *ctx, const In *in, int64_t *out, int64_t n) {
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  auto *tile_sums = static_cast<int64_t *>(scratch(ctx, kS15, sizeof(int64_t) * (tiles + 1)));
+  if (!tile_sums) return set_error(ctx, GBK_ENOMEM, "scan scratch");
+  if (tiles > 0) {
+    scan_reduce_tiles<In><<<tiles, kBlock, 0, ctx->stream>>>(in, n, tile_sums);
+  }
+  scan_tile_sums<<<1, 1024, 0, ctx->stream>>>(tile_sums, tiles, out + n);
+  if (tiles > 0) {
+    scan_tiles<In><<<tiles, kBlock, 0, ctx->stream>>>(in, n, tile_sums, out);
+  }
+  GBK_LAUNCH_CHECK(ctx, "exclusive_scan");
+  return GBK_OK;
+}
+
+int exclusive_scan_i64(gbk_ctx *ctx, const int64_t *in, int64_t *out, int64_t n) {
+  return exclusive_scan_impl<int64_t>(ctx, in, out, n);
+}
+
+__global__ void scatter_flagged(const int32_t *flags, const int64_t *pos, int64_t n, int32_t *out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && flags[i]) out[pos[i]] = static_cast<int32_t>(i);
+}
+
+int compact_flags(gbk_ctx *ctx, const int32_t *flags, int64_t n, int32_t *out, int64_t *count) {
+  auto *pos = static_cast<int64_t *>(scratch(ctx, kS14, sizeof(int64_t) * (n + 1)));
+  if (!pos) return set_error(ctx, GBK_ENOMEM, "compact scratch");
+  int rc = exclusive_scan_impl<int32_t>(ctx, flags, pos, n);
+  if (rc) return rc;
+  if (n > 0) scatter_flagged<<<grid_for(n), kBlock, 0, ctx->stream>>>(flags, pos, n, out);
+  GBK_LAUNCH_CHECK(ctx, "scatter_flagged");
+  GBK_HIP(ctx, hipMemcpyAsync(count, pos + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  GBK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GBK_OK;
+}
+
+// ------------------------------------------------------------ heavy lists
+__global__ void flag_heavy(const int64_t *off, int64_t n, int64_t threshold, int32_t *flags,
+                           unsigned long long *max_deg) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t d = 0;
+  if (i < n) {
+    d = off[i + 1] - off[i];
+    flags[i] = d > threshold ? 1 : 0;
+  }
+  d = wave_max(d);
+  if (lane_id() == 0) atomicMax(max_deg, static_cast<unsigned long long>(d));
+}
+
+int build_heavy_list(gbk_ctx *ctx, const int64_t *off, int64_t n, int64_t threshold,
+                     int32_t **list, int64_t *count, int64_t *max_degree,
+                     std::vector<void *> &owned) {
+  auto *flags = static_cast<int32_t *>(scratch(ctx, kS13, sizeof(int32_t) * (n + 1)));
+  auto *mx = static_cast<unsigned long long *>(scratch(ctx, kS12, 64));
+  if (!flags || !mx) return set_error(ctx, GBK_ENOMEM, "heavy scratch");
+  fill<unsigned long long>(ctx, mx, 0ull, 1);
+  if (n > 0) flag_heavy<<<grid_for(n), kBlock, 0, ctx->stream>>>(off, n, threshold, flags, mx);
+  GBK_LAUNCH_CHECK(ctx, "flag_heavy");
+  int32_t *out = nullptr;
+  // worst case n entries; most graphs have few heavy rows, so compact into
+  // scratch first and copy the exact size.
+  auto *tmp = static_cast<int32_t *>(scratch(ctx, kS11, sizeof(int32_t) * (n + 1)));
+  if (!tmp) return set_error(ctx, GBK_ENOMEM, "heavy scratch");
+  int64_t cnt = 0;
+  int rc = compact_flags(ctx, flags, n, tmp, &cnt);
+  if (rc) return rc;
+  GBK_HIP(ctx, hipMalloc(&out, sizeof(int32_t) * (cnt + 1)));
+  owned.push_back(out);
+  if (cnt > 0)
+    GBK_HIP(ctx, hipMemcpyAsync(out, tmp, sizeof(int32_t) * cnt, hipMemcpyDeviceToDevice, ctx->stream));
+  unsigned long long h = 0;
+  GBK_HIP(ctx, hipMemcpyAsync(&h, mx, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  GBK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *list = out;
+  *count = cnt;
+  *max_degree = static_cast<int64_t>(h);
+  return GBK_OK;
+}
+
+}  // namespace gbk
+
+using namespace gbk;
+
+// ----------------------------------------------------------------- C ABI
+extern "C" {
+
+int gbk_ctx_create(int device, gbk_ctx **out) {
+  if (!out) return GBK_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return GBK_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return GBK_EHIP;
+  auto *ctx = new gbk_ctx();
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    delete ctx;
+    return GBK_EHIP;
+  }
+  *out = ctx;
+  return GBK_OK;
+}
+
+int gbk_ctx_destroy(gbk_ctx *ctx) {
+  if (!ctx) return GBK_EINVAL;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto &s : ctx->slot)
+    if (s.p) hipFree(s.p);
+  if (ctx->pinned) hipHostFree(ctx->pinned);
+  hipEventDestroy(ctx->ev0);
+  hipEventDestroy(ctx->ev1);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return GBK_OK;
+}
+
+const char *gbk_last_error(const gbk_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void *gbk_ctx_stream(gbk_ctx *ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
+
+int gbk_ctx_sync(gbk_ctx *ctx) {
+  if (!ctx) return GBK_EINVAL;
+  GBK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GBK_OK;
+}
+
+int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value) {
+  if (!ctx || !name) return GBK_EINVAL;
+  const std::string k(name);
+  auto &t = ctx->tune;
+  if (k == "heavy_degree" && value >= 64) t.heavy_degree = value;
+  else if (k == "bu_group" && (value == 1 || value == 2 || value == 4 || value == 8 ||
+                               value == 16 || value == 32 || value == 64))
+    t.bu_group = static_cast<int>(value);
+  else if (k == "bfs_alpha" && value > 0) t.bfs_alpha = static_cast<int>(value);
+  else if (k == "bfs_beta" && value > 0) t.bfs_beta = static_cast<int>(value);
+  else if (k == "xcd_swizzle" && (value == 0 || value == 1)) t.xcd_swizzle = static_cast<int>(value);
+  else return set_error(ctx, GBK_EINVAL, "unknown parameter or bad value: " + k);
+  return GBK_OK;
+}
+
+int gbk_ctx_get_param(gbk_ctx *ctx, const char *name, int64_t *value) {
+  if (!ctx || !name || !value) return GBK_EINVAL;
+  const std::string k(name);
+  const auto &t = ctx->tune;
+  if (k == "heavy_degree") *value = t.heavy_degree;
+  else if (k == "bu_group") *value = t.bu_group;
+  else if (k == "bfs_alpha") *value = t.bfs_alpha;
+  else if (k == "bfs_beta") *value = t.bfs_beta;
+  else if (k == "xcd_swizzle") *value = t.xcd_swizzle;
+  else return set_error(ctx, GBK_EINVAL, "unknown parameter: " + k);
+  return GBK_OK;
+}
+
+int gbk_graph_info(const gbk_graph *g, int64_t *n, int64_t *m, int *directed, int *weighted) {
+  if (!g) return GBK_EINVAL;
+  if (n) *n = g->n;
+  if (m) *m = g->m;
+  if (directed) *directed = g->directed;
+  if (weighted) *weighted = g->weighted;
+  return GBK_OK;
+}
+
+int gbk_graph_free(gbk_ctx *ctx, gbk_graph *g) {
+  if (!ctx || !g) return GBK_EINVAL;
+  hipStreamSynchronize(ctx->stream);
+  for (void *p : g->owned) hipFree(p);
+  delete g;
+  return GBK_OK;
+}
+
+}  // extern "C"
+
+// Builds heavy lists for both CSRs of a freshly filled device graph.
+int gbk_graph_finalize(gbk_ctx *ctx, gbk_graph *g) {
+  int rc = build_heavy_list(ctx, g->off, g->n, ctx->tune.heavy_degree, &g->heavy_out,
+                            &g->n_heavy_out, &g->max_out_degree, g->owned);
+  if (rc) return rc;
+  if (g->directed) {
+    rc = build_heavy_list(ctx, g->in_off, g->n, ctx->tune.heavy_degree, &g->heavy_in,
+                          &g->n_heavy_in, &g->max_in_degree, g->owned);
+    if (rc) return rc;
+  } else {
+    g->heavy_in = g->heavy_out;
+    g->n_heavy_in = g->n_heavy_out;
+    g->max_in_degree = g->max_out_degree;
+  }
+  return GBK_OK;
+}
+
+// Allocates the device arrays of a graph with the given shape.
+int gbk_graph_alloc(gbk_ctx *ctx, int64_t n, int64_t m, int64_t m_in, int directed, int weighted,
+                    gbk_graph **out) {
+  auto *g = new gbk_graph();
+  g->n = n;
+  g->m = m;
+  g->directed = directed;
+  g->weighted = weighted;
+  auto alloc = [&](void **p, size_t bytes) {
+    if (bytes == 0) bytes = 4;
+    if (hipMalloc(p, bytes) != hipSuccess) return false;
+    g->owned.push_back(*p);
+    return true;
+  };
+  bool ok = alloc(reinterpret_cast<void **>(&g->off), sizeof(int64_t) * (n + 1)) &&
+            alloc(reinterpret_cast<void **>(&g->nbr), sizeof(int32_t) * m) &&
+            alloc(reinterpret_cast<void **>(&g->org_ids), sizeof(int32_t) * n);
+  if (ok && weighted) ok = alloc(reinterpret_cast<void **>(&g->wgt), sizeof(int32_t) * m);
+  if (ok && directed) {
+    ok = alloc(reinterpret_cast<void **>(&g->in_off), sizeof(int64_t) * (n + 1)) &&
+         alloc(reinterpret_cast<void **>(&g->in_nbr), sizeof(int32_t) * m_in);
+    if (ok && weighted) ok = alloc(reinterpret_cast<void **>(&g->in_wgt), sizeof(int32_t) * m_in);
+  } else if (ok) {
+    g->in_off = g->off;
+    g->in_nbr = g->nbr;
+    g->in_wgt = g->wgt;
+  }
+  if (!ok) {
+    for (void *p : g->owned) hipFree(p);
+    delete g;
+    return set_error(ctx, GBK_ENOMEM, "graph allocation");
+  }
+  *out = g;
+  return GBK_OK;
+}
+
+__global__ void iota_i32(int32_t *a, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = static_cast<int32_t>(i);
+}
+
+extern "C" int gbk_graph_upload(gbk_ctx *ctx, int64_t n, int directed, const int64_t *off,
+                                const int32_t *nbr, const int32_t *wgt, const int64_t *in_off,
+                                const int32_t *in_nbr, const int32_t *in_wgt,
+                                const int32_t *org_ids, gbk_graph **out) {
+  if (!ctx || !out || !off || n < 0 || n > INT32_MAX) return set_error(ctx, GBK_EINVAL, "upload: bad arguments");
+  *out = nullptr;
+  if (off[0] != 0) return set_error(ctx, GBK_EINVAL, "upload: off[0] must be 0");
+  const int64_t m = off[n];
+  if (m < 0 || (m > 0 && !nbr)) return set_error(ctx, GBK_EINVAL, "upload: bad edge count");
+  int64_t m_in = m;
+  if (directed) {
+    if (!in_off || (in_off[n] > 0 && !in_nbr) || in_off[0] != 0)
+      return set_error(ctx, GBK_EINVAL, "upload: directed graph needs its in-CSR");
+    m_in = in_off[n];
+    if (wgt && !in_wgt) return set_error(ctx, GBK_EINVAL, "upload: in-CSR weights missing");
+  }
+  hipSetDevice(ctx->device);
+  gbk_graph *g = nullptr;
+  int rc = gbk_graph_alloc(ctx, n, m, m_in, directed, wgt != nullptr, &g);
+  if (rc) return rc;
+  auto cp = [&](void *d, const void *h, size_t b) {
+    return b == 0 ? hipSuccess : hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, ctx->stream);
+  };
+  hipError_t e = cp(g->off, off, sizeof(int64_t) * (n + 1));
+  if (e == hipSuccess) e = cp(g->nbr, nbr, sizeof(int32_t) * m);
+  if (e == hipSuccess && wgt) e = cp(g->wgt, wgt, sizeof(int32_t) * m);
+  if (e == hipSuccess && directed) {
+    e = cp(g->in_off, in_off, sizeof(int64_t) * (n + 1));
+    if (e == hipSuccess) e = cp(g->in_nbr, in_nbr, sizeof(int32_t) * m_in);
+    if (e == hipSuccess && wgt) e = cp(g->in_wgt, in_wgt, sizeof(int32_t) * m_in);
+  }
+  if (e == hipSuccess) {
+    if (org_ids) e = cp(g->org_ids, org_ids, sizeof(int32_t) * n);
+    else if (n > 0) iota_i32<<<grid_for(n), kBlock, 0, ctx->stream>>>(g->org_ids, n);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    gbk_graph_free(ctx, g);
+    return check_hip(ctx, e, "upload copy");
+  }
+  rc = gbk_graph_finalize(ctx, g);
+  if (rc) {
+    gbk_graph_free(ctx, g);
+    return rc;
+  }
+  *out = g;
+  return GBK_OK;
+}
+
+extern "C" int gbk_graph_download(gbk_ctx *ctx, const gbk_graph *g, int64_t *off, int32_t *nbr,
+                                  int32_t *wgt, int64_t *in_off, int32_t *in_nbr, int32_t *in_wgt,
+                                  int32_t *org_ids) {
+  if (!ctx || !g) return GBK_EINVAL;
+  auto cp = [&](void *h, const void *d, size_t b) {
+    return (!h || b == 0) ? hipSuccess : hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, ctx->stream);
+  };
+  const int64_t m_in = g->directed ? -1 : g->m;
+  GBK_HIP(ctx, cp(off, g->off, sizeof(int64_t) * (g->n + 1)));
+  GBK_HIP(ctx, cp(nbr, g->nbr, sizeof(int32_t) * g->m));
+  if (wgt && !g->weighted) return set_error(ctx, GBK_ESTATE, "download: graph is unweighted");
+  if (g->weighted) GBK_HIP(ctx, cp(wgt, g->wgt, sizeof(int32_t) * g->m));
+  GBK_HIP(ctx, cp(org_ids, g->org_ids, sizeof(int32_t) * g->n));
+  if (g->directed) {
+    int64_t mi = 0;
+    GBK_HIP(ctx, hipMemcpyAsync(&mi, g->in_off + g->n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    GBK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GBK_HIP(ctx, cp(in_off, g->in_off, sizeof(int64_t) * (g->n + 1)));
+    GBK_HIP(ctx, cp(in_nbr, g->in_nbr, sizeof(int32_t) * mi));
+    if (g->weighted) GBK_HIP(ctx, cp(in_wgt, g->in_wgt, sizeof(int32_t) * mi));
+  } else {
+    (void)m_in;
+    GBK_HIP(ctx, cp(in_off, g->off, sizeof(int64_t) * (g->n + 1)));
+    GBK_HIP(ctx, cp(in_nbr, g->nbr, sizeof(int32_t) * g->m));
+    if (g->weighted) GBK_HIP(ctx, cp(in_wgt, g->wgt, sizeof(int32_t) * g->m));
+  }
+  GBK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GBK_OK;
+}

@@ -1,0 +1,271 @@
+"""ctypes binding of libgbk.so (include/gbk.h).
+
+The library is the only compute path: there is no CPU fallback.  Importing
+this module does not load it; the first Context() does, and raises
+GbkUnavailable when libgbk.so is missing or no GPU is visible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libgbk.so")
+
+GBK_OK, GBK_EINVAL, GBK_EPERM_MAP, GBK_EHIP, GBK_ENOMEM, GBK_ESTATE = 0, -1, -2, -3, -4, -5
+PR_GS, PR_SPMV = 0, 1
+CC_AFFOREST, CC_SV = 0, 1
+DIST_INF = (2 ** 31 - 1) // 2
+
+
+class GbkUnavailable(RuntimeError):
+    pass
+
+
+class GbkError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gbk error {code}: {msg}")
+        self.code = code
+
+
+class PrStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("iterations", C.c_int), ("converged_iter", C.c_int),
+                ("final_error", C.c_double)]
+
+
+class BfsStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("steps_td", C.c_int), ("steps_bu", C.c_int),
+                ("reached", C.c_int64), ("edges_reached", C.c_int64)]
+
+
+class CcStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("iterations", C.c_int), ("skipped_label", C.c_int32)]
+
+
+class SsspStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("buckets", C.c_int64), ("relax_rounds", C.c_int64),
+                ("reached", C.c_int64), ("edges_reached", C.c_int64)]
+
+
+class BcStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("edges_reached", C.c_int64), ("max_depth", C.c_int)]
+
+
+class RelabelStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("n", C.c_int64), ("m", C.c_int64)]
+
+
+_lib = None
+VP = C.c_void_p
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(VP)
+
+
+def lib():
+    """Loads libgbk.so and declares its signatures (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GbkUnavailable(f"{LIB_PATH} is not built; run graphbrew_amd.build.build()")
+    L = C.CDLL(LIB_PATH)
+    i32, i64, f64, pp = C.c_int, C.c_int64, C.c_double, C.POINTER(VP)
+    sig = {
+        "gbk_ctx_create": (i32, [i32, pp]),
+        "gbk_ctx_destroy": (i32, [VP]),
+        "gbk_last_error": (C.c_char_p, [VP]),
+        "gbk_ctx_stream": (VP, [VP]),
+        "gbk_ctx_sync": (i32, [VP]),
+        "gbk_ctx_set_param": (i32, [VP, C.c_char_p, i64]),
+        "gbk_ctx_get_param": (i32, [VP, C.c_char_p, C.POINTER(i64)]),
+        "gbk_graph_upload": (i32, [VP, i64, i32, VP, VP, VP, VP, VP, VP, VP, pp]),
+        "gbk_graph_info": (i32, [VP, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32), C.POINTER(i32)]),
+        "gbk_graph_download": (i32, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+        "gbk_graph_free": (i32, [VP, VP]),
+        "gbk_relabel": (i32, [VP, VP, VP, pp, C.POINTER(RelabelStats)]),
+        "gbk_pr": (i32, [VP, VP, i32, i32, f64, i32, VP, C.POINTER(PrStats)]),
+        "gbk_bfs": (i32, [VP, VP, C.c_int32, i32, i32, VP, C.POINTER(BfsStats)]),
+        "gbk_cc": (i32, [VP, VP, i32, VP, C.POINTER(CcStats)]),
+        "gbk_sssp": (i32, [VP, VP, C.c_int32, C.c_int32, VP, C.POINTER(SsspStats)]),
+        "gbk_bc": (i32, [VP, VP, VP, i32, VP, C.POINTER(BcStats)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols() -> list:
+    """Names declared in include/gbk.h (the C ABI surface)."""
+    import re
+    hdr = open(os.path.join(os.path.dirname(_PKG), "include", "gbk.h")).read()
+    return sorted(set(re.findall(r"^(?:int|const char \*|void \*)\s*\*?(gbk_\w+)\(", hdr, re.M)))
+
+
+@dataclass
+class DeviceGraph:
+    ctx: "Context"
+    handle: int
+    n: int
+    m: int
+    directed: bool
+    weighted: bool
+
+    def download(self) -> dict:
+        L = lib()
+        out = {"off": np.empty(self.n + 1, np.int64), "nbr": np.empty(max(self.m, 1), np.int32),
+               "org_ids": np.empty(max(self.n, 1), np.int32)}
+        out["wgt"] = np.empty(max(self.m, 1), np.int32) if self.weighted else None
+        if self.directed:
+            out["in_off"] = np.empty(self.n + 1, np.int64)
+        self.ctx._check(L.gbk_graph_download(self.ctx.h, self.handle, _ptr(out["off"]), _ptr(out["nbr"]),
+                                             _ptr(out["wgt"]), _ptr(out.get("in_off")), None, None,
+                                             _ptr(out["org_ids"])))
+        if self.directed:
+            mi = int(out["in_off"][-1])
+            out["in_nbr"] = np.empty(max(mi, 1), np.int32)
+            out["in_wgt"] = np.empty(max(mi, 1), np.int32) if self.weighted else None
+            self.ctx._check(L.gbk_graph_download(self.ctx.h, self.handle, None, None, None, None,
+                                                 _ptr(out["in_nbr"]), _ptr(out["in_wgt"]), None))
+            out["in_nbr"] = out["in_nbr"][:mi]
+            if out["in_wgt"] is not None:
+                out["in_wgt"] = out["in_wgt"][:mi]
+        out["nbr"] = out["nbr"][:self.m]
+        if out["wgt"] is not None:
+            out["wgt"] = out["wgt"][:self.m]
+        out["org_ids"] = out["org_ids"][:self.n]
+        return out
+
+    def free(self):
+        if self.handle:
+            lib().gbk_graph_free(self.ctx.h, self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+def _a(x, dt):
+    return None if x is None else np.ascontiguousarray(x, dt)
+
+
+class Context:
+    """One HIP stream on one device; every call is stream-ordered on it."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = VP()
+        rc = L.gbk_ctx_create(device, C.byref(h))
+        if rc != GBK_OK:
+            raise GbkUnavailable(f"gbk_ctx_create({device}) failed with {rc}: no usable GPU")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().gbk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != GBK_OK:
+            msg = lib().gbk_last_error(self.h)
+            raise GbkError(rc, msg.decode() if msg else "")
+
+    @property
+    def stream(self) -> int:
+        return lib().gbk_ctx_stream(self.h)
+
+    def sync(self):
+        self._check(lib().gbk_ctx_sync(self.h))
+
+    def set_param(self, name: str, value: int):
+        self._check(lib().gbk_ctx_set_param(self.h, name.encode(), int(value)))
+
+    def get_param(self, name: str) -> int:
+        v = C.c_int64(0)
+        self._check(lib().gbk_ctx_get_param(self.h, name.encode(), C.byref(v)))
+        return v.value
+
+    # ------------------------------------------------------------- graphs
+    def upload(self, n, off, nbr, wgt=None, in_off=None, in_nbr=None, in_wgt=None,
+               org_ids=None, directed=False) -> DeviceGraph:
+        off, nbr, wgt = _a(off, np.int64), _a(nbr, np.int32), _a(wgt, np.int32)
+        in_off, in_nbr, in_wgt = _a(in_off, np.int64), _a(in_nbr, np.int32), _a(in_wgt, np.int32)
+        org_ids = _a(org_ids, np.int32)
+        h = VP()
+        self._check(lib().gbk_graph_upload(self.h, int(n), int(bool(directed)), _ptr(off), _ptr(nbr),
+                                           _ptr(wgt), _ptr(in_off), _ptr(in_nbr), _ptr(in_wgt),
+                                           _ptr(org_ids), C.byref(h)))
+        return self._wrap(h)
+
+    def upload_graph(self, g) -> DeviceGraph:
+        """Uploads any object with the oracle/host Graph attributes."""
+        return self.upload(g.n, g.off, g.nbr, g.wgt, g.in_off if g.directed else None,
+                           g.in_nbr if g.directed else None, g.in_wgt if g.directed else None,
+                           getattr(g, "org_ids", None), g.directed)
+
+    def _wrap(self, h) -> DeviceGraph:
+        n, m, d, w = C.c_int64(), C.c_int64(), C.c_int(), C.c_int()
+        self._check(lib().gbk_graph_info(h, C.byref(n), C.byref(m), C.byref(d), C.byref(w)))
+        return DeviceGraph(self, h.value, n.value, m.value, bool(d.value), bool(w.value))
+
+    # ------------------------------------------------------------ kernels
+    def relabel(self, g: DeviceGraph, new_ids):
+        ids = _a(new_ids, np.int32)
+        if len(ids) != g.n:
+            raise ValueError("mapping length differs from the vertex count")
+        h = VP()
+        st = RelabelStats()
+        self._check(lib().gbk_relabel(self.h, g.handle, _ptr(ids), C.byref(h), C.byref(st)))
+        return self._wrap(h), st
+
+    def pr(self, g: DeviceGraph, variant=PR_GS, max_iters=20, tol=1e-4, fixed=False, download=True):
+        out = np.empty(g.n, np.float32) if download else None
+        st = PrStats()
+        self._check(lib().gbk_pr(self.h, g.handle, variant, max_iters, tol, int(fixed), _ptr(out),
+                                 C.byref(st)))
+        return out, st
+
+    def bfs(self, g: DeviceGraph, source: int, alpha=0, beta=0, download=True):
+        out = np.empty(g.n, np.int32) if download else None
+        st = BfsStats()
+        self._check(lib().gbk_bfs(self.h, g.handle, int(source), alpha, beta, _ptr(out), C.byref(st)))
+        return out, st
+
+    def cc(self, g: DeviceGraph, variant=CC_AFFOREST, download=True):
+        out = np.empty(g.n, np.int32) if download else None
+        st = CcStats()
+        self._check(lib().gbk_cc(self.h, g.handle, variant, _ptr(out), C.byref(st)))
+        return out, st
+
+    def sssp(self, g: DeviceGraph, source: int, delta: int = 1, download=True):
+        out = np.empty(g.n, np.int32) if download else None
+        st = SsspStats()
+        self._check(lib().gbk_sssp(self.h, g.handle, int(source), int(delta), _ptr(out), C.byref(st)))
+        return out, st
+
+    def bc(self, g: DeviceGraph, sources: Sequence[int], download=True):
+        s = np.ascontiguousarray(sources, np.int32)
+        out = np.empty(g.n, np.float32) if download else None
+        st = BcStats()
+        self._check(lib().gbk_bc(self.h, g.handle, _ptr(s), len(s), _ptr(out), C.byref(st)))
+        return out, st

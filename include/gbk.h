@@ -1,0 +1,150 @@
+/*
+ * gbk.h -- C ABI of the MI355X (gfx950) graph-kernel library libgbk.so.
+ *
+ * Drop-in boundary for GraphBrew's hot path: the builder's relabel step, the
+ * five GAPBS kernels (seven binaries) and the timing harness.  Host arrays
+ * are plain pointers + sizes; no C++ or torch types cross this boundary and
+ * no exception escapes it.  Every call returns GBK_OK (0) or a negative
+ * GBK_E* code; gbk_last_error() gives the message.
+ *
+ * Reference interface each entry point replaces (paths relative to the
+ * reference root; gapbs/ = bench/include/external/gapbs/):
+ *   gbk_graph_upload       CSRGraph(num_nodes, index, neighs[, inv...])  gapbs/graph.h:310-330
+ *   gbk_relabel            BuilderBase::RelabelByMapping                 gapbs/builder.h:1046-1206
+ *   gbk_pr (GS)            PageRankPullGS                                bench/src/pr.cc:40-94
+ *   gbk_pr (SPMV)          PageRankPull                                  bench/src/pr_spmv.cc:40-83
+ *   gbk_bfs                DOBFS                                         bench/src/bfs.cc:164-269
+ *   gbk_cc (AFFOREST)      Afforest                                      bench/src/cc.cc:153-264
+ *   gbk_cc (SV)            ShiloachVishkin                               bench/src/cc_sv.cc:64-136
+ *   gbk_sssp               DeltaStep                                     bench/src/sssp.cc:160-283
+ *   gbk_bc                 Brandes                                       bench/src/bc.cc:135-237
+ * Results are written to caller-owned host buffers after the device work;
+ * the stats structs carry the device time of the kernel region (hipEvents on
+ * the context stream), which is what the harness prints as kernel time.
+ *
+ * Threading: one context = one HIP stream; a context is not thread-safe.
+ * Graphs belong to the context that created them.
+ */
+#ifndef GBK_H_
+#define GBK_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GBK_OK 0
+#define GBK_EINVAL -1      /* bad argument (null pointer, size, source id) */
+#define GBK_EPERM_MAP -2   /* mapping is not a permutation of [0, n) */
+#define GBK_EHIP -3        /* HIP runtime error */
+#define GBK_ENOMEM -4      /* device allocation failed */
+#define GBK_ESTATE -5      /* graph lacks what the call needs (e.g. weights) */
+
+typedef struct gbk_ctx gbk_ctx;
+typedef struct gbk_graph gbk_graph;
+
+enum gbk_pr_variant { GBK_PR_GS = 0, GBK_PR_SPMV = 1 };
+enum gbk_cc_variant { GBK_CC_AFFOREST = 0, GBK_CC_SV = 1 };
+
+typedef struct {
+  double device_ms;        /* device time of the whole call's kernels */
+  int iterations;          /* executed iterations */
+  int converged_iter;      /* first iteration with error < tol, -1 if none */
+  double final_error;      /* L1 change of the last iteration (pr.cc:62-76) */
+} gbk_pr_stats;
+
+typedef struct {
+  double device_ms;
+  int steps_td, steps_bu;  /* top-down / bottom-up steps executed */
+  int64_t reached;         /* vertices with parent >= 0 */
+  int64_t edges_reached;   /* sum of out-degrees of reached vertices (m_r) */
+} gbk_bfs_stats;
+
+typedef struct {
+  double device_ms;
+  int iterations;          /* SV hook rounds, or Afforest neighbour rounds */
+  int32_t skipped_label;   /* Afforest's sampled largest component label */
+} gbk_cc_stats;
+
+typedef struct {
+  double device_ms;
+  int64_t buckets;         /* buckets settled */
+  int64_t relax_rounds;    /* frontier relaxation launches that did work */
+  int64_t reached;
+  int64_t edges_reached;
+} gbk_sssp_stats;
+
+typedef struct {
+  double device_ms;
+  int64_t edges_reached;   /* sum over sources of m_r */
+  int max_depth;
+} gbk_bc_stats;
+
+typedef struct {
+  double device_ms;
+  int64_t n, m;
+} gbk_relabel_stats;
+
+/* ---- context --------------------------------------------------------- */
+int gbk_ctx_create(int device, gbk_ctx **out);
+int gbk_ctx_destroy(gbk_ctx *ctx);
+const char *gbk_last_error(const gbk_ctx *ctx);
+/* hipStream_t of the context, for callers that time or order work on it. */
+void *gbk_ctx_stream(gbk_ctx *ctx);
+int gbk_ctx_sync(gbk_ctx *ctx);
+/* Tunables: "heavy_degree", "bu_group", "bfs_alpha", "bfs_beta",
+ * "xcd_swizzle".  Returns GBK_EINVAL for an unknown name or bad value. */
+int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value);
+int gbk_ctx_get_param(gbk_ctx *ctx, const char *name, int64_t *value);
+
+/* ---- graphs ------------------------------------------------------------ */
+/* Copies a host CSR to the device.  n vertices; off has n+1 entries with
+ * off[0] == 0; rows sorted.  wgt may be NULL (unweighted).  For directed
+ * graphs the in-CSR (in_off/in_nbr[/in_wgt]) is required; for undirected
+ * graphs pass NULL and the out arrays serve as in arrays.  org_ids (length
+ * n, new id -> original id) may be NULL for the identity. */
+int gbk_graph_upload(gbk_ctx *ctx, int64_t n, int directed,
+                     const int64_t *off, const int32_t *nbr, const int32_t *wgt,
+                     const int64_t *in_off, const int32_t *in_nbr,
+                     const int32_t *in_wgt, const int32_t *org_ids,
+                     gbk_graph **out);
+int gbk_graph_info(const gbk_graph *g, int64_t *n, int64_t *m, int *directed,
+                   int *weighted);
+/* Copies the device CSR back.  Any pointer may be NULL to skip that array. */
+int gbk_graph_download(gbk_ctx *ctx, const gbk_graph *g, int64_t *off,
+                       int32_t *nbr, int32_t *wgt, int64_t *in_off,
+                       int32_t *in_nbr, int32_t *in_wgt, int32_t *org_ids);
+int gbk_graph_free(gbk_ctx *ctx, gbk_graph *g);
+
+/* ---- relabel (RelabelByMapping) --------------------------------------- */
+/* new_ids[old] = new (host array, n entries).  Builds a new device graph
+ * whose rows are the old rows moved to their new ids, neighbours remapped
+ * and each row sorted by (neighbour, weight); org_ids are composed as in
+ * update_org_ids (gapbs/graph.h:535-564).  A non-permutation returns
+ * GBK_EPERM_MAP and creates nothing. */
+int gbk_relabel(gbk_ctx *ctx, const gbk_graph *in, const int32_t *new_ids,
+                gbk_graph **out, gbk_relabel_stats *stats);
+
+/* ---- kernels ----------------------------------------------------------- */
+/* scores: host float[n] (may be NULL to leave the result on the device). */
+int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_iters,
+           double tol, int fixed_work, float *scores, gbk_pr_stats *stats);
+/* parent: host int32[n], -1 for unreached; alpha/beta <= 0 use defaults. */
+int gbk_bfs(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha,
+            int beta, int32_t *parent, gbk_bfs_stats *stats);
+/* comp: host int32[n]; labels are the smallest vertex id of each component. */
+int gbk_cc(gbk_ctx *ctx, const gbk_graph *g, int variant, int32_t *comp,
+           gbk_cc_stats *stats);
+/* dist: host int32[n], INT32_MAX/2 for unreached (sssp.cc:67). */
+int gbk_sssp(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t delta,
+             int32_t *dist, gbk_sssp_stats *stats);
+/* scores: host float[n], normalised by the maximum (bc.cc:224-230). */
+int gbk_bc(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources,
+           int num_sources, float *scores, gbk_bc_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GBK_H_ */

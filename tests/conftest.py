@@ -8,3 +8,15 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+
+
+import pytest  # noqa: E402
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    """One gbk context for the whole GPU session (fails loudly without a GPU)."""
+    from graphbrew_amd import gbk
+    c = gbk.Context(0)
+    yield c
+    c.close()
