@@ -1,0 +1,270 @@
+#!/usr/bin/env python3
+"""Benchmark of the GraphBrew hot path on one MI355X.
+
+Workload (one "step"): on a seeded symmetrized Kronecker graph (scale 22,
+edge factor 16, a=.57 b=.19 c=.19; SSSP weights U[1,255]) resident in HBM:
+relabel under the chosen ordering, then PR pull (Gauss-Seidel, <=20 iters,
+tol 1e-4), DO-BFS from 64 sources, CC Afforest, CC Shiloach-Vishkin,
+delta-stepping SSSP from 64 sources and Brandes BC from 4 sources, sources
+picked by the reference's SourcePicker (original ids, ordering-invariant).
+
+value = whole-step traversed edges / step time, in GTEPS (each kernel's
+TEPS numerator as the reference defines it).  Per-kernel GTEPS and the
+fraction of the 8 TB/s HBM roofline (algorithmic bytes) are reported too;
+'roofline' describes the kernel with the largest share of device time.
+cpu_baseline times the CPU oracle (an OpenMP restatement of the reference
+kernels) on a bounded sample of the same graph on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--scale", type=int, default=22)
+    p.add_argument("--degree", type=int, default=16)
+    p.add_argument("--uniform", action="store_true")
+    p.add_argument("--order", default="identity", choices=["identity", "random", "sort"])
+    p.add_argument("--bfs-sources", type=int, default=64)
+    p.add_argument("--sssp-sources", type=int, default=64)
+    p.add_argument("--bc-sources", type=int, default=4)
+    p.add_argument("--delta", type=int, default=64)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU sample work")
+    p.add_argument("--only", default="", help="comma list of kernels to run (profiling)")
+    p.add_argument("--verbose", action="store_true")
+    return p.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# Algorithmic HBM bytes (compulsory traffic; o = offset width = 8 B here).
+def bytes_relabel(n, m, weighted=False):
+    return 8 * m + (2 * 8 + 8) * n + (8 * m if weighted else 0)
+
+
+def bytes_pr_iter(n, m):
+    return 4 * m + (8 + 16) * n
+
+
+def bytes_bfs(n, m_r):
+    return 4 * m_r + (8 + 8) * n + n / 4
+
+
+def bytes_cc(n, m):
+    return 4 * m + (8 + 8) * n + 8 * n
+
+
+def bytes_sv_iter(n, m):
+    return 4 * m + (8 + 8) * n
+
+
+def bytes_sssp(n, m_r):
+    return 8 * m_r + (8 + 8) * n
+
+
+def bytes_bc(n, m_r):
+    return 8 * m_r + m_r / 4 + (2 * 8 + 40) * n
+
+
+class Workload:
+    def __init__(self, args, ctx, gbk, H):
+        self.args, self.ctx, self.gbk, self.H = args, ctx, gbk, H
+        t0 = time.time()
+        u, v = H.generate_el(args.scale, args.degree, args.uniform)
+        log(f"generated {len(u)} edges in {time.time() - t0:.1f}s")
+        self.g, ms = ctx.build(u, v, None, -1, True)
+        w = H.insert_weights(len(u))
+        self.gw, ms2 = ctx.build(u, v, w, self.g.n, True)
+        del u, v, w
+        log(f"device CSR build: n={self.g.n} m={self.g.m} ({ms:.1f} ms, weighted {ms2:.1f} ms)")
+        host_off = self.g.download()["off"] if args.order != "random" else None
+        n = self.g.n
+        if args.order == "identity":
+            self.ids = H.mapping_identity(n)
+        elif args.order == "random":
+            self.ids = H.mapping_random(n)
+        else:
+            self.ids = H.mapping_sort(host_off)
+        self.fingerprint = H.mapping_fingerprint(self.ids)
+        self.kernels = [k for k in args.only.split(",") if k] or \
+            ["relabel", "pr", "bfs", "cc", "cc_sv", "sssp", "bc"]
+        self.rg = None
+        self.rgw = None
+
+    def step(self, rec):
+        """One pass of the hot path; rec[name] accumulates (device ms, edges, bytes, calls)."""
+        ctx, gbk, H, a = self.ctx, self.gbk, self.H, self.args
+
+        def add(name, ms, edges, nbytes, calls=1):
+            r = rec.setdefault(name, [0.0, 0, 0.0, 0])
+            r[0] += ms
+            r[1] += edges
+            r[2] += nbytes
+            r[3] += calls
+
+        n = self.g.n
+        if self.rg is not None:
+            self.rg.free()
+            self.rgw.free()
+        self.rg, st = ctx.relabel(self.g, self.ids)
+        if "relabel" in self.kernels:
+            add("relabel", st.device_ms, self.g.m, bytes_relabel(n, self.g.m))
+        self.rgw, stw = ctx.relabel(self.gw, self.ids)
+        g, gw = self.rg, self.rgw
+        m = g.m
+        if not hasattr(self, "srcs"):
+            off = g.download()["off"]
+            org = g.download()["org_ids"]
+            self.srcs = H.source_picker(off, max(a.bfs_sources, a.sssp_sources, a.bc_sources), org)[1]
+            self.deg = np.diff(off)
+        if "pr" in self.kernels:
+            _, st = ctx.pr(g, gbk.PR_GS, 20, 1e-4, False, download=False)
+            add("pr", st.device_ms, m * st.iterations, bytes_pr_iter(n, m) * st.iterations, st.iterations)
+        if "bfs" in self.kernels:
+            for s in self.srcs[:a.bfs_sources]:
+                _, st = ctx.bfs(g, int(s), download=False)
+                add("bfs", st.device_ms, st.edges_reached, bytes_bfs(n, st.edges_reached))
+        if "cc" in self.kernels:
+            _, st = ctx.cc(g, gbk.CC_AFFOREST, download=False)
+            add("cc", st.device_ms, m, bytes_cc(n, m))
+        if "cc_sv" in self.kernels:
+            _, st = ctx.cc(g, gbk.CC_SV, download=False)
+            add("cc_sv", st.device_ms, m * st.iterations, bytes_sv_iter(n, m) * st.iterations, st.iterations)
+        if "sssp" in self.kernels:
+            for s in self.srcs[:a.sssp_sources]:
+                _, st = ctx.sssp(gw, int(s), a.delta, download=False)
+                add("sssp", st.device_ms, st.edges_reached, bytes_sssp(n, st.edges_reached))
+        if "bc" in self.kernels:
+            _, st = ctx.bc(g, self.srcs[:a.bc_sources], download=False)
+            add("bc", st.device_ms, 2 * st.edges_reached, bytes_bc(n, st.edges_reached) , a.bc_sources)
+
+
+def cpu_baseline(work, budget_s):
+    """CPU oracle (OpenMP restatement of the reference kernels) on a bounded
+    sample of the same relabelled graph: PR 2 fixed iterations, then BFS and
+    SSSP sources one at a time, CC once, BC one source, until the budget."""
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    d = work.rg.download()
+    g = O.Graph(n=work.rg.n, off=d["off"], nbr=d["nbr"], org_ids=d["org_ids"])
+    dw = work.rgw.download()
+    gw = O.Graph(n=work.rgw.n, off=dw["off"], nbr=dw["nbr"], wgt=dw["wgt"], org_ids=dw["org_ids"])
+    srcs = work.srcs
+    deg = np.diff(g.off)
+    edges, secs, parts = 0, 0.0, []
+    t = time.time()
+    O.pr_gs(g, max_iters=2, tol=0.0, fixed=True, threads=threads)
+    dt = time.time() - t
+    edges += 2 * g.m
+    secs += dt
+    parts.append("pr x2 iters")
+    t = time.time()
+    O.cc_afforest(g, threads=threads)
+    secs += time.time() - t
+    edges += g.m
+    parts.append("cc afforest")
+    i = 0
+    while secs < budget_s and i < len(srcs):
+        s = int(srcs[i])
+        t = time.time()
+        p = O.bfs_do(g, s, threads=threads)
+        secs += time.time() - t
+        edges += int(deg[p >= 0].sum())
+        t = time.time()
+        dist = O.sssp_delta(gw, s, work.args.delta, threads=threads)
+        secs += time.time() - t
+        edges += int(deg[dist != O.K_DIST_INF].sum())
+        i += 1
+    parts.append(f"bfs+sssp x{i} sources")
+    return {"value": edges / secs / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"scale-{work.args.scale} graph, {', '.join(parts)}; {secs:.1f}s of CPU work"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
+        return 2
+    from graphbrew_amd import gbk
+    from graphbrew_amd import host as H
+    import torch
+
+    ctx = gbk.Context(0)
+    work = Workload(args, ctx, gbk, H)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    for w in range(args.warmup):
+        work.step({})
+        log(f"warmup step {w} done")
+    ctx.sync()
+    torch.cuda.synchronize()
+    rec = {}
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.time()
+    ev0.record(stream)
+    for s in range(args.steps):
+        work.step(rec)
+        log(f"step {s} done")
+    ev1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.time() - t0
+    total_ms = ev0.elapsed_time(ev1)
+    n, m = work.g.n, work.g.m
+    kern = {}
+    total_edges = 0
+    for name, (ms, edges, nbytes, calls) in rec.items():
+        secs = ms / 1e3
+        total_edges += edges
+        kern[name] = {"ms_per_step": ms / args.steps, "gteps": edges / secs / 1e9 if secs else 0.0,
+                      "gbps": nbytes / secs / 1e9 if secs else 0.0,
+                      "hbm_frac": nbytes / secs / HBM_PEAK if secs else 0.0,
+                      "calls_per_step": calls / args.steps}
+    dom = max(rec.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dedges, dbytes, dcalls) = dom
+    achieved = dbytes / (dms / 1e3) / 1e9
+    value = total_edges / (total_ms / 1e3) / 1e9
+    out = {
+        "metric": "GTEPS (all five kernels + GPU relabel, aggregate) with %HBM roofline per kernel",
+        "value": round(value, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(total_ms / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (seeded Kronecker generator of the reference, weights U[1,255])",
+        "config": {"workload": f"{'uniform' if args.uniform else 'kronecker'}-scale{args.scale}-ef{args.degree}"
+                   f" relabel+pr+bfs{args.bfs_sources}+cc+cc_sv+sssp{args.sssp_sources}+bc{args.bc_sources}",
+                   "n": n, "m_directed": m, "order": args.order, "mapping_fingerprint": work.fingerprint,
+                   "delta": args.delta},
+        "kernels": kern,
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
+                     "traffic": None},
+        "wall_s": round(wall, 2),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(work, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -60,30 +60,43 @@ def build_lib(jobs: int = 8) -> str:
     return LIB
 
 
+HOST_LIB = os.path.join(PKG, "libgbkhost.so")
+
+
+def build_host_lib(jobs: int = 8) -> str:
+    """libgbkhost.so: generator, readers, mappings, sources (include/gbk_host.h)."""
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = [s for s in sorted(glob.glob(os.path.join(HOST, "*.cc")))
+            if os.path.basename(s)[:-3] not in DRIVERS and not os.path.basename(s).startswith("drv_")]
+    cxx = shutil.which("g++") or "g++"
+    objs, cmds = [], []
+    for s in srcs:
+        o = os.path.join(OBJ, "host_" + os.path.basename(s)[:-3] + ".o")
+        objs.append(o)
+        cmds.append([cxx, *HOST_FLAGS, "-fPIC", "-c", s, "-o", o])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
+    tmp = HOST_LIB + ".tmp"
+    _run([cxx, "-shared", "-fopenmp", *objs, "-o", tmp])
+    os.replace(tmp, HOST_LIB)
+    return HOST_LIB
+
+
 def build_drivers(jobs: int = 8) -> list:
-    """C++17 host drivers mirroring bench/bin/<kernel> over libgbk.so."""
+    """C++17 drivers mirroring bench/bin/<kernel> over libgbk.so + libgbkhost.so."""
     srcs = sorted(glob.glob(os.path.join(HOST, "*.cc")))
     mains = {os.path.basename(s)[:-3] for s in srcs} & set(DRIVERS)
     if not mains:
         return []
     os.makedirs(BIN, exist_ok=True)
-    common = [s for s in srcs if os.path.basename(s)[:-3] not in DRIVERS]
     cxx = shutil.which("g++") or "g++"
-    common_objs = []
-    cmds = []
-    for s in common:
-        o = os.path.join(OBJ, "host_" + os.path.basename(s)[:-3] + ".o")
-        common_objs.append(o)
-        cmds.append([cxx, *HOST_FLAGS, "-fPIC", "-c", s, "-o", o])
-    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        list(ex.map(_run, cmds))
-    out = []
-    cmds = []
+    shared = [s for s in sorted(glob.glob(os.path.join(HOST, "drv_*.cc")))]
+    out, cmds = [], []
     for name in sorted(mains):
         exe = os.path.join(BIN, name)
         out.append(exe)
-        cmds.append([cxx, *HOST_FLAGS, os.path.join(HOST, name + ".cc"), *common_objs,
-                     "-L", PKG, "-lgbk", f"-Wl,-rpath,$ORIGIN/..", "-o", exe])
+        cmds.append([cxx, *HOST_FLAGS, os.path.join(HOST, name + ".cc"), *shared,
+                     "-L", PKG, "-lgbk", "-lgbkhost", "-Wl,-rpath,$ORIGIN/..", "-o", exe])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     return out
@@ -91,6 +104,7 @@ def build_drivers(jobs: int = 8) -> list:
 
 def build(jobs: int = 8) -> None:
     build_lib(jobs)
+    build_host_lib(jobs)
     build_drivers(jobs)
 
 

@@ -18,7 +18,10 @@ namespace gbk {
 
 struct BcCounters {
   unsigned long long tail;
-  unsigned long long pad[3];
+  unsigned long long reach;   // sum over sources of m_r
+  unsigned long long max_bits;
+  int overflow;
+  int pad;
 };
 
 __global__ void bc_init_source(int64_t n, int32_t source, int32_t *depth, double *sigma,
@@ -32,6 +35,7 @@ __global__ void bc_init_source(int64_t n, int32_t source, int32_t *depth, double
   if (v == 0) {
     queue[0] = source;
     c->tail = 1;
+    c->overflow = 0;
   }
 }
 
@@ -40,8 +44,7 @@ struct BcForwardEdge {
   int32_t *depth;
   double *sigma;
   uint32_t *succ;
-  int32_t *queue;
-  unsigned long long *tail;
+  WaveQueue q;
   int64_t edge_base;
   int32_t d;
   __device__ void operator()(int64_t u, int64_t k, bool active) {
@@ -49,27 +52,42 @@ struct BcForwardEdge {
     int32_t v = 0;
     if (active) {
       v = nbr[k];
-      if (ld_agent(&depth[v]) == -1 && atomicCAS(&depth[v], -1, d) == -1) claimed = true;
-      if (ld_agent(&depth[v]) == d) {
+      // A plain load sees either a depth fixed in an earlier level (final)
+      // or -1 / d from this level; only -1 needs the CAS, whose return value
+      // is the coherent depth (bc.cc:104-112 test it twice).
+      int32_t x = depth[v];
+      if (x == -1) {
+        x = atomicCAS(&depth[v], -1, d);
+        claimed = x == -1;
+      }
+      if (x == -1 || x == d) {
         bit_set_atomic(succ, k - edge_base);
         unsafeAtomicAdd(&sigma[v], sigma[u]);
       }
     }
-    const int64_t at = wave_append(tail, claimed);
-    if (claimed) queue[at] = v;
+    q.push(claimed, v);
   }
 };
 
 __global__ void __launch_bounds__(kBlock)
 bc_forward_level(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigma,
-                 uint32_t *succ, int32_t *queue, int64_t lb, int64_t le, int32_t d,
+                 uint32_t *succ, int32_t *queue, int64_t lb, int64_t le, int64_t cap, int32_t d,
                  int64_t heavy_deg, BcCounters *c) {
-  BcForwardEdge f{nbr, depth, sigma, succ, queue, &c->tail, off[0], d};
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  BcForwardEdge f;
+  f.nbr = nbr;
+  f.depth = depth;
+  f.sigma = sigma;
+  f.succ = succ;
+  f.q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
+  f.edge_base = off[0];
+  f.d = d;
   auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
     s = off[u];
     l = off[u + 1] - s;
   };
   expand_list(queue + lb, le - lb, heavy_deg, rowlen, f);
+  f.q.flush();
 }
 
 // One level of the dependency accumulation.
@@ -119,14 +137,20 @@ bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, i
       const int64_t ostart = __shfl(start, lo);
       const int64_t oexcl = __shfl(excl, lo);
       const double osig = __shfl(su, lo);
-      if (e < total) {
+      const bool act = e < total;
+      float term = 0.0f;
+      if (act) {
         const int64_t k = ostart + (e - oexcl);
         if (bit_get(succ, k - edge_base)) {
           const int32_t v = nbr[k];
-          atomicAdd(&acc[wv * kWave + lo], static_cast<float>(osig / sigma[v] * (1.0 + delta[v])));
+          term = static_cast<float>(osig / sigma[v] * (1.0 + delta[v]));
         }
       }
+      bool tail;
+      const float run = seg_run_sum(term, act ? lo : kWave, tail);
+      if (act && tail) acc[wv * kWave + lo] += run;
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     __syncthreads();
     if (light) {
       const float du = acc[threadIdx.x];
@@ -156,25 +180,41 @@ bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, i
 }
 
 __global__ void bc_reach_edges(const int64_t *off, const int32_t *depth, int64_t n,
-                               unsigned long long *acc) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+                               unsigned long long *partial) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   unsigned long long e = 0;
-  if (v < n && depth[v] >= 0) e = static_cast<unsigned long long>(off[v + 1] - off[v]);
-  e = wave_sum(e);
-  if (lane_id() == 0 && e) atomicAdd(acc, e);
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride)
+    if (depth[v] >= 0) e += static_cast<unsigned long long>(off[v + 1] - off[v]);
+  block_partial(e, partial);
 }
 
-__global__ void bc_max(const float *scores, int64_t n, unsigned int *mx) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  float x = v < n ? scores[v] : 0.0f;
+// Max over non-negative floats via their bit patterns (ordered like the values).
+__global__ void bc_max(const float *scores, int64_t n, unsigned int *partial) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  unsigned int x = 0;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride)
+    x = max(x, __float_as_uint(scores[v]));
   x = wave_max(x);
-  // non-negative floats order like their bit patterns
-  if (lane_id() == 0) atomicMax(mx, __float_as_uint(x));
+  __shared__ unsigned int red[kWavesPerBlock];
+  if (lane_id() == 0) red[threadIdx.x / kWave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < kWavesPerBlock; i++) x = max(x, red[i]);
+    partial[blockIdx.x] = x;
+  }
 }
 
-__global__ void bc_normalize(float *scores, int64_t n, const unsigned int *mx) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v < n) scores[v] = scores[v] / __uint_as_float(*mx);
+__global__ void bc_normalize(float *scores, int64_t n, const unsigned int *partial, int count) {
+  __shared__ float biggest;
+  if (threadIdx.x == 0) {
+    unsigned int x = 0;
+    for (int i = 0; i < count; i++) x = max(x, partial[i]);
+    biggest = __uint_as_float(x);
+  }
+  __syncthreads();
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride)
+    scores[v] = scores[v] / biggest;
 }
 
 }  // namespace gbk
@@ -195,12 +235,14 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
   auto *hc = static_cast<BcCounters *>(pinned(ctx, sizeof(BcCounters)));
   if (!depth || !sigma || !delta || !queue || !succ || !c || !hc)
     return set_error(ctx, GBK_ENOMEM, "bc scratch");
-  auto *reach = reinterpret_cast<unsigned long long *>(&c->pad[0]);
-  auto *mx = reinterpret_cast<unsigned int *>(&c->pad[1]);
+  auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS11, sizeof(unsigned long long) * kGridCap));
+  if (!partial) return set_error(ctx, GBK_ENOMEM, "bc partials");
   const int64_t H = ctx->tune.heavy_degree;
+  const int64_t cap = n + 64;
+  const unsigned ng = expand_grid(n);
   EventTimer timer(ctx);
   fill<float>(ctx, scores, 0.0f, n);
-  fill<unsigned long long>(ctx, reinterpret_cast<unsigned long long *>(&c->pad[0]), 0ull, 2);
+  fill<unsigned long long>(ctx, &c->reach, 0ull, 1);
   int max_depth = 0;
   std::vector<int64_t> level;
   for (int it = 0; it < num_sources; it++) {
@@ -213,12 +255,12 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
       d++;
       const int64_t lb = level[level.size() - 2], le = level.back();
       bc_forward_level<<<expand_grid(le - lb), kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ,
-                                                              queue, lb, le, d, H, c);
+                                                              queue, lb, le, cap, d, H, c);
       GBK_LAUNCH_CHECK(ctx, "bc_forward_level");
       GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcCounters), hipMemcpyDeviceToHost, s));
       GBK_HIP(ctx, hipStreamSynchronize(s));
       const int64_t tail = static_cast<int64_t>(hc->tail);
-      if (tail > n + 1) return set_error(ctx, GBK_EHIP, "bc: queue overflow");
+      if (hc->overflow || tail > n + 1) return set_error(ctx, GBK_EHIP, "bc: queue overflow");
       if (tail == le) break;
       level.push_back(tail);
     }
@@ -228,18 +270,20 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
       bc_backward_level<<<expand_grid(cnt), kBlock, 0, s>>>(g->off, g->nbr, queue + lb, cnt, sigma,
                                                            succ, delta, scores, H);
     }
-    bc_reach_edges<<<grid_for(n), kBlock, 0, s>>>(g->off, depth, n, reach);
+    bc_reach_edges<<<ng, kBlock, 0, s>>>(g->off, depth, n, partial);
+    sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial, static_cast<int>(ng), &c->reach, 1);
     GBK_LAUNCH_CHECK(ctx, "bc_backward_level");
   }
-  bc_max<<<grid_for(n), kBlock, 0, s>>>(scores, n, mx);
-  bc_normalize<<<grid_for(n), kBlock, 0, s>>>(scores, n, mx);
+  auto *pmax = reinterpret_cast<unsigned int *>(partial);
+  bc_max<<<ng, kBlock, 0, s>>>(scores, n, pmax);
+  bc_normalize<<<ng, kBlock, 0, s>>>(scores, n, pmax, static_cast<int>(ng));
   GBK_LAUNCH_CHECK(ctx, "bc_normalize");
   const double ms = timer.stop_ms();
   GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcCounters), hipMemcpyDeviceToHost, s));
   GBK_HIP(ctx, hipStreamSynchronize(s));
   if (stats) {
     stats->device_ms = ms;
-    stats->edges_reached = static_cast<int64_t>(hc->pad[0]);
+    stats->edges_reached = static_cast<int64_t>(hc->reach);
     stats->max_depth = max_depth;
   }
   return GBK_OK;

@@ -6,13 +6,14 @@
 // edges_to_check / alpha, and returns to top-down once the awake count
 // shrinks and is at most n / beta.
 //   top-down   frontier queue -> coalesced expansion (gbk_expand.h), a CAS
-//              claims each newly seen vertex; winners are appended with one
-//              atomic per wave (ballot + mbcnt).
+//              claims each newly seen vertex; winners are staged per wave in
+//              LDS and flushed to the next queue in batches.
 //   bottom-up  bitmap frontier; 'bu_group' lanes per unvisited vertex read
 //              its in-neighbours in chunks and stop at the first frontier
 //              hit (masked ballot inside the lane group).
-// The host reads one small counter block per step to apply the switching
-// rule; kernels never wait on each other inside a launch.
+// Counts (scout, awake) are per-workgroup partials summed by a one-block
+// finish launch; the host reads one small counter block per step to apply
+// the switching rule.
 #include "gbk_expand.h"
 
 namespace gbk {
@@ -21,7 +22,8 @@ struct BfsCounters {
   unsigned long long tail;   // queue tail
   unsigned long long scout;  // top-down scout count
   unsigned long long awake;  // bottom-up awake count
-  unsigned long long pad;
+  int overflow;
+  int pad;
 };
 
 __global__ void bfs_init(const int64_t *out_off, int64_t n, int32_t source, int32_t *parent,
@@ -36,14 +38,14 @@ __global__ void bfs_init(const int64_t *out_off, int64_t n, int32_t source, int3
     c->tail = 1;
     c->scout = 0;
     c->awake = 0;
+    c->overflow = 0;
   }
 }
 
 struct TdEdge {
   const int32_t *nbr;
   int32_t *parent;
-  int32_t *queue;
-  unsigned long long *tail;
+  WaveQueue q;
   unsigned long long scout;
   __device__ void operator()(int64_t u, int64_t k, bool active) {
     bool won = false;
@@ -56,62 +58,68 @@ struct TdEdge {
         scout += static_cast<unsigned long long>(-static_cast<int64_t>(cur));
       }
     }
-    const int64_t at = wave_append(tail, won);
-    if (won) queue[at] = v;
+    q.push(won, v);
   }
 };
 
 __global__ void __launch_bounds__(kBlock)
 bfs_td_step(const int64_t *out_off, const int32_t *out_nbr, int32_t *parent, int32_t *queue,
-            int64_t qb, int64_t qe, int64_t heavy_deg, BfsCounters *c) {
-  TdEdge f{out_nbr, parent, queue, &c->tail, 0};
+            int64_t qb, int64_t qe, int64_t cap, int64_t heavy_deg, BfsCounters *c,
+            unsigned long long *partial) {
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  TdEdge f;
+  f.nbr = out_nbr;
+  f.parent = parent;
+  f.scout = 0;
+  f.q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
   auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
     s = out_off[u];
     l = out_off[u + 1] - s;
   };
   expand_list(queue + qb, qe - qb, heavy_deg, rowlen, f);
-  const unsigned long long sc = wave_sum(f.scout);
-  if (lane_id() == 0 && sc) atomicAdd(&c->scout, sc);
+  f.q.flush();
+  block_partial(f.scout, partial);
 }
 
 template <int G>
 __global__ void __launch_bounds__(kBlock)
 bfs_bu_step(const int64_t *in_off, const int32_t *in_nbr, int64_t n, int32_t *parent,
-            const uint32_t *front, uint32_t *next, BfsCounters *c, int swz) {
+            const uint32_t *front, uint32_t *next, unsigned long long *partial, int swz) {
   const int lane = lane_id();
   const int glane = lane % G;
   const int gbase = lane - glane;
   const int64_t groups_per_block = kBlock / G;
-  const int64_t b = swizzled_block(swz);
-  const int64_t u = b * groups_per_block + threadIdx.x / G;
+  const int64_t tiles = (n + groups_per_block - 1) / groups_per_block;
   unsigned long long awake = 0;
-  bool todo = false;
-  int64_t s = 0, e = 0;
-  if (u < n && parent[u] < 0) {
-    s = in_off[u];
-    e = in_off[u + 1];
-    todo = s < e;
-  }
-  // group-uniform loop
-  for (int64_t base = s; todo; base += G) {
-    const int64_t k = base + glane;
-    const bool hit = k < e && bit_get(front, in_nbr[k]);
-    const uint64_t m = __ballot(hit);
-    const uint64_t gm = (G == 64) ? m : ((m >> gbase) & ((1ull << G) - 1));
-    if (gm) {
-      const int first = __ffsll(static_cast<long long>(gm)) - 1;
-      if (glane == first) {
-        parent[u] = in_nbr[k];
-        bit_set_atomic(next, u);
-        awake = 1;
+  for (int64_t t = swizzled_block(swz); t < tiles; t += gridDim.x) {
+    const int64_t u = t * groups_per_block + threadIdx.x / G;
+    bool todo = false;
+    int64_t s = 0, e = 0;
+    if (u < n && parent[u] < 0) {
+      s = in_off[u];
+      e = in_off[u + 1];
+      todo = s < e;
+    }
+    // group-uniform loop
+    for (int64_t base = s; todo; base += G) {
+      const int64_t k = base + glane;
+      const bool hit = k < e && bit_get(front, in_nbr[k]);
+      const uint64_t m = __ballot(hit);
+      const uint64_t gm = (G == 64) ? m : ((m >> gbase) & ((1ull << G) - 1));
+      if (gm) {
+        const int first = __ffsll(static_cast<long long>(gm)) - 1;
+        if (glane == first) {
+          parent[u] = in_nbr[k];
+          bit_set_atomic(next, u);
+          awake++;
+        }
+        todo = false;
+      } else if (base + G >= e) {
+        todo = false;
       }
-      todo = false;
-    } else if (base + G >= e) {
-      todo = false;
     }
   }
-  awake = wave_sum(awake);
-  if (lane == 0 && awake) atomicAdd(&c->awake, awake);
+  block_partial(awake, partial);
 }
 
 __global__ void bfs_queue_to_bitmap(const int32_t *queue, int64_t qb, int64_t qe, uint32_t *bm) {
@@ -119,47 +127,65 @@ __global__ void bfs_queue_to_bitmap(const int32_t *queue, int64_t qb, int64_t qe
   if (i < qe) bit_set_atomic(bm, queue[i]);
 }
 
-__global__ void bfs_bitmap_to_queue(const uint32_t *bm, int64_t n, int32_t *queue, BfsCounters *c) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool set = v < n && bit_get(bm, v);
-  const int64_t at = wave_append(&c->tail, set);
-  if (set) queue[at] = static_cast<int32_t>(v);
+__global__ void __launch_bounds__(kBlock)
+bfs_bitmap_to_queue(const uint32_t *bm, int64_t n, int32_t *queue, int64_t cap, BfsCounters *c) {
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  WaveQueue q;
+  q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
+    const int64_t v = base + threadIdx.x;
+    const bool set = v < n && bit_get(bm, v);
+    q.push(set, static_cast<int32_t>(v));
+  }
+  q.flush();
+}
+
+// Writes the summed partial into a counter (one workgroup).
+__global__ void bfs_store_count(const unsigned long long *partial, int count,
+                                unsigned long long *dst) {
+  __shared__ unsigned long long red[1024 / kWave];
+  unsigned long long s = 0;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) s += partial[i];
+  s = wave_sum(s);
+  if (lane_id() == 0) red[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < static_cast<int>(blockDim.x) / kWave; i++) t += red[i];
+    *dst = t;
+  }
 }
 
 __global__ void bfs_finish(const int64_t *out_off, int64_t n, int32_t *parent,
-                           unsigned long long *stats) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+                           unsigned long long *p_reached, unsigned long long *p_edges) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   unsigned long long reached = 0, edges = 0;
-  if (v < n) {
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
     int32_t p = parent[v];
     if (p < -1) parent[v] = p = -1;
     if (p >= 0) {
-      reached = 1;
-      edges = static_cast<unsigned long long>(out_off[v + 1] - out_off[v]);
+      reached++;
+      edges += static_cast<unsigned long long>(out_off[v + 1] - out_off[v]);
     }
   }
-  reached = wave_sum(reached);
-  edges = wave_sum(edges);
-  if (lane_id() == 0) {
-    if (reached) atomicAdd(&stats[0], reached);
-    if (edges) atomicAdd(&stats[1], edges);
-  }
+  block_partial(reached, p_reached);
+  block_partial(edges, p_edges);
 }
 
 static void launch_bu(gbk_ctx *ctx, int G, const gbk_graph *g, int32_t *parent,
-                      const uint32_t *front, uint32_t *next, BfsCounters *c) {
-  const int64_t groups_per_block = kBlock / G;
-  const unsigned grid = static_cast<unsigned>((g->n + groups_per_block - 1) / groups_per_block);
+                      const uint32_t *front, uint32_t *next, unsigned long long *partial,
+                      unsigned grid) {
   const int swz = ctx->tune.xcd_swizzle;
   hipStream_t s = ctx->stream;
   switch (G) {
-    case 1: bfs_bu_step<1><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, c, swz); break;
-    case 2: bfs_bu_step<2><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, c, swz); break;
-    case 4: bfs_bu_step<4><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, c, swz); break;
-    case 16: bfs_bu_step<16><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, c, swz); break;
-    case 32: bfs_bu_step<32><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, c, swz); break;
-    case 64: bfs_bu_step<64><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, c, swz); break;
-    default: bfs_bu_step<8><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, c, swz); break;
+    case 1: bfs_bu_step<1><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, partial, swz); break;
+    case 2: bfs_bu_step<2><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, partial, swz); break;
+    case 4: bfs_bu_step<4><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, partial, swz); break;
+    case 16: bfs_bu_step<16><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, partial, swz); break;
+    case 32: bfs_bu_step<32><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, partial, swz); break;
+    case 64: bfs_bu_step<64><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, partial, swz); break;
+    default: bfs_bu_step<8><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, parent, front, next, partial, swz); break;
   }
 }
 
@@ -167,20 +193,22 @@ static void launch_bu(gbk_ctx *ctx, int G, const gbk_graph *g, int32_t *parent,
 
 using namespace gbk;
 
-// Runs one DO-BFS on the device; parent stays in 'parent' (device).
+// Runs one DO-BFS on the device; the parent array stays in 'parent'.
 int gbk_bfs_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha, int beta,
                    int32_t *parent, gbk_bfs_stats *stats) {
   const int64_t n = g->n;
   hipStream_t s = ctx->stream;
   const int64_t words = (n + 31) / 32 + 1;
-  auto *queue = static_cast<int32_t *>(scratch(ctx, kS5, sizeof(int32_t) * (n + 64)));
+  const int64_t cap = n + 64;
+  auto *queue = static_cast<int32_t *>(scratch(ctx, kS5, sizeof(int32_t) * cap));
   auto *bm0 = static_cast<uint32_t *>(scratch(ctx, kS6, sizeof(uint32_t) * words));
   auto *bm1 = static_cast<uint32_t *>(scratch(ctx, kS7, sizeof(uint32_t) * words));
   auto *c = static_cast<BfsCounters *>(scratch(ctx, kS8, sizeof(BfsCounters) + 64));
+  auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS9, sizeof(unsigned long long) * 2 * kGridCap));
+  auto *fin = static_cast<unsigned long long *>(scratch(ctx, kS10, 64));
   auto *hc = static_cast<BfsCounters *>(pinned(ctx, sizeof(BfsCounters) * 2));
-  if (!queue || !bm0 || !bm1 || !c || !hc) return set_error(ctx, GBK_ENOMEM, "bfs scratch");
-  auto *sstat = reinterpret_cast<unsigned long long *>(c + 1);  // 2 words after counters
-  (void)sstat;
+  if (!queue || !bm0 || !bm1 || !c || !partial || !fin || !hc)
+    return set_error(ctx, GBK_ENOMEM, "bfs scratch");
   int64_t out_deg_src = 0;
   {
     int64_t o[2];
@@ -189,11 +217,14 @@ int gbk_bfs_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha, 
     out_deg_src = o[1] - o[0];
   }
   const int G = ctx->tune.bu_group;
+  const unsigned bu_grid = expand_grid((n + (kBlock / G) - 1) / (kBlock / G) * kBlock);
+  const unsigned n_grid = expand_grid(n);
   EventTimer timer(ctx);
   bfs_init<<<grid_for(n), kBlock, 0, s>>>(g->off, n, source, parent, queue, c);
   auto read_counters = [&]() -> int {
     GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BfsCounters), hipMemcpyDeviceToHost, s));
     GBK_HIP(ctx, hipStreamSynchronize(s));
+    if (hc->overflow) return set_error(ctx, GBK_EHIP, "bfs: queue overflow");
     return GBK_OK;
   };
   int64_t qb = 0, qe = 1;
@@ -210,24 +241,25 @@ int gbk_bfs_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha, 
       do {
         old_awake_count = awake_count;
         fill<uint32_t>(ctx, curr, 0u, words);
-        fill<unsigned long long>(ctx, &c->awake, 0ull, 1);
-        launch_bu(ctx, G, g, parent, front, curr, c);
+        launch_bu(ctx, G, g, parent, front, curr, partial, bu_grid);
+        bfs_store_count<<<1, 1024, 0, s>>>(partial, static_cast<int>(bu_grid), &c->awake);
         GBK_LAUNCH_CHECK(ctx, "bfs_bu_step");
         if (int rc = read_counters()) return rc;
         awake_count = static_cast<int64_t>(hc->awake);
         std::swap(front, curr);
         steps_bu++;
       } while ((awake_count >= old_awake_count) || (awake_count > n / beta));
-      bfs_bitmap_to_queue<<<grid_for(n), kBlock, 0, s>>>(front, n, queue, c);
+      bfs_bitmap_to_queue<<<n_grid, kBlock, 0, s>>>(front, n, queue, cap, c);
       GBK_LAUNCH_CHECK(ctx, "bfs_bitmap_to_queue");
       if (int rc = read_counters()) return rc;
       qe = static_cast<int64_t>(hc->tail);
       scout_count = 1;
     } else {
       edges_to_check -= scout_count;
-      fill<unsigned long long>(ctx, &c->scout, 0ull, 1);
-      bfs_td_step<<<expand_grid(qe - qb), kBlock, 0, s>>>(g->off, g->nbr, parent, queue, qb, qe,
-                                                         ctx->tune.heavy_degree, c);
+      const unsigned grid = expand_grid(qe - qb);
+      bfs_td_step<<<grid, kBlock, 0, s>>>(g->off, g->nbr, parent, queue, qb, qe, cap,
+                                          ctx->tune.heavy_degree, c, partial);
+      bfs_store_count<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), &c->scout);
       GBK_LAUNCH_CHECK(ctx, "bfs_td_step");
       if (int rc = read_counters()) return rc;
       scout_count = static_cast<int64_t>(hc->scout);
@@ -235,14 +267,10 @@ int gbk_bfs_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha, 
       qe = static_cast<int64_t>(hc->tail);
       steps_td++;
     }
-    if (qe > n + 1) return set_error(ctx, GBK_EHIP, "bfs: queue overflow");
   }
-  unsigned long long *st2 = reinterpret_cast<unsigned long long *>(&c->pad);
-  (void)st2;
-  auto *fin = static_cast<unsigned long long *>(scratch(ctx, kS9, 64));
-  if (!fin) return set_error(ctx, GBK_ENOMEM, "bfs stats");
-  fill<unsigned long long>(ctx, fin, 0ull, 2);
-  bfs_finish<<<grid_for(n), kBlock, 0, s>>>(g->off, n, parent, fin);
+  bfs_finish<<<n_grid, kBlock, 0, s>>>(g->off, n, parent, partial, partial + kGridCap);
+  bfs_store_count<<<1, 1024, 0, s>>>(partial, static_cast<int>(n_grid), fin);
+  bfs_store_count<<<1, 1024, 0, s>>>(partial + kGridCap, static_cast<int>(n_grid), fin + 1);
   GBK_LAUNCH_CHECK(ctx, "bfs_finish");
   const double ms = timer.stop_ms();
   unsigned long long hf[2] = {0, 0};

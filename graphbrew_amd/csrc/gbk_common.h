@@ -191,6 +191,11 @@ inline void fill(gbk_ctx *ctx, T *p, T v, int64_t n) {
 
 // Device-wide exclusive scan of int64 counts (out[i] = sum in[0..i), out[n] = total).
 int exclusive_scan_i64(gbk_ctx *ctx, const int64_t *in, int64_t *out, int64_t n);
+int exclusive_scan_i32(gbk_ctx *ctx, const int32_t *in, int64_t *out, int64_t n);
+// Sorts every CSR row by (neighbour, weight) into new arrays (identity relabel).
+int sort_rows(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_t *nbr, const int32_t *wgt,
+              const int32_t *iota, int64_t *deg, int64_t *off_out, int32_t *nbr_out,
+              int32_t *wgt_out);
 // Stable compaction of ids [0,n) where flag(i) != 0 (flags given as int32).
 int compact_flags(gbk_ctx *ctx, const int32_t *flags, int64_t n, int32_t *out, int64_t *count);
 // Heavy-row list for a CSR.

@@ -51,6 +51,25 @@ __device__ __forceinline__ void wave_walk(int64_t row, int64_t start, int64_t le
   }
 }
 
+// Segmented inclusive sum across a 64-edge chunk whose owners ('seg', one
+// per lane, non-decreasing across lanes) form contiguous runs.  Returns the
+// run total in the last lane of each run (other lanes hold partial sums);
+// 'tail' tells the caller which lanes end a run.  Replaces same-address LDS
+// atomics, which serialise when many lanes of one wave share an owner.
+template <typename T>
+__device__ __forceinline__ T seg_run_sum(T x, int seg, bool &tail) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const T y = __shfl_up(x, o);
+    const int s = __shfl_up(seg, o);
+    if (lane >= o && s == seg) x += y;
+  }
+  const int next = __shfl_down(seg, 1);
+  tail = (lane == kWave - 1) || (next != seg);
+  return x;
+}
+
 // Block-cooperative walk of one heavy row: uniform trip count per wave.
 template <class F>
 __device__ __forceinline__ void block_walk(int64_t row, int64_t start, int64_t len, F &f) {
@@ -128,11 +147,86 @@ __device__ void expand_all(int64_t n, int64_t heavy_deg, RowLen rowlen, F &f) {
 }
 
 // Number of workgroups for a grid-stride expansion over 'items' entries.
+// Capped so that per-workgroup epilogues (queue flushes, partial sums) stay
+// few: same-address global atomics serialise in L2 at tens of ns each.
+constexpr int64_t kGridCap = 2048;
 inline unsigned expand_grid(int64_t items) {
   int64_t g = (items + kBlock - 1) / kBlock;
   if (g < 1) g = 1;
-  if (g > 8192) g = 8192;
+  if (g > kGridCap) g = kGridCap;
   return static_cast<unsigned>(g);
+}
+
+// Per-wave queue staged in LDS.  push() is called by every lane of the wave
+// (uniform control flow); entries gather in the wave's LDS segment and go to
+// the global queue in batches of up to kSeg with one atomic per batch, as
+// contiguous coalesced stores.  flush() must be called once at the end.
+constexpr int kSeg = 512;
+
+struct WaveQueue {
+  int32_t *lds;                 // this wave's kSeg-entry segment
+  int cnt;                      // entries staged (wave-uniform)
+  unsigned long long *tail;     // global tail counter
+  int32_t *out;                 // global queue
+  int64_t cap;                  // capacity of 'out'
+  int *overflow;
+
+  __device__ void init(int32_t *seg, unsigned long long *t, int32_t *o, int64_t c, int *ovf) {
+    lds = seg;
+    cnt = 0;
+    tail = t;
+    out = o;
+    cap = c;
+    overflow = ovf;
+  }
+  __device__ void flush() {
+    if (cnt == 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long base = 0;
+    if (lane_id() == 0) base = atomicAdd(tail, static_cast<unsigned long long>(cnt));
+    base = __shfl(base, 0);
+    for (int i = lane_id(); i < cnt; i += kWave) {
+      const int64_t at = static_cast<int64_t>(base) + i;
+      if (at < cap) out[at] = lds[i];
+      else *overflow = 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    cnt = 0;
+  }
+  __device__ void push(bool pred, int32_t v) {
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return;
+    const int k = __popcll(mask);
+    if (cnt + k > kSeg) flush();
+    if (pred) lds[cnt + lanemask_lt_count(mask)] = v;
+    cnt += k;
+  }
+};
+
+// Block-wide partial sum written to partial[blockIdx.x]; a one-workgroup
+// finalize kernel adds the partials (no same-address atomics).
+template <typename T>
+__device__ __forceinline__ void block_partial(T v, T *partial) {
+  __shared__ T red[kWavesPerBlock];
+  const T s = block_sum(v, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void sum_partials(const T *partial, int count, T *out, int accumulate) {
+  __shared__ T red[1024 / kWave];
+  T s = 0;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) s += partial[i];
+  s = wave_sum(s);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) red[w] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T t = 0;
+    for (int i = 0; i < static_cast<int>(blockDim.x) / kWave; i++) t += red[i];
+    *out = accumulate ? *out + t : t;
+  }
 }
 
 // "Last workgroup done" helper: returns true in exactly one thread (thread 0

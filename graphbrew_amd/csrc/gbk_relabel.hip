@@ -9,17 +9,16 @@
 // (2) device exclusive scan -> new offsets; (3) rows are binned by degree
 // and each bin is gathered, remapped and sorted in one fused kernel, so the
 // neighbour array is read once and the relabelled array written once:
-//   deg <= 16    windows of 128 rows sorted together in LDS
-//   deg <= 128   windows of 16 rows
-//   deg <= 2048  one row per workgroup
+//   deg <= 64    a group of 8/16/32/64 lanes per row, one entry per lane,
+//                bitonic network over shuffles (registers only)
+//   deg <= 2048  one row per workgroup, bitonic sort in LDS
 //   larger       2048-element chunks sorted in LDS, then merge-path passes
 //                over a compact staging buffer, then copied into place.
-// Sorting keys: the window kernels sort (row-in-window, neighbour[, weight]);
-// since rows of a window are sorted by row first, each element's output slot
-// is new_off[row] + its rank inside the row.
+// Keys compare as (neighbour, weight), the order of NodeWeight::operator<
+// (gapbs/graph.h:136-140).
 #include <algorithm>
 
-#include "gbk_common.h"
+#include "gbk_expand.h"
 
 int gbk_graph_alloc(gbk_ctx *ctx, int64_t n, int64_t m, int64_t m_in, int directed, int weighted,
                     gbk_graph **out);
@@ -47,16 +46,75 @@ __global__ void relabel_invert(const int64_t *off, const int32_t *new_ids, const
   org_new[r] = org_old[u];
 }
 
-// Degree classes 0..3 (see file comment); empty rows need no work.
-__global__ void relabel_bin_rows(const int64_t *deg_new, int64_t n, int32_t *l0, int32_t *l1,
-                                 int32_t *l2, int32_t *l3, unsigned long long *counts) {
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t d = r < n ? deg_new[r] : 0;
-  const int cls = d == 0 ? -1 : d <= 16 ? 0 : d <= 128 ? 1 : d <= kSortCap ? 2 : 3;
-  int32_t *lists[4] = {l0, l1, l2, l3};
-  for (int c = 0; c < 4; c++) {
-    const int64_t at = wave_append(&counts[c], cls == c);
-    if (cls == c) lists[c][at] = static_cast<int32_t>(r);
+// Degree classes: <=8, <=16, <=32, <=64 (lane-group register sorts),
+// <=2048 (workgroup LDS sort), larger (chunk sort + merge).  Empty rows need
+// no work.  Each wave stages its class lists in LDS and flushes in batches.
+constexpr int kClasses = 6;
+__global__ void __launch_bounds__(kBlock)
+relabel_bin_rows(const int64_t *deg_new, int64_t n, int32_t *l0, int32_t *l1, int32_t *l2,
+                 int32_t *l3, int32_t *l4, int32_t *l5, unsigned long long *counts,
+                 int *overflow) {
+  __shared__ int32_t seg[kClasses * kWavesPerBlock * kSeg];
+  const int w = threadIdx.x / kWave;
+  int32_t *lists[kClasses] = {l0, l1, l2, l3, l4, l5};
+  WaveQueue q[kClasses];
+  for (int c = 0; c < kClasses; c++)
+    q[c].init(seg + (c * kWavesPerBlock + w) * kSeg, &counts[c], lists[c], n + 1, overflow);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
+    const int64_t r = base + threadIdx.x;
+    const int64_t d = r < n ? deg_new[r] : 0;
+    const int cls = d == 0 ? -1 : d <= 8 ? 0 : d <= 16 ? 1 : d <= 32 ? 2 : d <= 64 ? 3
+                  : d <= kSortCap ? 4 : 5;
+    for (int c = 0; c < kClasses; c++) q[c].push(cls == c, static_cast<int32_t>(r));
+  }
+  for (int c = 0; c < kClasses; c++) q[c].flush();
+}
+
+// Rows of at most G entries: G lanes hold one entry each and run a bitonic
+// network over shuffles (no LDS, no barriers).  Keys compare as (v, w).
+template <bool W, int G>
+__global__ void __launch_bounds__(kBlock)
+relabel_group_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const int64_t *off,
+                   const int32_t *nbr, const int32_t *wgt, const int32_t *new_ids,
+                   const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out) {
+  using K = typename std::conditional<W, uint64_t, uint32_t>::type;
+  const int glane = threadIdx.x % G;
+  const int64_t ri = static_cast<int64_t>(blockIdx.x) * (kBlock / G) + threadIdx.x / G;
+  int64_t len = 0, src = 0, dst = 0;
+  if (ri < nrows) {
+    const int32_t r = rows[ri];
+    const int32_t u = inv[r];
+    src = off[u];
+    len = off[u + 1] - src;
+    dst = new_off[r];
+  }
+  K key = static_cast<K>(~static_cast<K>(0));
+  if (glane < len) {
+    const int32_t v = new_ids[nbr[src + glane]];
+    if constexpr (W)
+      key = (static_cast<uint64_t>(static_cast<uint32_t>(v)) << 32) |
+            (static_cast<uint32_t>(wgt[src + glane]) ^ 0x80000000u);
+    else
+      key = static_cast<uint32_t>(v);
+  }
+#pragma unroll
+  for (int k = 2; k <= G; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const K other = __shfl_xor(key, j);
+      const bool up = (glane & k) == 0;
+      const bool lower = (glane & j) == 0;
+      key = (lower == up) ? (other < key ? other : key) : (other > key ? other : key);
+    }
+  }
+  if (glane < len) {
+    if constexpr (W) {
+      nbr_out[dst + glane] = static_cast<int32_t>(key >> 32);
+      wgt_out[dst + glane] = static_cast<int32_t>(static_cast<uint32_t>(key) ^ 0x80000000u);
+    } else {
+      nbr_out[dst + glane] = static_cast<int32_t>(key);
+    }
   }
 }
 
@@ -290,44 +348,59 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
   if (!degrees_ready) return set_error(ctx, GBK_EINVAL, "relabel: degrees not computed");
   int rc = exclusive_scan_i64(ctx, deg_new, off_out, n);
   if (rc) return rc;
-  int32_t *lists[4];
-  for (int c = 0; c < 4; c++) {
-    lists[c] = static_cast<int32_t *>(scratch(ctx, kS4 + c, sizeof(int32_t) * (n + 1)));
+  int32_t *lists[kClasses];
+  const int slots[kClasses] = {kS4, kS5, kS6, kS7, kS12, kS13};
+  for (int c = 0; c < kClasses; c++) {
+    lists[c] = static_cast<int32_t *>(scratch(ctx, slots[c], sizeof(int32_t) * (n + 1)));
     if (!lists[c]) return set_error(ctx, GBK_ENOMEM, "relabel bins");
   }
-  auto *counts_d = static_cast<unsigned long long *>(scratch(ctx, kS8, 64));
+  auto *counts_d = static_cast<unsigned long long *>(scratch(ctx, kS8, 128));
   if (!counts_d) return set_error(ctx, GBK_ENOMEM, "relabel counts");
-  fill<unsigned long long>(ctx, counts_d, 0ull, 4);
+  fill<unsigned long long>(ctx, counts_d, 0ull, kClasses + 1);
   if (n > 0)
-    relabel_bin_rows<<<grid_for(n), kBlock, 0, s>>>(deg_new, n, lists[0], lists[1], lists[2],
-                                                    lists[3], counts_d);
+    relabel_bin_rows<<<expand_grid(n), kBlock, 0, s>>>(deg_new, n, lists[0], lists[1], lists[2],
+                                                       lists[3], lists[4], lists[5], counts_d,
+                                                       reinterpret_cast<int *>(counts_d + kClasses));
   GBK_LAUNCH_CHECK(ctx, "relabel_bin_rows");
-  unsigned long long counts[4];
+  unsigned long long counts[kClasses + 1];
   GBK_HIP(ctx, hipMemcpyAsync(counts, counts_d, sizeof(counts), hipMemcpyDeviceToHost, s));
   GBK_HIP(ctx, hipStreamSynchronize(s));
-  const int R[3] = {128, 16, 1};
-  for (int c = 0; c < 3; c++) {
+  if (counts[kClasses]) return set_error(ctx, GBK_EHIP, "relabel: bin overflow");
+  // rows of <= 64 entries: lane groups sort in registers
+  const int groups[4] = {8, 16, 32, 64};
+  for (int c = 0; c < 4; c++) {
     const int64_t cnt = static_cast<int64_t>(counts[c]);
     if (cnt == 0) continue;
-    const int64_t blocks = (cnt + R[c] - 1) / R[c];
-    relabel_window_sort<W><<<blocks, kBlock, 0, s>>>(lists[c], cnt, R[c], inv, off, nbr, wgt,
-                                                      new_ids_d, off_out, nbr_out, wgt_out);
+    const int64_t blocks = (cnt + kBlock / groups[c] - 1) / (kBlock / groups[c]);
+    switch (groups[c]) {
+      case 8: relabel_group_sort<W, 8><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+      case 16: relabel_group_sort<W, 16><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+      case 32: relabel_group_sort<W, 32><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+      default: relabel_group_sort<W, 64><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+    }
+    GBK_LAUNCH_CHECK(ctx, "relabel_group_sort");
+  }
+  if (counts[4] > 0) {
+    const int64_t cnt = static_cast<int64_t>(counts[4]);
+    relabel_window_sort<W><<<cnt, kBlock, 0, s>>>(lists[4], cnt, 1, inv, off, nbr, wgt, new_ids_d,
+                                                   off_out, nbr_out, wgt_out);
     GBK_LAUNCH_CHECK(ctx, "relabel_window_sort");
   }
-  const int nbig = static_cast<int>(counts[3]);
+  const int nbig = static_cast<int>(counts[5]);
+  int32_t *big_list = lists[5];
   if (nbig == 0) return GBK_OK;
   // Big rows: host builds the staging layout and chunk table (few rows).
   std::vector<int32_t> big(nbig);
   std::vector<int64_t> bdeg(nbig);
-  GBK_HIP(ctx, hipMemcpyAsync(big.data(), lists[3], sizeof(int32_t) * nbig, hipMemcpyDeviceToHost, s));
+  GBK_HIP(ctx, hipMemcpyAsync(big.data(), big_list, sizeof(int32_t) * nbig, hipMemcpyDeviceToHost, s));
   GBK_HIP(ctx, hipStreamSynchronize(s));
   std::sort(big.begin(), big.end());
   {
     // degrees of the big rows: deg_new[row], gathered on the device
     auto *tmp = static_cast<int64_t *>(scratch(ctx, kS9, sizeof(int64_t) * nbig));
     if (!tmp) return set_error(ctx, GBK_ENOMEM, "relabel big degrees");
-    GBK_HIP(ctx, hipMemcpyAsync(lists[3], big.data(), sizeof(int32_t) * nbig, hipMemcpyHostToDevice, s));
-    gather_i64<<<grid_for(nbig), kBlock, 0, s>>>(deg_new, lists[3], nbig, tmp);
+    GBK_HIP(ctx, hipMemcpyAsync(big_list, big.data(), sizeof(int32_t) * nbig, hipMemcpyHostToDevice, s));
+    gather_i64<<<grid_for(nbig), kBlock, 0, s>>>(deg_new, big_list, nbig, tmp);
     GBK_LAUNCH_CHECK(ctx, "gather_i64");
     GBK_HIP(ctx, hipMemcpyAsync(bdeg.data(), tmp, sizeof(int64_t) * nbig, hipMemcpyDeviceToHost, s));
     GBK_HIP(ctx, hipStreamSynchronize(s));
@@ -385,6 +458,15 @@ __global__ void degrees_through_inverse(const int64_t *off, const int32_t *inv, 
     const int32_t u = inv[r];
     deg_new[r] = off[u + 1] - off[u];
   }
+}
+
+// Sorts every row of a CSR by (neighbour, weight) into new arrays: the
+// relabel machinery with the identity mapping (used by the graph builder).
+int sort_rows(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_t *nbr, const int32_t *wgt,
+              const int32_t *iota, int64_t *deg, int64_t *off_out, int32_t *nbr_out,
+              int32_t *wgt_out) {
+  if (wgt) return relabel_csr<true>(ctx, n, off, nbr, wgt, iota, iota, deg, off_out, nbr_out, wgt_out, true);
+  return relabel_csr<false>(ctx, n, off, nbr, wgt, iota, iota, deg, off_out, nbr_out, wgt_out, true);
 }
 
 }  // namespace gbk

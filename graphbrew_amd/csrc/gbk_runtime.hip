@@ -1,5 +1,6 @@
 // gbk_runtime.hip -- context, scratch memory, device scan/compaction and
 // graph upload/download for libgbk.so.
+#include <algorithm>
 #include <cstring>
 
 #include "gbk_common.h"
@@ -147,6 +148,10 @@ int exclusive_scan_i64(gbk_ctx *ctx, const int64_t *in, int64_t *out, int64_t n)
   return exclusive_scan_impl<int64_t>(ctx, in, out, n);
 }
 
+int exclusive_scan_i32(gbk_ctx *ctx, const int32_t *in, int64_t *out, int64_t n) {
+  return exclusive_scan_impl<int32_t>(ctx, in, out, n);
+}
+
 __global__ void scatter_flagged(const int32_t *flags, const int64_t *pos, int64_t n, int32_t *out) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n && flags[i]) out[pos[i]] = static_cast<int32_t>(i);
@@ -166,25 +171,33 @@ int compact_flags(gbk_ctx *ctx, const int32_t *flags, int64_t n, int32_t *out, i
 
 // ------------------------------------------------------------ heavy lists
 __global__ void flag_heavy(const int64_t *off, int64_t n, int64_t threshold, int32_t *flags,
-                           unsigned long long *max_deg) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t d = 0;
-  if (i < n) {
-    d = off[i + 1] - off[i];
+                           unsigned long long *partial_max) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  unsigned long long mx = 0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t d = off[i + 1] - off[i];
     flags[i] = d > threshold ? 1 : 0;
+    mx = max(mx, static_cast<unsigned long long>(d));
   }
-  d = wave_max(d);
-  if (lane_id() == 0) atomicMax(max_deg, static_cast<unsigned long long>(d));
+  mx = wave_max(mx);
+  __shared__ unsigned long long red[kWavesPerBlock];
+  if (lane_id() == 0) red[threadIdx.x / kWave] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < kWavesPerBlock; i++) mx = max(mx, red[i]);
+    partial_max[blockIdx.x] = mx;
+  }
 }
 
 int build_heavy_list(gbk_ctx *ctx, const int64_t *off, int64_t n, int64_t threshold,
                      int32_t **list, int64_t *count, int64_t *max_degree,
                      std::vector<void *> &owned) {
+  const unsigned grid = grid_for(n, kBlock, 2048);
   auto *flags = static_cast<int32_t *>(scratch(ctx, kS13, sizeof(int32_t) * (n + 1)));
-  auto *mx = static_cast<unsigned long long *>(scratch(ctx, kS12, 64));
+  auto *mx = static_cast<unsigned long long *>(scratch(ctx, kS12, sizeof(unsigned long long) * grid));
   if (!flags || !mx) return set_error(ctx, GBK_ENOMEM, "heavy scratch");
-  fill<unsigned long long>(ctx, mx, 0ull, 1);
-  if (n > 0) flag_heavy<<<grid_for(n), kBlock, 0, ctx->stream>>>(off, n, threshold, flags, mx);
+  fill<unsigned long long>(ctx, mx, 0ull, grid);
+  if (n > 0) flag_heavy<<<grid, kBlock, 0, ctx->stream>>>(off, n, threshold, flags, mx);
   GBK_LAUNCH_CHECK(ctx, "flag_heavy");
   int32_t *out = nullptr;
   // worst case n entries; most graphs have few heavy rows, so compact into
@@ -198,9 +211,12 @@ int build_heavy_list(gbk_ctx *ctx, const int64_t *off, int64_t n, int64_t thresh
   owned.push_back(out);
   if (cnt > 0)
     GBK_HIP(ctx, hipMemcpyAsync(out, tmp, sizeof(int32_t) * cnt, hipMemcpyDeviceToDevice, ctx->stream));
-  unsigned long long h = 0;
-  GBK_HIP(ctx, hipMemcpyAsync(&h, mx, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<unsigned long long> hm(grid);
+  GBK_HIP(ctx, hipMemcpyAsync(hm.data(), mx, sizeof(unsigned long long) * grid, hipMemcpyDeviceToHost,
+                              ctx->stream));
   GBK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  unsigned long long h = 0;
+  for (auto x : hm) h = std::max(h, x);
   *list = out;
   *count = cnt;
   *max_degree = static_cast<int64_t>(h);

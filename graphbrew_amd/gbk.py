@@ -85,6 +85,7 @@ def lib():
         "gbk_ctx_set_param": (i32, [VP, C.c_char_p, i64]),
         "gbk_ctx_get_param": (i32, [VP, C.c_char_p, C.POINTER(i64)]),
         "gbk_graph_upload": (i32, [VP, i64, i32, VP, VP, VP, VP, VP, VP, VP, pp]),
+        "gbk_graph_build": (i32, [VP, i64, i64, VP, VP, VP, i32, i32, pp, C.POINTER(f64)]),
         "gbk_graph_info": (i32, [VP, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32), C.POINTER(i32)]),
         "gbk_graph_download": (i32, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
         "gbk_graph_free": (i32, [VP, VP]),
@@ -216,6 +217,16 @@ class Context:
                                            _ptr(wgt), _ptr(in_off), _ptr(in_nbr), _ptr(in_wgt),
                                            _ptr(org_ids), C.byref(h)))
         return self._wrap(h)
+
+    def build(self, u, v, w=None, n: int = -1, symmetrize=True, keep_self=False):
+        """Device CSR construction from an edge list (returns graph, device ms)."""
+        u, v, w = _a(u, np.int32), _a(v, np.int32), _a(w, np.int32)
+        h = VP()
+        ms = C.c_double(0)
+        self._check(lib().gbk_graph_build(self.h, int(n), len(u), _ptr(u), _ptr(v), _ptr(w),
+                                          int(bool(symmetrize)), int(bool(keep_self)), C.byref(h),
+                                          C.byref(ms)))
+        return self._wrap(h), ms.value
 
     def upload_graph(self, g) -> DeviceGraph:
         """Uploads any object with the oracle/host Graph attributes."""

@@ -1,0 +1,160 @@
+"""ctypes binding of libgbkhost.so (include/gbk_host.h): the host substrate
+of the bench path -- synthetic graphs, edge-list files, mapping producers and
+files, source picking, the Tier-0 features and the frozen rule."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libgbkhost.so")
+_lib = None
+VP = C.c_void_p
+
+# BenchmarkType codes (reorder_types.h:1214-1224)
+BENCH_CODES = {"pr": 1, "bfs": 2, "cc": 3, "sssp": 4, "bc": 5, "tc": 6, "pr_spmv": 7, "cc_sv": 8}
+TIER0_NAMES = ("avg_degree", "degree_cv", "hub_concentration", "normalized_edge_span",
+               "window_neighbor_overlap", "packing_factor", "internal_density")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run graphbrew_amd.build.build()")
+        L = C.CDLL(LIB_PATH)
+        i32, i64, f64 = C.c_int, C.c_int64, C.c_double
+        sig = {
+            "gbh_generate_el": (i32, [i32, i32, i32, VP, VP]),
+            "gbh_insert_weights": (i32, [i64, VP]),
+            "gbh_read_el": (i32, [C.c_char_p, i32, C.POINTER(i64), VP, VP, VP]),
+            "gbh_mapping_identity": (i32, [i64, VP]),
+            "gbh_mapping_random": (i32, [i64, VP]),
+            "gbh_mapping_sort": (i32, [i64, VP, VP]),
+            "gbh_mapping_fingerprint": (C.c_uint64, [i64, VP]),
+            "gbh_validate_permutation": (i32, [i64, VP]),
+            "gbh_mapping_read": (i32, [C.c_char_p, C.POINTER(i64), VP]),
+            "gbh_mapping_write": (i32, [C.c_char_p, i64, VP]),
+            "gbh_source_picker": (i32, [i64, VP, VP, i32, i32, VP, VP]),
+            "gbh_afforest_samples": (i32, [i64, i32, VP]),
+            "gbh_tier0_features": (i32, [i64, VP, VP, i64, C.c_uint64, VP]),
+            "gbh_modeled_property_bytes": (i64, [i32, i64, i64]),
+            "gbh_frozen_rule": (i32, [i32, f64, i64, f64, f64, f64]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(VP)
+
+
+def _chk(rc, what):
+    if rc < 0:
+        raise ValueError(f"{what} failed with code {rc}")
+
+
+def generate_el(scale: int, degree: int = 16, uniform: bool = False):
+    m = (1 << scale) * degree
+    u = np.empty(m, np.int32)
+    v = np.empty(m, np.int32)
+    _chk(lib().gbh_generate_el(scale, degree, int(uniform), _p(u), _p(v)), "generate_el")
+    return u, v
+
+
+def insert_weights(m: int) -> np.ndarray:
+    w = np.empty(m, np.int32)
+    _chk(lib().gbh_insert_weights(m, _p(w)), "insert_weights")
+    return w
+
+
+def read_el(path: str, weighted: bool = False):
+    m = C.c_int64(0)
+    _chk(lib().gbh_read_el(path.encode(), int(weighted), C.byref(m), None, None, None), "read_el")
+    u = np.empty(m.value, np.int32)
+    v = np.empty(m.value, np.int32)
+    w = np.empty(m.value, np.int32) if weighted else None
+    _chk(lib().gbh_read_el(path.encode(), int(weighted), C.byref(m), _p(u), _p(v), _p(w)), "read_el")
+    return u, v, w
+
+
+def mapping_identity(n: int) -> np.ndarray:
+    out = np.empty(n, np.int32)
+    _chk(lib().gbh_mapping_identity(n, _p(out)), "identity")
+    return out
+
+
+def mapping_random(n: int) -> np.ndarray:
+    out = np.empty(n, np.int32)
+    _chk(lib().gbh_mapping_random(n, _p(out)), "random")
+    return out
+
+
+def mapping_sort(off: np.ndarray) -> np.ndarray:
+    off = np.ascontiguousarray(off, np.int64)
+    out = np.empty(len(off) - 1, np.int32)
+    _chk(lib().gbh_mapping_sort(len(off) - 1, _p(off), _p(out)), "sort")
+    return out
+
+
+def mapping_fingerprint(ids) -> str:
+    ids = np.ascontiguousarray(ids, np.int32)
+    return f"{lib().gbh_mapping_fingerprint(len(ids), _p(ids)):016x}"
+
+
+def validate_permutation(ids) -> int:
+    ids = np.ascontiguousarray(ids, np.int32)
+    return lib().gbh_validate_permutation(len(ids), _p(ids))
+
+
+def mapping_read(path: str) -> np.ndarray:
+    n = C.c_int64(0)
+    _chk(lib().gbh_mapping_read(path.encode(), C.byref(n), None), "mapping_read")
+    out = np.empty(n.value, np.int32)
+    _chk(lib().gbh_mapping_read(path.encode(), C.byref(n), _p(out)), "mapping_read")
+    return out
+
+
+def mapping_write(path: str, ids) -> None:
+    ids = np.ascontiguousarray(ids, np.int32)
+    _chk(lib().gbh_mapping_write(path.encode(), len(ids), _p(ids)), "mapping_write")
+
+
+def source_picker(off, count: int, org_ids=None, repeats: int = 1):
+    off = np.ascontiguousarray(off, np.int64)
+    org = None if org_ids is None else np.ascontiguousarray(org_ids, np.int32)
+    o = np.empty(count, np.int32)
+    i = np.empty(count, np.int32)
+    _chk(lib().gbh_source_picker(len(off) - 1, _p(off), _p(org), count, repeats, _p(o), _p(i)),
+         "source_picker")
+    return o, i
+
+
+def afforest_samples(n: int, count: int = 1024) -> np.ndarray:
+    out = np.empty(count, np.int32)
+    _chk(lib().gbh_afforest_samples(n, count, _p(out)), "afforest_samples")
+    return out
+
+
+def tier0_features(off, nbr, sample_size=0, seed=0) -> dict:
+    off = np.ascontiguousarray(off, np.int64)
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    f = np.zeros(7, np.float64)
+    _chk(lib().gbh_tier0_features(len(off) - 1, _p(off), _p(nbr), sample_size, seed, _p(f)), "tier0")
+    return dict(zip(TIER0_NAMES, f.tolist()))
+
+
+def modeled_property_bytes(bench: str, n: int, m: int) -> int:
+    return int(lib().gbh_modeled_property_bytes(BENCH_CODES[bench], n, m))
+
+
+def frozen_rule(bench: str, reuse: float, n: int, avg_degree: float, degree_cv: float,
+                wsr: float) -> bool:
+    return bool(lib().gbh_frozen_rule(BENCH_CODES[bench], reuse, n, avg_degree, degree_cv, wsr))

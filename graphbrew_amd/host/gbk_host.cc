@@ -1,0 +1,414 @@
+// gbk_host.cc -- host substrate of the bench drivers (libgbkhost.so).
+//
+// Deterministic pieces that must reproduce the reference bit for bit (the
+// generator streams, the random mapping, the source order) use the same
+// engines and the same draw order as the reference; everything else is
+// organised for the host side of a GPU pipeline (block-parallel generation,
+// buffered file parsing).
+#include "gbk_host.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <numeric>
+#include <random>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+namespace {
+
+constexpr uint32_t kSeed = 27491095u;            // gapbs/util.h:47 (kRandSeed)
+constexpr int64_t kEdgesPerStream = int64_t(1) << 18;  // generator.h:192
+
+// Draws in [0, bound) from a 32-bit engine the way the reference's UniDist
+// does (generator.h:32-62): reject the top partial bucket, then reduce.
+struct BoundedDraw {
+  uint32_t bound;
+  uint32_t reject_from;  // 0 = never reject
+  explicit BoundedDraw(uint32_t b) : bound(b) {
+    const uint64_t range = 4294967296ull;
+    const uint32_t tail = static_cast<uint32_t>((range - 1) % b);
+    reject_from = (tail == b - 1) ? 0u : static_cast<uint32_t>(4294967295u - tail);
+  }
+  uint32_t operator()(std::mt19937 &rng) const {
+    uint32_t x = static_cast<uint32_t>(rng());
+    if (bound == 0) return x;  // full 32-bit range
+    if (reject_from)
+      while (x >= reject_from) x = static_cast<uint32_t>(rng());
+    return x % bound;
+  }
+};
+
+// One R-MAT edge: 'scale' quadrant choices with a=.57, b=c=.19, d=.05
+// expressed as 32-bit thresholds (generator.h:120-121, 131-145).
+struct RmatQuadrants {
+  uint32_t a, ab, abc;
+  RmatQuadrants() {
+    const uint32_t top = std::numeric_limits<uint32_t>::max();
+    const uint32_t A = 0.57 * top, B = 0.19 * top, C = 0.19 * top;
+    a = A;
+    ab = A + B;
+    abc = A + B + C;
+  }
+  void edge(std::mt19937 &rng, int scale, int32_t &src, int32_t &dst) const {
+    int32_t s = 0, d = 0;
+    for (int level = 0; level < scale; level++) {
+      const uint32_t r = static_cast<uint32_t>(rng());
+      const int32_t row_bit = r >= ab;
+      const int32_t col_bit = row_bit ? (r > abc) : (r > a);
+      s = (s << 1) | row_bit;
+      d = (d << 1) | col_bit;
+    }
+    src = s;
+    dst = d;
+  }
+};
+
+inline uint64_t splitmix_next(uint64_t &state) {
+  uint64_t z = (state += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+bool ends_with(const std::string &s, const char *suf) {
+  const size_t k = std::strlen(suf);
+  return s.size() >= k && s.compare(s.size() - k, k, suf) == 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gbh_generate_el(int scale, int degree, int uniform, int32_t *u, int32_t *v) {
+  if (scale < 0 || scale > 30 || degree <= 0 || !u || !v) return -1;
+  const int64_t n = int64_t(1) << scale;
+  const int64_t m = n * degree;
+  const int64_t streams = (m + kEdgesPerStream - 1) / kEdgesPerStream;
+  if (uniform) {
+    // MakeUniformEL (generator.h:101-117): no id permutation afterwards.
+    const BoundedDraw draw(static_cast<uint32_t>(n));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t sidx = 0; sidx < streams; sidx++) {
+      std::mt19937 rng(kSeed + static_cast<uint32_t>(sidx));
+      const int64_t e1 = std::min(m, (sidx + 1) * kEdgesPerStream);
+      for (int64_t e = sidx * kEdgesPerStream; e < e1; e++) {
+        u[e] = static_cast<int32_t>(draw(rng));
+        v[e] = static_cast<int32_t>(draw(rng));
+      }
+    }
+    return 0;
+  }
+  const RmatQuadrants q;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t sidx = 0; sidx < streams; sidx++) {
+    std::mt19937 rng(kSeed + static_cast<uint32_t>(sidx));
+    const int64_t e1 = std::min(m, (sidx + 1) * kEdgesPerStream);
+    for (int64_t e = sidx * kEdgesPerStream; e < e1; e++) q.edge(rng, scale, u[e], v[e]);
+  }
+  // PermuteIDs (generator.h:89-99): one shuffle of the identity.
+  std::vector<int32_t> relabel(n);
+  std::iota(relabel.begin(), relabel.end(), 0);
+  std::mt19937 prng(kSeed);
+  std::shuffle(relabel.begin(), relabel.end(), prng);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < m; e++) {
+    u[e] = relabel[u[e]];
+    v[e] = relabel[v[e]];
+  }
+  return 0;
+}
+
+int gbh_insert_weights(int64_t m, int32_t *w) {
+  if (m < 0 || (m > 0 && !w)) return -1;
+  const BoundedDraw draw(255u);  // U[0,254] + 1 (generator.h:171-186)
+  const int64_t streams = (m + kEdgesPerStream - 1) / kEdgesPerStream;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t sidx = 0; sidx < streams; sidx++) {
+    std::mt19937 rng(kSeed + static_cast<uint32_t>(sidx));
+    const int64_t e1 = std::min(m, (sidx + 1) * kEdgesPerStream);
+    for (int64_t e = sidx * kEdgesPerStream; e < e1; e++) w[e] = static_cast<int32_t>(draw(rng) + 1);
+  }
+  return 0;
+}
+
+// ReadInEL / ReadInWEL (reader.h:91-185): skips empty lines and lines that
+// start with '%' or '#'; a line contributes an edge when it starts with two
+// (three) integers.
+int gbh_read_el(const char *path, int weighted, int64_t *m, int32_t *u, int32_t *v, int32_t *w) {
+  if (!path || !m) return -1;
+  FILE *f = std::fopen(path, "r");
+  if (!f) return -2;
+  int64_t count = 0;
+  char line[4096];
+  while (std::fgets(line, sizeof(line), f)) {
+    if (line[0] == '\0' || line[0] == '\n' || line[0] == '\r' || line[0] == '%' || line[0] == '#')
+      continue;
+    char *p = line, *end = nullptr;
+    errno = 0;
+    const long a = std::strtol(p, &end, 10);
+    if (end == p) continue;
+    p = end;
+    const long b = std::strtol(p, &end, 10);
+    if (end == p) continue;
+    p = end;
+    long c = 1;
+    if (weighted) {
+      c = std::strtol(p, &end, 10);
+      if (end == p) continue;
+    }
+    if (u) {
+      if (count >= *m) break;
+      u[count] = static_cast<int32_t>(a);
+      v[count] = static_cast<int32_t>(b);
+      if (w) w[count] = static_cast<int32_t>(c);
+    }
+    count++;
+  }
+  std::fclose(f);
+  *m = count;
+  return 0;
+}
+
+int gbh_mapping_identity(int64_t n, int32_t *new_ids) {
+  if (n < 0 || (n > 0 && !new_ids)) return -1;
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; i++) new_ids[i] = static_cast<int32_t>(i);
+  return 0;
+}
+
+// GenerateRandomMapping (reorder_basic.h:104-135): Fisher-Yates from the
+// back with SplitMix64 (state 0) and rejection of the biased low range.
+int gbh_mapping_random(int64_t n, int32_t *new_ids) {
+  if (n < 0 || (n > 0 && !new_ids)) return -1;
+  for (int64_t i = 0; i < n; i++) new_ids[i] = static_cast<int32_t>(i);
+  uint64_t state = 0;
+  for (uint64_t left = static_cast<uint64_t>(n); left > 1; --left) {
+    const uint64_t floor_reject = (0 - left) % left;
+    uint64_t r = splitmix_next(state);
+    while (r < floor_reject) r = splitmix_next(state);
+    std::swap(new_ids[left - 1], new_ids[r % left]);
+  }
+  return 0;
+}
+
+// GenerateSortMapping (reorder_basic.h:213-258): stable, degree descending.
+int gbh_mapping_sort(int64_t n, const int64_t *off, int32_t *new_ids) {
+  if (n < 0 || (n > 0 && (!off || !new_ids))) return -1;
+  std::vector<int32_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [off](int32_t a, int32_t b) {
+    return (off[a + 1] - off[a]) > (off[b + 1] - off[b]);
+  });
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; i++) new_ids[order[i]] = static_cast<int32_t>(i);
+  return 0;
+}
+
+// MappingFingerprint (builder.h:1398-1411).  The offset basis is the one the
+// reference writes, 1469598103934665603.
+uint64_t gbh_mapping_fingerprint(int64_t n, const int32_t *new_ids) {
+  uint64_t h = 1469598103934665603ULL;
+  const uint64_t prime = 1099511628211ULL;
+  for (int64_t i = 0; i < n; i++) {
+    h = (h ^ static_cast<uint64_t>(i)) * prime;
+    h = (h ^ static_cast<uint64_t>(static_cast<int64_t>(new_ids[i]))) * prime;
+  }
+  return h;
+}
+
+int gbh_validate_permutation(int64_t n, const int32_t *ids) {
+  if (n < 0 || (n > 0 && !ids)) return -3;
+  std::vector<uint64_t> seen((n + 63) / 64, 0);
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t x = ids[i];
+    if (x < 0 || x >= n) return -1;
+    uint64_t &word = seen[x >> 6];
+    const uint64_t bit = uint64_t(1) << (x & 63);
+    if (word & bit) return -2;
+    word |= bit;
+  }
+  return 0;
+}
+
+int gbh_mapping_read(const char *path, int64_t *n, int32_t *ids) {
+  if (!path || !n) return -1;
+  const std::string p(path);
+  if (ends_with(p, ".so")) {
+    std::ifstream f(p, std::ios::binary | std::ios::ate);
+    if (!f) return -2;
+    const int64_t bytes = f.tellg();
+    if (bytes % 4) return -3;
+    if (!ids) {
+      *n = bytes / 4;
+      return 0;
+    }
+    f.seekg(0);
+    f.read(reinterpret_cast<char *>(ids), std::min<int64_t>(bytes, *n * 4));
+    return f ? 0 : -3;
+  }
+  std::ifstream f(p);
+  if (!f) return -2;
+  int64_t count = 0;
+  long long x;
+  while (f >> x) {
+    if (ids) {
+      if (count >= *n) break;
+      ids[count] = static_cast<int32_t>(x);
+    }
+    count++;
+  }
+  *n = count;
+  return 0;
+}
+
+int gbh_mapping_write(const char *path, int64_t n, const int32_t *ids) {
+  if (!path || n < 0 || (n > 0 && !ids)) return -1;
+  const std::string p(path);
+  if (ends_with(p, ".so")) {
+    std::ofstream f(p, std::ios::binary);
+    if (!f) return -2;
+    f.write(reinterpret_cast<const char *>(ids), n * 4);
+    return f ? 0 : -3;
+  }
+  std::ofstream f(p);
+  if (!f) return -2;
+  for (int64_t i = 0; i < n; i++) f << ids[i] << '\n';
+  return f ? 0 : -3;
+}
+
+// SourcePicker::GenerateConsistentSources (benchmark.h:115-156): original
+// ids of non-isolated vertices, shuffled by mt19937_64(kRandSeed), grouped
+// into runs of 'repeats', resolved to internal ids through org_ids.
+int gbh_source_picker(int64_t n, const int64_t *off, const int32_t *org_ids, int count,
+                      int repeats, int32_t *orig_out, int32_t *internal_out) {
+  if (repeats <= 0 || count < 0 || !off) return -2;
+  std::vector<int32_t> internal(n);
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t o = org_ids ? org_ids[i] : i;
+    if (o < 0 || o >= n) return -3;
+    internal[o] = static_cast<int32_t>(i);
+  }
+  std::vector<int32_t> candidates;
+  candidates.reserve(n);
+  for (int64_t o = 0; o < n; o++) {
+    const int32_t x = internal[o];
+    if (off[x + 1] > off[x]) candidates.push_back(static_cast<int32_t>(o));
+  }
+  if (candidates.empty()) return -1;
+  std::mt19937_64 rng(static_cast<uint64_t>(kSeed));
+  std::shuffle(candidates.begin(), candidates.end(), rng);
+  const int groups = (count + repeats - 1) / repeats;
+  int k = 0;
+  for (int gi = 0; gi < groups; gi++) {
+    const int32_t o = candidates[gi % candidates.size()];
+    for (int r = 0; r < repeats && k < count; r++, k++) {
+      orig_out[k] = o;
+      internal_out[k] = internal[o];
+    }
+  }
+  return 0;
+}
+
+int gbh_afforest_samples(int64_t n, int count, int32_t *idx) {
+  if (n <= 0 || count < 0 || (count > 0 && !idx)) return -1;
+  std::mt19937 gen;
+  std::uniform_int_distribution<int32_t> pick(0, static_cast<int32_t>(n - 1));
+  for (int i = 0; i < count; i++) idx[i] = pick(gen);
+  return 0;
+}
+
+// ComputeTier0SampledGraphFeatures (reorder_types.h:6066-6195).
+int gbh_tier0_features(int64_t n, const int64_t *off, const int32_t *nbr, int64_t sample_size,
+                       uint64_t seed, double feat[7]) {
+  if (!feat) return -1;
+  std::fill(feat, feat + 7, 0.0);
+  if (n <= 0) return 0;
+  if (sample_size == 0) {
+    const int64_t root = static_cast<int64_t>(std::sqrt(static_cast<double>(n)));
+    sample_size = std::max<int64_t>(1024, std::min<int64_t>(root, 8192));
+  }
+  sample_size = std::min(sample_size, n);
+  const uint64_t shift = (seed * 0x9E3779B97F4A7C15ULL) % static_cast<uint64_t>(n);
+  struct Sample {
+    int64_t degree, node;
+  };
+  std::vector<Sample> picks(sample_size);
+  double deg_total = 0;
+  for (int64_t i = 0; i < sample_size; i++) {
+    const uint64_t mid = ((2 * static_cast<uint64_t>(i) + 1) * static_cast<uint64_t>(n)) /
+                         (2 * static_cast<uint64_t>(sample_size));
+    const int64_t node = static_cast<int64_t>((mid + shift) % static_cast<uint64_t>(n));
+    picks[i] = {off[node + 1] - off[node], node};
+    deg_total += static_cast<double>(picks[i].degree);
+  }
+  const double mean = deg_total / sample_size;
+  double ss = 0;
+  for (const auto &p : picks) ss += (p.degree - mean) * (p.degree - mean);
+  const double sd = sample_size > 1 ? std::sqrt(ss / (sample_size - 1)) : 0.0;
+  std::sort(picks.begin(), picks.end(), [](const Sample &a, const Sample &b) {
+    return a.degree != b.degree ? a.degree > b.degree : a.node < b.node;
+  });
+  const int64_t hubs = std::max<int64_t>(1, sample_size / 10);
+  double hub_total = 0;
+  std::unordered_set<int64_t> hub_nodes, hub_lines;
+  for (int64_t i = 0; i < hubs; i++) {
+    hub_total += static_cast<double>(picks[i].degree);
+    hub_nodes.insert(picks[i].node);
+  }
+  constexpr int64_t kLine = 64 / sizeof(double);
+  for (int64_t h : hub_nodes) hub_lines.insert(h / kLine);
+  double span = 0;
+  int64_t edges = 0, near = 0;
+  const double denom = static_cast<double>(std::max<int64_t>(1, n - 1));
+  for (const auto &p : picks) {
+    for (int64_t k = off[p.node]; k < off[p.node + 1]; k++) {
+      const int64_t d = nbr[k];
+      span += static_cast<double>(std::llabs(d - p.node)) / denom;
+      edges++;
+      if (std::llabs(d / kLine - p.node / kLine) <= 1) near++;
+    }
+  }
+  feat[0] = mean;
+  feat[1] = mean > 0 ? sd / mean : 0.0;
+  feat[2] = deg_total > 0 ? hub_total / deg_total : 0.0;
+  feat[3] = edges ? span / edges : 0.0;
+  feat[4] = edges ? static_cast<double>(near) / edges : 0.0;
+  feat[5] = hub_lines.empty() ? 0.0 : static_cast<double>(hub_nodes.size()) / hub_lines.size();
+  feat[6] = n > 1 ? mean / (n - 1) : 0.0;
+  return 0;
+}
+
+// ModeledPropertyBytes (reorder_types.h:1226-1259); bench codes follow the
+// reference's BenchmarkType: 1 PR, 2 BFS, 3 CC, 4 SSSP, 5 BC, 6 TC,
+// 7 PR_SPMV, 8 CC_SV.
+int64_t gbh_modeled_property_bytes(int bench, int64_t n, int64_t m) {
+  const int64_t bits = (n + 7) / 8;
+  if (bench == 1 || bench == 7) return 8 * n;
+  if (bench == 2) return 8 * n + 2 * bits;
+  if (bench == 3 || bench == 8) return 4 * n + bits;
+  if (bench == 4) return 4 * n + 4 * m;
+  if (bench == 5) return 20 * n + (m + 7) / 8;
+  if (bench == 6) return 4 * n;
+  return -1;
+}
+
+// ShouldUseAllKernelLowReuseRule (reorder_adaptive.h:302-333).
+int gbh_frozen_rule(int bench, double reuse, int64_t n, double avg_degree, double degree_cv,
+                    double wsr) {
+  const bool kernel_ok = bench == 1 || bench == 2 || bench == 3 || bench == 4 || bench == 5 ||
+                         bench == 7 || bench == 8;
+  if (!kernel_ok || reuse > 2.0 || n < 1000 || wsr > 3.2) return 0;
+  if (degree_cv > 8.0) return 1;
+  return degree_cv <= 2.68 && (avg_degree <= 60.0 || wsr <= 0.82);
+}
+
+}  // extern "C"

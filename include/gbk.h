@@ -109,6 +109,15 @@ int gbk_graph_upload(gbk_ctx *ctx, int64_t n, int directed,
                      const int64_t *in_off, const int32_t *in_nbr,
                      const int32_t *in_wgt, const int32_t *org_ids,
                      gbk_graph **out);
+/* Builds the squished CSR of an edge list on the device: MakeGraphFromEL +
+ * MakeCSR + SquishGraph (gapbs/builder.h:186-318, 444-493).  n < 0 takes
+ * max id + 1 (FindMaxNodeID); symmetrize adds both directions; rows are
+ * sorted by (neighbour, weight), duplicates keep the smallest weight, self
+ * loops are dropped unless keep_self.  Directed graphs get their in-CSR.
+ * device_ms (may be NULL) receives the device time after the upload. */
+int gbk_graph_build(gbk_ctx *ctx, int64_t n, int64_t m_el, const int32_t *u,
+                    const int32_t *v, const int32_t *w, int symmetrize,
+                    int keep_self, gbk_graph **out, double *device_ms);
 int gbk_graph_info(const gbk_graph *g, int64_t *n, int64_t *m, int *directed,
                    int *weighted);
 /* Copies the device CSR back.  Any pointer may be NULL to skip that array. */

@@ -1,0 +1,65 @@
+/*
+ * gbk_host.h -- C ABI of libgbkhost.so, the host-side substrate of the
+ * bench drivers: synthetic graph generation, edge-list readers, mapping
+ * producers and files, source picking and the Tier-0 feature rule.  Pure
+ * C++17/OpenMP on the CPU; the device work lives in libgbk.so (gbk.h).
+ *
+ * Reference interface each entry point follows (gapbs/ =
+ * bench/include/external/gapbs/, reorder/ = bench/include/graphbrew/reorder/):
+ *   gbh_generate_el / gbh_insert_weights  Generator::GenerateEL, InsertWeights  gapbs/generator.h:89-186
+ *   gbh_read_el                           Reader::ReadInEL / ReadInWEL           gapbs/reader.h:91-185
+ *   gbh_mapping_*                         GenerateOriginal/Random/SortMapping    reorder/reorder_basic.h:65-258
+ *   gbh_mapping_fingerprint               BuilderBase::MappingFingerprint        gapbs/builder.h:1398-1411
+ *   gbh_validate_permutation              ValidatePermutationOrThrow             gapbs/builder.h:2042-2081
+ *   gbh_mapping_read / gbh_mapping_write  MAP (-o 13) files, WriteLabels         gapbs/builder.h:2125-2235, gapbs/writer.h:380-412
+ *   gbh_source_picker                     SourcePicker::GenerateConsistentSources gapbs/benchmark.h:115-156
+ *   gbh_tier0_features / gbh_frozen_rule  ComputeTier0SampledGraphFeatures,
+ *                                         ShouldUseAllKernelLowReuseRule         reorder/reorder_types.h:6066-6195, reorder/reorder_adaptive.h:302-333
+ * All calls return 0 on success and a negative code on bad input.
+ */
+#ifndef GBK_HOST_H_
+#define GBK_HOST_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* (1<<scale)*degree edges into u, v (caller-allocated). */
+int gbh_generate_el(int scale, int degree, int uniform, int32_t *u, int32_t *v);
+int gbh_insert_weights(int64_t m, int32_t *w);
+/* Reads a .el (weighted=0) or .wel (weighted=1) file.  First call with
+ * u == NULL returns the edge count in *m; second call fills the arrays. */
+int gbh_read_el(const char *path, int weighted, int64_t *m, int32_t *u, int32_t *v, int32_t *w);
+
+int gbh_mapping_identity(int64_t n, int32_t *new_ids);
+int gbh_mapping_random(int64_t n, int32_t *new_ids);
+/* use_out_degree=1 sorts by out-degree (off), else by in-degree (in_off). */
+int gbh_mapping_sort(int64_t n, const int64_t *off, int32_t *new_ids);
+uint64_t gbh_mapping_fingerprint(int64_t n, const int32_t *new_ids);
+/* 0 ok, -1 out-of-range id, -2 duplicate id. */
+int gbh_validate_permutation(int64_t n, const int32_t *ids);
+/* Mapping files: ".so" = binary int32 new_ids, ".lo" = one id per line.
+ * Read: first call with ids == NULL returns the length in *n. */
+int gbh_mapping_read(const char *path, int64_t *n, int32_t *ids);
+int gbh_mapping_write(const char *path, int64_t n, const int32_t *ids);
+
+int gbh_source_picker(int64_t n, const int64_t *off, const int32_t *org_ids, int count,
+                      int repeats, int32_t *orig_out, int32_t *internal_out);
+/* Afforest's sampled indices (default-seeded mt19937, 1024 draws). */
+int gbh_afforest_samples(int64_t n, int count, int32_t *idx);
+
+/* feat: avg_degree, degree_cv, hub_concentration, normalized_edge_span,
+ * window_neighbor_overlap, packing_factor, internal_density. */
+int gbh_tier0_features(int64_t n, const int64_t *off, const int32_t *nbr, int64_t sample_size,
+                       uint64_t seed, double feat[7]);
+int64_t gbh_modeled_property_bytes(int bench, int64_t n, int64_t m);
+int gbh_frozen_rule(int bench, double reuse, int64_t n, double avg_degree, double degree_cv,
+                    double wsr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GBK_HOST_H_ */

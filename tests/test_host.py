@@ -1,0 +1,89 @@
+"""CPU checks of the product host substrate (libgbkhost.so) against the
+oracle: generator streams, readers, mapping producers/files, sources,
+Tier-0 features and the frozen rule must be identical."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from graphbrew_amd import host as H
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "graphs")
+
+
+@pytest.mark.parametrize("scale,deg,uniform", [(10, 16, False), (12, 8, True), (15, 16, False)])
+def test_generator_matches_oracle(scale, deg, uniform):
+    u, v = H.generate_el(scale, deg, uniform)
+    ou, ov = O.generate_el(scale, deg, uniform)
+    assert np.array_equal(u, ou) and np.array_equal(v, ov)
+
+
+def test_insert_weights_matches_oracle():
+    m = (1 << 18) + 12345  # crosses a stream boundary
+    assert np.array_equal(H.insert_weights(m), O.insert_weights(m))
+
+
+@pytest.mark.parametrize("name,weighted", [("tiny.el", False), ("weighted.wel", True),
+                                           ("isolated.el", False), ("weighted_isolated.wel", True)])
+def test_read_el_fixtures(name, weighted, tmp_path):
+    u, v, w = H.read_el(os.path.join(GOLD, name), weighted)
+    from tests import bruteforce as B
+    bu, bv, bw = B.read_el(os.path.join(GOLD, name), weighted)
+    assert np.array_equal(u, bu) and np.array_equal(v, bv)
+    if weighted:
+        assert np.array_equal(w, bw)
+    p = tmp_path / "c.el"
+    p.write_text("# comment\n% other\n\n3 4\n5 6 7\n")
+    u, v, _ = H.read_el(str(p))
+    assert u.tolist() == [3, 5] and v.tolist() == [4, 6]
+
+
+def test_mappings_match_oracle():
+    g = O.kron(12)
+    assert np.array_equal(H.mapping_random(g.n), O.mapping_random(g.n))
+    assert np.array_equal(H.mapping_sort(g.off), O.mapping_sort(g))
+    assert np.array_equal(H.mapping_identity(7), np.arange(7))
+    ids = O.mapping_random(g.n)
+    assert H.mapping_fingerprint(ids) == O.mapping_fingerprint(ids)
+    assert H.validate_permutation(ids) == 0
+    bad = ids.copy()
+    bad[0] = bad[1]
+    assert H.validate_permutation(bad) == -2
+    bad[0] = -3
+    assert H.validate_permutation(bad) == -1
+
+
+def test_mapping_files_roundtrip(tmp_path):
+    ids = O.mapping_random(1000)
+    for ext in (".so", ".lo"):
+        p = str(tmp_path / ("map" + ext))
+        H.mapping_write(p, ids)
+        assert np.array_equal(H.mapping_read(p), ids)
+    assert (tmp_path / "map.so").stat().st_size == 4000
+
+
+def test_source_picker_matches_oracle():
+    g = O.kron(12)
+    r = O.relabel(g, O.mapping_random(g.n))
+    for graph in (g, r):
+        for reps in (1, 3):
+            ho, hi = H.source_picker(graph.off, 20, graph.org_ids, reps)
+            oo, oi = O.source_picker(graph, 20, reps)
+            assert np.array_equal(ho, oo) and np.array_equal(hi, oi)
+
+
+def test_afforest_samples_match_oracle():
+    assert np.array_equal(H.afforest_samples(65536), O.afforest_sample_indices(65536))
+
+
+def test_tier0_and_rule_match_oracle():
+    g = O.kron(13)
+    hf = H.tier0_features(g.off, g.nbr)
+    of = O.tier0_features(g)
+    for k in H.TIER0_NAMES:
+        assert hf[k] == pytest.approx(of[k], rel=1e-12, abs=1e-15)
+    for bench, code in H.BENCH_CODES.items():
+        assert H.modeled_property_bytes(bench, 1000, 20000) == O.modeled_property_bytes(code, 1000, 20000)
+        for args in ((1, 5000, 30, 1.0, 1.0), (3, 5000, 30, 1.0, 1.0), (1, 5000, 100, 9.0, 3.0)):
+            assert H.frozen_rule(bench, *args) == O.frozen_rule(code, *args)
