@@ -34,7 +34,15 @@ def lib():
             "gbh_mapping_identity": (i32, [i64, VP]),
             "gbh_mapping_random": (i32, [i64, VP]),
             "gbh_mapping_sort": (i32, [i64, VP, VP]),
+            "gbh_mapping_dbg": (i32, [i64, VP, VP]),
             "gbh_mapping_fingerprint": (C.c_uint64, [i64, VP]),
+            "gbh_verify_bfs": (i32, [i64, VP, VP, VP, VP, C.c_int32, VP]),
+            "gbh_verify_cc": (i32, [i64, VP, VP, VP, VP, i32, VP]),
+            "gbh_verify_sssp": (i64, [i64, VP, VP, VP, C.c_int32, VP]),
+            "gbh_verify_bc": (i32, [i64, VP, VP, VP, i32, VP, C.POINTER(f64)]),
+            "gbh_pr_residual": (f64, [i64, VP, VP, VP]),
+            "gbh_weight_checksum": (None, [i64, VP, VP, VP, VP, VP]),
+            "gbh_distance_fingerprint": (None, [i64, VP, VP, VP]),
             "gbh_validate_permutation": (i32, [i64, VP]),
             "gbh_mapping_read": (i32, [C.c_char_p, C.POINTER(i64), VP]),
             "gbh_mapping_write": (i32, [C.c_char_p, i64, VP]),
@@ -104,6 +112,13 @@ def mapping_sort(off: np.ndarray) -> np.ndarray:
     return out
 
 
+def mapping_dbg(off: np.ndarray) -> np.ndarray:
+    off = np.ascontiguousarray(off, np.int64)
+    out = np.empty(len(off) - 1, np.int32)
+    _chk(lib().gbh_mapping_dbg(len(off) - 1, _p(off), _p(out)), "dbg")
+    return out
+
+
 def mapping_fingerprint(ids) -> str:
     ids = np.ascontiguousarray(ids, np.int32)
     return f"{lib().gbh_mapping_fingerprint(len(ids), _p(ids)):016x}"
@@ -158,3 +173,48 @@ def modeled_property_bytes(bench: str, n: int, m: int) -> int:
 def frozen_rule(bench: str, reuse: float, n: int, avg_degree: float, degree_cv: float,
                 wsr: float) -> bool:
     return bool(lib().gbh_frozen_rule(BENCH_CODES[bench], reuse, n, avg_degree, degree_cv, wsr))
+
+
+# ------------------------------------------------------------- verifiers
+def _a32(x):
+    return None if x is None else np.ascontiguousarray(x, np.int32)
+
+
+def verify_bfs(off, nbr, in_off, in_nbr, source: int, parent) -> bool:
+    return bool(lib().gbh_verify_bfs(len(off) - 1, _p(off), _p(nbr), _p(in_off), _p(in_nbr), source,
+                                     _p(_a32(parent))))
+
+
+def verify_cc(off, nbr, in_off, in_nbr, directed: bool, comp) -> bool:
+    return bool(lib().gbh_verify_cc(len(off) - 1, _p(off), _p(nbr), _p(in_off), _p(in_nbr),
+                                    int(directed), _p(_a32(comp))))
+
+
+def verify_sssp(off, nbr, wgt, source: int, dist) -> int:
+    return int(lib().gbh_verify_sssp(len(off) - 1, _p(off), _p(nbr), _p(wgt), source, _p(_a32(dist))))
+
+
+def verify_bc(off, nbr, sources, scores):
+    s = _a32(sources)
+    sc = np.ascontiguousarray(scores, np.float32)
+    worst = C.c_double(0)
+    ok = lib().gbh_verify_bc(len(off) - 1, _p(off), _p(nbr), _p(s), len(s), _p(sc), C.byref(worst))
+    return bool(ok), worst.value
+
+
+def pr_residual(off, nbr, scores) -> float:
+    return float(lib().gbh_pr_residual(len(off) - 1, _p(off), _p(nbr),
+                                       _p(np.ascontiguousarray(scores, np.float32))))
+
+
+def weight_checksum(off, nbr, wgt, org_ids=None) -> str:
+    out = np.zeros(2, np.uint64)
+    lib().gbh_weight_checksum(len(off) - 1, _p(off), _p(nbr), _p(wgt), _p(_a32(org_ids)), _p(out))
+    return f"{int(out[0]):016x}{int(out[1]):016x}"
+
+
+def distance_fingerprint(dist, org_ids=None) -> str:
+    d = _a32(dist)
+    out = np.zeros(2, np.uint64)
+    lib().gbh_distance_fingerprint(len(d), _p(d), _p(_a32(org_ids)), _p(out))
+    return f"{int(out[0]):016x}{int(out[1]):016x}"

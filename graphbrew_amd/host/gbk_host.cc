@@ -213,6 +213,34 @@ int gbh_mapping_sort(int64_t n, const int64_t *off, int32_t *new_ids) {
   return 0;
 }
 
+// GenerateDBGMapping (reorder_hub.h:267-345): degree-based grouping into 8
+// buckets with thresholds avg/2, avg, 2avg, ..., 32avg, inf; the highest
+// bucket gets the lowest ids; inside a bucket vertices keep id order (the
+// reference's static schedule hands each thread a contiguous id range).
+int gbh_mapping_dbg(int64_t n, const int64_t *off, int32_t *new_ids) {
+  if (n < 0 || (n > 0 && (!off || !new_ids))) return -1;
+  if (n == 0) return 0;
+  const uint64_t avg = static_cast<uint64_t>(off[n] - off[0]) / static_cast<uint64_t>(n);
+  const uint64_t limit[8] = {avg / 2, avg, avg * 2, avg * 4, avg * 8, avg * 16, avg * 32,
+                             std::numeric_limits<uint64_t>::max()};
+  auto bucket_of = [&](int64_t v) {
+    const uint64_t d = static_cast<uint64_t>(off[v + 1] - off[v]);
+    int b = 0;
+    while (d > limit[b]) b++;
+    return b;
+  };
+  int64_t count[8] = {0};
+  for (int64_t v = 0; v < n; v++) count[bucket_of(v)]++;
+  int64_t start[8];
+  int64_t run = 0;
+  for (int b = 7; b >= 0; b--) {
+    start[b] = run;
+    run += count[b];
+  }
+  for (int64_t v = 0; v < n; v++) new_ids[v] = static_cast<int32_t>(start[bucket_of(v)]++);
+  return 0;
+}
+
 // MappingFingerprint (builder.h:1398-1411).  The offset basis is the one the
 // reference writes, 1469598103934665603.
 uint64_t gbh_mapping_fingerprint(int64_t n, const int32_t *new_ids) {

@@ -37,6 +37,8 @@ int gbh_mapping_identity(int64_t n, int32_t *new_ids);
 int gbh_mapping_random(int64_t n, int32_t *new_ids);
 /* use_out_degree=1 sorts by out-degree (off), else by in-degree (in_off). */
 int gbh_mapping_sort(int64_t n, const int64_t *off, int32_t *new_ids);
+/* -o 5: degree-based grouping (reorder/reorder_hub.h:267-345). */
+int gbh_mapping_dbg(int64_t n, const int64_t *off, int32_t *new_ids);
 uint64_t gbh_mapping_fingerprint(int64_t n, const int32_t *new_ids);
 /* 0 ok, -1 out-of-range id, -2 duplicate id. */
 int gbh_validate_permutation(int64_t n, const int32_t *ids);
@@ -57,6 +59,31 @@ int gbh_tier0_features(int64_t n, const int64_t *off, const int32_t *nbr, int64_
 int64_t gbh_modeled_property_bytes(int bench, int64_t n, int64_t m);
 int gbh_frozen_rule(int bench, double reuse, int64_t n, double avg_degree, double degree_cv,
                     double wsr);
+
+/* ---- built-in verifiers (gapbs/benchmark.h:475-485) and fingerprints ---- */
+/* BFSVerifier (graphbrew/bfs_common.h:31-97): 1 = PASS. */
+int gbh_verify_bfs(int64_t n, const int64_t *out_off, const int32_t *out_nbr,
+                   const int64_t *in_off, const int32_t *in_nbr, int32_t source,
+                   const int32_t *parent);
+/* CCVerifier (bench/src/cc.cc:290-330): 1 = PASS. */
+int gbh_verify_cc(int64_t n, const int64_t *out_off, const int32_t *out_nbr,
+                  const int64_t *in_off, const int32_t *in_nbr, int directed,
+                  const int32_t *comp);
+/* SSSPVerifier (bench/src/sssp.cc:294-324): number of mismatches (0 = PASS). */
+int64_t gbh_verify_sssp(int64_t n, const int64_t *off, const int32_t *nbr,
+                        const int32_t *wgt, int32_t source, const int32_t *dist);
+/* BCVerifier (bench/src/bc.cc:256-326): 1 = PASS. */
+int gbh_verify_bc(int64_t n, const int64_t *off, const int32_t *nbr,
+                  const int32_t *sources, int num_sources, const float *scores,
+                  double *max_abs_delta);
+/* PRVerifier's residual (bench/src/pr.cc:117-134). */
+double gbh_pr_residual(int64_t n, const int64_t *out_off, const int32_t *out_nbr,
+                       const float *scores);
+/* WeightChecksum / DistanceFingerprint (bench/src/sssp.cc:92-127). */
+void gbh_weight_checksum(int64_t n, const int64_t *off, const int32_t *nbr,
+                         const int32_t *wgt, const int32_t *org_ids, uint64_t out[2]);
+void gbh_distance_fingerprint(int64_t n, const int32_t *dist, const int32_t *org_ids,
+                              uint64_t out[2]);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,7 @@ def lib():
             "orc_validate_permutation": (i32, [i64, I32P]),
             "orc_mapping_random": (i32, [i64, I32P]),
             "orc_mapping_sort": (i32, [i64, I64P, I32P]),
+            "orc_mapping_dbg": (i32, [i64, I64P, I32P]),
             "orc_source_picker": (i32, [i64, I64P, VP, i32, i32, I32P, I32P]),
             "orc_pr_gs": (i32, [i64, I64P, I32P, I64P, i32, f64, i32, i32, F32P,
                                 C.POINTER(C.c_int), C.POINTER(C.c_double)]),
@@ -220,6 +221,12 @@ def mapping_random(n: int) -> np.ndarray:
 def mapping_sort(g: Graph) -> np.ndarray:
     out = np.empty(g.n, np.int32)
     _check(lib().orc_mapping_sort(g.n, g.off, out), "mapping_sort")
+    return out
+
+
+def mapping_dbg(g: Graph) -> np.ndarray:
+    out = np.empty(g.n, np.int32)
+    _check(lib().orc_mapping_dbg(g.n, g.off, out), "mapping_dbg")
     return out
 
 

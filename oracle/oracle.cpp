@@ -361,6 +361,40 @@ int orc_mapping_sort(int64_t n, const int64_t *off, int32_t *new_ids) {
   return 0;
 }
 
+// GenerateDBGMapping (reorder_hub.h:267-345) restated with the reference's
+// thread-local buckets and per-(bucket, thread) start offsets, serial.
+int orc_mapping_dbg(int64_t n, const int64_t *off, int32_t *new_ids) {
+  if (n == 0) return 0;
+  const uint64_t avg = static_cast<uint64_t>(off[n] - off[0]) / n;
+  const uint64_t thr[8] = {avg / 2, avg, avg * 2, avg * 4, avg * 8, avg * 16, avg * 32,
+                           std::numeric_limits<uint64_t>::max()};
+  const int T = 3;  // any team size gives the same answer with a static schedule
+  std::vector<std::vector<std::vector<int32_t>>> local(T, std::vector<std::vector<int32_t>>(8));
+  const int64_t chunk = (n + T - 1) / T;
+  for (int t = 0; t < T; t++)
+    for (int64_t i = t * chunk; i < std::min<int64_t>(n, (t + 1) * chunk); i++) {
+      const uint64_t d = static_cast<uint64_t>(Deg(off, i));
+      for (int j = 0; j < 8; j++)
+        if (d <= thr[j]) {
+          local[t][j].push_back(static_cast<int32_t>(i));
+          break;
+        }
+    }
+  int64_t k = 0;
+  std::vector<std::vector<int64_t>> start(T, std::vector<int64_t>(8));
+  for (int j = 7; j >= 0; j--)
+    for (int t = 0; t < T; t++) {
+      start[t][j] = k;
+      k += local[t][j].size();
+    }
+  for (int t = 0; t < T; t++)
+    for (int j = 7; j >= 0; j--) {
+      int64_t p = start[t][j];
+      for (int32_t v : local[t][j]) new_ids[v] = static_cast<int32_t>(p++);
+    }
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // A16: SourcePicker::GenerateConsistentSources (benchmark.h:115-156)
 // ---------------------------------------------------------------------------

@@ -63,6 +63,8 @@ int orc_validate_permutation(int64_t n, const int32_t *ids);
 int orc_mapping_random(int64_t n, int32_t *new_ids);
 /* -o 2: stable sort by degree, descending (reorder_basic.h:213-258). */
 int orc_mapping_sort(int64_t n, const int64_t *off, int32_t *new_ids);
+/* -o 5: degree-based grouping (reorder_hub.h:267-345). */
+int orc_mapping_dbg(int64_t n, const int64_t *off, int32_t *new_ids);
 
 /* ---- A16 SourcePicker (gapbs/benchmark.h:115-156, 301-309) ------------- */
 /* Writes 'count' (original, internal) source pairs.  org_ids may be NULL

@@ -87,3 +87,47 @@ def test_tier0_and_rule_match_oracle():
         assert H.modeled_property_bytes(bench, 1000, 20000) == O.modeled_property_bytes(code, 1000, 20000)
         for args in ((1, 5000, 30, 1.0, 1.0), (3, 5000, 30, 1.0, 1.0), (1, 5000, 100, 9.0, 3.0)):
             assert H.frozen_rule(bench, *args) == O.frozen_rule(code, *args)
+
+
+def test_dbg_mapping_matches_oracle_and_closed_form():
+    g = O.kron(12)
+    assert np.array_equal(H.mapping_dbg(g.off), O.mapping_dbg(g))
+    # two stars 6-{0,1,2}, 7-{3,4,5}: avg degree 1, leaves in bucket 1, hubs
+    # (degree 3) in bucket 3 -> hubs first, then leaves in id order
+    star = O.make_graph(np.array([6, 6, 6, 7, 7, 7], np.int32), np.arange(6, dtype=np.int32))
+    assert H.mapping_dbg(star.off).tolist() == [2, 3, 4, 5, 6, 7, 0, 1]
+
+
+def test_host_verifiers_agree_with_oracle():
+    g = O.kron(11)
+    gw = O.kron(10, weighted=True)
+    _, srcs = O.source_picker(g, 3)
+    for s in srcs:
+        p = O.bfs_do(g, int(s))
+        assert H.verify_bfs(g.off, g.nbr, g.off, g.nbr, int(s), p)
+        bad = p.copy()
+        far = int(np.argmax(O.bfs_depths(g, int(s))))
+        bad[far] = far
+        assert not H.verify_bfs(g.off, g.nbr, g.off, g.nbr, int(s), bad)
+    c = O.cc_afforest(g)
+    assert H.verify_cc(g.off, g.nbr, g.off, g.nbr, False, c)
+    c2 = c.copy()
+    c2[np.argmax(np.diff(g.off))] = -7
+    assert not H.verify_cc(g.off, g.nbr, g.off, g.nbr, False, c2)
+    sw = int(O.source_picker(gw, 1)[1][0])
+    d = O.dijkstra(gw, sw)
+    assert H.verify_sssp(gw.off, gw.nbr, gw.wgt, sw, d) == 0
+    d2 = d.copy()
+    d2[d2 < O.K_DIST_INF][:1] += 1
+    d2[np.flatnonzero(d < O.K_DIST_INF)[1]] += 1
+    assert H.verify_sssp(gw.off, gw.nbr, gw.wgt, sw, d2) == 1
+    assert H.weight_checksum(gw.off, gw.nbr, gw.wgt) == O.weight_checksum(gw)
+    assert H.distance_fingerprint(d) == O.distance_fingerprint(gw, d)
+    b = O.bc_serial(g, srcs)
+    ok, worst = H.verify_bc(g.off, g.nbr, srcs, b)
+    assert ok and worst < 1e-6
+    b2 = b.copy()
+    b2[np.argmax(b2)] *= 0.9
+    assert not H.verify_bc(g.off, g.nbr, srcs, b2)[0]
+    s, _, _ = O.pr_gs(g)
+    assert H.pr_residual(g.off, g.nbr, s) == pytest.approx(O.pr_residual(g, s), rel=1e-9)
