@@ -72,7 +72,7 @@ struct BcForwardEdge {
 __global__ void __launch_bounds__(kBlock)
 bc_forward_level(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigma,
                  uint32_t *succ, int32_t *queue, int64_t lb, int64_t le, int64_t cap, int32_t d,
-                 int64_t heavy_deg, BcCounters *c) {
+                 int64_t heavy_deg, BcCounters *c, HeavyQueue hq) {
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   BcForwardEdge f;
   f.nbr = nbr;
@@ -86,7 +86,25 @@ bc_forward_level(const int64_t *off, const int32_t *nbr, int32_t *depth, double 
     s = off[u];
     l = off[u + 1] - s;
   };
-  expand_list(queue + lb, le - lb, heavy_deg, rowlen, f);
+  expand_list_light(queue + lb, le - lb, heavy_deg, rowlen, f, hq);
+  f.q.flush();
+}
+
+// Heavy rows of the level, spread over all workgroups.
+__global__ void __launch_bounds__(kBlock)
+bc_forward_heavy(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigma,
+                 uint32_t *succ, int32_t *queue, int64_t cap, int32_t d, BcCounters *c,
+                 HeavyQueue hq) {
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  BcForwardEdge f;
+  f.nbr = nbr;
+  f.depth = depth;
+  f.sigma = sigma;
+  f.succ = succ;
+  f.q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
+  f.edge_base = off[0];
+  f.d = d;
+  expand_heavy(hq, f);
   f.q.flush();
 }
 
@@ -236,7 +254,9 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
   if (!depth || !sigma || !delta || !queue || !succ || !c || !hc)
     return set_error(ctx, GBK_ENOMEM, "bc scratch");
   auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS11, sizeof(unsigned long long) * kGridCap));
-  if (!partial) return set_error(ctx, GBK_ENOMEM, "bc partials");
+  HeavyQueue hq;
+  if (!partial || !make_heavy_queue(ctx, g, kS12, &hq)) return set_error(ctx, GBK_ENOMEM, "bc partials");
+  hq.overflow = &c->overflow;
   const int64_t H = ctx->tune.heavy_degree;
   const int64_t cap = n + 64;
   const unsigned ng = expand_grid(n);
@@ -254,8 +274,12 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
     for (;;) {
       d++;
       const int64_t lb = level[level.size() - 2], le = level.back();
+      fill<unsigned long long>(ctx, hq.count, 0ull, 1);
       bc_forward_level<<<expand_grid(le - lb), kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ,
-                                                              queue, lb, le, cap, d, H, c);
+                                                              queue, lb, le, cap, d, H, c, hq);
+      heavy_prefix<<<1, 1024, 0, s>>>(hq);
+      bc_forward_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ, queue, cap,
+                                                     d, c, hq);
       GBK_LAUNCH_CHECK(ctx, "bc_forward_level");
       GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcCounters), hipMemcpyDeviceToHost, s));
       GBK_HIP(ctx, hipStreamSynchronize(s));

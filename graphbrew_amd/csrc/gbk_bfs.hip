@@ -65,7 +65,7 @@ struct TdEdge {
 __global__ void __launch_bounds__(kBlock)
 bfs_td_step(const int64_t *out_off, const int32_t *out_nbr, int32_t *parent, int32_t *queue,
             int64_t qb, int64_t qe, int64_t cap, int64_t heavy_deg, BfsCounters *c,
-            unsigned long long *partial) {
+            unsigned long long *partial, HeavyQueue hq) {
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   TdEdge f;
   f.nbr = out_nbr;
@@ -76,7 +76,22 @@ bfs_td_step(const int64_t *out_off, const int32_t *out_nbr, int32_t *parent, int
     s = out_off[u];
     l = out_off[u + 1] - s;
   };
-  expand_list(queue + qb, qe - qb, heavy_deg, rowlen, f);
+  expand_list_light(queue + qb, qe - qb, heavy_deg, rowlen, f, hq);
+  f.q.flush();
+  block_partial(f.scout, partial);
+}
+
+// Heavy rows of the frontier, spread over all workgroups.
+__global__ void __launch_bounds__(kBlock)
+bfs_td_heavy(const int32_t *out_nbr, int32_t *parent, int32_t *queue, int64_t cap, BfsCounters *c,
+             unsigned long long *partial, HeavyQueue hq) {
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  TdEdge f;
+  f.nbr = out_nbr;
+  f.parent = parent;
+  f.scout = 0;
+  f.q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
+  expand_heavy(hq, f);
   f.q.flush();
   block_partial(f.scout, partial);
 }
@@ -141,12 +156,14 @@ bfs_bitmap_to_queue(const uint32_t *bm, int64_t n, int32_t *queue, int64_t cap, 
   q.flush();
 }
 
-// Writes the summed partial into a counter (one workgroup).
+// Writes the summed partials (one or two ranges) into a counter (one workgroup).
 __global__ void bfs_store_count(const unsigned long long *partial, int count,
-                                unsigned long long *dst) {
+                                unsigned long long *dst, const unsigned long long *partial2 = nullptr,
+                                int count2 = 0) {
   __shared__ unsigned long long red[1024 / kWave];
   unsigned long long s = 0;
   for (int i = threadIdx.x; i < count; i += blockDim.x) s += partial[i];
+  for (int i = threadIdx.x; i < count2; i += blockDim.x) s += partial2[i];
   s = wave_sum(s);
   if (lane_id() == 0) red[threadIdx.x / kWave] = s;
   __syncthreads();
@@ -207,8 +224,10 @@ int gbk_bfs_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha, 
   auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS9, sizeof(unsigned long long) * 2 * kGridCap));
   auto *fin = static_cast<unsigned long long *>(scratch(ctx, kS10, 64));
   auto *hc = static_cast<BfsCounters *>(pinned(ctx, sizeof(BfsCounters) * 2));
-  if (!queue || !bm0 || !bm1 || !c || !partial || !fin || !hc)
+  HeavyQueue hq;
+  if (!queue || !bm0 || !bm1 || !c || !partial || !fin || !hc || !make_heavy_queue(ctx, g, kS11, &hq))
     return set_error(ctx, GBK_ENOMEM, "bfs scratch");
+  hq.overflow = &c->overflow;  // reported with the step counters
   int64_t out_deg_src = 0;
   {
     int64_t o[2];
@@ -257,9 +276,13 @@ int gbk_bfs_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha, 
     } else {
       edges_to_check -= scout_count;
       const unsigned grid = expand_grid(qe - qb);
+      fill<unsigned long long>(ctx, hq.count, 0ull, 1);
       bfs_td_step<<<grid, kBlock, 0, s>>>(g->off, g->nbr, parent, queue, qb, qe, cap,
-                                          ctx->tune.heavy_degree, c, partial);
-      bfs_store_count<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), &c->scout);
+                                          ctx->tune.heavy_degree, c, partial, hq);
+      heavy_prefix<<<1, 1024, 0, s>>>(hq);
+      bfs_td_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, parent, queue, cap, c, partial + kGridCap, hq);
+      bfs_store_count<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), &c->scout,
+                                         partial + kGridCap, static_cast<int>(kHeavyGrid));
       GBK_LAUNCH_CHECK(ctx, "bfs_td_step");
       if (int rc = read_counters()) return rc;
       scout_count = static_cast<int64_t>(hc->scout);

@@ -16,6 +16,8 @@
 // wave, so they may use wave-aggregated appends.
 #pragma once
 
+#include <algorithm>
+
 #include "gbk_common.h"
 
 namespace gbk {
@@ -113,6 +115,124 @@ __device__ void expand_list(const int32_t *list, int64_t count, int64_t heavy_de
     const int nh = heavy_n;
     for (int h = 0; h < nh; h++) block_walk(heavy_rows[h], heavy_start[h], heavy_len[h], f);
     __syncthreads();
+  }
+}
+
+// Heavy rows found in a frontier, deferred to a second kernel that spreads
+// each of them over every workgroup (one hub no longer holds one workgroup
+// for hundreds of microseconds while the rest of the GPU idles).
+struct HeavyQueue {
+  int32_t *row;
+  int64_t *start;
+  int64_t *len;
+  int64_t *prefix;            // chunk prefix over the queued rows (cap + 1)
+  unsigned long long *count;  // reset to 0 before the light kernel
+  int64_t cap;
+  int *overflow;
+};
+
+constexpr int64_t kHeavyChunk = 1024;
+constexpr unsigned kHeavyGrid = 512;
+
+// Device storage for a HeavyQueue in scratch slot 'slot': at most
+// m / heavy_deg rows can exceed heavy_deg.  Returns false on allocation
+// failure.  The count and overflow words are cleared on the stream.
+inline bool make_heavy_queue(gbk_ctx *ctx, const gbk_graph *g, int slot, HeavyQueue *hq) {
+  const int64_t H = ctx->tune.heavy_degree;
+  const int64_t cap = std::min<int64_t>(g->n, g->m / (H > 0 ? H : 1) + 1) + 1;
+  const size_t bytes = 128 + cap * (4 + 8 + 8 + 8) + 64;
+  char *p = static_cast<char *>(scratch(ctx, slot, bytes));
+  if (!p) return false;
+  hq->count = reinterpret_cast<unsigned long long *>(p);
+  hq->overflow = reinterpret_cast<int *>(p + 64);
+  hq->start = reinterpret_cast<int64_t *>(p + 128);
+  hq->len = hq->start + cap;
+  hq->prefix = hq->len + cap;
+  hq->row = reinterpret_cast<int32_t *>(hq->prefix + cap + 1);
+  hq->cap = cap;
+  fill<unsigned long long>(ctx, hq->count, 0ull, 1);
+  fill<int>(ctx, hq->overflow, 0, 1);
+  return true;
+}
+
+// Light part of a frontier expansion: rows above heavy_deg go to 'hq'.
+template <class RowLen, class F>
+__device__ void expand_list_light(const int32_t *list, int64_t count, int64_t heavy_deg,
+                                  RowLen rowlen, F &f, const HeavyQueue &hq) {
+  for (int64_t tile = blockIdx.x; tile * kBlock < count; tile += gridDim.x) {
+    const int64_t i = tile * kBlock + threadIdx.x;
+    int64_t row = 0, start = 0, len = 0;
+    if (i < count) {
+      row = list[i];
+      rowlen(row, start, len);
+      if (len > heavy_deg) {
+        const unsigned long long at = atomicAdd(hq.count, 1ull);
+        if (static_cast<int64_t>(at) < hq.cap) {
+          hq.row[at] = static_cast<int32_t>(row);
+          hq.start[at] = start;
+          hq.len[at] = len;
+        } else {
+          *hq.overflow = 1;
+        }
+        len = 0;
+      }
+    }
+    wave_walk(row, start, len, f);
+  }
+}
+
+// One workgroup (1024 lanes): prefix[h] = chunks of rows [0, h); prefix[rows]
+// = total.  Run between the light and the heavy kernel.
+static __global__ void heavy_prefix(HeavyQueue hq) {
+  __shared__ int64_t part[1024];
+  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  const int t = threadIdx.x;
+  const int64_t per = (rows + blockDim.x - 1) / blockDim.x;
+  const int64_t b = t * per, e = min(rows, b + per);
+  int64_t s = 0;
+  for (int64_t h = b; h < e; h++) s += (hq.len[h] + kHeavyChunk - 1) / kHeavyChunk;
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int64_t run = 0;
+    for (int i = 0; i < static_cast<int>(blockDim.x); i++) {
+      const int64_t x = part[i];
+      part[i] = run;
+      run += x;
+    }
+    hq.prefix[rows] = run;
+  }
+  __syncthreads();
+  int64_t run = part[t];
+  for (int64_t h = b; h < e; h++) {
+    hq.prefix[h] = run;
+    run += (hq.len[h] + kHeavyChunk - 1) / kHeavyChunk;
+  }
+}
+
+// Heavy part: the chunks of all queued rows are numbered globally and
+// workgroup b takes chunks b, b+G, b+2G, ... (each found by binary search in
+// the chunk prefix), so the work spreads evenly however many rows there are.
+template <class F>
+__device__ void expand_heavy(const HeavyQueue &hq, F &f) {
+  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  if (rows == 0) return;
+  const int64_t total = hq.prefix[rows];
+  for (int64_t gc = blockIdx.x; gc < total; gc += gridDim.x) {
+    int64_t lo = 0, hi = rows;  // last row with prefix[row] <= gc
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (hq.prefix[mid] <= gc) lo = mid;
+      else hi = mid;
+    }
+    const int64_t len = hq.len[lo], start = hq.start[lo], row = hq.row[lo];
+    const int64_t first = (gc - hq.prefix[lo]) * kHeavyChunk;
+    const int64_t last = min(len, first + kHeavyChunk);
+    for (int64_t base = first; base < last; base += blockDim.x) {
+      const int64_t e = base + threadIdx.x;
+      const bool active = e < last;
+      f(row, active ? start + e : 0, active);
+    }
   }
 }
 

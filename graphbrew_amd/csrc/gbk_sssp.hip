@@ -2,20 +2,23 @@
 //
 // Buckets of width delta as in the reference: a vertex is relaxed from the
 // current bucket b only while dist[v] >= delta*b (stale copies are skipped,
-// sssp.cc:191-192), relaxations use an atomic minimum, and vertices that
-// land in bucket b are processed again in b before the next bucket opens
-// (the reference's bucket fusion does the same for small bins).  On the GPU:
-//   near[2]  ping-pong queues of the current bucket (deduplicated per round)
-//   far[2]   one pending list for every later bucket (deduplicated per list)
-// Three grid-wide kernels form one step -- relax the near queue, find the
-// smallest pending bucket, split the pending list -- each followed by a
-// one-workgroup kernel that advances the device state block; every kernel
-// acts only in its phase.  The host queues steps without waiting and checks
-// the 'done' flag every few steps, so the number of host round trips is
-// logarithmic in the number of steps.  Queue appends are staged per wave in
-// LDS (gbk_expand.h WaveQueue), so the shared tails see one atomic per
-// batch of up to 512 entries.  Distances are
-// integers, so the result is exact whatever the order of relaxations.
+// sssp.cc:191-192), relaxations take the minimum atomically, and vertices
+// that land in bucket b are processed again in b before the next bucket
+// opens (the reference's bucket fusion does the same for small bins).
+//
+// GPU organisation:
+//   near[2]   ping-pong queues of the current bucket, deduplicated per round
+//             with a per-vertex round tag, staged per wave in LDS.
+//   later buckets are not queued at all: when the current bucket drains, one
+//             dense pass over dist[] finds the smallest bucket still holding
+//             an unsettled vertex and a second pass queues that bucket.  Two
+//             streaming passes over 4n bytes per bucket cost far less than a
+//             pending list would in random atomics (one per improvement).
+// Kernels of one step: relax, then min-bucket and split (each acting only in
+// its phase), each followed by a one-workgroup kernel that advances the
+// device state block.  The host queues steps without waiting and checks the
+// 'done' flag after 2, 4, ... 64 steps.  Distances are integers, so the
+// result is exact whatever the order of relaxations.
 #include <climits>
 
 #include "gbk_expand.h"
@@ -26,40 +29,32 @@ constexpr int32_t kDistInf = INT32_MAX / 2;  // sssp.cc:67
 
 struct SsspState {
   unsigned long long near_cnt[2];
-  unsigned long long far_cnt[2];
   unsigned long long next_bucket;
   long long bucket;
   long long relax_rounds;
   long long buckets;
-  int cur, fcur, phase, done;
-  int overflow;
-  unsigned int blocks_done;
+  int cur, phase, done, overflow;
 };
 
-__global__ void sssp_init(int64_t n, int32_t source, int32_t *dist, int32_t *stamp,
-                          int32_t *fstamp, int32_t *near0, SsspState *st) {
+__global__ void sssp_init(int64_t n, int32_t source, int32_t *dist, int32_t *stamp, int32_t *near0,
+                          SsspState *st) {
   const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v < n) {
     dist[v] = v == source ? 0 : kDistInf;
     stamp[v] = -1;
-    fstamp[v] = -1;
   }
   if (v == 0) {
     near0[0] = source;
     st->near_cnt[0] = 1;
     st->near_cnt[1] = 0;
-    st->far_cnt[0] = 0;
-    st->far_cnt[1] = 0;
     st->next_bucket = ULLONG_MAX;
     st->bucket = 0;
     st->relax_rounds = 0;
     st->buckets = 0;
     st->cur = 0;
-    st->fcur = 0;
     st->phase = 0;
     st->done = 0;
     st->overflow = 0;
-    st->blocks_done = 0;
   }
 }
 
@@ -68,15 +63,12 @@ struct RelaxEdge {
   const int32_t *wgt;
   int32_t *dist;
   int32_t *stamp;
-  int32_t *fstamp;
   WaveQueue nearq;
-  WaveQueue farq;
   int64_t delta;
   int64_t bucket;
-  int32_t near_tag;
-  int32_t far_gen;
+  int32_t tag;
   __device__ void operator()(int64_t u, int64_t k, bool active) {
-    bool to_near = false, to_far = false;
+    bool to_near = false;
     int32_t v = 0;
     if (active) {
       v = nbr[k];
@@ -84,78 +76,90 @@ struct RelaxEdge {
       // test first: most relaxations do not improve, and a plain load can
       // only overstate dist[v] (values only decrease), never hide a win
       const int32_t old = nd < dist[v] ? atomicMin(&dist[v], nd) : nd;
-      if (nd < old) {
-        if (nd / delta == bucket) to_near = atomicExch(&stamp[v], near_tag) != near_tag;
-        else to_far = atomicExch(&fstamp[v], far_gen) != far_gen;
-      }
+      if (nd < old && nd / delta == bucket && stamp[v] != tag)
+        to_near = atomicExch(&stamp[v], tag) != tag;
     }
     nearq.push(to_near, v);
-    farq.push(to_far, v);
   }
 };
 
-__global__ void __launch_bounds__(kBlock)
-sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *dist,
-           int32_t *stamp, int32_t *fstamp, int32_t *near0, int32_t *near1, int32_t *far0,
-           int32_t *far1, int64_t cap, int64_t delta, int64_t heavy_deg, SsspState *st) {
+__device__ __forceinline__ void relax_setup(RelaxEdge &f, int32_t *seg, const int32_t *nbr,
+                                            const int32_t *wgt, int32_t *dist, int32_t *stamp,
+                                            int32_t *near0, int32_t *near1, int64_t cap,
+                                            int64_t delta, SsspState *st) {
   const volatile SsspState *vs = st;
-  if (vs->done || vs->phase != 0) return;
-  __shared__ int32_t seg[2 * kWavesPerBlock * kSeg];
-  const int cur = vs->cur, fcur = vs->fcur;
-  const int64_t bucket = vs->bucket;
-  const int64_t count = static_cast<int64_t>(vs->near_cnt[cur]);
-  const int32_t *list = cur ? near1 : near0;
-  const int w = threadIdx.x / kWave;
-  RelaxEdge f;
+  const int cur = vs->cur;
   f.nbr = nbr;
   f.wgt = wgt;
   f.dist = dist;
   f.stamp = stamp;
-  f.fstamp = fstamp;
-  f.nearq.init(seg + w * kSeg, &st->near_cnt[1 - cur], cur ? near0 : near1, cap, &st->overflow);
-  f.farq.init(seg + (kWavesPerBlock + w) * kSeg, &st->far_cnt[fcur], fcur ? far1 : far0, cap,
-              &st->overflow);
+  f.nearq.init(seg + (threadIdx.x / kWave) * kSeg, &st->near_cnt[1 - cur], cur ? near0 : near1, cap,
+               &st->overflow);
   f.delta = delta;
-  f.bucket = bucket;
-  f.near_tag = static_cast<int32_t>(2 * vs->relax_rounds + 2);
-  f.far_gen = static_cast<int32_t>(vs->buckets);
-  const int64_t lower = delta * bucket;
+  f.bucket = vs->bucket;
+  f.tag = static_cast<int32_t>(2 * vs->relax_rounds + 2);
+}
+
+__global__ void __launch_bounds__(kBlock)
+sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *dist,
+           int32_t *stamp, int32_t *near0, int32_t *near1, int64_t cap, int64_t delta,
+           int64_t heavy_deg, SsspState *st, HeavyQueue hq) {
+  const volatile SsspState *vs = st;
+  if (vs->done || vs->phase != 0) return;
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  const int cur = vs->cur;
+  const int64_t count = static_cast<int64_t>(vs->near_cnt[cur]);
+  RelaxEdge f;
+  relax_setup(f, seg, nbr, wgt, dist, stamp, near0, near1, cap, delta, st);
+  const int64_t lower = delta * f.bucket;
   auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
     s = off[u];
     l = dist[u] >= lower ? off[u + 1] - s : 0;
   };
-  expand_list(list, count, heavy_deg, rowlen, f);
+  expand_list_light(cur ? near1 : near0, count, heavy_deg, rowlen, f, hq);
   f.nearq.flush();
-  f.farq.flush();
 }
 
-// One workgroup: closes a relaxation round (swap the near queues; open the
-// bucket search when the next near queue is empty).
-__global__ void sssp_relax_done(SsspState *st) {
+// Heavy rows of the near queue, spread over all workgroups.
+__global__ void __launch_bounds__(kBlock)
+sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *stamp,
+                 int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, SsspState *st,
+                 HeavyQueue hq) {
+  const volatile SsspState *vs = st;
+  if (vs->done || vs->phase != 0) return;
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  RelaxEdge f;
+  relax_setup(f, seg, nbr, wgt, dist, stamp, near0, near1, cap, delta, st);
+  expand_heavy(hq, f);
+  f.nearq.flush();
+}
+
+// One workgroup: closes a relaxation round (swap the near queues; when the
+// next one is empty the bucket is done and the search for the next opens).
+__global__ void sssp_relax_done(SsspState *st, unsigned long long *heavy_count) {
   if (threadIdx.x != 0 || st->done || st->phase != 0) return;
+  *heavy_count = 0;
   const int cur = st->cur;
   st->near_cnt[cur] = 0;
   st->cur = 1 - cur;
   st->relax_rounds += 1;
-  if (st->near_cnt[1 - cur] == 0) {
-    st->phase = 1;
-    st->next_bucket = ULLONG_MAX;
-  }
+  if (st->near_cnt[1 - cur] == 0) st->phase = 1;
 }
 
-__global__ void sssp_min_bucket(const int32_t *dist, const int32_t *far0, const int32_t *far1,
-                                int64_t delta, SsspState *st, unsigned long long *partial) {
+// Smallest bucket above the current one that holds a reached vertex.
+__global__ void sssp_min_bucket(const int32_t *dist, int64_t n, int64_t delta, const SsspState *st,
+                                unsigned long long *partial) {
   const volatile SsspState *vs = st;
   if (vs->done || vs->phase != 1) return;
-  const int fcur = vs->fcur;
   const int64_t bucket = vs->bucket;
-  const int64_t count = static_cast<int64_t>(vs->far_cnt[fcur]);
-  const int32_t *far = fcur ? far1 : far0;
   unsigned long long best = ULLONG_MAX;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
-    const int64_t b = dist[far[i]] / delta;
-    if (b > bucket && static_cast<unsigned long long>(b) < best) best = static_cast<unsigned long long>(b);
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
+    const int32_t d = dist[v];
+    if (d < kDistInf) {
+      const int64_t b = d / delta;
+      if (b > bucket && static_cast<unsigned long long>(b) < best) best = static_cast<unsigned long long>(b);
+    }
   }
   for (int o = kWave / 2; o > 0; o >>= 1) {
     const unsigned long long x = __shfl_xor(best, o);
@@ -183,59 +187,41 @@ __global__ void sssp_min_done(const unsigned long long *partial, int count, Sssp
   if (lane_id() == 0) red[threadIdx.x / kWave] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
+    m = red[0];
     for (int i = 1; i < static_cast<int>(blockDim.x) / kWave; i++) m = red[i] < m ? red[i] : m;
-    m = red[0] < m ? red[0] : m;
     st->next_bucket = m;
-    if (m == ULLONG_MAX) {
-      st->done = 1;
-      st->far_cnt[st->fcur] = 0;
-    } else {
-      st->phase = 2;
-    }
+    if (m == ULLONG_MAX) st->done = 1;
+    else st->phase = 2;
   }
 }
 
+// Queues every vertex of the chosen bucket (each vertex once).
 __global__ void __launch_bounds__(kBlock)
-sssp_split(const int32_t *dist, int32_t *stamp, int32_t *fstamp, int32_t *near0,
-           int32_t *near1, int32_t *far0, int32_t *far1, int64_t cap, int64_t delta,
-           SsspState *st) {
+sssp_split(const int32_t *dist, int64_t n, int32_t *near0, int32_t *near1, int64_t cap,
+           int64_t delta, SsspState *st) {
   const volatile SsspState *vs = st;
   if (vs->done || vs->phase != 2) return;
-  __shared__ int32_t seg[2 * kWavesPerBlock * kSeg];
-  const int cur = vs->cur, fcur = vs->fcur;
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  const int cur = vs->cur;
   const int64_t nb = static_cast<int64_t>(vs->next_bucket);
-  const int64_t count = static_cast<int64_t>(vs->far_cnt[fcur]);
-  const int32_t *far = fcur ? far1 : far0;
-  const int32_t near_tag = static_cast<int32_t>(2 * vs->relax_rounds + 1);
-  const int32_t gen = static_cast<int32_t>(vs->buckets + 1);
-  const int w = threadIdx.x / kWave;
-  WaveQueue nq, fq;
-  nq.init(seg + w * kSeg, &st->near_cnt[cur], cur ? near1 : near0, cap, &st->overflow);
-  fq.init(seg + (kWavesPerBlock + w) * kSeg, &st->far_cnt[1 - fcur], fcur ? far0 : far1, cap,
-          &st->overflow);
+  WaveQueue q;
+  q.init(seg + (threadIdx.x / kWave) * kSeg, &st->near_cnt[cur], cur ? near1 : near0, cap,
+         &st->overflow);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  // uniform trip count per wave so the queue pushes can ballot
-  for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < count; base += stride) {
-    const int64_t i = base + threadIdx.x;
-    bool to_near = false, to_far = false;
-    int32_t v = 0;
-    if (i < count) {
-      v = far[i];
-      const int64_t b = dist[v] / delta;
-      if (b == nb) to_near = atomicExch(&stamp[v], near_tag) != near_tag;
-      else if (b > nb) to_far = atomicExch(&fstamp[v], gen) != gen;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
+    const int64_t v = base + threadIdx.x;
+    bool take = false;
+    if (v < n) {
+      const int32_t d = dist[v];
+      take = d < kDistInf && d / delta == nb;
     }
-    nq.push(to_near, v);
-    fq.push(to_far, v);
+    q.push(take, static_cast<int32_t>(v));
   }
-  nq.flush();
-  fq.flush();
+  q.flush();
 }
 
 __global__ void sssp_split_done(SsspState *st) {
   if (threadIdx.x != 0 || st->done || st->phase != 2) return;
-  st->far_cnt[st->fcur] = 0;
-  st->fcur = 1 - st->fcur;
   st->bucket = static_cast<long long>(st->next_bucket);
   st->buckets += 1;
   st->phase = 0;
@@ -265,29 +251,34 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   hipStream_t s = ctx->stream;
   const int64_t cap = n + 64;
   auto *stamp = static_cast<int32_t *>(scratch(ctx, kS5, sizeof(int32_t) * cap));
-  auto *fstamp = static_cast<int32_t *>(scratch(ctx, kS6, sizeof(int32_t) * cap));
-  auto *q = static_cast<int32_t *>(scratch(ctx, kS7, sizeof(int32_t) * cap * 4));
+  auto *q = static_cast<int32_t *>(scratch(ctx, kS7, sizeof(int32_t) * cap * 2));
   auto *st = static_cast<SsspState *>(scratch(ctx, kS8, sizeof(SsspState) + 64));
   auto *fin = static_cast<unsigned long long *>(scratch(ctx, kS9, 64));
-  auto *hs = static_cast<SsspState *>(pinned(ctx, sizeof(SsspState)));
-  if (!stamp || !fstamp || !q || !st || !fin || !hs) return set_error(ctx, GBK_ENOMEM, "sssp scratch");
-  int32_t *near0 = q, *near1 = q + cap, *far0 = q + 2 * cap, *far1 = q + 3 * cap;
-  const int64_t H = ctx->tune.heavy_degree;
-  const unsigned grid = 1024;
   auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS10, sizeof(unsigned long long) * 2 * kGridCap));
-  if (!partial) return set_error(ctx, GBK_ENOMEM, "sssp partials");
+  auto *hs = static_cast<SsspState *>(pinned(ctx, sizeof(SsspState)));
+  HeavyQueue hq;
+  if (!stamp || !q || !st || !fin || !partial || !hs || !make_heavy_queue(ctx, g, kS11, &hq))
+    return set_error(ctx, GBK_ENOMEM, "sssp scratch");
+  hq.overflow = &st->overflow;
+  int32_t *near0 = q, *near1 = q + cap;
+  const int64_t H = ctx->tune.heavy_degree;
+  const unsigned relax_grid = 1024;
+  const unsigned scan_grid = expand_grid(n);
   EventTimer timer(ctx);
-  sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, dist, stamp, fstamp, near0, st);
+  sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, dist, stamp, near0, st);
   int batch = 2;
   int64_t launched = 0;
   for (;;) {
     for (int j = 0; j < batch; j++) {
-      sssp_relax<<<grid, kBlock, 0, s>>>(g->off, g->nbr, g->wgt, dist, stamp, fstamp, near0, near1,
-                                         far0, far1, cap, delta, H, st);
-      sssp_relax_done<<<1, kWave, 0, s>>>(st);
-      sssp_min_bucket<<<grid, kBlock, 0, s>>>(dist, far0, far1, delta, st, partial);
-      sssp_min_done<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), st);
-      sssp_split<<<grid, kBlock, 0, s>>>(dist, stamp, fstamp, near0, near1, far0, far1, cap, delta, st);
+      sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->nbr, g->wgt, dist, stamp, near0, near1, cap,
+                                               delta, H, st, hq);
+      heavy_prefix<<<1, 1024, 0, s>>>(hq);
+      sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, g->wgt, dist, stamp, near0, near1, cap,
+                                                     delta, st, hq);
+      sssp_relax_done<<<1, kWave, 0, s>>>(st, hq.count);
+      sssp_min_bucket<<<scan_grid, kBlock, 0, s>>>(dist, n, delta, st, partial);
+      sssp_min_done<<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), st);
+      sssp_split<<<scan_grid, kBlock, 0, s>>>(dist, n, near0, near1, cap, delta, st);
       sssp_split_done<<<1, kWave, 0, s>>>(st);
     }
     launched += batch;
@@ -297,12 +288,14 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
     if (hs->overflow) return set_error(ctx, GBK_EHIP, "sssp: queue overflow");
     if (hs->done) break;
     if (batch < 64) batch *= 2;
+    // every step either settles a bucket or shrinks the work of the current
+    // one; more steps than buckets times rounds would mean a broken state
     if (launched > 4 * (n + 16) + 1024) return set_error(ctx, GBK_EHIP, "sssp: no convergence");
   }
-  const unsigned ng = expand_grid(n);
-  sssp_finish<<<ng, kBlock, 0, s>>>(g->off, dist, n, partial, partial + kGridCap);
-  sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial, static_cast<int>(ng), fin, 0);
-  sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial + kGridCap, static_cast<int>(ng), fin + 1, 0);
+  sssp_finish<<<scan_grid, kBlock, 0, s>>>(g->off, dist, n, partial, partial + kGridCap);
+  sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), fin, 0);
+  sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial + kGridCap, static_cast<int>(scan_grid), fin + 1, 0);
+  GBK_LAUNCH_CHECK(ctx, "sssp_finish");
   const double ms = timer.stop_ms();
   unsigned long long hf[2] = {0, 0};
   GBK_HIP(ctx, hipMemcpyAsync(hf, fin, sizeof(hf), hipMemcpyDeviceToHost, s));

@@ -14,7 +14,10 @@
 // GOrder, COrder, RCM, GraphBrew/Leiden, ...) enter through -o 13:<file>.
 #pragma once
 
+#include <execinfo.h>
 #include <getopt.h>
+#include <signal.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <chrono>
@@ -74,7 +77,19 @@ struct Cli {
   int device = 0;
 
   // extra: the per-kernel getopt letters beyond CLApp's
+  static void OnFault(int sig) {
+    void *frames[48];
+    const int k = backtrace(frames, 48);
+    const char msg[] = "driver: fatal signal, host backtrace:\n";
+    (void)!write(2, msg, sizeof(msg) - 1);
+    backtrace_symbols_fd(frames, k, 2);
+    signal(sig, SIG_DFL);
+    raise(sig);
+  }
+
   bool Parse(int argc, char **argv, const std::string &extra) {
+    setvbuf(stdout, nullptr, _IOLBF, 0);  // every line reaches a pipe as it is printed
+    signal(SIGSEGV, OnFault);
     std::string opts = "f:g:hk:su:m:o:zj:SlD:W:an:r:R:v" + extra;
     int c;
     optind = 1;
