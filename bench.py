@@ -48,6 +48,10 @@ def parse():
     p.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU sample work")
     p.add_argument("--only", default="", help="comma list of kernels to run (profiling)")
     p.add_argument("--verbose", action="store_true")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="first copy MIB MiB device to device (calibrates HBM byte counters)")
+    p.add_argument("--autotune", action="store_true",
+                   help="pick delta / BU group / alpha-beta with the autotuner before timing")
     return p.parse_args()
 
 
@@ -128,6 +132,8 @@ class Workload:
         if "relabel" in self.kernels:
             add("relabel", st.device_ms, self.g.m, bytes_relabel(n, self.g.m))
         self.rgw, stw = ctx.relabel(self.gw, self.ids)
+        if "relabel" in self.kernels:
+            add("relabel", stw.device_ms, self.gw.m, bytes_relabel(n, self.gw.m, True))
         g, gw = self.rg, self.rgw
         m = g.m
         if not hasattr(self, "srcs"):
@@ -155,6 +161,44 @@ class Workload:
         if "bc" in self.kernels:
             _, st = ctx.bc(g, self.srcs[:a.bc_sources], download=False)
             add("bc", st.device_ms, 2 * st.edges_reached, bytes_bc(n, st.edges_reached) , a.bc_sources)
+
+
+# Kernels followed one launch at a time (HIP events recorded by the library
+# on its stream around each launch) for the roofline of the dominant family.
+def kt_table(gbk):
+    return {
+        "sssp": (gbk.KT_SSSP_RELAX, "sssp_relax+heavy_prefix+sssp_relax_heavy (one relaxation round)"),
+        "pr": (gbk.KT_PR_ITER, "pr_pull_iter<GS> (one pull iteration)"),
+        "bfs": (gbk.KT_BFS_STEP, "bfs_td_step+heavy_prefix+bfs_td_heavy | bfs_bu_step (one BFS step)"),
+        "bc": (gbk.KT_BC_LEVEL, "bc_forward_level+heavy_prefix+bc_forward_heavy | bc_backward_level"),
+    }
+
+
+def launch_bytes(family, kt, work, rec):
+    """Algorithmic bytes of one launch of the followed kernel.
+
+    PR: one iteration reads 4 B per in-edge, the offsets (8 B per row), the
+    contributions and scores (16 B per row): 4m + 24n.  SSSP: a relaxation
+    round reads 8 B per scanned edge (neighbour + weight) and 16 B per
+    non-stale frontier row (queue entry, offsets, distance); rows and edges
+    are counted on the device outside the timed launches.  BFS and BC: the
+    per-source figure spread evenly over the source's launches."""
+    n, m = work.g.n, work.g.m
+    if family == "pr":
+        return float(bytes_pr_iter(n, m)), "exact"
+    if family == "sssp":
+        return (8.0 * kt["edges"] + 16.0 * kt["rows"]) / max(kt["launches"], 1), "counted"
+    return rec[family][2] / max(kt["launches"], 1), "per-source figure / launches"
+
+
+def load_traffic(workload, family):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    e = t.get(f"{workload}|{family}")
+    return None if e is None else e.get("bytes_per_launch")
 
 
 def cpu_baseline(work, budget_s):
@@ -209,13 +253,49 @@ def main():
     import torch
 
     ctx = gbk.Context(0)
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        a = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy probe: {nb} bytes read, {nb} bytes written")
+        del a, b
     work = Workload(args, ctx, gbk, H)
     stream = torch.cuda.ExternalStream(ctx.stream)
+    wrec = {}
     for w in range(args.warmup):
-        work.step({})
+        work.step(wrec)
         log(f"warmup step {w} done")
     ctx.sync()
     torch.cuda.synchronize()
+    tuned = None
+    if args.autotune:
+        from graphbrew_amd.autotune import Autotuner
+        if work.rg is None:
+            work.step({})
+        tuner = Autotuner(ctx)
+        t0 = time.time()
+        srcs = work.srcs[:4]
+        best_s = tuner.tune_sssp(work.rgw, srcs) if "sssp" in work.kernels else None
+        best_b = tuner.tune_bfs(work.rg, srcs) if "bfs" in work.kernels else None
+        if best_s:
+            args.delta = best_s["delta"]
+        if best_b:
+            tuner.apply(best_b)
+        tuned = {"sssp": best_s, "bfs": best_b, "seconds": round(time.time() - t0, 1),
+                 "sources": len(srcs)}
+        log(f"autotune: {tuned}")
+        for row in tuner.log:
+            log(f"  {row}")
+    kts = kt_table(gbk)
+    if wrec:
+        family = max((f for f in wrec if f in kts), key=lambda f: wrec[f][0], default=None)
+    else:
+        family = next((f for f in ("sssp", "bc", "bfs", "pr") if f in work.kernels), None)
+    if family is not None:
+        ctx.ktime(kts[family][0])
+        ctx.ktime_read()
     rec = {}
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -229,6 +309,8 @@ def main():
     torch.cuda.synchronize()
     wall = time.time() - t0
     total_ms = ev0.elapsed_time(ev1)
+    kt = ctx.ktime_read() if family is not None else None
+    ctx.ktime(gbk.KT_NONE)
     n, m = work.g.n, work.g.m
     kern = {}
     total_edges = 0
@@ -239,24 +321,36 @@ def main():
                       "gbps": nbytes / secs / 1e9 if secs else 0.0,
                       "hbm_frac": nbytes / secs / HBM_PEAK if secs else 0.0,
                       "calls_per_step": calls / args.steps}
-    dom = max(rec.items(), key=lambda kv: kv[1][0])
-    dname, (dms, dedges, dbytes, dcalls) = dom
-    achieved = dbytes / (dms / 1e3) / 1e9
     value = total_edges / (total_ms / 1e3) / 1e9
+    workload = (f"{'uniform' if args.uniform else 'kronecker'}-scale{args.scale}-ef{args.degree}"
+                f" relabel+pr+bfs{args.bfs_sources}+cc+cc_sv+sssp{args.sssp_sources}+bc{args.bc_sources}")
+    roof = None
+    if family is not None and kt and kt["launches"]:
+        per_launch, how = launch_bytes(family, kt, work, rec)
+        avg_s = kt["ms"] / kt["launches"] / 1e3
+        achieved = per_launch / avg_s / 1e9
+        dom = max(rec, key=lambda f: rec[f][0])
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(achieved * 1e9 / HBM_PEAK, 4),
+                "traffic": load_traffic(workload, family),
+                "kernel": kts[family][1], "family": family, "dominant_family": dom,
+                "launches": kt["launches"], "avg_launch_us": round(kt["ms"] / kt["launches"] * 1e3, 2),
+                "bytes_per_launch": round(per_launch), "bytes_basis": how,
+                "launch_share_of_step": round(kt["ms"] / total_ms, 4)}
+        if family == "sssp":
+            roof["rows_scanned"] = kt["rows"]
+            roof["edges_scanned"] = kt["edges"]
     out = {
         "metric": "GTEPS (all five kernels + GPU relabel, aggregate) with %HBM roofline per kernel",
         "value": round(value, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(total_ms / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic (seeded Kronecker generator of the reference, weights U[1,255])",
-        "config": {"workload": f"{'uniform' if args.uniform else 'kronecker'}-scale{args.scale}-ef{args.degree}"
-                   f" relabel+pr+bfs{args.bfs_sources}+cc+cc_sv+sssp{args.sssp_sources}+bc{args.bc_sources}",
+        "config": {"workload": workload,
                    "n": n, "m_directed": m, "order": args.order, "mapping_fingerprint": work.fingerprint,
-                   "delta": args.delta},
+                   "delta": args.delta, "autotuned": tuned},
         "kernels": kern,
-        "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
-                     "traffic": None},
+        "roofline": roof,
         "wall_s": round(wall, 2),
     }
     if not args.no_cpu_baseline:

@@ -275,11 +275,13 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
       d++;
       const int64_t lb = level[level.size() - 2], le = level.back();
       fill<unsigned long long>(ctx, hq.count, 0ull, 1);
+      const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
       bc_forward_level<<<expand_grid(le - lb), kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ,
                                                               queue, lb, le, cap, d, H, c, hq);
       heavy_prefix<<<1, 1024, 0, s>>>(hq);
       bc_forward_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ, queue, cap,
                                                      d, c, hq);
+      kt_end(ctx, kt);
       GBK_LAUNCH_CHECK(ctx, "bc_forward_level");
       GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcCounters), hipMemcpyDeviceToHost, s));
       GBK_HIP(ctx, hipStreamSynchronize(s));
@@ -291,8 +293,10 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
     max_depth = std::max<int>(max_depth, static_cast<int>(level.size()) - 2);
     for (int lv = static_cast<int>(level.size()) - 2; lv >= 0; lv--) {
       const int64_t lb = level[lv], cnt = level[lv + 1] - lb;
+      const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
       bc_backward_level<<<expand_grid(cnt), kBlock, 0, s>>>(g->off, g->nbr, queue + lb, cnt, sigma,
                                                            succ, delta, scores, H);
+      kt_end(ctx, kt);
     }
     bc_reach_edges<<<ng, kBlock, 0, s>>>(g->off, depth, n, partial);
     sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial, static_cast<int>(ng), &c->reach, 1);

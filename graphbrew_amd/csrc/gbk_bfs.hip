@@ -260,7 +260,9 @@ int gbk_bfs_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha, 
       do {
         old_awake_count = awake_count;
         fill<uint32_t>(ctx, curr, 0u, words);
+        const int kt = kt_begin(ctx, GBK_KT_BFS_STEP);
         launch_bu(ctx, G, g, parent, front, curr, partial, bu_grid);
+        kt_end(ctx, kt);
         bfs_store_count<<<1, 1024, 0, s>>>(partial, static_cast<int>(bu_grid), &c->awake);
         GBK_LAUNCH_CHECK(ctx, "bfs_bu_step");
         if (int rc = read_counters()) return rc;
@@ -277,10 +279,12 @@ int gbk_bfs_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha, 
       edges_to_check -= scout_count;
       const unsigned grid = expand_grid(qe - qb);
       fill<unsigned long long>(ctx, hq.count, 0ull, 1);
+      const int kt = kt_begin(ctx, GBK_KT_BFS_STEP);
       bfs_td_step<<<grid, kBlock, 0, s>>>(g->off, g->nbr, parent, queue, qb, qe, cap,
                                           ctx->tune.heavy_degree, c, partial, hq);
       heavy_prefix<<<1, 1024, 0, s>>>(hq);
       bfs_td_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, parent, queue, cap, c, partial + kGridCap, hq);
+      kt_end(ctx, kt);
       bfs_store_count<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), &c->scout,
                                          partial + kGridCap, static_cast<int>(kHeavyGrid));
       GBK_LAUNCH_CHECK(ctx, "bfs_td_step");

@@ -34,7 +34,9 @@ struct Tuning {
   int bfs_alpha = 15;           // bfs.cc:165 defaults
   int bfs_beta = 18;
   int xcd_swizzle = 1;          // contiguous vertex ranges per XCD
+  int relax_grid = 1024;        // workgroups of a frontier (queue) launch
 };
+constexpr int kGridCapRuntime = 2048;  // upper bound of relax_grid
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -167,6 +169,17 @@ struct gbk_ctx {
   Slot slot[16];
   void *pinned = nullptr;      // small pinned staging for counters
   size_t pinned_bytes = 0;
+  // Per-launch timing of one selected kernel (GBK_KT_*): a start/stop event
+  // pair is recorded on the stream around each of its launches and read back
+  // after the entry point's closing synchronisation.  kt_dev holds device
+  // counters [rows, edges] plus per-workgroup partials for kernels whose work
+  // per launch is data dependent.
+  int ktime = 0;
+  std::vector<hipEvent_t> kt_ev;
+  size_t kt_used = 0;
+  double kt_ms = 0;
+  int64_t kt_launches = 0;
+  unsigned long long *kt_dev = nullptr;
 };
 
 namespace gbk {
@@ -206,8 +219,16 @@ int build_heavy_list(gbk_ctx *ctx, const int64_t *off, int64_t n, int64_t thresh
 struct EventTimer {
   gbk_ctx *ctx;
   explicit EventTimer(gbk_ctx *c);
-  double stop_ms();  // synchronises
+  double stop_ms();  // synchronises, then collects kernel timer pairs
 };
+
+// Kernel timer: kt_begin returns a pair index when 'id' is the selected
+// kernel (else -1); kt_end closes it; kt_collect reads finished pairs after
+// the stream has been synchronised.
+int kt_begin(gbk_ctx *ctx, int id);
+void kt_end(gbk_ctx *ctx, int pair);
+void kt_collect(gbk_ctx *ctx);
+constexpr int kKtPartials = 2048;  // partial slots per counter in kt_dev
 
 }  // namespace gbk
 

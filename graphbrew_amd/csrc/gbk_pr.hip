@@ -173,6 +173,7 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   EventTimer timer(ctx);
   pr_init<<<grid_for(n), kBlock, 0, s>>>(g->off, n, init, scores, c0, st);
   for (int it = 0; it < max_iters; it++) {
+    const int kt = kt_begin(ctx, GBK_KT_PR_ITER);
     if (variant == GBK_PR_GS) {
       pr_pull_iter<true><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->off, n, g->heavy_in,
                                                  g->n_heavy_in, ctx->tune.heavy_degree, scores, c0,
@@ -184,6 +185,7 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
                                                   g->n_heavy_in, ctx->tune.heavy_degree, scores, cur,
                                                   nxt, st, partial, base, kDamp);
     }
+    kt_end(ctx, kt);
     pr_finish<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), st, errs, it, tol, fixed_work);
   }
   GBK_LAUNCH_CHECK(ctx, "pr_pull_iter");

@@ -55,7 +55,40 @@ double EventTimer::stop_ms() {
   hipEventSynchronize(ctx->ev1);
   float ms = 0;
   hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  kt_collect(ctx);
   return ms;
+}
+
+int kt_begin(gbk_ctx *ctx, int id) {
+  if (ctx->ktime == 0 || ctx->ktime != id) return -1;
+  if (ctx->kt_used * 2 + 2 > ctx->kt_ev.size()) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess) return -1;
+    if (hipEventCreate(&b) != hipSuccess) {
+      hipEventDestroy(a);
+      return -1;
+    }
+    ctx->kt_ev.push_back(a);
+    ctx->kt_ev.push_back(b);
+  }
+  const int pair = static_cast<int>(ctx->kt_used++);
+  hipEventRecord(ctx->kt_ev[2 * pair], ctx->stream);
+  return pair;
+}
+
+void kt_end(gbk_ctx *ctx, int pair) {
+  if (pair >= 0) hipEventRecord(ctx->kt_ev[2 * pair + 1], ctx->stream);
+}
+
+void kt_collect(gbk_ctx *ctx) {
+  for (size_t i = 0; i < ctx->kt_used; i++) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->kt_ev[2 * i], ctx->kt_ev[2 * i + 1]) == hipSuccess) {
+      ctx->kt_ms += ms;
+      ctx->kt_launches += 1;
+    }
+  }
+  ctx->kt_used = 0;
 }
 
 // ------------------------------------------------------------------- scan
@@ -254,6 +287,8 @@ int gbk_ctx_destroy(gbk_ctx *ctx) {
   for (auto &s : ctx->slot)
     if (s.p) hipFree(s.p);
   if (ctx->pinned) hipHostFree(ctx->pinned);
+  if (ctx->kt_dev) hipFree(ctx->kt_dev);
+  for (hipEvent_t e : ctx->kt_ev) hipEventDestroy(e);
   hipEventDestroy(ctx->ev0);
   hipEventDestroy(ctx->ev1);
   hipStreamDestroy(ctx->stream);
@@ -282,7 +317,19 @@ int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value) {
   else if (k == "bfs_alpha" && value > 0) t.bfs_alpha = static_cast<int>(value);
   else if (k == "bfs_beta" && value > 0) t.bfs_beta = static_cast<int>(value);
   else if (k == "xcd_swizzle" && (value == 0 || value == 1)) t.xcd_swizzle = static_cast<int>(value);
-  else return set_error(ctx, GBK_EINVAL, "unknown parameter or bad value: " + k);
+  else if (k == "relax_grid" && value >= 64 && value <= kGridCapRuntime) t.relax_grid = static_cast<int>(value);
+  else if (k == "ktime" && value >= GBK_KT_NONE && value <= GBK_KT_LAST) {
+    if (value != GBK_KT_NONE && !ctx->kt_dev) {
+      const size_t bytes = sizeof(unsigned long long) * (2 + 2 * kKtPartials);
+      hipSetDevice(ctx->device);
+      if (hipMalloc(&ctx->kt_dev, bytes) != hipSuccess) {
+        ctx->kt_dev = nullptr;
+        return set_error(ctx, GBK_ENOMEM, "kernel timer counters");
+      }
+      if (hipMemset(ctx->kt_dev, 0, bytes) != hipSuccess) return set_error(ctx, GBK_EHIP, "kernel timer counters");
+    }
+    ctx->ktime = static_cast<int>(value);
+  } else return set_error(ctx, GBK_EINVAL, "unknown parameter or bad value: " + k);
   return GBK_OK;
 }
 
@@ -295,7 +342,28 @@ int gbk_ctx_get_param(gbk_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "bfs_alpha") *value = t.bfs_alpha;
   else if (k == "bfs_beta") *value = t.bfs_beta;
   else if (k == "xcd_swizzle") *value = t.xcd_swizzle;
+  else if (k == "relax_grid") *value = t.relax_grid;
+  else if (k == "ktime") *value = ctx->ktime;
   else return set_error(ctx, GBK_EINVAL, "unknown parameter: " + k);
+  return GBK_OK;
+}
+
+int gbk_ktime_read(gbk_ctx *ctx, double *total_ms, int64_t *launches, int64_t *rows, int64_t *edges) {
+  if (!ctx) return GBK_EINVAL;
+  hipSetDevice(ctx->device);
+  GBK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  kt_collect(ctx);
+  unsigned long long u[2] = {0, 0};
+  if (ctx->kt_dev) {
+    GBK_HIP(ctx, hipMemcpy(u, ctx->kt_dev, sizeof(u), hipMemcpyDeviceToHost));
+    GBK_HIP(ctx, hipMemset(ctx->kt_dev, 0, sizeof(u)));
+  }
+  if (total_ms) *total_ms = ctx->kt_ms;
+  if (launches) *launches = ctx->kt_launches;
+  if (rows) *rows = static_cast<int64_t>(u[0]);
+  if (edges) *edges = static_cast<int64_t>(u[1]);
+  ctx->kt_ms = 0;
+  ctx->kt_launches = 0;
   return GBK_OK;
 }
 

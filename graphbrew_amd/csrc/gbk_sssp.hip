@@ -134,6 +134,32 @@ sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t 
   f.nearq.flush();
 }
 
+// Work of the coming relaxation round, for the kernel timer only: rows of
+// the near queue that are not stale and their edges, as per-workgroup
+// partials (rows in partial[0, grid), edges in partial[grid, 2 grid)).
+__global__ void sssp_count_round(const int64_t *off, const int32_t *dist, const int32_t *near0,
+                                 const int32_t *near1, int64_t delta, const SsspState *st,
+                                 unsigned long long *partial) {
+  const volatile SsspState *vs = st;
+  unsigned long long rows = 0, edges = 0;
+  if (!vs->done && vs->phase == 0) {
+    const int cur = vs->cur;
+    const int32_t *q = cur ? near1 : near0;
+    const int64_t count = static_cast<int64_t>(vs->near_cnt[cur]);
+    const int64_t lower = delta * vs->bucket;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+      const int32_t u = q[i];
+      if (dist[u] >= lower) {
+        rows++;
+        edges += static_cast<unsigned long long>(off[u + 1] - off[u]);
+      }
+    }
+  }
+  block_partial(rows, partial);
+  block_partial(edges, partial + gridDim.x);
+}
+
 // One workgroup: closes a relaxation round (swap the near queues; when the
 // next one is empty the bucket is done and the search for the next opens).
 __global__ void sssp_relax_done(SsspState *st, unsigned long long *heavy_count) {
@@ -262,7 +288,9 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   hq.overflow = &st->overflow;
   int32_t *near0 = q, *near1 = q + cap;
   const int64_t H = ctx->tune.heavy_degree;
-  const unsigned relax_grid = 1024;
+  const unsigned relax_grid = static_cast<unsigned>(ctx->tune.relax_grid);
+  const bool count = ctx->ktime == GBK_KT_SSSP_RELAX && ctx->kt_dev;
+  const unsigned count_grid = static_cast<unsigned>(kKtPartials < 256 ? kKtPartials : 256);
   const unsigned scan_grid = expand_grid(n);
   EventTimer timer(ctx);
   sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, dist, stamp, near0, st);
@@ -270,11 +298,20 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   int64_t launched = 0;
   for (;;) {
     for (int j = 0; j < batch; j++) {
+      if (count) {
+        unsigned long long *part = ctx->kt_dev + 2;
+        sssp_count_round<<<count_grid, kBlock, 0, s>>>(g->off, dist, near0, near1, delta, st, part);
+        sum_partials<unsigned long long><<<1, 1024, 0, s>>>(part, static_cast<int>(count_grid), ctx->kt_dev, 1);
+        sum_partials<unsigned long long><<<1, 1024, 0, s>>>(part + count_grid, static_cast<int>(count_grid),
+                                                          ctx->kt_dev + 1, 1);
+      }
+      const int kt = kt_begin(ctx, GBK_KT_SSSP_RELAX);
       sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->nbr, g->wgt, dist, stamp, near0, near1, cap,
                                                delta, H, st, hq);
       heavy_prefix<<<1, 1024, 0, s>>>(hq);
       sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, g->wgt, dist, stamp, near0, near1, cap,
                                                      delta, st, hq);
+      kt_end(ctx, kt);
       sssp_relax_done<<<1, kWave, 0, s>>>(st, hq.count);
       sssp_min_bucket<<<scan_grid, kBlock, 0, s>>>(dist, n, delta, st, partial);
       sssp_min_done<<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), st);
@@ -285,6 +322,7 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
     GBK_LAUNCH_CHECK(ctx, "sssp step");
     GBK_HIP(ctx, hipMemcpyAsync(hs, st, sizeof(SsspState), hipMemcpyDeviceToHost, s));
     GBK_HIP(ctx, hipStreamSynchronize(s));
+    kt_collect(ctx);
     if (hs->overflow) return set_error(ctx, GBK_EHIP, "sssp: queue overflow");
     if (hs->done) break;
     if (batch < 64) batch *= 2;

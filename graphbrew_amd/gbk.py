@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_PKG, "libgbk.so")
 GBK_OK, GBK_EINVAL, GBK_EPERM_MAP, GBK_EHIP, GBK_ENOMEM, GBK_ESTATE = 0, -1, -2, -3, -4, -5
 PR_GS, PR_SPMV = 0, 1
 CC_AFFOREST, CC_SV = 0, 1
+KT_NONE, KT_PR_ITER, KT_SSSP_RELAX, KT_BFS_STEP, KT_BC_LEVEL = 0, 1, 2, 3, 4
 DIST_INF = (2 ** 31 - 1) // 2
 
 
@@ -84,6 +85,7 @@ def lib():
         "gbk_ctx_sync": (i32, [VP]),
         "gbk_ctx_set_param": (i32, [VP, C.c_char_p, i64]),
         "gbk_ctx_get_param": (i32, [VP, C.c_char_p, C.POINTER(i64)]),
+        "gbk_ktime_read": (i32, [VP, C.POINTER(f64), C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
         "gbk_graph_upload": (i32, [VP, i64, i32, VP, VP, VP, VP, VP, VP, VP, pp]),
         "gbk_graph_build": (i32, [VP, i64, i64, VP, VP, VP, i32, i32, pp, C.POINTER(f64)]),
         "gbk_graph_info": (i32, [VP, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32), C.POINTER(i32)]),
@@ -205,6 +207,17 @@ class Context:
         v = C.c_int64(0)
         self._check(lib().gbk_ctx_get_param(self.h, name.encode(), C.byref(v)))
         return v.value
+
+    def ktime(self, kernel: int):
+        """Selects the kernel (KT_*) whose launches are timed one by one."""
+        self.set_param("ktime", kernel)
+
+    def ktime_read(self) -> dict:
+        """Reads and resets the per-launch timer."""
+        ms = C.c_double(0)
+        n, rows, edges = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        self._check(lib().gbk_ktime_read(self.h, C.byref(ms), C.byref(n), C.byref(rows), C.byref(edges)))
+        return {"ms": ms.value, "launches": n.value, "rows": rows.value, "edges": edges.value}
 
     # ------------------------------------------------------------- graphs
     def upload(self, n, off, nbr, wgt=None, in_off=None, in_nbr=None, in_wgt=None,

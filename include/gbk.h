@@ -46,6 +46,15 @@ typedef struct gbk_graph gbk_graph;
 
 enum gbk_pr_variant { GBK_PR_GS = 0, GBK_PR_SPMV = 1 };
 enum gbk_cc_variant { GBK_CC_AFFOREST = 0, GBK_CC_SV = 1 };
+/* Kernels the per-launch timer can follow (parameter "ktime"). */
+enum gbk_ktime_kernel {
+  GBK_KT_NONE = 0,
+  GBK_KT_PR_ITER = 1,     /* one pull iteration of PR (GS or SpMV) */
+  GBK_KT_SSSP_RELAX = 2,  /* one relaxation round: light rows + heavy rows */
+  GBK_KT_BFS_STEP = 3,    /* one top-down or bottom-up BFS step */
+  GBK_KT_BC_LEVEL = 4,    /* one forward or backward Brandes level */
+  GBK_KT_LAST = 4
+};
 
 typedef struct {
   double device_ms;        /* device time of the whole call's kernels */
@@ -94,9 +103,19 @@ const char *gbk_last_error(const gbk_ctx *ctx);
 void *gbk_ctx_stream(gbk_ctx *ctx);
 int gbk_ctx_sync(gbk_ctx *ctx);
 /* Tunables: "heavy_degree", "bu_group", "bfs_alpha", "bfs_beta",
- * "xcd_swizzle".  Returns GBK_EINVAL for an unknown name or bad value. */
+ * "xcd_swizzle", "relax_grid"; and "ktime" (a gbk_ktime_kernel) selecting
+ * the kernel whose launches are timed one by one with HIP events.  Returns
+ * GBK_EINVAL for an unknown name or bad value.  No reference counterpart:
+ * the reference's kernels have no launch shape to tune. */
 int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value);
 int gbk_ctx_get_param(gbk_ctx *ctx, const char *name, int64_t *value);
+/* Reads and resets the per-launch timer: summed launch time, launch count,
+ * and, for GBK_KT_SSSP_RELAX only, the non-stale frontier rows and their
+ * edges those launches scanned (counted by separate kernels outside the
+ * timed event pairs; 0 otherwise).  No reference counterpart (the
+ * reference's Timer, timer.h, times whole trials). */
+int gbk_ktime_read(gbk_ctx *ctx, double *total_ms, int64_t *launches,
+                   int64_t *rows, int64_t *edges);
 
 /* ---- graphs ------------------------------------------------------------ */
 /* Copies a host CSR to the device.  n vertices; off has n+1 entries with
