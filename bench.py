@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="first copy MIB MiB device to device (calibrates HBM byte counters)")
+    p.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
+                   help="library tunable to set before the run (repeatable)")
     p.add_argument("--autotune", action="store_true",
                    help="pick delta / BU group / alpha-beta with the autotuner before timing")
     return p.parse_args()
@@ -253,6 +255,9 @@ def main():
     import torch
 
     ctx = gbk.Context(0)
+    for kv in args.param:
+        name, value = kv.split("=", 1)
+        ctx.set_param(name, int(value))
     if args.copy_probe:
         nb = args.copy_probe << 20
         a = torch.empty(nb, dtype=torch.uint8, device="cuda")
@@ -281,6 +286,7 @@ def main():
         best_b = tuner.tune_bfs(work.rg, srcs) if "bfs" in work.kernels else None
         if best_s:
             args.delta = best_s["delta"]
+            tuner.apply(best_s)
         if best_b:
             tuner.apply(best_b)
         tuned = {"sssp": best_s, "bfs": best_b, "seconds": round(time.time() - t0, 1),
@@ -348,7 +354,9 @@ def main():
         "data": "synthetic (seeded Kronecker generator of the reference, weights U[1,255])",
         "config": {"workload": workload,
                    "n": n, "m_directed": m, "order": args.order, "mapping_fingerprint": work.fingerprint,
-                   "delta": args.delta, "autotuned": tuned},
+                   "delta": args.delta, "autotuned": tuned,
+                   "params": {k: ctx.get_param(k) for k in ("heavy_degree", "bu_group", "bfs_alpha", "bfs_beta",
+                                                            "relax_grid")}},
         "kernels": kern,
         "roofline": roof,
         "wall_s": round(wall, 2),

@@ -6,6 +6,7 @@ keeps the fastest candidate whose output equals the default's (integer
 results: bit-exact; PR: 1e-4 relative).  Swept, per kernel:
 
   sssp   delta in {1, 8, 32, 64, 128, 256, 512}           (CLDelta -d)
+         then relax_grid in {256, 512, 1024, 2048} workgroups
   bfs    bu_group in {1, 2, 4, 8, 16} x (alpha, beta) grid (bfs.cc:165)
   all    heavy_degree in {1024, 2048, 4096, 8192}          (rows per workgroup)
 
@@ -28,6 +29,7 @@ DELTAS = (1, 8, 32, 64, 128, 256, 512)
 BU_GROUPS = (1, 2, 4, 8, 16)
 ALPHA_BETA = ((15, 18), (8, 18), (30, 18), (15, 9), (15, 36))
 HEAVY = (1024, 2048, 4096, 8192)
+GRIDS = (256, 512, 1024, 2048)
 
 
 def _median(xs: Sequence[float]) -> float:
@@ -75,6 +77,24 @@ class Autotuner:
             self.log.append({"kernel": "sssp", "delta": d, "ms": ms, "wall": time.time() - t0})
             if best is None or ms < best["ms"]:
                 best = {"delta": d, "ms": ms}
+        # then the launch width of the relaxation kernel at the chosen delta
+        saved = self.ctx.get_param("relax_grid")
+        best["relax_grid"] = saved
+        for grid in GRIDS:
+            if grid == saved:
+                continue
+            self.ctx.set_param("relax_grid", grid)
+            ms = 0.0
+            for s, r in zip(srcs, ref):
+                out, _ = self.ctx.sssp(g, s, best["delta"])
+                if not np.array_equal(out, r):
+                    raise AssertionError(f"sssp relax_grid={grid} changed distances from source {s}")
+                ms += _median([self.ctx.sssp(g, s, best["delta"], download=False)[1].device_ms
+                               for _ in range(self.repeats)])
+            self.log.append({"kernel": "sssp", "delta": best["delta"], "relax_grid": grid, "ms": ms})
+            if ms < best["ms"]:
+                best = {"delta": best["delta"], "relax_grid": grid, "ms": ms}
+        self.ctx.set_param("relax_grid", saved)
         self.cache[k] = best
         self._save()
         return best
@@ -128,7 +148,7 @@ class Autotuner:
         return best
 
     def apply(self, tuned: dict) -> None:
-        for name in ("bu_group", "heavy_degree"):
+        for name in ("bu_group", "heavy_degree", "relax_grid"):
             if name in tuned:
                 self.ctx.set_param(name, tuned[name])
         if "alpha" in tuned:

@@ -30,7 +30,7 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 // ----------------------------------------------------------------- tunables
 struct Tuning {
   int64_t heavy_degree = 2048;  // rows above this get a whole workgroup
-  int bu_group = 8;             // lanes per vertex in the bottom-up BFS step
+  int bu_group = 1;             // lanes per vertex in the bottom-up BFS step
   int bfs_alpha = 15;           // bfs.cc:165 defaults
   int bfs_beta = 18;
   int xcd_swizzle = 1;          // contiguous vertex ranges per XCD

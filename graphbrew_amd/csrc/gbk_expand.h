@@ -53,6 +53,39 @@ __device__ __forceinline__ void wave_walk(int64_t row, int64_t start, int64_t le
   }
 }
 
+// Batched walk: each lane takes U edges per trip (U consecutive 64-edge
+// chunks of the union), so the functor's batch<U>() can issue U independent
+// loads and atomics before it waits on any.  Inactive lanes get edge 0 and
+// their row's id, both valid indices whenever the walk runs.
+template <int U, class F>
+__device__ __forceinline__ void wave_walk_batch(int64_t row, int64_t start, int64_t len, F &f) {
+  const int lane = lane_id();
+  const int64_t incl = wave_incl_scan(len);
+  const int64_t excl = incl - len;
+  const int64_t total = __shfl(incl, kWave - 1);
+  for (int64_t c = 0; c < total; c += U * kWave) {
+    int64_t rows[U], edges[U];
+    bool act[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int64_t e = c + j * kWave + lane;
+      int lo = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {
+        const int probe = lo + step;
+        const int64_t x = __shfl(excl, probe & (kWave - 1));
+        if (probe < kWave && x <= e) lo = probe;
+      }
+      act[j] = e < total;
+      rows[j] = __shfl(row, lo);
+      const int64_t ostart = __shfl(start, lo);
+      const int64_t oexcl = __shfl(excl, lo);
+      edges[j] = act[j] ? ostart + (e - oexcl) : 0;
+    }
+    f.template batch<U>(rows, edges, act);
+  }
+}
+
 // Segmented inclusive sum across a 64-edge chunk whose owners ('seg', one
 // per lane, non-decreasing across lanes) form contiguous runs.  Returns the
 // run total in the last lane of each run (other lanes hold partial sums);
@@ -129,6 +162,7 @@ struct HeavyQueue {
   unsigned long long *count;  // reset to 0 before the light kernel
   int64_t cap;
   int *overflow;
+  int64_t chunk;              // edges of a row per chunk
 };
 
 constexpr int64_t kHeavyChunk = 1024;
@@ -150,6 +184,7 @@ inline bool make_heavy_queue(gbk_ctx *ctx, const gbk_graph *g, int slot, HeavyQu
   hq->prefix = hq->len + cap;
   hq->row = reinterpret_cast<int32_t *>(hq->prefix + cap + 1);
   hq->cap = cap;
+  hq->chunk = kHeavyChunk;
   fill<unsigned long long>(ctx, hq->count, 0ull, 1);
   fill<int>(ctx, hq->overflow, 0, 1);
   return true;
@@ -181,32 +216,61 @@ __device__ void expand_list_light(const int32_t *list, int64_t count, int64_t he
   }
 }
 
+// expand_list_light with the batched walk.
+template <int U, class RowLen, class F>
+__device__ void expand_list_light_batch(const int32_t *list, int64_t count, int64_t heavy_deg,
+                                        RowLen rowlen, F &f, const HeavyQueue &hq) {
+  for (int64_t tile = blockIdx.x; tile * kBlock < count; tile += gridDim.x) {
+    const int64_t i = tile * kBlock + threadIdx.x;
+    int64_t row = 0, start = 0, len = 0;
+    if (i < count) {
+      row = list[i];
+      rowlen(row, start, len);
+      if (len > heavy_deg) {
+        const unsigned long long at = atomicAdd(hq.count, 1ull);
+        if (static_cast<int64_t>(at) < hq.cap) {
+          hq.row[at] = static_cast<int32_t>(row);
+          hq.start[at] = start;
+          hq.len[at] = len;
+        } else {
+          *hq.overflow = 1;
+        }
+        len = 0;
+      }
+    }
+    wave_walk_batch<U>(row, start, len, f);
+  }
+}
+
 // One workgroup (1024 lanes): prefix[h] = chunks of rows [0, h); prefix[rows]
 // = total.  Run between the light and the heavy kernel.
 static __global__ void heavy_prefix(HeavyQueue hq) {
-  __shared__ int64_t part[1024];
+  __shared__ int64_t wsum[1024 / kWave];
   const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  if (rows == 0) return;  // expand_heavy reads nothing else then
+  const int64_t chunk = hq.chunk;
   const int t = threadIdx.x;
+  const int nw = static_cast<int>(blockDim.x) / kWave;
   const int64_t per = (rows + blockDim.x - 1) / blockDim.x;
   const int64_t b = t * per, e = min(rows, b + per);
   int64_t s = 0;
-  for (int64_t h = b; h < e; h++) s += (hq.len[h] + kHeavyChunk - 1) / kHeavyChunk;
-  part[t] = s;
+  for (int64_t h = b; h < e; h++) s += (hq.len[h] + chunk - 1) / chunk;
+  // block-wide exclusive scan of the per-thread chunk counts
+  const int64_t incl = wave_incl_scan(s);
+  const int w = t / kWave;
+  if (lane_id() == kWave - 1) wsum[w] = incl;
   __syncthreads();
-  if (t == 0) {
-    int64_t run = 0;
-    for (int i = 0; i < static_cast<int>(blockDim.x); i++) {
-      const int64_t x = part[i];
-      part[i] = run;
-      run += x;
-    }
-    hq.prefix[rows] = run;
+  if (w == 0) {
+    int64_t x = lane_id() < nw ? wsum[lane_id()] : 0;
+    x = wave_incl_scan(x);
+    if (lane_id() < nw) wsum[lane_id()] = x;
   }
   __syncthreads();
-  int64_t run = part[t];
+  int64_t run = incl - s + (w > 0 ? wsum[w - 1] : 0);
+  if (t == static_cast<int>(blockDim.x) - 1) hq.prefix[rows] = run + s;
   for (int64_t h = b; h < e; h++) {
     hq.prefix[h] = run;
-    run += (hq.len[h] + kHeavyChunk - 1) / kHeavyChunk;
+    run += (hq.len[h] + chunk - 1) / chunk;
   }
 }
 
@@ -226,12 +290,44 @@ __device__ void expand_heavy(const HeavyQueue &hq, F &f) {
       else hi = mid;
     }
     const int64_t len = hq.len[lo], start = hq.start[lo], row = hq.row[lo];
-    const int64_t first = (gc - hq.prefix[lo]) * kHeavyChunk;
-    const int64_t last = min(len, first + kHeavyChunk);
+    const int64_t first = (gc - hq.prefix[lo]) * hq.chunk;
+    const int64_t last = min(len, first + hq.chunk);
     for (int64_t base = first; base < last; base += blockDim.x) {
       const int64_t e = base + threadIdx.x;
       const bool active = e < last;
       f(row, active ? start + e : 0, active);
+    }
+  }
+}
+
+// expand_heavy with U edges per lane per trip (rows of the batch are the
+// same row; edges past the chunk end are inactive and read edge 0).
+template <int U, class F>
+__device__ void expand_heavy_batch(const HeavyQueue &hq, F &f) {
+  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  if (rows == 0) return;
+  const int64_t total = hq.prefix[rows];
+  for (int64_t gc = blockIdx.x; gc < total; gc += gridDim.x) {
+    int64_t lo = 0, hi = rows;  // last row with prefix[row] <= gc
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (hq.prefix[mid] <= gc) lo = mid;
+      else hi = mid;
+    }
+    const int64_t len = hq.len[lo], start = hq.start[lo], row = hq.row[lo];
+    const int64_t first = (gc - hq.prefix[lo]) * hq.chunk;
+    const int64_t last = min(len, first + hq.chunk);
+    for (int64_t base = first; base < last; base += U * static_cast<int64_t>(blockDim.x)) {
+      int64_t rws[U], edges[U];
+      bool act[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int64_t e = base + j * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        act[j] = e < last;
+        rws[j] = row;
+        edges[j] = act[j] ? start + e : 0;
+      }
+      f.template batch<U>(rws, edges, act);
     }
   }
 }

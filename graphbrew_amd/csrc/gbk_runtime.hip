@@ -317,7 +317,8 @@ int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value) {
   else if (k == "bfs_alpha" && value > 0) t.bfs_alpha = static_cast<int>(value);
   else if (k == "bfs_beta" && value > 0) t.bfs_beta = static_cast<int>(value);
   else if (k == "xcd_swizzle" && (value == 0 || value == 1)) t.xcd_swizzle = static_cast<int>(value);
-  else if (k == "relax_grid" && value >= 64 && value <= kGridCapRuntime) t.relax_grid = static_cast<int>(value);
+  else if (k == "relax_grid" && value >= 64 && value <= kGridCapRuntime && value % 8 == 0)
+    t.relax_grid = static_cast<int>(value);
   else if (k == "ktime" && value >= GBK_KT_NONE && value <= GBK_KT_LAST) {
     if (value != GBK_KT_NONE && !ctx->kt_dev) {
       const size_t bytes = sizeof(unsigned long long) * (2 + 2 * kKtPartials);
@@ -385,6 +386,7 @@ int gbk_graph_free(gbk_ctx *ctx, gbk_graph *g) {
 }
 
 }  // extern "C"
+
 
 // Builds heavy lists for both CSRs of a freshly filled device graph.
 int gbk_graph_finalize(gbk_ctx *ctx, gbk_graph *g) {

@@ -7,13 +7,25 @@
 // opens (the reference's bucket fusion does the same for small bins).
 //
 // GPU organisation:
-//   near[2]   ping-pong queues of the current bucket, deduplicated per round
-//             with a per-vertex round tag, staged per wave in LDS.
+//   word[v]   one 64-bit word per vertex: distance in the high half, in the
+//             low half the relaxation round that last queued v (0 = none).
+//             A relaxation is a single 64-bit atomic minimum of
+//             (candidate << 32 | tag): it lowers the distance and records
+//             the round together, and the value it returns says both
+//             whether the distance improved and whether v is already queued
+//             for the next round -- one atomic instead of a minimum plus a
+//             separate test-and-set of a queued flag.  The tag is the round
+//             number only when the candidate falls in the current bucket,
+//             so a later improvement into the bucket still queues v.
+//   near[2]   ping-pong queues of the current bucket, staged per wave in LDS.
 //   later buckets are not queued at all: when the current bucket drains, one
-//             dense pass over dist[] finds the smallest bucket still holding
-//             an unsettled vertex and a second pass queues that bucket.  Two
-//             streaming passes over 4n bytes per bucket cost far less than a
+//             dense pass over the words finds the smallest bucket still
+//             holding an unsettled vertex and a second pass queues that
+//             bucket.  Two streaming passes per bucket cost far less than a
 //             pending list would in random atomics (one per improvement).
+// Each lane relaxes four edges per trip (independent loads and atomics in
+// flight together): the path is bound by the latency of dependent random
+// accesses, not by bandwidth.
 // Kernels of one step: relax, then min-bucket and split (each acting only in
 // its phase), each followed by a one-workgroup kernel that advances the
 // device state block.  The host queues steps without waiting and checks the
@@ -26,6 +38,12 @@
 namespace gbk {
 
 constexpr int32_t kDistInf = INT32_MAX / 2;  // sssp.cc:67
+constexpr int kRelaxBatch = 4;               // edges per lane per trip
+
+__device__ __forceinline__ int64_t word_dist(unsigned long long w) { return static_cast<int64_t>(w >> 32); }
+__device__ __forceinline__ unsigned long long make_word(int64_t d, uint32_t tag) {
+  return (static_cast<unsigned long long>(d) << 32) | tag;
+}
 
 struct SsspState {
   unsigned long long near_cnt[2];
@@ -36,13 +54,10 @@ struct SsspState {
   int cur, phase, done, overflow;
 };
 
-__global__ void sssp_init(int64_t n, int32_t source, int32_t *dist, int32_t *stamp, int32_t *near0,
+__global__ void sssp_init(int64_t n, int32_t source, unsigned long long *word, int32_t *near0,
                           SsspState *st) {
   const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v < n) {
-    dist[v] = v == source ? 0 : kDistInf;
-    stamp[v] = -1;
-  }
+  if (v < n) word[v] = make_word(v == source ? 0 : kDistInf, 0);
   if (v == 0) {
     near0[0] = source;
     st->near_cnt[0] = 1;
@@ -61,85 +76,99 @@ __global__ void sssp_init(int64_t n, int32_t source, int32_t *dist, int32_t *sta
 struct RelaxEdge {
   const int32_t *nbr;
   const int32_t *wgt;
-  int32_t *dist;
-  int32_t *stamp;
+  unsigned long long *word;
   WaveQueue nearq;
   int64_t delta;
   int64_t bucket;
-  int32_t tag;
-  __device__ void operator()(int64_t u, int64_t k, bool active) {
-    bool to_near = false;
-    int32_t v = 0;
-    if (active) {
-      v = nbr[k];
-      const int32_t nd = dist[u] + wgt[k];
+  uint32_t tag;  // this round's tag, never 0
+
+  template <int U>
+  __device__ __forceinline__ void batch(const int64_t (&u)[U], const int64_t (&k)[U], const bool (&act)[U]) {
+    int32_t v[U];
+    int64_t cand[U];
+    unsigned long long wv[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      v[j] = nbr[k[j]];
+      cand[j] = wgt[k[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      cand[j] += word_dist(word[u[j]]);
+      wv[j] = word[v[j]];
+    }
+    bool push[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      push[j] = false;
       // test first: most relaxations do not improve, and a plain load can
       // only overstate dist[v] (values only decrease), never hide a win
-      const int32_t old = nd < dist[v] ? atomicMin(&dist[v], nd) : nd;
-      if (nd < old && nd / delta == bucket && stamp[v] != tag)
-        to_near = atomicExch(&stamp[v], tag) != tag;
+      if (act[j] && cand[j] < word_dist(wv[j])) {
+        const bool here = cand[j] / delta == bucket;
+        const unsigned long long old = atomicMin(&word[v[j]], make_word(cand[j], here ? tag : 0u));
+        push[j] = here && cand[j] < word_dist(old) && static_cast<uint32_t>(old) != tag;
+      }
     }
-    nearq.push(to_near, v);
+#pragma unroll
+    for (int j = 0; j < U; j++) nearq.push(push[j], v[j]);
   }
 };
 
 __device__ __forceinline__ void relax_setup(RelaxEdge &f, int32_t *seg, const int32_t *nbr,
-                                            const int32_t *wgt, int32_t *dist, int32_t *stamp,
+                                            const int32_t *wgt, unsigned long long *word,
                                             int32_t *near0, int32_t *near1, int64_t cap,
                                             int64_t delta, SsspState *st) {
   const volatile SsspState *vs = st;
   const int cur = vs->cur;
   f.nbr = nbr;
   f.wgt = wgt;
-  f.dist = dist;
-  f.stamp = stamp;
+  f.word = word;
   f.nearq.init(seg + (threadIdx.x / kWave) * kSeg, &st->near_cnt[1 - cur], cur ? near0 : near1, cap,
                &st->overflow);
   f.delta = delta;
   f.bucket = vs->bucket;
-  f.tag = static_cast<int32_t>(2 * vs->relax_rounds + 2);
+  f.tag = static_cast<uint32_t>(vs->relax_rounds + 1);
 }
 
 __global__ void __launch_bounds__(kBlock)
-sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *dist,
-           int32_t *stamp, int32_t *near0, int32_t *near1, int64_t cap, int64_t delta,
-           int64_t heavy_deg, SsspState *st, HeavyQueue hq) {
+sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, unsigned long long *word,
+           int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, int64_t heavy_deg,
+           SsspState *st, HeavyQueue hq) {
   const volatile SsspState *vs = st;
   if (vs->done || vs->phase != 0) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   const int cur = vs->cur;
   const int64_t count = static_cast<int64_t>(vs->near_cnt[cur]);
   RelaxEdge f;
-  relax_setup(f, seg, nbr, wgt, dist, stamp, near0, near1, cap, delta, st);
+  relax_setup(f, seg, nbr, wgt, word, near0, near1, cap, delta, st);
   const int64_t lower = delta * f.bucket;
   auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
     s = off[u];
-    l = dist[u] >= lower ? off[u + 1] - s : 0;
+    l = word_dist(word[u]) >= lower ? off[u + 1] - s : 0;
   };
-  expand_list_light(cur ? near1 : near0, count, heavy_deg, rowlen, f, hq);
+  expand_list_light_batch<kRelaxBatch>(cur ? near1 : near0, count, heavy_deg, rowlen, f, hq);
   f.nearq.flush();
 }
 
 // Heavy rows of the near queue, spread over all workgroups.
 __global__ void __launch_bounds__(kBlock)
-sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *stamp,
-                 int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, SsspState *st,
-                 HeavyQueue hq) {
+sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, unsigned long long *word, int32_t *near0,
+                 int32_t *near1, int64_t cap, int64_t delta, SsspState *st, HeavyQueue hq) {
   const volatile SsspState *vs = st;
   if (vs->done || vs->phase != 0) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   RelaxEdge f;
-  relax_setup(f, seg, nbr, wgt, dist, stamp, near0, near1, cap, delta, st);
-  expand_heavy(hq, f);
+  relax_setup(f, seg, nbr, wgt, word, near0, near1, cap, delta, st);
+  expand_heavy_batch<kRelaxBatch>(hq, f);
   f.nearq.flush();
 }
 
 // Work of the coming relaxation round, for the kernel timer only: rows of
 // the near queue that are not stale and their edges, as per-workgroup
 // partials (rows in partial[0, grid), edges in partial[grid, 2 grid)).
-__global__ void sssp_count_round(const int64_t *off, const int32_t *dist, const int32_t *near0,
-                                 const int32_t *near1, int64_t delta, const SsspState *st,
-                                 unsigned long long *partial) {
+__global__ void sssp_count_round(const int64_t *off, const unsigned long long *word,
+                                 const int32_t *near0, const int32_t *near1, int64_t delta,
+                                 const SsspState *st, unsigned long long *partial) {
   const volatile SsspState *vs = st;
   unsigned long long rows = 0, edges = 0;
   if (!vs->done && vs->phase == 0) {
@@ -150,7 +179,7 @@ __global__ void sssp_count_round(const int64_t *off, const int32_t *dist, const 
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
       const int32_t u = q[i];
-      if (dist[u] >= lower) {
+      if (word_dist(word[u]) >= lower) {
         rows++;
         edges += static_cast<unsigned long long>(off[u + 1] - off[u]);
       }
@@ -173,15 +202,15 @@ __global__ void sssp_relax_done(SsspState *st, unsigned long long *heavy_count) 
 }
 
 // Smallest bucket above the current one that holds a reached vertex.
-__global__ void sssp_min_bucket(const int32_t *dist, int64_t n, int64_t delta, const SsspState *st,
-                                unsigned long long *partial) {
+__global__ void sssp_min_bucket(const unsigned long long *word, int64_t n, int64_t delta,
+                                const SsspState *st, unsigned long long *partial) {
   const volatile SsspState *vs = st;
   if (vs->done || vs->phase != 1) return;
   const int64_t bucket = vs->bucket;
   unsigned long long best = ULLONG_MAX;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
-    const int32_t d = dist[v];
+    const int64_t d = word_dist(word[v]);
     if (d < kDistInf) {
       const int64_t b = d / delta;
       if (b > bucket && static_cast<unsigned long long>(b) < best) best = static_cast<unsigned long long>(b);
@@ -223,7 +252,7 @@ __global__ void sssp_min_done(const unsigned long long *partial, int count, Sssp
 
 // Queues every vertex of the chosen bucket (each vertex once).
 __global__ void __launch_bounds__(kBlock)
-sssp_split(const int32_t *dist, int64_t n, int32_t *near0, int32_t *near1, int64_t cap,
+sssp_split(const unsigned long long *word, int64_t n, int32_t *near0, int32_t *near1, int64_t cap,
            int64_t delta, SsspState *st) {
   const volatile SsspState *vs = st;
   if (vs->done || vs->phase != 2) return;
@@ -238,7 +267,7 @@ sssp_split(const int32_t *dist, int64_t n, int32_t *near0, int32_t *near1, int64
     const int64_t v = base + threadIdx.x;
     bool take = false;
     if (v < n) {
-      const int32_t d = dist[v];
+      const int64_t d = word_dist(word[v]);
       take = d < kDistInf && d / delta == nb;
     }
     q.push(take, static_cast<int32_t>(v));
@@ -253,12 +282,15 @@ __global__ void sssp_split_done(SsspState *st) {
   st->phase = 0;
 }
 
-__global__ void sssp_finish(const int64_t *off, const int32_t *dist, int64_t n,
-                            unsigned long long *p_reached, unsigned long long *p_edges) {
+// Distances out of the words, plus reached vertices and their out-edges.
+__global__ void sssp_finish(const int64_t *off, const unsigned long long *word, int64_t n,
+                            int32_t *dist, unsigned long long *p_reached, unsigned long long *p_edges) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   unsigned long long r = 0, e = 0;
   for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
-    if (dist[v] != kDistInf) {
+    const int32_t d = static_cast<int32_t>(word_dist(word[v]));
+    dist[v] = d;
+    if (d != kDistInf) {
       r++;
       e += static_cast<unsigned long long>(off[v + 1] - off[v]);
     }
@@ -276,14 +308,14 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   const int64_t n = g->n;
   hipStream_t s = ctx->stream;
   const int64_t cap = n + 64;
-  auto *stamp = static_cast<int32_t *>(scratch(ctx, kS5, sizeof(int32_t) * cap));
+  auto *word = static_cast<unsigned long long *>(scratch(ctx, kS5, sizeof(unsigned long long) * cap));
   auto *q = static_cast<int32_t *>(scratch(ctx, kS7, sizeof(int32_t) * cap * 2));
   auto *st = static_cast<SsspState *>(scratch(ctx, kS8, sizeof(SsspState) + 64));
   auto *fin = static_cast<unsigned long long *>(scratch(ctx, kS9, 64));
   auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS10, sizeof(unsigned long long) * 2 * kGridCap));
   auto *hs = static_cast<SsspState *>(pinned(ctx, sizeof(SsspState)));
   HeavyQueue hq;
-  if (!stamp || !q || !st || !fin || !partial || !hs || !make_heavy_queue(ctx, g, kS11, &hq))
+  if (!word || !q || !st || !fin || !partial || !hs || !make_heavy_queue(ctx, g, kS11, &hq))
     return set_error(ctx, GBK_ENOMEM, "sssp scratch");
   hq.overflow = &st->overflow;
   int32_t *near0 = q, *near1 = q + cap;
@@ -293,29 +325,29 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   const unsigned count_grid = static_cast<unsigned>(kKtPartials < 256 ? kKtPartials : 256);
   const unsigned scan_grid = expand_grid(n);
   EventTimer timer(ctx);
-  sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, dist, stamp, near0, st);
+  sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, word, near0, st);
   int batch = 2;
   int64_t launched = 0;
   for (;;) {
     for (int j = 0; j < batch; j++) {
       if (count) {
         unsigned long long *part = ctx->kt_dev + 2;
-        sssp_count_round<<<count_grid, kBlock, 0, s>>>(g->off, dist, near0, near1, delta, st, part);
+        sssp_count_round<<<count_grid, kBlock, 0, s>>>(g->off, word, near0, near1, delta, st, part);
         sum_partials<unsigned long long><<<1, 1024, 0, s>>>(part, static_cast<int>(count_grid), ctx->kt_dev, 1);
         sum_partials<unsigned long long><<<1, 1024, 0, s>>>(part + count_grid, static_cast<int>(count_grid),
                                                           ctx->kt_dev + 1, 1);
       }
       const int kt = kt_begin(ctx, GBK_KT_SSSP_RELAX);
-      sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->nbr, g->wgt, dist, stamp, near0, near1, cap,
-                                               delta, H, st, hq);
+      sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->nbr, g->wgt, word, near0, near1, cap, delta, H,
+                                               st, hq);
       heavy_prefix<<<1, 1024, 0, s>>>(hq);
-      sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, g->wgt, dist, stamp, near0, near1, cap,
-                                                     delta, st, hq);
+      sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, g->wgt, word, near0, near1, cap, delta, st,
+                                                     hq);
       kt_end(ctx, kt);
       sssp_relax_done<<<1, kWave, 0, s>>>(st, hq.count);
-      sssp_min_bucket<<<scan_grid, kBlock, 0, s>>>(dist, n, delta, st, partial);
+      sssp_min_bucket<<<scan_grid, kBlock, 0, s>>>(word, n, delta, st, partial);
       sssp_min_done<<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), st);
-      sssp_split<<<scan_grid, kBlock, 0, s>>>(dist, n, near0, near1, cap, delta, st);
+      sssp_split<<<scan_grid, kBlock, 0, s>>>(word, n, near0, near1, cap, delta, st);
       sssp_split_done<<<1, kWave, 0, s>>>(st);
     }
     launched += batch;
@@ -330,7 +362,7 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
     // one; more steps than buckets times rounds would mean a broken state
     if (launched > 4 * (n + 16) + 1024) return set_error(ctx, GBK_EHIP, "sssp: no convergence");
   }
-  sssp_finish<<<scan_grid, kBlock, 0, s>>>(g->off, dist, n, partial, partial + kGridCap);
+  sssp_finish<<<scan_grid, kBlock, 0, s>>>(g->off, word, n, dist, partial, partial + kGridCap);
   sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), fin, 0);
   sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial + kGridCap, static_cast<int>(scan_grid), fin + 1, 0);
   GBK_LAUNCH_CHECK(ctx, "sssp_finish");

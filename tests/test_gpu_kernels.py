@@ -77,12 +77,15 @@ def test_bfs_bottom_up_group_sizes(ctx):
     g = O.kron(13)
     dg = ctx.upload_graph(g)
     s = int(O.source_picker(g, 1)[1][0])
-    for G in (1, 4, 8, 16, 64):
-        ctx.set_param("bu_group", G)
-        parent, st = ctx.bfs(dg, s)
-        assert O.bfs_verify(g, s, parent)
-        assert st.steps_bu > 0
-    ctx.set_param("bu_group", 8)
+    saved = ctx.get_param("bu_group")
+    try:
+        for G in (1, 2, 4, 8, 16, 32, 64):
+            ctx.set_param("bu_group", G)
+            parent, st = ctx.bfs(dg, s)
+            assert O.bfs_verify(g, s, parent)
+            assert st.steps_bu > 0
+    finally:
+        ctx.set_param("bu_group", saved)
 
 
 CC_GRAPHS = {
