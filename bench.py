@@ -43,13 +43,17 @@ def parse():
     p.add_argument("--bfs-sources", type=int, default=64)
     p.add_argument("--sssp-sources", type=int, default=64)
     p.add_argument("--bc-sources", type=int, default=4)
-    p.add_argument("--delta", type=int, default=64)
+    p.add_argument("--delta", type=int, default=64, help="delta of one-source SSSP runs")
+    p.add_argument("--batch-delta", type=int, default=1024,
+                   help="bucket width of the 64-source SSSP batch (distances do not depend on it)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU sample work")
     p.add_argument("--only", default="", help="comma list of kernels to run (profiling)")
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="first copy MIB MiB device to device (calibrates HBM byte counters)")
+    p.add_argument("--single-source", action="store_true",
+                   help="run BFS and SSSP one source per call instead of 64-source batches")
     p.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
                    help="library tunable to set before the run (repeatable)")
     p.add_argument("--autotune", action="store_true",
@@ -146,20 +150,28 @@ class Workload:
         if "pr" in self.kernels:
             _, st = ctx.pr(g, gbk.PR_GS, 20, 1e-4, False, download=False)
             add("pr", st.device_ms, m * st.iterations, bytes_pr_iter(n, m) * st.iterations, st.iterations)
-        if "bfs" in self.kernels:
+        if "bfs" in self.kernels and a.single_source:
             for s in self.srcs[:a.bfs_sources]:
                 _, st = ctx.bfs(g, int(s), download=False)
                 add("bfs", st.device_ms, st.edges_reached, bytes_bfs(n, st.edges_reached))
+        elif "bfs" in self.kernels:
+            _, _, edges, st = ctx.bfs_batch(g, self.srcs[:a.bfs_sources], download=False)
+            add("bfs", st.device_ms, st.edges_reached, sum(bytes_bfs(n, int(e)) for e in edges),
+                len(edges))
         if "cc" in self.kernels:
             _, st = ctx.cc(g, gbk.CC_AFFOREST, download=False)
             add("cc", st.device_ms, m, bytes_cc(n, m))
         if "cc_sv" in self.kernels:
             _, st = ctx.cc(g, gbk.CC_SV, download=False)
             add("cc_sv", st.device_ms, m * st.iterations, bytes_sv_iter(n, m) * st.iterations, st.iterations)
-        if "sssp" in self.kernels:
+        if "sssp" in self.kernels and a.single_source:
             for s in self.srcs[:a.sssp_sources]:
                 _, st = ctx.sssp(gw, int(s), a.delta, download=False)
                 add("sssp", st.device_ms, st.edges_reached, bytes_sssp(n, st.edges_reached))
+        elif "sssp" in self.kernels:
+            _, _, edges, st = ctx.sssp_batch(gw, self.srcs[:a.sssp_sources], a.batch_delta, download=False)
+            add("sssp", st.device_ms, st.edges_reached, sum(bytes_sssp(n, int(e)) for e in edges),
+                len(edges))
         if "bc" in self.kernels:
             _, st = ctx.bc(g, self.srcs[:a.bc_sources], download=False)
             add("bc", st.device_ms, 2 * st.edges_reached, bytes_bc(n, st.edges_reached) , a.bc_sources)
@@ -167,13 +179,19 @@ class Workload:
 
 # Kernels followed one launch at a time (HIP events recorded by the library
 # on its stream around each launch) for the roofline of the dominant family.
-def kt_table(gbk):
-    return {
+def kt_table(gbk, single):
+    t = {
         "sssp": (gbk.KT_SSSP_RELAX, "sssp_relax+heavy_prefix+sssp_relax_heavy (one relaxation round)"),
         "pr": (gbk.KT_PR_ITER, "pr_pull_iter<GS> (one pull iteration)"),
         "bfs": (gbk.KT_BFS_STEP, "bfs_td_step+heavy_prefix+bfs_td_heavy | bfs_bu_step (one BFS step)"),
         "bc": (gbk.KT_BC_LEVEL, "bc_forward_level+heavy_prefix+bc_forward_heavy | bc_backward_level"),
     }
+    if not single:
+        t["sssp"] = (gbk.KT_SSSP_RELAX, "sb_relax<64>+heavy_prefix+sb_relax_heavy<64> "
+                                        "(one relaxation round of a 64-source batch)")
+        t["bfs"] = (gbk.KT_BFS_STEP, "msbfs_front_queue+msbfs_td+heavy_prefix+msbfs_td_heavy | msbfs_bu<8> "
+                                     "(one level of a 64-source batch)")
+    return t
 
 
 def launch_bytes(family, kt, work, rec):
@@ -183,23 +201,27 @@ def launch_bytes(family, kt, work, rec):
     contributions and scores (16 B per row): 4m + 24n.  SSSP: a relaxation
     round reads 8 B per scanned edge (neighbour + weight) and 16 B per
     non-stale frontier row (queue entry, offsets, distance); rows and edges
-    are counted on the device outside the timed launches.  BFS and BC: the
-    per-source figure spread evenly over the source's launches."""
+    are counted on the device outside the timed launches.  BFS and BC, and
+    the 64-source batches: the per-source figures of all sources spread
+    evenly over the launches (a batch reads each edge once for all its
+    sources, so this counts the bytes separate runs would need)."""
     n, m = work.g.n, work.g.m
     if family == "pr":
         return float(bytes_pr_iter(n, m)), "exact"
-    if family == "sssp":
+    if family == "sssp" and work.args.single_source:
         return (8.0 * kt["edges"] + 16.0 * kt["rows"]) / max(kt["launches"], 1), "counted"
     return rec[family][2] / max(kt["launches"], 1), "per-source figure / launches"
 
 
-def load_traffic(workload, family):
+def load_traffic(workload, family, single):
+    """HBM bytes per launch of the followed kernel, measured with rocprofv3
+    PMC counters for this workload and mode (profiles/traffic.json)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    e = t.get(f"{workload}|{family}")
+    e = t.get(f"{workload}|{family}|{'single' if single else 'batched'}")
     return None if e is None else e.get("bytes_per_launch")
 
 
@@ -282,10 +304,17 @@ def main():
         tuner = Autotuner(ctx)
         t0 = time.time()
         srcs = work.srcs[:4]
-        best_s = tuner.tune_sssp(work.rgw, srcs) if "sssp" in work.kernels else None
+        best_s = None
+        if "sssp" in work.kernels and args.single_source:
+            best_s = tuner.tune_sssp(work.rgw, srcs)
+        elif "sssp" in work.kernels:
+            best_s = tuner.tune_sssp_batch(work.rgw, work.srcs[:args.sssp_sources])
         best_b = tuner.tune_bfs(work.rg, srcs) if "bfs" in work.kernels else None
-        if best_s:
+        if best_s and args.single_source:
             args.delta = best_s["delta"]
+        elif best_s:
+            args.batch_delta = best_s["delta"]
+        if best_s:
             tuner.apply(best_s)
         if best_b:
             tuner.apply(best_b)
@@ -294,7 +323,7 @@ def main():
         log(f"autotune: {tuned}")
         for row in tuner.log:
             log(f"  {row}")
-    kts = kt_table(gbk)
+    kts = kt_table(gbk, args.single_source)
     if wrec:
         family = max((f for f in wrec if f in kts), key=lambda f: wrec[f][0], default=None)
     else:
@@ -338,12 +367,12 @@ def main():
         dom = max(rec, key=lambda f: rec[f][0])
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(achieved * 1e9 / HBM_PEAK, 4),
-                "traffic": load_traffic(workload, family),
+                "traffic": load_traffic(workload, family, args.single_source),
                 "kernel": kts[family][1], "family": family, "dominant_family": dom,
                 "launches": kt["launches"], "avg_launch_us": round(kt["ms"] / kt["launches"] * 1e3, 2),
                 "bytes_per_launch": round(per_launch), "bytes_basis": how,
                 "launch_share_of_step": round(kt["ms"] / total_ms, 4)}
-        if family == "sssp":
+        if family == "sssp" and args.single_source:
             roof["rows_scanned"] = kt["rows"]
             roof["edges_scanned"] = kt["edges"]
     out = {
@@ -354,7 +383,8 @@ def main():
         "data": "synthetic (seeded Kronecker generator of the reference, weights U[1,255])",
         "config": {"workload": workload,
                    "n": n, "m_directed": m, "order": args.order, "mapping_fingerprint": work.fingerprint,
-                   "delta": args.delta, "autotuned": tuned,
+                   "delta": args.delta if args.single_source else args.batch_delta,
+                   "batched": not args.single_source, "autotuned": tuned,
                    "params": {k: ctx.get_param(k) for k in ("heavy_degree", "bu_group", "bfs_alpha", "bfs_beta",
                                                             "relax_grid")}},
         "kernels": kern,

@@ -7,6 +7,7 @@ results: bit-exact; PR: 1e-4 relative).  Swept, per kernel:
 
   sssp   delta in {1, 8, 32, 64, 128, 256, 512}           (CLDelta -d)
          then relax_grid in {256, 512, 1024, 2048} workgroups
+  sssp batch  bucket width in {64, 256, 1024, 4096, 2^20}
   bfs    bu_group in {1, 2, 4, 8, 16} x (alpha, beta) grid (bfs.cc:165)
   all    heavy_degree in {1024, 2048, 4096, 8192}          (rows per workgroup)
 
@@ -30,6 +31,7 @@ BU_GROUPS = (1, 2, 4, 8, 16)
 ALPHA_BETA = ((15, 18), (8, 18), (30, 18), (15, 9), (15, 36))
 HEAVY = (1024, 2048, 4096, 8192)
 GRIDS = (256, 512, 1024, 2048)
+BATCH_DELTAS = (64, 256, 1024, 4096, 1 << 20)
 
 
 def _median(xs: Sequence[float]) -> float:
@@ -95,6 +97,28 @@ class Autotuner:
             if ms < best["ms"]:
                 best = {"delta": best["delta"], "relax_grid": grid, "ms": ms}
         self.ctx.set_param("relax_grid", saved)
+        self.cache[k] = best
+        self._save()
+        return best
+
+    def tune_sssp_batch(self, g: gbk.DeviceGraph, sources: Iterable[int],
+                        deltas: Sequence[int] = BATCH_DELTAS) -> dict:
+        """Bucket width of the multi-source SSSP batch (whole batch timed)."""
+        k = self.key(g, "sssp_batch")
+        if k in self.cache:
+            return self.cache[k]
+        srcs = np.asarray([int(s) for s in sources], np.int32)
+        ref = self.ctx.sssp_batch(g, srcs, deltas[-1])[0]
+        best = None
+        for d in deltas:
+            out = self.ctx.sssp_batch(g, srcs, d)[0]
+            if not np.array_equal(out, ref):
+                raise AssertionError(f"sssp batch delta={d} changed distances")
+            ms = _median([self.ctx.sssp_batch(g, srcs, d, download=False)[3].device_ms
+                          for _ in range(self.repeats)])
+            self.log.append({"kernel": "sssp_batch", "delta": d, "ms": ms})
+            if best is None or ms < best["ms"]:
+                best = {"delta": d, "ms": ms}
         self.cache[k] = best
         self._save()
         return best

@@ -1,0 +1,835 @@
+// gbk_batch.hip -- many BFS / SSSP sources in one traversal.
+//
+// The reference runs one source per trial (the trial loop of BenchmarkKernel,
+// gapbs/benchmark.h:450, calling bench/src/bfs.cc:285-287 and sssp.cc:342-344
+// once per source).  Here up to 64 sources share every edge
+// scan, so that one random access to a vertex's state serves all of them:
+//
+//   BFS   bit-parallel: seen / frontier / next are one 64-bit word per
+//         vertex, bit k standing for source k.  A top-down step ORs the
+//         frontier word of a row into next[x] of each neighbour x (atomic OR
+//         only when it adds bits); a bottom-up step ORs the frontier words of
+//         a vertex's in-neighbours until every source has seen it.  Depths
+//         equal those of separate BFS runs: bit k of the frontier after d
+//         steps is exactly the set of vertices at depth d from source k.
+//   SSSP  lane-parallel: the distances of one vertex to the Kp (8..64)
+//         sources of a batch are stored together (vertex-major), and a group
+//         of Kp lanes relaxes one edge for all sources with one coalesced
+//         access.  Buckets of width delta are shared: lane k relaxes from a
+//         row only while its own distance lies in the current bucket; an
+//         improvement into the current bucket queues the vertex for the next
+//         round (per-vertex round tag, one queue entry per vertex), one into
+//         a later bucket lowers the vertex's pending-bucket word
+//         pend[v] with an atomic minimum.  When the current bucket drains,
+//         the next one is the minimum of pend[] (dense scan) and its
+//         vertices are queued.  A queued row re-registers the later buckets
+//         of its other lanes when it is visited.  Integer distances make the
+//         result exact whatever the order, so every source's distances equal
+//         a separate delta-stepping run.
+//
+// Outputs are written vertex-major during the run and transposed through
+// LDS tiles at the end into one array per source, while the reached
+// vertices and their out-edges are counted per source.
+#include <algorithm>
+#include <climits>
+#include <vector>
+
+#include "gbk_expand.h"
+
+namespace gbk {
+
+constexpr int kBatchMax = 64;
+constexpr int32_t kBatchInf = INT32_MAX / 2;  // sssp.cc:67
+constexpr int64_t kBatchHeavy = 256;          // rows above this are chunked over groups
+constexpr int64_t kBatchChunk = 256;
+constexpr int kBatchUnroll = 4;
+
+inline int batch_width(int k) { return k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64; }
+
+// ------------------------------------------------------------------ shared
+template <int Kp>
+__device__ __forceinline__ int32_t group_min(int32_t v) {
+#pragma unroll
+  for (int o = Kp / 2; o >= 1; o >>= 1) {
+    const int32_t w = __shfl_xor(v, o);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+template <int Kp>
+__device__ __forceinline__ bool group_any(bool p) {
+  const uint64_t m = __ballot(p);
+  if (Kp == 64) return m != 0;
+  const int shift = (lane_id() / Kp) * Kp;
+  return ((m >> shift) & ((1ull << Kp) - 1)) != 0;
+}
+
+// Transposes a vertex-major [n][Kp] int32 array into K source-major arrays
+// (out may be null) and counts, per source, the vertices whose value is not
+// 'unreached' and the sum of their out-degrees: partial[b * 64 + k] and
+// partial[(grid + b) * 64 + k] for workgroup b.
+__global__ void __launch_bounds__(kBlock)
+batch_collect(const int32_t *vm, int64_t n, int Kp, int K, int32_t unreached, const int64_t *off,
+              int32_t *out, unsigned long long *partial) {
+  __shared__ int32_t tile[64][65];
+  __shared__ int64_t degs[64];
+  __shared__ unsigned long long cnt[2][kBatchMax];
+  if (threadIdx.x < kBatchMax) {
+    cnt[0][threadIdx.x] = 0;
+    cnt[1][threadIdx.x] = 0;
+  }
+  const int w = threadIdx.x / kWave;
+  for (int64_t t0 = static_cast<int64_t>(blockIdx.x) * 64; t0 < n; t0 += static_cast<int64_t>(gridDim.x) * 64) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * Kp; i += kBlock) {
+      const int64_t v = t0 + i / Kp;
+      tile[i / Kp][i % Kp] = v < n ? vm[v * Kp + i % Kp] : unreached;
+    }
+    if (threadIdx.x < 64) {
+      const int64_t v = t0 + threadIdx.x;
+      degs[threadIdx.x] = v < n ? off[v + 1] - off[v] : 0;
+    }
+    __syncthreads();
+    // wave w handles sources w, w+4, ...; lane = vertex of the tile
+    for (int k = w; k < K; k += kWavesPerBlock) {
+      const int vv = lane_id();
+      const int64_t v = t0 + vv;
+      const int32_t val = tile[vv][k];
+      const bool ok = v < n && val != unreached;
+      if (v < n && out) out[static_cast<int64_t>(k) * n + v] = val;
+      const unsigned long long r = wave_sum(static_cast<unsigned long long>(ok ? 1 : 0));
+      const unsigned long long e = wave_sum(static_cast<unsigned long long>(ok ? degs[vv] : 0));
+      if (lane_id() == 0) {
+        cnt[0][k] += r;
+        cnt[1][k] += e;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kBatchMax) {
+    partial[static_cast<int64_t>(blockIdx.x) * kBatchMax + threadIdx.x] = cnt[0][threadIdx.x];
+    partial[static_cast<int64_t>(gridDim.x + blockIdx.x) * kBatchMax + threadIdx.x] = cnt[1][threadIdx.x];
+  }
+}
+
+// One workgroup of 1024 lanes: sums the collect partials per source.
+__global__ void batch_collect_sum(const unsigned long long *partial, int grid, unsigned long long *sums) {
+  __shared__ unsigned long long red[2][16][kBatchMax];
+  const int k = threadIdx.x % kBatchMax, part = threadIdx.x / kBatchMax;  // 16 parts
+  unsigned long long r = 0, e = 0;
+  for (int b = part; b < grid; b += 16) {
+    r += partial[static_cast<int64_t>(b) * kBatchMax + k];
+    e += partial[static_cast<int64_t>(grid + b) * kBatchMax + k];
+  }
+  red[0][part][k] = r;
+  red[1][part][k] = e;
+  __syncthreads();
+  if (threadIdx.x < kBatchMax) {
+    unsigned long long a = 0, b2 = 0;
+    for (int p = 0; p < 16; p++) {
+      a += red[0][p][threadIdx.x];
+      b2 += red[1][p][threadIdx.x];
+    }
+    sums[threadIdx.x] = a;
+    sums[kBatchMax + threadIdx.x] = b2;
+  }
+}
+
+// ------------------------------------------------------------------- BFS
+struct MsCounters {
+  unsigned long long tail;
+  unsigned long long fverts;  // vertices in the new frontier
+  unsigned long long fedges;  // their out-degree sum
+  int overflow;
+  int pad;
+};
+
+__global__ void msbfs_sources(const int32_t *src, int K, int Kp, unsigned long long *seen,
+                              unsigned long long *front, int32_t *depth_vm, MsCounters *c) {
+  const int j = threadIdx.x;
+  if (j < K) {
+    const int64_t s = src[j];
+    atomicOr(&seen[s], 1ull << j);
+    atomicOr(&front[s], 1ull << j);
+    depth_vm[s * Kp + j] = 0;
+  }
+  if (j == 0) {
+    c->tail = 0;
+    c->fverts = 0;
+    c->fedges = 0;
+    c->overflow = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+msbfs_front_queue(const unsigned long long *front, int64_t n, int32_t *queue, int64_t cap, MsCounters *c) {
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  WaveQueue q;
+  q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
+    const int64_t v = base + threadIdx.x;
+    q.push(v < n && front[v] != 0, static_cast<int32_t>(v));
+  }
+  q.flush();
+}
+
+struct MsTdEdge {
+  const int32_t *nbr;
+  const unsigned long long *front;
+  const unsigned long long *seen;
+  unsigned long long *next;
+  __device__ void operator()(int64_t r, int64_t k, bool active) {
+    if (!active) return;
+    const int32_t x = nbr[k];
+    const unsigned long long add = front[r] & ~seen[x];
+    if (add & ~next[x]) atomicOr(&next[x], add);
+  }
+};
+
+__global__ void __launch_bounds__(kBlock)
+msbfs_td(const int64_t *off, const int32_t *nbr, const int32_t *queue, const MsCounters *c,
+         const unsigned long long *front, const unsigned long long *seen, unsigned long long *next,
+         int64_t heavy_deg, HeavyQueue hq) {
+  MsTdEdge f{nbr, front, seen, next};
+  auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
+    s = off[u];
+    l = off[u + 1] - s;
+  };
+  expand_list_light(queue, static_cast<int64_t>(c->tail), heavy_deg, rowlen, f, hq);
+}
+
+__global__ void __launch_bounds__(kBlock)
+msbfs_td_heavy(const int32_t *nbr, const unsigned long long *front, const unsigned long long *seen,
+               unsigned long long *next, HeavyQueue hq) {
+  MsTdEdge f{nbr, front, seen, next};
+  expand_heavy(hq, f);
+}
+
+// Bottom-up: G lanes per vertex OR the frontier words of its in-neighbours
+// until every source has reached it; only this group writes next[u].
+template <int G>
+__global__ void __launch_bounds__(kBlock)
+msbfs_bu(const int64_t *in_off, const int32_t *in_nbr, int64_t n, const unsigned long long *seen,
+         const unsigned long long *front, unsigned long long *next, unsigned long long full, int swz) {
+  const int glane = lane_id() % G;
+  const int64_t groups_per_block = kBlock / G;
+  const int64_t tiles = (n + groups_per_block - 1) / groups_per_block;
+  for (int64_t t = swizzled_block(swz); t < tiles; t += gridDim.x) {
+    const int64_t u = t * groups_per_block + threadIdx.x / G;
+    unsigned long long sv = full;
+    int64_t s = 0, e = 0;
+    if (u < n) {
+      sv = seen[u];
+      s = in_off[u];
+      e = in_off[u + 1];
+    }
+    bool todo = (sv & full) != full && s < e;
+    const bool any = todo;
+    unsigned long long acc = 0;
+    for (int64_t base = s; todo; base += G) {  // group-uniform loop
+      const int64_t k = base + glane;
+      unsigned long long x = k < e ? front[in_nbr[k]] : 0ull;
+#pragma unroll
+      for (int o = G / 2; o >= 1; o >>= 1) x |= __shfl_xor(x, o);
+      acc |= x;
+      if (((acc | sv) & full) == full || base + G >= e) todo = false;
+    }
+    if (any && glane == 0) {
+      const unsigned long long add = acc & ~sv & full;
+      if (add) next[u] = add;
+    }
+  }
+}
+
+__global__ void msbfs_finish(const int64_t *off, int64_t n, int Kp, unsigned long long *seen,
+                             unsigned long long *front, unsigned long long *next, int32_t *depth_vm,
+                             int32_t level, unsigned long long *p_verts, unsigned long long *p_edges) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  unsigned long long verts = 0, edges = 0;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
+    const unsigned long long nx = next[v];
+    const unsigned long long sv = seen[v];
+    const unsigned long long add = nx & ~sv;
+    if (front[v] != add) front[v] = add;
+    if (nx) next[v] = 0;
+    if (add) {
+      seen[v] = sv | add;
+      verts++;
+      edges += static_cast<unsigned long long>(off[v + 1] - off[v]);
+      unsigned long long b = add;
+      while (b) {
+        const int k = __ffsll(static_cast<long long>(b)) - 1;
+        depth_vm[v * Kp + k] = level;
+        b &= b - 1;
+      }
+    }
+  }
+  block_partial(verts, p_verts);
+  block_partial(edges, p_edges);
+}
+
+__global__ void msbfs_store(const unsigned long long *partial, int count, MsCounters *c) {
+  __shared__ unsigned long long red[2][1024 / kWave];
+  unsigned long long a = 0, b = 0;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+    a += partial[i];
+    b += partial[kGridCap + i];
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if (lane_id() == 0) {
+    red[0][threadIdx.x / kWave] = a;
+    red[1][threadIdx.x / kWave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long x = 0, y = 0;
+    for (int i = 0; i < static_cast<int>(blockDim.x) / kWave; i++) {
+      x += red[0][i];
+      y += red[1][i];
+    }
+    c->fverts = x;
+    c->fedges = y;
+    c->tail = 0;
+  }
+}
+
+static void launch_msbfs_bu(gbk_ctx *ctx, int G, const gbk_graph *g, const unsigned long long *seen,
+                            const unsigned long long *front, unsigned long long *next,
+                            unsigned long long full, unsigned grid) {
+  const int swz = ctx->tune.xcd_swizzle;
+  hipStream_t s = ctx->stream;
+  switch (G) {
+    case 1: msbfs_bu<1><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
+    case 2: msbfs_bu<2><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
+    case 4: msbfs_bu<4><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
+    case 16: msbfs_bu<16><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
+    default: msbfs_bu<8><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
+  }
+}
+
+// ------------------------------------------------------------------ SSSP
+struct BatchState {
+  unsigned long long near_cnt[2];
+  long long bucket;
+  long long next_bucket;
+  long long relax_rounds;
+  long long buckets;
+  int cur, phase, done, overflow;
+};
+
+__global__ void sb_init_state(BatchState *st, long long first) {
+  if (threadIdx.x == 0) {
+    st->near_cnt[0] = static_cast<unsigned long long>(first);
+    st->near_cnt[1] = 0;
+    st->bucket = 0;
+    st->next_bucket = 0;
+    st->relax_rounds = 0;
+    st->buckets = 0;
+    st->cur = 0;
+    st->phase = 0;
+    st->done = 0;
+    st->overflow = 0;
+  }
+}
+
+__global__ void sb_sources(const int32_t *src, int K, int Kp, int32_t *dist) {
+  const int j = threadIdx.x;
+  if (j < K) dist[static_cast<int64_t>(src[j]) * Kp + j] = 0;
+}
+
+// Per-lane relaxation context of a group of Kp lanes (lane k = source k).
+template <int Kp>
+struct GroupRelax {
+  const int32_t *nbr;
+  const int32_t *wgt;
+  int32_t *dist;
+  int32_t *pend;
+  uint32_t *vtag;
+  WaveQueue nearq;
+  int64_t delta, lower, upper;
+  uint32_t tag;
+  int k;
+  bool leader;
+
+  __device__ void setup(int32_t *seg, const int32_t *nb, const int32_t *wg, int32_t *d, int32_t *p,
+                        uint32_t *vt, int32_t *near0, int32_t *near1, int64_t cap, int64_t dl,
+                        BatchState *st) {
+    const volatile BatchState *vs = st;
+    const int cur = vs->cur;
+    nbr = nb;
+    wgt = wg;
+    dist = d;
+    pend = p;
+    vtag = vt;
+    nearq.init(seg + (threadIdx.x / kWave) * kSeg, &st->near_cnt[1 - cur], cur ? near0 : near1, cap,
+               &st->overflow);
+    delta = dl;
+    lower = dl * vs->bucket;
+    upper = lower + dl;
+    tag = static_cast<uint32_t>(vs->relax_rounds + 1);
+    k = lane_id() % Kp;
+    leader = k == 0;
+  }
+
+  // Lane k's distance of a row and whether it relaxes from it; a later
+  // bucket of this lane is registered in pend[row] (by the group leader).
+  __device__ void row_lane(int64_t row, bool valid, bool reg, int64_t &du, bool &act) {
+    du = valid ? dist[row * Kp + k] : kBatchInf;
+    act = valid && du >= lower && du < upper;
+    const int32_t fut = (valid && reg && du >= upper && du < kBatchInf) ? static_cast<int32_t>(du / delta) : INT32_MAX;
+    const int32_t f = group_min<Kp>(fut);
+    if (leader && valid && f != INT32_MAX && f < pend[row]) atomicMin(&pend[row], f);
+  }
+
+  // Relaxes edges [s, s + len) of a row for every source of the group;
+  // len is group-uniform, the trip count wave-uniform.
+  __device__ void walk(int64_t s, int64_t len, int64_t du, bool act) {
+    const int64_t trips = wave_max(len);
+    for (int64_t j0 = 0; j0 < trips; j0 += kBatchUnroll) {
+      int32_t x[kBatchUnroll], w[kBatchUnroll], dv[kBatchUnroll];
+      bool valid[kBatchUnroll];
+#pragma unroll
+      for (int j = 0; j < kBatchUnroll; j++) {
+        valid[j] = j0 + j < len;
+        const int64_t e = valid[j] ? s + j0 + j : 0;
+        x[j] = nbr[e];
+        w[j] = wgt[e];
+      }
+#pragma unroll
+      for (int j = 0; j < kBatchUnroll; j++) dv[j] = dist[static_cast<int64_t>(x[j]) * Kp + k];
+#pragma unroll
+      for (int j = 0; j < kBatchUnroll; j++) {
+        bool here = false;
+        int32_t fut = INT32_MAX;
+        const int64_t cand = du + w[j];
+        if (valid[j] && act && cand < dv[j]) {
+          const int32_t old = atomicMin(&dist[static_cast<int64_t>(x[j]) * Kp + k], static_cast<int32_t>(cand));
+          if (cand < old) {
+            if (cand < upper) here = true;
+            else fut = static_cast<int32_t>(cand / delta);
+          }
+        }
+        const bool anyh = group_any<Kp>(here);
+        const int32_t f = group_min<Kp>(fut);
+        bool push = false;
+        if (leader && valid[j]) {
+          if (anyh) push = atomicExch(&vtag[x[j]], tag) != tag;
+          if (f != INT32_MAX && f < pend[x[j]]) atomicMin(&pend[x[j]], f);
+        }
+        nearq.push(push, x[j]);
+      }
+    }
+  }
+};
+
+template <int Kp>
+__global__ void __launch_bounds__(kBlock)
+sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *pend,
+         uint32_t *vtag, int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, BatchState *st,
+         HeavyQueue hq) {
+  const volatile BatchState *vs = st;
+  if (vs->done || vs->phase != 0) return;
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  constexpr int G = kWave / Kp;
+  const int grp = lane_id() / Kp;
+  const int cur = vs->cur;
+  const int64_t count = static_cast<int64_t>(vs->near_cnt[cur]);
+  const int32_t *list = cur ? near1 : near0;
+  GroupRelax<Kp> f;
+  f.setup(seg, nbr, wgt, dist, pend, vtag, near0, near1, cap, delta, st);
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  for (int64_t base = gw * G; base < count; base += waves * G) {
+    const int64_t i = base + grp;
+    const bool valid = i < count;
+    int64_t row = 0, s = 0, len = 0;
+    if (valid) {
+      row = list[i];
+      s = off[row];
+      len = off[row + 1] - s;
+    }
+    const bool heavy = len > kBatchHeavy;
+    if (heavy && f.leader) {
+      const unsigned long long at = atomicAdd(hq.count, 1ull);
+      if (static_cast<int64_t>(at) < hq.cap) {
+        hq.row[at] = static_cast<int32_t>(row);
+        hq.start[at] = s;
+        hq.len[at] = len;
+      } else {
+        *hq.overflow = 1;
+      }
+    }
+    int64_t du;
+    bool act;
+    f.row_lane(row, valid, !heavy, du, act);
+    f.walk(s, heavy ? 0 : len, du, act);
+  }
+  f.nearq.flush();
+}
+
+template <int Kp>
+__global__ void __launch_bounds__(kBlock)
+sb_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *pend, uint32_t *vtag,
+               int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, BatchState *st,
+               HeavyQueue hq) {
+  const volatile BatchState *vs = st;
+  if (vs->done || vs->phase != 0) return;
+  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  if (rows == 0) return;
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  constexpr int G = kWave / Kp;
+  const int grp = lane_id() / Kp;
+  GroupRelax<Kp> f;
+  f.setup(seg, nbr, wgt, dist, pend, vtag, near0, near1, cap, delta, st);
+  const int64_t total = hq.prefix[rows];
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
+  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
+    const int64_t c = c0 + grp;
+    const bool valid = c < total;
+    int64_t row = 0, first = 0, last = 0, start = 0;
+    if (valid) {
+      int64_t lo = 0, hi = rows;  // last row with prefix[row] <= c
+      while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (hq.prefix[mid] <= c) lo = mid;
+        else hi = mid;
+      }
+      row = hq.row[lo];
+      start = hq.start[lo];
+      first = (c - hq.prefix[lo]) * hq.chunk;
+      last = min(hq.len[lo], first + hq.chunk);
+    }
+    int64_t du;
+    bool act;
+    f.row_lane(row, valid, first == 0, du, act);
+    f.walk(start + first, valid ? last - first : 0, du, act);
+  }
+  f.nearq.flush();
+}
+
+__global__ void sb_relax_done(BatchState *st, unsigned long long *heavy_count) {
+  if (threadIdx.x != 0 || st->done || st->phase != 0) return;
+  *heavy_count = 0;
+  const int cur = st->cur;
+  st->near_cnt[cur] = 0;
+  st->cur = 1 - cur;
+  st->relax_rounds += 1;
+  if (st->near_cnt[1 - cur] == 0) st->phase = 1;
+}
+
+__global__ void sb_min_pend(const int32_t *pend, int64_t n, const BatchState *st, unsigned long long *partial) {
+  const volatile BatchState *vs = st;
+  if (vs->done || vs->phase != 1) return;
+  int32_t best = INT32_MAX;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
+    const int32_t p = pend[v];
+    best = p < best ? p : best;
+  }
+  best = group_min<kWave>(best);
+  __shared__ int32_t red[kWavesPerBlock];
+  if (lane_id() == 0) red[threadIdx.x / kWave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t m = red[0];
+    for (int i = 1; i < kWavesPerBlock; i++) m = red[i] < m ? red[i] : m;
+    partial[blockIdx.x] = static_cast<unsigned long long>(m);
+  }
+}
+
+__global__ void sb_min_done(const unsigned long long *partial, int count, BatchState *st) {
+  if (st->done || st->phase != 1) return;
+  __shared__ unsigned long long red[1024 / kWave];
+  unsigned long long m = INT32_MAX;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) m = partial[i] < m ? partial[i] : m;
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const unsigned long long x = __shfl_xor(m, o);
+    m = x < m ? x : m;
+  }
+  if (lane_id() == 0) red[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = red[0];
+    for (int i = 1; i < static_cast<int>(blockDim.x) / kWave; i++) m = red[i] < m ? red[i] : m;
+    st->next_bucket = static_cast<long long>(m);
+    if (m >= static_cast<unsigned long long>(INT32_MAX)) st->done = 1;
+    else st->phase = 2;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+sb_split(int32_t *pend, int64_t n, int32_t *near0, int32_t *near1, int64_t cap, BatchState *st) {
+  const volatile BatchState *vs = st;
+  if (vs->done || vs->phase != 2) return;
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  const int cur = vs->cur;
+  const int32_t nb = static_cast<int32_t>(vs->next_bucket);
+  WaveQueue q;
+  q.init(seg + (threadIdx.x / kWave) * kSeg, &st->near_cnt[cur], cur ? near1 : near0, cap, &st->overflow);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
+    const int64_t v = base + threadIdx.x;
+    bool take = false;
+    if (v < n && pend[v] == nb) {
+      pend[v] = INT32_MAX;
+      take = true;
+    }
+    q.push(take, static_cast<int32_t>(v));
+  }
+  q.flush();
+}
+
+__global__ void sb_split_done(BatchState *st) {
+  if (threadIdx.x != 0 || st->done || st->phase != 2) return;
+  st->bucket = st->next_bucket;
+  st->buckets += 1;
+  st->phase = 0;
+}
+
+template <int Kp>
+static void launch_sb_relax(gbk_ctx *ctx, const gbk_graph *g, int32_t *dist, int32_t *pend, uint32_t *vtag,
+                            int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, BatchState *st,
+                            const HeavyQueue &hq, unsigned grid) {
+  hipStream_t s = ctx->stream;
+  sb_relax<Kp><<<grid, kBlock, 0, s>>>(g->off, g->nbr, g->wgt, dist, pend, vtag, near0, near1, cap, delta,
+                                       st, hq);
+  heavy_prefix<<<1, 1024, 0, s>>>(hq);
+  sb_relax_heavy<Kp><<<grid, kBlock, 0, s>>>(g->nbr, g->wgt, dist, pend, vtag, near0, near1, cap, delta,
+                                             st, hq);
+}
+
+}  // namespace gbk
+
+using namespace gbk;
+
+namespace {
+
+// Per-source reached / edge counts (and optional source-major output) from
+// a vertex-major array; adds into the caller's arrays at offset 'first'.
+int collect(gbk_ctx *ctx, const gbk_graph *g, const int32_t *vm, int Kp, int K, int32_t unreached,
+            int32_t *out_dev, int64_t *reached, int64_t *edges, int64_t first, int64_t *edge_total) {
+  hipStream_t s = ctx->stream;
+  const unsigned grid = 1024;
+  auto *part = static_cast<unsigned long long *>(scratch(ctx, kS14, sizeof(unsigned long long) * 2 * grid * kBatchMax + 1024));
+  auto *sums = static_cast<unsigned long long *>(pinned(ctx, sizeof(unsigned long long) * 2 * kBatchMax));
+  if (!part || !sums) return set_error(ctx, GBK_ENOMEM, "batch collect scratch");
+  batch_collect<<<grid, kBlock, 0, s>>>(vm, g->n, Kp, K, unreached, g->off, out_dev, part);
+  batch_collect_sum<<<1, 1024, 0, s>>>(part, static_cast<int>(grid), part + 2 * grid * kBatchMax);
+  GBK_LAUNCH_CHECK(ctx, "batch_collect");
+  GBK_HIP(ctx, hipMemcpyAsync(sums, part + 2 * grid * kBatchMax, sizeof(unsigned long long) * 2 * kBatchMax,
+                              hipMemcpyDeviceToHost, s));
+  GBK_HIP(ctx, hipStreamSynchronize(s));
+  for (int k = 0; k < K; k++) {
+    if (reached) reached[first + k] = static_cast<int64_t>(sums[k]);
+    if (edges) edges[first + k] = static_cast<int64_t>(sums[kBatchMax + k]);
+    *edge_total += static_cast<int64_t>(sums[kBatchMax + k]);
+  }
+  return GBK_OK;
+}
+
+int check_sources(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int num) {
+  if (!ctx || !g || !sources || num <= 0) return set_error(ctx, GBK_EINVAL, "batch: bad arguments");
+  for (int i = 0; i < num; i++)
+    if (sources[i] < 0 || sources[i] >= g->n) return set_error(ctx, GBK_EINVAL, "batch: source out of range");
+  return GBK_OK;
+}
+
+int upload_sources(gbk_ctx *ctx, const int32_t *src, int K, int32_t **dev) {
+  *dev = static_cast<int32_t *>(scratch(ctx, kS13, sizeof(int32_t) * kBatchMax));
+  if (!*dev) return set_error(ctx, GBK_ENOMEM, "batch sources");
+  GBK_HIP(ctx, hipMemcpyAsync(*dev, src, sizeof(int32_t) * K, hipMemcpyHostToDevice, ctx->stream));
+  return GBK_OK;
+}
+
+int msbfs_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, int32_t *depth_host,
+                int64_t *reached, int64_t *edges, int64_t first, gbk_batch_stats *acc) {
+  const int64_t n = g->n;
+  hipStream_t s = ctx->stream;
+  const int Kp = batch_width(K);
+  const int64_t cap = n + 64;
+  auto *seen = static_cast<unsigned long long *>(scratch(ctx, kS0, sizeof(unsigned long long) * (n + 1)));
+  auto *front = static_cast<unsigned long long *>(scratch(ctx, kS1, sizeof(unsigned long long) * (n + 1)));
+  auto *next = static_cast<unsigned long long *>(scratch(ctx, kS2, sizeof(unsigned long long) * (n + 1)));
+  auto *dvm = static_cast<int32_t *>(scratch(ctx, kS3, sizeof(int32_t) * n * Kp + 64));
+  auto *queue = static_cast<int32_t *>(scratch(ctx, kS4, sizeof(int32_t) * cap));
+  auto *c = static_cast<MsCounters *>(scratch(ctx, kS5, sizeof(MsCounters) + 64));
+  auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS6, sizeof(unsigned long long) * 2 * kGridCap));
+  auto *out = depth_host ? static_cast<int32_t *>(scratch(ctx, kS8, sizeof(int32_t) * n * K + 64)) : nullptr;
+  auto *hc = static_cast<MsCounters *>(pinned(ctx, sizeof(MsCounters)));
+  HeavyQueue hq;
+  if (!seen || !front || !next || !dvm || !queue || !c || !partial || !hc || (depth_host && !out) ||
+      !make_heavy_queue(ctx, g, kS7, &hq))
+    return set_error(ctx, GBK_ENOMEM, "bfs batch scratch");
+  hq.overflow = &c->overflow;
+  int32_t *dsrc = nullptr;
+  if (int rc = upload_sources(ctx, src, K, &dsrc)) return rc;
+  const unsigned long long full = K == 64 ? ~0ull : ((1ull << K) - 1);
+  const unsigned n_grid = expand_grid(n);
+  const unsigned bu_grid = expand_grid((n + kBlock / 8 - 1) / (kBlock / 8) * kBlock);
+  const int64_t alpha = ctx->tune.bfs_alpha;
+  EventTimer timer(ctx);
+  fill<unsigned long long>(ctx, seen, 0ull, n);
+  fill<unsigned long long>(ctx, front, 0ull, n);
+  fill<unsigned long long>(ctx, next, 0ull, n);
+  fill<int32_t>(ctx, dvm, -1, n * Kp);
+  msbfs_sources<<<1, kBatchMax, 0, s>>>(dsrc, K, Kp, seen, front, dvm, c);
+  int64_t fedges = 0, levels = 0;
+  for (int32_t level = 1; level <= n + 1; level++) {
+    const int kt = kt_begin(ctx, GBK_KT_BFS_STEP);
+    if (fedges > g->m / alpha) {
+      launch_msbfs_bu(ctx, 8, g, seen, front, next, full, bu_grid);
+    } else {
+      fill<unsigned long long>(ctx, hq.count, 0ull, 1);
+      msbfs_front_queue<<<n_grid, kBlock, 0, s>>>(front, n, queue, cap, c);
+      msbfs_td<<<kGridCap, kBlock, 0, s>>>(g->off, g->nbr, queue, c, front, seen, next,
+                                           ctx->tune.heavy_degree, hq);
+      heavy_prefix<<<1, 1024, 0, s>>>(hq);
+      msbfs_td_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, front, seen, next, hq);
+    }
+    kt_end(ctx, kt);
+    msbfs_finish<<<n_grid, kBlock, 0, s>>>(g->off, n, Kp, seen, front, next, dvm, level, partial,
+                                           partial + kGridCap);
+    msbfs_store<<<1, 1024, 0, s>>>(partial, static_cast<int>(n_grid), c);
+    GBK_LAUNCH_CHECK(ctx, "bfs batch level");
+    GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(MsCounters), hipMemcpyDeviceToHost, s));
+    GBK_HIP(ctx, hipStreamSynchronize(s));
+    if (hc->overflow) return set_error(ctx, GBK_EHIP, "bfs batch: queue overflow");
+    levels++;
+    if (hc->fverts == 0) break;
+    fedges = static_cast<int64_t>(hc->fedges);
+  }
+  int64_t etot = 0;
+  if (int rc = collect(ctx, g, dvm, Kp, K, -1, out, reached, edges, first, &etot)) return rc;
+  const double ms = timer.stop_ms();
+  acc->device_ms += ms;
+  acc->rounds += levels;
+  acc->edges_reached += etot;
+  if (depth_host) {
+    GBK_HIP(ctx, hipMemcpyAsync(depth_host + first * n, out, sizeof(int32_t) * n * K, hipMemcpyDeviceToHost, s));
+    GBK_HIP(ctx, hipStreamSynchronize(s));
+  }
+  return GBK_OK;
+}
+
+int sssp_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, int32_t delta,
+                     int32_t *dist_host, int64_t *reached, int64_t *edges, int64_t first,
+                     gbk_batch_stats *acc) {
+  const int64_t n = g->n;
+  hipStream_t s = ctx->stream;
+  const int Kp = batch_width(K);
+  const int64_t cap = n + 64;
+  auto *dist = static_cast<int32_t *>(scratch(ctx, kS0, sizeof(int32_t) * n * Kp + 64));
+  auto *pend = static_cast<int32_t *>(scratch(ctx, kS1, sizeof(int32_t) * (n + 1)));
+  auto *vtag = static_cast<uint32_t *>(scratch(ctx, kS2, sizeof(uint32_t) * (n + 1)));
+  auto *q = static_cast<int32_t *>(scratch(ctx, kS4, sizeof(int32_t) * cap * 2));
+  auto *st = static_cast<BatchState *>(scratch(ctx, kS5, sizeof(BatchState) + 64));
+  auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS6, sizeof(unsigned long long) * kGridCap));
+  auto *out = dist_host ? static_cast<int32_t *>(scratch(ctx, kS8, sizeof(int32_t) * n * K + 64)) : nullptr;
+  auto *hs = static_cast<BatchState *>(pinned(ctx, sizeof(BatchState)));
+  HeavyQueue hq;
+  if (!dist || !pend || !vtag || !q || !st || !partial || !hs || (dist_host && !out) ||
+      !make_heavy_queue(ctx, g, kS7, &hq))
+    return set_error(ctx, GBK_ENOMEM, "sssp batch scratch");
+  // rows above kBatchHeavy are queued: at most m / kBatchHeavy of them
+  if (hq.cap < std::min<int64_t>(n, g->m / kBatchHeavy + 1) + 1) {
+    const int64_t H = ctx->tune.heavy_degree;
+    ctx->tune.heavy_degree = kBatchHeavy;
+    const bool ok = make_heavy_queue(ctx, g, kS7, &hq);
+    ctx->tune.heavy_degree = H;
+    if (!ok) return set_error(ctx, GBK_ENOMEM, "sssp batch heavy queue");
+  }
+  hq.overflow = &st->overflow;
+  hq.chunk = kBatchChunk;
+  int32_t *near0 = q, *near1 = q + cap;
+  // distinct sources form the first near queue
+  std::vector<int32_t> uniq(src, src + K);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  int32_t *dsrc = nullptr;
+  if (int rc = upload_sources(ctx, src, K, &dsrc)) return rc;
+  const unsigned grid = static_cast<unsigned>(ctx->tune.relax_grid);
+  const unsigned scan_grid = expand_grid(n);
+  EventTimer timer(ctx);
+  GBK_HIP(ctx, hipMemcpyAsync(near0, uniq.data(), sizeof(int32_t) * uniq.size(), hipMemcpyHostToDevice, s));
+  fill<int32_t>(ctx, dist, kBatchInf, n * Kp);
+  fill<int32_t>(ctx, pend, INT32_MAX, n);
+  fill<uint32_t>(ctx, vtag, 0u, n);
+  sb_sources<<<1, kBatchMax, 0, s>>>(dsrc, K, Kp, dist);
+  sb_init_state<<<1, 1, 0, s>>>(st, static_cast<long long>(uniq.size()));
+  int batch = 2;
+  int64_t launched = 0;
+  for (;;) {
+    for (int j = 0; j < batch; j++) {
+      const int kt = kt_begin(ctx, GBK_KT_SSSP_RELAX);
+      switch (Kp) {
+        case 8: launch_sb_relax<8>(ctx, g, dist, pend, vtag, near0, near1, cap, delta, st, hq, grid); break;
+        case 16: launch_sb_relax<16>(ctx, g, dist, pend, vtag, near0, near1, cap, delta, st, hq, grid); break;
+        case 32: launch_sb_relax<32>(ctx, g, dist, pend, vtag, near0, near1, cap, delta, st, hq, grid); break;
+        default: launch_sb_relax<64>(ctx, g, dist, pend, vtag, near0, near1, cap, delta, st, hq, grid); break;
+      }
+      kt_end(ctx, kt);
+      sb_relax_done<<<1, kWave, 0, s>>>(st, hq.count);
+      sb_min_pend<<<scan_grid, kBlock, 0, s>>>(pend, n, st, partial);
+      sb_min_done<<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), st);
+      sb_split<<<scan_grid, kBlock, 0, s>>>(pend, n, near0, near1, cap, st);
+      sb_split_done<<<1, kWave, 0, s>>>(st);
+    }
+    launched += batch;
+    GBK_LAUNCH_CHECK(ctx, "sssp batch step");
+    GBK_HIP(ctx, hipMemcpyAsync(hs, st, sizeof(BatchState), hipMemcpyDeviceToHost, s));
+    GBK_HIP(ctx, hipStreamSynchronize(s));
+    kt_collect(ctx);
+    if (hs->overflow) return set_error(ctx, GBK_EHIP, "sssp batch: queue overflow");
+    if (hs->done) break;
+    if (batch < 64) batch *= 2;
+    if (launched > 4 * (n + 16) + 1024) return set_error(ctx, GBK_EHIP, "sssp batch: no convergence");
+  }
+  const BatchState fin = *hs;  // collect() may regrow the pinned staging buffer
+  int64_t etot = 0;
+  if (int rc = collect(ctx, g, dist, Kp, K, kBatchInf, out, reached, edges, first, &etot)) return rc;
+  const double ms = timer.stop_ms();
+  acc->device_ms += ms;
+  acc->rounds += fin.relax_rounds;
+  acc->buckets += fin.buckets + 1;
+  acc->edges_reached += etot;
+  if (dist_host) {
+    GBK_HIP(ctx, hipMemcpyAsync(dist_host + first * n, out, sizeof(int32_t) * n * K, hipMemcpyDeviceToHost, s));
+    GBK_HIP(ctx, hipStreamSynchronize(s));
+  }
+  return GBK_OK;
+}
+
+}  // namespace
+
+extern "C" int gbk_bfs_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int num_sources,
+                             int32_t *depth, int64_t *reached, int64_t *edges_reached,
+                             gbk_batch_stats *stats) {
+  if (int rc = check_sources(ctx, g, sources, num_sources)) return rc;
+  gbk_batch_stats acc{};
+  for (int first = 0; first < num_sources; first += kBatchMax) {
+    const int K = std::min(kBatchMax, num_sources - first);
+    if (int rc = msbfs_chunk(ctx, g, sources + first, K, depth, reached, edges_reached, first, &acc)) return rc;
+  }
+  if (stats) *stats = acc;
+  return GBK_OK;
+}
+
+extern "C" int gbk_sssp_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int num_sources,
+                              int32_t delta, int32_t *dist, int64_t *reached, int64_t *edges_reached,
+                              gbk_batch_stats *stats) {
+  if (int rc = check_sources(ctx, g, sources, num_sources)) return rc;
+  if (!g->weighted) return set_error(ctx, GBK_ESTATE, "sssp batch: graph has no weights");
+  if (delta <= 0) return set_error(ctx, GBK_EINVAL, "sssp batch: delta must be positive");
+  gbk_batch_stats acc{};
+  for (int first = 0; first < num_sources; first += kBatchMax) {
+    const int K = std::min(kBatchMax, num_sources - first);
+    if (int rc = sssp_batch_chunk(ctx, g, sources + first, K, delta, dist, reached, edges_reached, first, &acc))
+      return rc;
+  }
+  if (stats) *stats = acc;
+  return GBK_OK;
+}

@@ -56,6 +56,11 @@ class BcStats(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("edges_reached", C.c_int64), ("max_depth", C.c_int)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("rounds", C.c_int64), ("buckets", C.c_int64),
+                ("edges_reached", C.c_int64)]
+
+
 class RelabelStats(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("n", C.c_int64), ("m", C.c_int64)]
 
@@ -97,6 +102,8 @@ def lib():
         "gbk_cc": (i32, [VP, VP, i32, VP, C.POINTER(CcStats)]),
         "gbk_sssp": (i32, [VP, VP, C.c_int32, C.c_int32, VP, C.POINTER(SsspStats)]),
         "gbk_bc": (i32, [VP, VP, VP, i32, VP, C.POINTER(BcStats)]),
+        "gbk_bfs_batch": (i32, [VP, VP, VP, i32, VP, VP, VP, C.POINTER(BatchStats)]),
+        "gbk_sssp_batch": (i32, [VP, VP, VP, i32, C.c_int32, VP, VP, VP, C.POINTER(BatchStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name, None)
@@ -293,3 +300,27 @@ class Context:
         st = BcStats()
         self._check(lib().gbk_bc(self.h, g.handle, _ptr(s), len(s), _ptr(out), C.byref(st)))
         return out, st
+
+    def bfs_batch(self, g: DeviceGraph, sources: Sequence[int], download=True):
+        """Depths (sources x n, -1 unreached), per-source reached counts and
+        out-edge sums, stats."""
+        s = np.ascontiguousarray(sources, np.int32)
+        depth = np.empty((len(s), g.n), np.int32) if download else None
+        reached = np.empty(len(s), np.int64)
+        edges = np.empty(len(s), np.int64)
+        st = BatchStats()
+        self._check(lib().gbk_bfs_batch(self.h, g.handle, _ptr(s), len(s), _ptr(depth), _ptr(reached),
+                                        _ptr(edges), C.byref(st)))
+        return depth, reached, edges, st
+
+    def sssp_batch(self, g: DeviceGraph, sources: Sequence[int], delta: int = 1, download=True):
+        """Distances (sources x n, INT32_MAX/2 unreached), per-source reached
+        counts and out-edge sums, stats."""
+        s = np.ascontiguousarray(sources, np.int32)
+        dist = np.empty((len(s), g.n), np.int32) if download else None
+        reached = np.empty(len(s), np.int64)
+        edges = np.empty(len(s), np.int64)
+        st = BatchStats()
+        self._check(lib().gbk_sssp_batch(self.h, g.handle, _ptr(s), len(s), int(delta), _ptr(dist),
+                                         _ptr(reached), _ptr(edges), C.byref(st)))
+        return dist, reached, edges, st

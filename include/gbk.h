@@ -171,6 +171,33 @@ int gbk_sssp(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t delta,
 int gbk_bc(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources,
            int num_sources, float *scores, gbk_bc_stats *stats);
 
+/* ---- multi-source batches ---------------------------------------------- */
+/* Many sources in one traversal; every source's result equals the single-
+ * source call's (BFS depths, SSSP distances).  No reference counterpart as a
+ * single call: the reference runs one source per trial (the trial loop of
+ * BenchmarkKernel, gapbs/benchmark.h:450, around the BFSBound / SSSPBound
+ * lambdas of bench/src/bfs.cc:285-287 and sssp.cc:342-344); these replace
+ * that loop for callers that want the batch's throughput.  Sources are
+ * processed 64 at a time. */
+typedef struct {
+  double device_ms;
+  int64_t rounds;         /* BFS levels or SSSP relaxation rounds, summed */
+  int64_t buckets;        /* SSSP buckets opened (0 for BFS) */
+  int64_t edges_reached;  /* sum over sources of the out-degrees of reached vertices */
+} gbk_batch_stats;
+
+/* depth: host int32[num_sources * n] (source-major, -1 unreached) or NULL;
+ * reached / edges_reached: host int64[num_sources] or NULL. */
+int gbk_bfs_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources,
+                  int num_sources, int32_t *depth, int64_t *reached,
+                  int64_t *edges_reached, gbk_batch_stats *stats);
+/* dist: host int32[num_sources * n] (source-major, INT32_MAX/2 unreached)
+ * or NULL. */
+int gbk_sssp_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources,
+                   int num_sources, int32_t delta, int32_t *dist,
+                   int64_t *reached, int64_t *edges_reached,
+                   gbk_batch_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,146 @@
+"""Multi-source batches against one-source-at-a-time oracles (GPU).
+
+Bars: BFS depths and SSSP distances of every source bit-exact to the
+oracle's single-source BFS / Dijkstra (every delta); per-source reached
+counts and out-edge sums equal to those recomputed from the oracle's
+answer.  Batches cover 1, odd, exactly 64 and more than 64 sources (two
+chunks), repeated sources, unreachable parts and directed graphs.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from graphbrew_amd import gbk
+from tests import bruteforce as B
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "graphs")
+
+
+def _fixture(name, weighted=False, symmetrize=True):
+    u, v, w = B.read_el(os.path.join(GOLD, name), weighted)
+    return O.make_graph(u, v, w, symmetrize=symmetrize)
+
+
+def _star(leaves, weighted=False):
+    u = np.zeros(leaves, np.int32)
+    v = np.arange(1, leaves + 1, dtype=np.int32)
+    w = (np.arange(leaves, dtype=np.int32) % 251 + 1) if weighted else None
+    return O.make_graph(u, v, w)
+
+
+def _directed(n=3000, m=30000, seed=5, weighted=False):
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, n, m).astype(np.int32)
+    v = (rng.zipf(1.4, m) % n).astype(np.int32)
+    w = rng.integers(1, 256, m).astype(np.int32) if weighted else None
+    return O.make_graph(u, v, w, symmetrize=False, n=n)
+
+
+def _sources(g, k, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, g.n, k).astype(np.int32)
+
+
+def _check_counts(g, vals, unreached, reached, edges):
+    deg = np.diff(g.off)
+    for j, row in enumerate(vals):
+        ok = row != unreached
+        assert reached[j] == int(ok.sum())
+        assert edges[j] == int(deg[ok].sum())
+
+
+BFS_GRAPHS = {
+    "kron13": lambda: O.kron(13),
+    "uniform12": lambda: O.kron(12, uniform=True),
+    "star5000": lambda: _star(5000),
+    "path": lambda: _fixture("path.el"),
+    "disconnected": lambda: _fixture("disconnected.el"),
+    "directed": lambda: _directed(),
+}
+
+
+@pytest.mark.parametrize("name", list(BFS_GRAPHS))
+@pytest.mark.parametrize("k", [1, 7, 64, 70])
+def test_bfs_batch_depths(ctx, name, k):
+    g = BFS_GRAPHS[name]()
+    dg = ctx.upload_graph(g)
+    srcs = _sources(g, k, seed=k)
+    srcs[-1] = srcs[0]  # a repeated source
+    depth, reached, edges, st = ctx.bfs_batch(dg, srcs)
+    for j, s in enumerate(srcs):
+        ref = O.bfs_depths(g, int(s))
+        assert np.array_equal(depth[j], ref), (name, k, j, int(s))
+    _check_counts(g, depth, -1, reached, edges)
+    assert st.edges_reached == int(edges.sum())
+    assert st.rounds >= 1
+    dg.free()
+
+
+def test_bfs_batch_matches_single_source_parents(ctx):
+    # the single-source kernel's parents define the same depths
+    g = O.kron(12)
+    dg = ctx.upload_graph(g)
+    srcs = O.source_picker(g, 16)[1]
+    depth, _, edges, _ = ctx.bfs_batch(dg, srcs)
+    for j, s in enumerate(srcs):
+        parent, st = ctx.bfs(dg, int(s))
+        assert O.bfs_verify(g, int(s), parent)
+        assert np.array_equal(parent >= 0, depth[j] >= 0)
+        assert st.edges_reached == edges[j]
+    dg.free()
+
+
+SSSP_GRAPHS = {
+    "kron12w": lambda: O.kron(12, weighted=True),
+    "uniform11w": lambda: O.kron(11, uniform=True, weighted=True),
+    "star3000w": lambda: _star(3000, weighted=True),
+    "weighted_path": lambda: _fixture("weighted_path.wel", weighted=True, symmetrize=False),
+    "directed_w": lambda: _directed(weighted=True),
+}
+
+
+@pytest.mark.parametrize("name", list(SSSP_GRAPHS))
+@pytest.mark.parametrize("k", [1, 3, 17, 64, 65])
+def test_sssp_batch_distances(ctx, name, k):
+    g = SSSP_GRAPHS[name]()
+    dg = ctx.upload_graph(g)
+    srcs = _sources(g, k, seed=100 + k)
+    srcs[-1] = srcs[0]
+    refs = [O.dijkstra(g, int(s)) for s in srcs]
+    for delta in (1, 32, 100000):
+        dist, reached, edges, st = ctx.sssp_batch(dg, srcs, delta)
+        for j in range(k):
+            assert np.array_equal(dist[j], refs[j]), (name, k, delta, j)
+        _check_counts(g, dist, O.K_DIST_INF, reached, edges)
+        assert st.rounds >= 1 and st.buckets >= 1
+    dg.free()
+
+
+def test_sssp_batch_equals_single_source_counts(ctx):
+    g = O.kron(13, weighted=True)
+    dg = ctx.upload_graph(g)
+    srcs = O.source_picker(g, 8)[1]
+    _, reached, edges, _ = ctx.sssp_batch(dg, srcs, 64, download=False)
+    for j, s in enumerate(srcs):
+        _, st = ctx.sssp(dg, int(s), 64, download=False)
+        assert st.reached == reached[j] and st.edges_reached == edges[j]
+    dg.free()
+
+
+def test_batch_rejects_bad_input(ctx):
+    g = O.kron(8, weighted=True)
+    dg = ctx.upload_graph(g)
+    with pytest.raises(gbk.GbkError):
+        ctx.bfs_batch(dg, [0, g.n])
+    with pytest.raises(gbk.GbkError):
+        ctx.sssp_batch(dg, [-1], 4)
+    with pytest.raises(gbk.GbkError):
+        ctx.sssp_batch(dg, [0], 0)
+    gu = ctx.upload_graph(O.kron(8))
+    with pytest.raises(gbk.GbkError):
+        ctx.sssp_batch(gu, [0], 4)
+    gu.free()
+    dg.free()

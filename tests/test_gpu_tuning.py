@@ -148,3 +148,18 @@ def test_autotuner_sssp_and_bfs(ctx, tmp_path):
         ctx.set_param("bfs_alpha", saved[1])
         ctx.set_param("bfs_beta", saved[2])
     dg.free()
+
+
+def test_autotuner_sssp_batch(ctx):
+    from graphbrew_amd.autotune import BATCH_DELTAS
+    g = O.kron(11, weighted=True)
+    dg = ctx.upload_graph(g)
+    t = Autotuner(ctx, repeats=1)
+    srcs = O.source_picker(g, 8)[1]
+    best = t.tune_sssp_batch(dg, srcs)
+    assert best["delta"] in BATCH_DELTAS
+    assert len([r for r in t.log if r["kernel"] == "sssp_batch"]) == len(BATCH_DELTAS)
+    dist = ctx.sssp_batch(dg, srcs, best["delta"])[0]
+    for j, s in enumerate(srcs):
+        assert np.array_equal(dist[j], O.dijkstra(g, int(s)))
+    dg.free()
