@@ -35,6 +35,10 @@ def lib():
             "gbh_mapping_random": (i32, [i64, VP]),
             "gbh_mapping_sort": (i32, [i64, VP, VP]),
             "gbh_mapping_dbg": (i32, [i64, VP, VP]),
+            "gbh_mapping_hubsort": (i32, [i64, VP, VP]),
+            "gbh_mapping_hubcluster": (i32, [i64, VP, VP]),
+            "gbh_mapping_hubsortdbg": (i32, [i64, VP, VP]),
+            "gbh_mapping_hubclusterdbg": (i32, [i64, VP, VP]),
             "gbh_mapping_fingerprint": (C.c_uint64, [i64, VP]),
             "gbh_verify_bfs": (i32, [i64, VP, VP, VP, VP, C.c_int32, VP]),
             "gbh_verify_cc": (i32, [i64, VP, VP, VP, VP, i32, VP]),
@@ -117,6 +121,44 @@ def mapping_dbg(off: np.ndarray) -> np.ndarray:
     out = np.empty(len(off) - 1, np.int32)
     _chk(lib().gbh_mapping_dbg(len(off) - 1, _p(off), _p(out)), "dbg")
     return out
+
+
+def _hub_mapping(name, off):
+    off = np.ascontiguousarray(off, np.int64)
+    out = np.empty(len(off) - 1, np.int32)
+    _chk(getattr(lib(), "gbh_mapping_" + name)(len(off) - 1, _p(off), _p(out)), name)
+    return out
+
+
+def mapping_hubsort(off: np.ndarray) -> np.ndarray:
+    return _hub_mapping("hubsort", off)
+
+
+def mapping_hubcluster(off: np.ndarray) -> np.ndarray:
+    return _hub_mapping("hubcluster", off)
+
+
+def mapping_hubsortdbg(off: np.ndarray) -> np.ndarray:
+    return _hub_mapping("hubsortdbg", off)
+
+
+def mapping_hubclusterdbg(off: np.ndarray) -> np.ndarray:
+    return _hub_mapping("hubclusterdbg", off)
+
+
+def mapping(algo: int, off: np.ndarray) -> np.ndarray:
+    """new_ids of the cheap orderings -o 0..7 (original, random, sort,
+    hubsort, hubcluster, dbg, hubsortdbg, hubclusterdbg)."""
+    n = len(off) - 1
+    if algo == 0:
+        return mapping_identity(n)
+    if algo == 1:
+        return mapping_random(n)
+    fn = {2: mapping_sort, 3: mapping_hubsort, 4: mapping_hubcluster, 5: mapping_dbg,
+          6: mapping_hubsortdbg, 7: mapping_hubclusterdbg}.get(algo)
+    if fn is None:
+        raise ValueError(f"ordering {algo} is not produced here")
+    return fn(off)
 
 
 def mapping_fingerprint(ids) -> str:

@@ -247,10 +247,30 @@ struct Session {
         t.Stop();
         PrintTime("Sort Map Time", t.Seconds());
         break;
+      case 3:
+        gbh_mapping_hubsort(h.n, h.off.data(), ids.data());
+        t.Stop();
+        PrintTime("HubSort Map Time", t.Seconds());
+        break;
+      case 4:
+        gbh_mapping_hubcluster(h.n, h.off.data(), ids.data());
+        t.Stop();
+        PrintTime("HubCluster Map Time", t.Seconds());
+        break;
       case 5:
         gbh_mapping_dbg(h.n, h.off.data(), ids.data());
         t.Stop();
         PrintTime("DBG Map Time", t.Seconds());
+        break;
+      case 6:
+        gbh_mapping_hubsortdbg(h.n, h.off.data(), ids.data());
+        t.Stop();
+        PrintTime("HubSortDBG Map Time", t.Seconds());
+        break;
+      case 7:
+        gbh_mapping_hubclusterdbg(h.n, h.off.data(), ids.data());
+        t.Stop();
+        PrintTime("HubClusterDBG Map Time", t.Seconds());
         break;
       case 13: {
         if (opt.empty()) Die("MAP ordering needs a file: -o 13:<mapping.lo|.so>");

@@ -440,3 +440,104 @@ int gbh_frozen_rule(int bench, double reuse, int64_t n, double avg_degree, doubl
 }
 
 }  // extern "C"
+
+// ---- hub-based mappings (reorder/reorder_hub.h) ---------------------------
+// Hubs are vertices whose degree exceeds the integer average m / n.
+namespace {
+
+int64_t hub_avg(int64_t n, const int64_t *off) {
+  return n > 0 ? (off[n] - off[0]) / n : 0;
+}
+
+// Places hubs[j] at new id j; every other vertex keeps its own id unless
+// that id is taken by a hub, in which case it moves into a position a hub
+// vacated (AssignHubOrderPreservingSourceIds, reorder_hub.h:36-86): a
+// non-hub at position p < h whose slot went to a hub takes the old id of the
+// hub that now sits at p when that hub came from beyond h; the rest fill the
+// remaining vacated positions in order.
+int place_hubs(int64_t n, const std::vector<int32_t> &hubs, int32_t *new_ids) {
+  const int64_t h = static_cast<int64_t>(hubs.size());
+  std::vector<uint8_t> hub(n, 0);
+  std::fill(new_ids, new_ids + n, -1);
+  for (int64_t j = 0; j < h; j++) {
+    hub[hubs[j]] = 1;
+    new_ids[hubs[j]] = static_cast<int32_t>(j);
+  }
+  std::vector<int32_t> spill;
+  for (int64_t v = h; v < n; v++) {
+    if (!hub[v]) {
+      new_ids[v] = static_cast<int32_t>(v);
+      continue;
+    }
+    const int32_t p = new_ids[v];  // the hub's new position, below h
+    if (!hub[p] && new_ids[p] == -1) new_ids[p] = static_cast<int32_t>(v);
+    else spill.push_back(static_cast<int32_t>(v));
+  }
+  size_t next = 0;
+  for (int64_t v = 0; v < h; v++) {
+    if (new_ids[v] != -1) continue;
+    if (next >= spill.size()) return -2;
+    new_ids[v] = spill[next++];
+  }
+  return next == spill.size() ? 0 : -2;
+}
+
+// Hubs ordered by degree, then id, both descending (std::greater on
+// (degree, id) pairs, reorder_hub.h:151-153).
+std::vector<int32_t> hubs_by_degree(int64_t n, const int64_t *off) {
+  const int64_t avg = hub_avg(n, off);
+  std::vector<int32_t> hubs;
+  for (int64_t v = 0; v < n; v++)
+    if (off[v + 1] - off[v] > avg) hubs.push_back(static_cast<int32_t>(v));
+  std::sort(hubs.begin(), hubs.end(), [off](int32_t a, int32_t b) {
+    const int64_t da = off[a + 1] - off[a], db = off[b + 1] - off[b];
+    return da != db ? da > db : a > b;
+  });
+  return hubs;
+}
+
+std::vector<int32_t> hubs_by_id(int64_t n, const int64_t *off) {
+  const int64_t avg = hub_avg(n, off);
+  std::vector<int32_t> hubs;
+  for (int64_t v = 0; v < n; v++)
+    if (off[v + 1] - off[v] > avg) hubs.push_back(static_cast<int32_t>(v));
+  return hubs;
+}
+
+// Hubs first in the given order, then every other vertex in id order
+// (the DBG variants, reorder_hub.h:380-502 and 505-582).
+void hubs_then_rest(int64_t n, const std::vector<int32_t> &hubs, int32_t *new_ids) {
+  std::fill(new_ids, new_ids + n, -1);
+  int32_t next = 0;
+  for (int32_t v : hubs) new_ids[v] = next++;
+  for (int64_t v = 0; v < n; v++)
+    if (new_ids[v] == -1) new_ids[v] = next++;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gbh_mapping_hubsort(int64_t n, const int64_t *off, int32_t *new_ids) {
+  if (n < 0 || (n > 0 && (!off || !new_ids))) return -1;
+  return place_hubs(n, hubs_by_degree(n, off), new_ids);
+}
+
+int gbh_mapping_hubcluster(int64_t n, const int64_t *off, int32_t *new_ids) {
+  if (n < 0 || (n > 0 && (!off || !new_ids))) return -1;
+  return place_hubs(n, hubs_by_id(n, off), new_ids);
+}
+
+int gbh_mapping_hubsortdbg(int64_t n, const int64_t *off, int32_t *new_ids) {
+  if (n < 0 || (n > 0 && (!off || !new_ids))) return -1;
+  hubs_then_rest(n, hubs_by_degree(n, off), new_ids);
+  return 0;
+}
+
+int gbh_mapping_hubclusterdbg(int64_t n, const int64_t *off, int32_t *new_ids) {
+  if (n < 0 || (n > 0 && (!off || !new_ids))) return -1;
+  hubs_then_rest(n, hubs_by_id(n, off), new_ids);
+  return 0;
+}
+
+}  // extern "C"

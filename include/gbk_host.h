@@ -39,6 +39,13 @@ int gbh_mapping_random(int64_t n, int32_t *new_ids);
 int gbh_mapping_sort(int64_t n, const int64_t *off, int32_t *new_ids);
 /* -o 5: degree-based grouping (reorder/reorder_hub.h:267-345). */
 int gbh_mapping_dbg(int64_t n, const int64_t *off, int32_t *new_ids);
+/* Hub mappings of reorder/reorder_hub.h (degree > integer average m/n):
+ * HubSort (-o 3, :125-199), HubCluster (-o 4, :203-245), HubSortDBG
+ * (-o 6, :380-502), HubClusterDBG (-o 7, :505-582). */
+int gbh_mapping_hubsort(int64_t n, const int64_t *off, int32_t *new_ids);
+int gbh_mapping_hubcluster(int64_t n, const int64_t *off, int32_t *new_ids);
+int gbh_mapping_hubsortdbg(int64_t n, const int64_t *off, int32_t *new_ids);
+int gbh_mapping_hubclusterdbg(int64_t n, const int64_t *off, int32_t *new_ids);
 uint64_t gbh_mapping_fingerprint(int64_t n, const int32_t *new_ids);
 /* 0 ok, -1 out-of-range id, -2 duplicate id. */
 int gbh_validate_permutation(int64_t n, const int32_t *ids);

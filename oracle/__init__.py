@@ -230,6 +230,79 @@ def mapping_dbg(g: Graph) -> np.ndarray:
     return out
 
 
+def _hubs(off: np.ndarray):
+    """Vertices with degree above the integer average (reorder_hub.h:141-149)."""
+    n = len(off) - 1
+    deg = np.diff(off)
+    avg = (int(off[-1]) - int(off[0])) // n if n else 0
+    return np.nonzero(deg > avg)[0].astype(np.int64), deg
+
+
+def _assign_hubs(n: int, hubs) -> np.ndarray:
+    """AssignHubOrderPreservingSourceIds, reorder_hub.h:36-86, step by step."""
+    new = np.full(n, -1, np.int64)
+    is_hub = np.zeros(n, bool)
+    for pos, v in enumerate(hubs):
+        is_hub[v] = True
+        new[v] = pos
+    deferred = []
+    for old in range(len(hubs), n):
+        if not is_hub[old]:
+            new[old] = old
+            continue
+        p = new[old]
+        if not is_hub[p] and new[p] == -1:
+            new[p] = old
+        else:
+            deferred.append(old)
+    it = iter(deferred)
+    for old in range(len(hubs)):
+        if new[old] == -1:
+            new[old] = next(it)
+    return new.astype(np.int32)
+
+
+def _hubs_by_degree(off):
+    hubs, deg = _hubs(off)
+    # std::greater on (degree, id) pairs: degree descending, then id descending
+    return sorted(hubs.tolist(), key=lambda v: (-int(deg[v]), -v))
+
+
+def mapping_hubsort(off) -> np.ndarray:
+    """GenerateHubSortMapping, reorder_hub.h:125-199."""
+    off = np.asarray(off, np.int64)
+    return _assign_hubs(len(off) - 1, _hubs_by_degree(off))
+
+
+def mapping_hubcluster(off) -> np.ndarray:
+    """GenerateHubClusterMapping, reorder_hub.h:203-245 (hubs in id order)."""
+    off = np.asarray(off, np.int64)
+    return _assign_hubs(len(off) - 1, _hubs(off)[0].tolist())
+
+
+def _hubs_then_rest(n, hubs):
+    new = np.full(n, -1, np.int64)
+    new[np.asarray(hubs, np.int64)] = np.arange(len(hubs))
+    rest = np.nonzero(new < 0)[0]
+    new[rest] = len(hubs) + np.arange(len(rest))
+    return new.astype(np.int32)
+
+
+def mapping_hubsortdbg(off) -> np.ndarray:
+    """GenerateHubSortDBGMapping, reorder_hub.h:380-502: sorted hubs first,
+    then the other vertices in id order (the per-thread slices are
+    contiguous and numbered in thread order)."""
+    off = np.asarray(off, np.int64)
+    return _hubs_then_rest(len(off) - 1, _hubs_by_degree(off))
+
+
+def mapping_hubclusterdbg(off) -> np.ndarray:
+    """GenerateHubClusterDBGMapping, reorder_hub.h:505-582: two buckets
+    (degree <= avg, above), the upper first, each in id order."""
+    off = np.asarray(off, np.int64)
+    return _hubs_then_rest(len(off) - 1, _hubs(off)[0].tolist())
+
+
 def source_picker(g: Graph, count: int, repeats: int = 1):
     o = np.empty(count, np.int32)
     i = np.empty(count, np.int32)

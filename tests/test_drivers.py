@@ -48,7 +48,7 @@ def test_help_exits_zero():
 
 # ------------------------------------------------------------------ GPU
 @pytest.mark.gpu
-@pytest.mark.parametrize("order", ["0", "1", "2", "5"])
+@pytest.mark.parametrize("order", ["0", "1", "2", "3", "4", "5", "6", "7"])
 def test_bfs_driver_verifies(order):
     r = run("bfs", "-g", 12, "-n", 4, "-v", "-a", "-o", order)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -59,7 +59,9 @@ def test_bfs_driver_verifies(order):
     orig, _ = O.source_picker(g, 4)
     assert [int(x) for x in L["Source Original"]] == orig.tolist()
     ids = {"0": np.arange(g.n, dtype=np.int32), "1": O.mapping_random(g.n),
-           "2": O.mapping_sort(g), "5": O.mapping_dbg(g)}[order]
+           "2": O.mapping_sort(g), "3": O.mapping_hubsort(g.off), "4": O.mapping_hubcluster(g.off),
+           "5": O.mapping_dbg(g), "6": O.mapping_hubsortdbg(g.off),
+           "7": O.mapping_hubclusterdbg(g.off)}[order]
     assert L["Mapping Fingerprint"] == [O.mapping_fingerprint(ids)]
     assert re.search(r"^Average Time:\s+\d+\.\d{5}$", r.stdout, re.M)
 

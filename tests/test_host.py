@@ -131,3 +131,66 @@ def test_host_verifiers_agree_with_oracle():
     assert not H.verify_bc(g.off, g.nbr, srcs, b2)[0]
     s, _, _ = O.pr_gs(g)
     assert H.pr_residual(g.off, g.nbr, s) == pytest.approx(O.pr_residual(g, s), rel=1e-9)
+
+
+# ---- hub mappings (reorder/reorder_hub.h) ---------------------------------
+def _off(degrees):
+    return np.concatenate([[0], np.cumsum(degrees)]).astype(np.int64)
+
+
+def test_hub_mappings_known_answers():
+    # degrees 2,100,5,50 (the example of reorder_hub.h:111-115): average 39,
+    # hubs 1 and 3; vertex 1 gets id 0, vertex 3 id 1
+    off = _off([2, 100, 5, 50])
+    assert H.mapping_hubsort(off).tolist() == [3, 0, 2, 1]
+    assert H.mapping_hubcluster(off).tolist() == [3, 0, 2, 1]
+    # degrees 50,2,100,5: hubs 0 and 2.  HubSort puts 2 first; 2 sits beyond
+    # the hub range and its slot 0 belongs to a hub, so 2 is deferred into
+    # the free slot of vertex 1.  HubCluster keeps hub order 0,2 and vertex 1
+    # takes the slot vertex 2 left.
+    off = _off([50, 2, 100, 5])
+    assert H.mapping_hubsort(off).tolist() == [1, 2, 0, 3]
+    assert H.mapping_hubcluster(off).tolist() == [0, 2, 1, 3]
+    assert H.mapping_hubsortdbg(off).tolist() == [1, 2, 0, 3]
+    assert H.mapping_hubclusterdbg(off).tolist() == [0, 2, 1, 3]
+    # equal degrees: no vertex exceeds the average, every mapping is the identity
+    off = _off([3, 3, 3, 3, 3])
+    for f in (H.mapping_hubsort, H.mapping_hubcluster, H.mapping_hubsortdbg, H.mapping_hubclusterdbg):
+        assert f(off).tolist() == [0, 1, 2, 3, 4]
+
+
+@pytest.mark.parametrize("scale,uniform", [(10, False), (12, False), (11, True)])
+def test_hub_mappings_match_oracle(scale, uniform):
+    g = O.kron(scale, uniform=uniform)
+    for name in ("hubsort", "hubcluster", "hubsortdbg", "hubclusterdbg"):
+        got = getattr(H, "mapping_" + name)(g.off)
+        ref = getattr(O, "mapping_" + name)(g.off)
+        assert np.array_equal(got, ref), name
+        assert H.validate_permutation(got) == 0
+        # hubs (degree above m/n) occupy exactly the first ids
+        deg = np.diff(g.off)
+        hubs = deg > (int(g.off[-1]) // g.n)
+        assert set(got[hubs].tolist()) == set(range(int(hubs.sum())))
+
+
+def test_hubsort_orders_hubs_by_degree():
+    g = O.kron(12)
+    deg = np.diff(g.off)
+    new = H.mapping_hubsortdbg(g.off)
+    inv = np.empty_like(new)
+    inv[new] = np.arange(g.n)
+    h = int((deg > int(g.off[-1]) // g.n).sum())
+    d = deg[inv[:h]]
+    assert np.all(d[:-1] >= d[1:])
+    # the non-hubs keep their relative order
+    rest = inv[h:]
+    assert np.all(np.diff(rest) > 0)
+
+
+def test_mapping_dispatch_covers_cheap_orderings():
+    g = O.kron(10)
+    for a in range(8):
+        ids = H.mapping(a, g.off)
+        assert H.validate_permutation(ids) == 0
+    with pytest.raises(ValueError):
+        H.mapping(12, g.off)
