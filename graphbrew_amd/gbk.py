@@ -103,6 +103,15 @@ def lib():
         "gbk_sssp": (i32, [VP, VP, C.c_int32, C.c_int32, VP, C.POINTER(SsspStats)]),
         "gbk_bc": (i32, [VP, VP, VP, i32, VP, C.POINTER(BcStats)]),
         "gbk_bfs_batch": (i32, [VP, VP, VP, i32, VP, VP, VP, C.POINTER(BatchStats)]),
+        "gbk_graph_features": (i32, [VP, VP, VP, i32]),
+        "gbk_mlp_create": (i32, [VP, i32, C.c_uint64, pp]),
+        "gbk_mlp_destroy": (i32, [VP, VP]),
+        "gbk_mlp_num_params": (i64, [VP]),
+        "gbk_mlp_get_params": (i32, [VP, VP, VP]),
+        "gbk_mlp_set_params": (i32, [VP, VP, VP]),
+        "gbk_mlp_forward": (i32, [VP, VP, VP, i64, VP]),
+        "gbk_mlp_grad": (i32, [VP, VP, VP, VP, i64, VP, C.POINTER(f64)]),
+        "gbk_mlp_train": (i32, [VP, VP, VP, VP, i64, i32, i32, C.c_float, C.c_uint64, VP]),
         "gbk_sssp_batch": (i32, [VP, VP, VP, i32, C.c_int32, VP, VP, VP, C.POINTER(BatchStats)]),
     }
     for name, (res, args) in sig.items():
@@ -119,7 +128,7 @@ def exported_symbols() -> list:
     """Names declared in include/gbk.h (the C ABI surface)."""
     import re
     hdr = open(os.path.join(os.path.dirname(_PKG), "include", "gbk.h")).read()
-    return sorted(set(re.findall(r"^(?:int|const char \*|void \*)\s*\*?(gbk_\w+)\(", hdr, re.M)))
+    return sorted(set(re.findall(r"^(?:u?int64_t|int|const char \*|void \*)\s*\*?(gbk_\w+)\(", hdr, re.M)))
 
 
 @dataclass
@@ -324,3 +333,86 @@ class Context:
         self._check(lib().gbk_sssp_batch(self.h, g.handle, _ptr(s), len(s), int(delta), _ptr(dist),
                                          _ptr(reached), _ptr(edges), C.byref(st)))
         return dist, reached, edges, st
+
+    # -------------------------------------------------- selector support
+    def features(self, g: DeviceGraph) -> np.ndarray:
+        """GBK_NUM_FEATURES graph statistics computed on the device."""
+        out = np.empty(NUM_FEATURES, np.float32)
+        rc = lib().gbk_graph_features(self.h, g.handle, _ptr(out), NUM_FEATURES)
+        if rc < 0:
+            self._check(rc)
+        return out
+
+    def mlp(self, in_dim: int, seed: int = 0) -> "Mlp":
+        return Mlp(self, in_dim, seed)
+
+
+NUM_FEATURES = 12
+FEATURE_NAMES = ("log2_n", "log2_m", "mean_degree", "log2_max_degree", "degree_cv", "hub_edge_share",
+                 "clustering", "log_span", "near64", "near4096", "ws_l2_log2", "ws_mall_log2")
+
+
+class Mlp:
+    """fp32 MLP in -> 64 -> 64 -> 1 (ReLU) living on the device; forward,
+    gradient and Adam training run in the library's own HIP kernels."""
+
+    def __init__(self, ctx: Context, in_dim: int, seed: int = 0):
+        self.ctx = ctx
+        self.in_dim = in_dim
+        h = VP()
+        ctx._check(lib().gbk_mlp_create(ctx.h, int(in_dim), int(seed), C.byref(h)))
+        self.h = h.value
+        self.num_params = int(lib().gbk_mlp_num_params(self.h))
+
+    def close(self):
+        if self.h:
+            lib().gbk_mlp_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _x(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        if X.ndim != 2 or X.shape[1] != self.in_dim:
+            raise ValueError(f"expected rows x {self.in_dim} features")
+        return X
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.empty(self.num_params, np.float32)
+        self.ctx._check(lib().gbk_mlp_get_params(self.ctx.h, self.h, _ptr(p)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.num_params:
+            raise ValueError("parameter count differs")
+        self.ctx._check(lib().gbk_mlp_set_params(self.ctx.h, self.h, _ptr(p)))
+
+    def forward(self, X) -> np.ndarray:
+        X = self._x(X)
+        out = np.empty(len(X), np.float32)
+        self.ctx._check(lib().gbk_mlp_forward(self.ctx.h, self.h, _ptr(X), len(X), _ptr(out)))
+        return out
+
+    def grad(self, X, y):
+        X = self._x(X)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.empty(self.num_params, np.float32)
+        loss = C.c_double(0)
+        self.ctx._check(lib().gbk_mlp_grad(self.ctx.h, self.h, _ptr(X), _ptr(y), len(X), _ptr(g),
+                                           C.byref(loss)))
+        return loss.value, g
+
+    def train(self, X, y, epochs=100, batch=256, lr=1e-3, seed=0) -> np.ndarray:
+        X = self._x(X)
+        y = np.ascontiguousarray(y, np.float32)
+        curve = np.empty(epochs, np.float64)
+        self.ctx._check(lib().gbk_mlp_train(self.ctx.h, self.h, _ptr(X), _ptr(y), len(X), int(epochs),
+                                            int(batch), float(lr), int(seed), _ptr(curve)))
+        return curve

@@ -198,6 +198,43 @@ int gbk_sssp_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources,
                    int64_t *reached, int64_t *edges_reached,
                    gbk_batch_stats *stats);
 
+/* ---- learned ordering selector ------------------------------------------ */
+/* No reference counterpart as a call: the reference's selector computes its
+ * features on the CPU (ComputeTier0SampledGraphFeatures, reorder/
+ * reorder_types.h:6066-6195) and applies a frozen threshold rule
+ * (reorder/reorder_adaptive.h:302-333).  These compute graph statistics on
+ * the device and train / run a small fp32 MLP (in -> 64 -> 64 -> 1, ReLU)
+ * with this library's own kernels. */
+#define GBK_NUM_FEATURES 12
+/* Writes GBK_NUM_FEATURES floats: log2 n, log2 m, mean degree, log2(1 +
+ * max degree), degree coefficient of variation, share of edges owned by
+ * rows above the mean degree, sampled local clustering coefficient (1024
+ * vertices, first 64 neighbours), mean log2(1 + |u - v|) / log2 n over all
+ * edges, share of edges with |u - v| < 64 and < 4096, log2 of the working
+ * set over 4 MiB (one XCD's L2) and over 256 MiB (Infinity Cache).  Returns
+ * GBK_NUM_FEATURES, or a negative error code. */
+int gbk_graph_features(gbk_ctx *ctx, const gbk_graph *g, float *features, int count);
+
+typedef struct gbk_mlp gbk_mlp;
+/* in_dim in [1, 48]; Glorot-uniform weights from 'seed', zero biases. */
+int gbk_mlp_create(gbk_ctx *ctx, int in_dim, uint64_t seed, gbk_mlp **out);
+int gbk_mlp_destroy(gbk_ctx *ctx, gbk_mlp *m);
+/* Parameters as one flat array: W1[64][in], b1[64], W2[64][64], b2[64],
+ * w3[64], b3. */
+int64_t gbk_mlp_num_params(const gbk_mlp *m);
+int gbk_mlp_get_params(gbk_ctx *ctx, const gbk_mlp *m, float *params);
+int gbk_mlp_set_params(gbk_ctx *ctx, gbk_mlp *m, const float *params);
+/* X: rows x in_dim (row-major host floats); pred: rows. */
+int gbk_mlp_forward(gbk_ctx *ctx, const gbk_mlp *m, const float *X, int64_t rows, float *pred);
+/* Mean squared error over the rows and its gradient w.r.t. the parameters
+ * (grad may be NULL). */
+int gbk_mlp_grad(gbk_ctx *ctx, gbk_mlp *m, const float *X, const float *y, int64_t rows,
+                 float *grad, double *loss);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over shuffled minibatches; optional
+ * mean loss per epoch. */
+int gbk_mlp_train(gbk_ctx *ctx, gbk_mlp *m, const float *X, const float *y, int64_t rows,
+                  int epochs, int batch, float lr, uint64_t seed, double *loss_per_epoch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -452,3 +452,52 @@ def modeled_property_bytes(bench: int, n: int, m: int) -> int:
 def frozen_rule(bench: int, reuse: float, n: int, avg_degree: float, degree_cv: float,
                 wsr: float) -> bool:
     return bool(lib().orc_frozen_rule(bench, reuse, n, avg_degree, degree_cv, wsr))
+
+
+# ---- selector MLP (test infrastructure for the device kernels) -----------
+HIDDEN = 64
+
+
+def mlp_unpack(p: np.ndarray, d: int):
+    """Flat parameters W1[64][d], b1, W2[64][64], b2, w3, b3 as float64."""
+    p = np.asarray(p, np.float64)
+    i = 0
+    W1 = p[i:i + HIDDEN * d].reshape(HIDDEN, d); i += HIDDEN * d
+    b1 = p[i:i + HIDDEN]; i += HIDDEN
+    W2 = p[i:i + HIDDEN * HIDDEN].reshape(HIDDEN, HIDDEN); i += HIDDEN * HIDDEN
+    b2 = p[i:i + HIDDEN]; i += HIDDEN
+    w3 = p[i:i + HIDDEN]; i += HIDDEN
+    b3 = p[i]
+    return W1, b1, W2, b2, w3, b3
+
+
+def mlp_forward(p, d, X):
+    W1, b1, W2, b2, w3, b3 = mlp_unpack(p, d)
+    a1 = np.maximum(np.asarray(X, np.float64) @ W1.T + b1, 0)
+    a2 = np.maximum(a1 @ W2.T + b2, 0)
+    return a2 @ w3 + b3
+
+
+def mlp_grad(p, d, X, y):
+    """Mean squared error and its gradient, by the chain rule in float64."""
+    X = np.asarray(X, np.float64)
+    y = np.asarray(y, np.float64)
+    W1, b1, W2, b2, w3, b3 = mlp_unpack(p, d)
+    z1 = X @ W1.T + b1
+    a1 = np.maximum(z1, 0)
+    z2 = a1 @ W2.T + b2
+    a2 = np.maximum(z2, 0)
+    yh = a2 @ w3 + b3
+    B = len(X)
+    e = yh - y
+    gy = 2.0 * e / B
+    gw3 = a2.T @ gy
+    gb3 = gy.sum()
+    dz2 = np.outer(gy, w3) * (z2 > 0)
+    gW2 = dz2.T @ a1
+    gb2 = dz2.sum(0)
+    dz1 = (dz2 @ W2) * (z1 > 0)
+    gW1 = dz1.T @ X
+    gb1 = dz1.sum(0)
+    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gw3, [gb3]])
+    return float((e * e).mean()), g

@@ -477,3 +477,33 @@ def test_frozen_rule_truth_table():
     assert O.modeled_property_bytes(1, 10, 100) == 80
     assert O.modeled_property_bytes(4, 10, 100) == 440
     assert O.modeled_property_bytes(5, 10, 100) == 10 * 20 + 13
+
+
+# ---- selector MLP oracle pinned by finite differences --------------------
+def test_mlp_oracle_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    d, B = 5, 9
+    P = 64 * d + 64 + 64 * 64 + 64 + 64 + 1
+    p = rng.normal(0, 0.3, P)
+    X = rng.normal(0, 1, (B, d))
+    y = rng.normal(0, 1, B)
+    loss, g = O.mlp_grad(p, d, X, y)
+    assert np.isclose(loss, ((O.mlp_forward(p, d, X) - y) ** 2).mean())
+    idx = rng.choice(P, 60, replace=False)
+    idx = np.concatenate([idx, [P - 1, P - 2, 0, 64 * d]])  # b3, w3, W1, b1 always
+    for i in idx:
+        e = np.zeros(P)
+        e[i] = 1e-6
+        num = (O.mlp_grad(p + e, d, X, y)[0] - O.mlp_grad(p - e, d, X, y)[0]) / 2e-6
+        assert abs(num - g[i]) <= 1e-6 + 1e-5 * abs(num), i
+
+
+def test_mlp_oracle_known_network():
+    # a network whose hidden units copy a positive input: y = sum(x) + b3
+    d = 3
+    W1 = np.zeros((64, d)); W1[:d] = np.eye(d)
+    W2 = np.zeros((64, 64)); W2[:d, :d] = np.eye(d)
+    w3 = np.zeros(64); w3[:d] = 1
+    p = np.concatenate([W1.ravel(), np.zeros(64), W2.ravel(), np.zeros(64), w3, [0.5]])
+    X = np.array([[1.0, 2.0, 3.0], [0.5, 0.0, 4.0]])
+    assert np.allclose(O.mlp_forward(p, d, X), [6.5, 5.0])
