@@ -72,15 +72,14 @@ struct LinkEdge {
 // component links its remaining edges (skip = rounds for the out-CSR, 0 for
 // the in-CSR of a directed graph).
 __global__ void __launch_bounds__(kBlock)
-cc_final_link(const int64_t *off, const int32_t *nbr, int64_t n, int64_t skip, int32_t c,
-              int32_t *comp, int64_t heavy_deg) {
+cc_final_link(RowTiles rt, const int32_t *nbr, int64_t skip, int32_t c, int32_t *comp) {
   LinkEdge f{nbr, comp};
   auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
-    s = off[u] + skip;
-    l = off[u + 1] - s;
+    s = rt.off[u] + skip;
+    l = rt.off[u + 1] - s;
     if (l < 0 || ld_agent(&comp[u]) == c) l = 0;
   };
-  expand_all(n, heavy_deg, rowlen, f);
+  expand_rows(rt, rowlen, f);
 }
 
 struct HookEdge {
@@ -100,14 +99,13 @@ struct HookEdge {
 
 // SV hook over every edge (cc_sv.cc:86-106).
 __global__ void __launch_bounds__(kBlock)
-cc_sv_hook(const int64_t *off, const int32_t *nbr, int64_t n, int32_t *comp, int64_t heavy_deg,
-           int *change) {
+cc_sv_hook(RowTiles rt, const int32_t *nbr, int32_t *comp, int *change) {
   HookEdge f{nbr, comp, 0};
   auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
-    s = off[u];
-    l = off[u + 1] - s;
+    s = rt.off[u];
+    l = rt.off[u + 1] - s;
   };
-  expand_all(n, heavy_deg, rowlen, f);
+  expand_rows(rt, rowlen, f);
   if (f.changed) *change = 1;
 }
 
@@ -134,11 +132,11 @@ int gbk_cc_device(gbk_ctx *ctx, const gbk_graph *g, int variant, int32_t *comp,
                   gbk_cc_stats *stats) {
   const int64_t n = g->n;
   hipStream_t s = ctx->stream;
-  const int64_t H = ctx->tune.heavy_degree;
   EventTimer timer(ctx);
   cc_init<<<grid_for(n), kBlock, 0, s>>>(comp, n);
   int iterations = 0;
   int32_t skipped = -1;
+  const RowTiles out_rows = row_tiles(g, false);
   if (variant == GBK_CC_AFFOREST) {
     const int rounds = 2;
     for (int r = 0; r < rounds; r++) {
@@ -160,9 +158,11 @@ int gbk_cc_device(gbk_ctx *ctx, const gbk_graph *g, int variant, int32_t *comp,
     for (auto it = counts.begin(); it != counts.end(); ++it)
       if (best->second < it->second) best = it;
     skipped = best->first;
-    cc_final_link<<<expand_grid(n), kBlock, 0, s>>>(g->off, g->nbr, n, rounds, skipped, comp, H);
-    if (g->directed)
-      cc_final_link<<<expand_grid(n), kBlock, 0, s>>>(g->in_off, g->in_nbr, n, 0, skipped, comp, H);
+    cc_final_link<<<rows_grid(out_rows), kBlock, 0, s>>>(out_rows, g->nbr, rounds, skipped, comp);
+    if (g->directed) {
+      const RowTiles in_rows = row_tiles(g, true);
+      cc_final_link<<<rows_grid(in_rows), kBlock, 0, s>>>(in_rows, g->in_nbr, 0, skipped, comp);
+    }
     cc_compress<<<grid_for(n), kBlock, 0, s>>>(comp, n);
     GBK_LAUNCH_CHECK(ctx, "afforest");
   } else {
@@ -171,7 +171,7 @@ int gbk_cc_device(gbk_ctx *ctx, const gbk_graph *g, int variant, int32_t *comp,
     if (!change || !hc) return set_error(ctx, GBK_ENOMEM, "sv flag");
     for (;;) {
       fill<int>(ctx, change, 0, 1);
-      cc_sv_hook<<<expand_grid(n), kBlock, 0, s>>>(g->off, g->nbr, n, comp, H, change);
+      cc_sv_hook<<<rows_grid(out_rows), kBlock, 0, s>>>(out_rows, g->nbr, comp, change);
       cc_compress<<<grid_for(n), kBlock, 0, s>>>(comp, n);
       GBK_LAUNCH_CHECK(ctx, "sv");
       GBK_HIP(ctx, hipMemcpyAsync(hc, change, sizeof(int), hipMemcpyDeviceToHost, s));

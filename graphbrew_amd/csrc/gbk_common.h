@@ -37,6 +37,7 @@ struct Tuning {
   int relax_grid = 1024;        // workgroups of a frontier (queue) launch
 };
 constexpr int kGridCapRuntime = 2048;  // upper bound of relax_grid
+constexpr int64_t kHeavyChunkEdges = 1024;  // heavy-row chunk of the all-rows kernels
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -152,6 +153,13 @@ struct gbk_graph {
   int64_t n_heavy_in = 0;
   int64_t max_out_degree = 0;
   int64_t max_in_degree = 0;
+  int64_t heavy_deg = 0;       // threshold the heavy lists were built with
+  // Edge-balanced work units of the all-rows kernels, per CSR: tile starts
+  // (ntiles + 1 entries, the last = n; a tile has at most 64 rows and about
+  // 2048 light edges) and the chunk prefix of the heavy rows (nheavy + 1).
+  int32_t *tiles_out = nullptr, *tiles_in = nullptr;
+  int64_t ntiles_out = 0, ntiles_in = 0;
+  int64_t *hchunk_out = nullptr, *hchunk_in = nullptr;
   std::vector<void *> owned;   // device allocations freed with the graph
 };
 

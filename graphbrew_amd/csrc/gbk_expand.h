@@ -332,6 +332,85 @@ __device__ void expand_heavy_batch(const HeavyQueue &hq, F &f) {
   }
 }
 
+// ------------------------------------------------------- all rows, balanced
+// Work units of an all-rows pass over one CSR, fixed per graph: heavy rows
+// (degree above the graph's threshold) cut into kHeavyChunkEdges-edge chunks
+// numbered by hprefix, and tiles of at most 64 rows and about 2048 light
+// edges.  Workgroups take heavy chunks, then waves take tiles, so the work of
+// a unit is bounded whatever the order of the rows (a window of consecutive
+// high-degree rows no longer lands on one wave).
+struct RowTiles {
+  const int64_t *off;
+  const int32_t *tiles;    // ntiles + 1 starts
+  int64_t ntiles;
+  const int32_t *heavy;    // heavy rows
+  const int64_t *hprefix;  // nheavy + 1 chunk prefix
+  int64_t nheavy;
+  int64_t heavy_deg;
+};
+
+inline RowTiles row_tiles(const gbk_graph *g, bool in_csr) {
+  RowTiles t;
+  t.off = in_csr ? g->in_off : g->off;
+  t.tiles = in_csr ? g->tiles_in : g->tiles_out;
+  t.ntiles = in_csr ? g->ntiles_in : g->ntiles_out;
+  t.heavy = in_csr ? g->heavy_in : g->heavy_out;
+  t.hprefix = in_csr ? g->hchunk_in : g->hchunk_out;
+  t.nheavy = in_csr ? g->n_heavy_in : g->n_heavy_out;
+  t.heavy_deg = g->heavy_deg;
+  return t;
+}
+
+// Heavy chunk gc of the row list: the row's index h in the list and the
+// chunk's edge range [first, last) relative to the row's start.
+__device__ __forceinline__ void heavy_chunk_of(const RowTiles &rt, int64_t gc, int64_t &h, int64_t &first,
+                                               int64_t &last) {
+  int64_t lo = 0, hi = rt.nheavy;  // last row with hprefix[row] <= gc
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (rt.hprefix[mid] <= gc) lo = mid;
+    else hi = mid;
+  }
+  h = lo;
+  const int32_t u = rt.heavy[lo];
+  const int64_t deg = rt.off[u + 1] - rt.off[u];
+  first = (gc - rt.hprefix[lo]) * kHeavyChunkEdges;
+  last = min(deg, first + kHeavyChunkEdges);
+}
+
+// Every row of the CSR: rowlen(row, &s, &l) gives the edge range to walk
+// (a sub-range of the row, or l = 0 to skip it); heavy rows are walked by
+// whole workgroups chunk by chunk (positions relative to s, clipped to l),
+// light rows by waves tile by tile.
+template <class RowLen, class F>
+__device__ void expand_rows(const RowTiles &rt, RowLen rowlen, F &f) {
+  const int64_t chunks = rt.nheavy ? rt.hprefix[rt.nheavy] : 0;
+  for (int64_t gc = blockIdx.x; gc < chunks; gc += gridDim.x) {
+    int64_t h, first, last;
+    heavy_chunk_of(rt, gc, h, first, last);
+    const int64_t u = rt.heavy[h];
+    int64_t s, l;
+    rowlen(u, s, l);
+    const int64_t hi = min(last, l);
+    for (int64_t base = first; base < hi; base += blockDim.x) {
+      const int64_t e = base + threadIdx.x;
+      const bool active = e < hi;
+      f(u, active ? s + e : 0, active);
+    }
+  }
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave; t < rt.ntiles;
+       t += waves) {
+    const int64_t r = rt.tiles[t] + lane_id();
+    int64_t start = 0, len = 0;
+    if (r < rt.tiles[t + 1]) {
+      const int64_t deg = rt.off[r + 1] - rt.off[r];
+      if (deg <= rt.heavy_deg) rowlen(r, start, len);
+    }
+    wave_walk(r, start, len, f);
+  }
+}
+
 // Same over all rows [0, n) (row = list index).
 template <class RowLen, class F>
 __device__ void expand_all(int64_t n, int64_t heavy_deg, RowLen rowlen, F &f) {
@@ -371,6 +450,12 @@ inline unsigned expand_grid(int64_t items) {
   if (g < 1) g = 1;
   if (g > kGridCap) g = kGridCap;
   return static_cast<unsigned>(g);
+}
+
+// Number of workgroups for a balanced all-rows pass.
+inline unsigned rows_grid(const RowTiles &rt) {
+  const int64_t units = rt.ntiles / kWavesPerBlock + 1;
+  return static_cast<unsigned>(units > kGridCap ? kGridCap : units);
 }
 
 // Per-wave queue staged in LDS.  push() is called by every lane of the wave

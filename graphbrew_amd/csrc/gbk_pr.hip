@@ -7,17 +7,21 @@
 // of the next iteration is fused into this iteration's store (contrib_next =
 // score / out_degree), which is the same float the reference recomputes.
 //
-// One launch per iteration plus a one-workgroup finish launch.  A block
-// takes either a heavy row (more than heavy_degree in-edges, summed by the
-// whole workgroup) or 4 windows of 64 consecutive vertices, one per wave;
-// a window's in-edges are contiguous in the CSR, so the wave streams them in
-// 64-edge chunks with coalesced neighbour reads, maps each edge to its
-// owner lane by a 6-step shuffle search over the window's edge prefix,
-// gathers contrib[v] and accumulates into the owner's LDS slot.  Blocks
-// grid-stride so that only ~2K per-block partial L1 changes are written;
-// the finish launch adds them, decides convergence and raises a device flag
-// that makes the remaining queued launches return at once -- no host
-// synchronisation inside the iteration loop and no same-address atomics.
+// One launch per iteration plus a one-workgroup finish launch.  The pull
+// launch works on the graph's fixed, edge-balanced units: heavy in-rows (more
+// than the graph's heavy threshold) in 1024-edge chunks, each summed by a
+// workgroup and added into the row's accumulator, then tiles of at most 64
+// rows and about 2048 edges, one per wave.  A tile's in-edges are contiguous
+// in the CSR, so the wave streams them in 64-edge chunks with coalesced
+// neighbour reads, maps each edge to its owner lane by a 6-step shuffle
+// search over the tile's edge prefix, gathers contrib[v] and accumulates
+// into the owner's LDS slot.  However the rows are ordered (degree-sorted
+// orders put all hubs side by side), no unit is larger than a chunk or a
+// tile.  The finish launch applies the heavy rows, adds the per-workgroup L1
+// changes, decides convergence and raises a device flag that makes the
+// remaining queued launches return at once -- no host synchronisation
+// inside the iteration loop and no same-address atomics beyond one per
+// heavy chunk.
 #include "gbk_expand.h"
 
 namespace gbk {
@@ -46,14 +50,29 @@ __global__ void pr_init(const int64_t *out_off, int64_t n, float init, float *sc
   }
 }
 
-// One workgroup: sums the per-workgroup L1 changes of iteration 'iter' and
-// decides convergence (pr.cc:77-88).
+// One workgroup: applies the heavy rows (their sums were accumulated chunk
+// by chunk into hacc by the pull launch), adds their L1 change to the
+// per-workgroup partials of iteration 'iter' and decides convergence
+// (pr.cc:77-88).
 __global__ void pr_finish(const double *partial, int count, PrState *st, double *errs, int iter,
-                          double tol, int fixed) {
+                          double tol, int fixed, const int32_t *heavy, int64_t nheavy, float *hacc,
+                          const int64_t *out_off, float *scores, float *contrib_next, float base,
+                          float damp) {
   if (*reinterpret_cast<volatile int *>(&st->done)) return;
   __shared__ double red[1024 / kWave];
   double s = 0;
   for (int i = threadIdx.x; i < count; i += blockDim.x) s += partial[i];
+  for (int64_t h = threadIdx.x; h < nheavy; h += blockDim.x) {
+    const int32_t u = heavy[h];
+    const float sum = hacc[h];
+    hacc[h] = 0.0f;
+    const float old = scores[u];
+    const float nw = base + damp * sum;
+    scores[u] = nw;
+    s += fabs(static_cast<double>(nw - old));
+    const int64_t d = out_off[u + 1] - out_off[u];
+    contrib_next[u] = d ? nw / static_cast<float>(d) : 0.0f;
+  }
   s = wave_sum(s);
   if (lane_id() == 0) red[threadIdx.x / kWave] = s;
   __syncthreads();
@@ -68,47 +87,40 @@ __global__ void pr_finish(const double *partial, int count, PrState *st, double 
   }
 }
 
-// Grid-stride over [heavy rows][64-row windows per wave].
+// Heavy in-rows chunk by chunk (workgroup sums added into hacc), then light
+// rows tile by tile (one wave per tile of at most 64 rows, the owner of each
+// edge found by a shuffle search over the tile's edge prefix).
 template <bool GS>
 __global__ void __launch_bounds__(kBlock)
-pr_pull_iter(const int64_t *in_off, const int32_t *in_nbr, const int64_t *out_off, int64_t n,
-             const int32_t *heavy, int64_t nheavy, int64_t heavy_deg, float *scores,
+pr_pull_iter(RowTiles rt, const int32_t *in_nbr, const int64_t *out_off, float *scores,
              const float *contrib_cur, float *contrib_next, const PrState *st, double *partial,
-             float base, float damp) {
+             float *hacc, float base, float damp) {
   if (*reinterpret_cast<const volatile int *>(&st->done)) return;
   __shared__ float acc[kBlock];
   __shared__ float hred[kWavesPerBlock];
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
-  const int64_t windows = (n + kBlock - 1) / kBlock;
-  const int64_t items = nheavy + windows;
+  const int64_t chunks = rt.nheavy ? rt.hprefix[rt.nheavy] : 0;
+  for (int64_t gc = blockIdx.x; gc < chunks; gc += gridDim.x) {
+    int64_t h, first, last;
+    heavy_chunk_of(rt, gc, h, first, last);
+    const int64_t s = rt.off[rt.heavy[h]];
+    float part = 0;
+    for (int64_t k = first + threadIdx.x; k < last; k += blockDim.x) part += contrib_cur[in_nbr[s + k]];
+    const float sum = block_sum(part, hred);
+    if (threadIdx.x == 0) atomicAdd(&hacc[h], sum);
+  }
   double err = 0;
-  for (int64_t b = blockIdx.x; b < items; b += gridDim.x) {
-    if (b < nheavy) {
-      // one heavy row per workgroup
-      const int32_t u = heavy[b];
-      const int64_t s = in_off[u], e = in_off[u + 1];
-      float part = 0;
-      for (int64_t k = s + threadIdx.x; k < e; k += blockDim.x) part += contrib_cur[in_nbr[k]];
-      const float sum = block_sum(part, hred);
-      if (threadIdx.x == 0) {
-        const float old = scores[u];
-        const float nw = base + damp * sum;
-        scores[u] = nw;
-        err += fabs(static_cast<double>(nw - old));
-        const int64_t d = out_off[u + 1] - out_off[u];
-        contrib_next[u] = d ? nw / static_cast<float>(d) : 0.0f;
-      }
-      continue;
-    }
-    const int64_t u = ((b - nheavy) * kWavesPerBlock + wv) * kWave + lane;
-    const bool valid = u < n;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wv; t < rt.ntiles; t += waves) {
+    const int64_t u = rt.tiles[t] + lane;
+    const bool valid = u < rt.tiles[t + 1];
     int64_t start = 0, len = 0;
     if (valid) {
-      start = in_off[u];
-      len = in_off[u + 1] - start;
+      start = rt.off[u];
+      len = rt.off[u + 1] - start;
     }
-    const bool light = valid && len <= heavy_deg;
+    const bool light = valid && len <= rt.heavy_deg;
     if (!light) len = 0;
     acc[threadIdx.x] = 0.0f;
     const int64_t incl = wave_incl_scan(len);
@@ -158,35 +170,38 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   const int64_t n = g->n;
   if (n == 0) return set_error(ctx, GBK_EINVAL, "pr: empty graph");
   hipStream_t s = ctx->stream;
-  const int64_t windows = (n + kBlock - 1) / kBlock;
-  const unsigned grid = expand_grid((windows + g->n_heavy_in) * kBlock);
+  const RowTiles rt = row_tiles(g, true);
+  const unsigned grid = rows_grid(rt);
   auto *scores = static_cast<float *>(scratch(ctx, kS0, sizeof(float) * n));
   auto *c0 = static_cast<float *>(scratch(ctx, kS1, sizeof(float) * n));
   auto *c1 = static_cast<float *>(scratch(ctx, kS2, sizeof(float) * n));
   auto *st = static_cast<PrState *>(scratch(ctx, kS3, sizeof(PrState)));
   auto *errs = static_cast<double *>(scratch(ctx, kS4, sizeof(double) * max_iters));
   auto *partial = static_cast<double *>(scratch(ctx, kS5, sizeof(double) * grid));
-  if (!scores || !c0 || !c1 || !st || !errs || !partial) return set_error(ctx, GBK_ENOMEM, "pr scratch");
+  auto *hacc = static_cast<float *>(scratch(ctx, kS6, sizeof(float) * (rt.nheavy + 1)));
+  if (!scores || !c0 || !c1 || !st || !errs || !partial || !hacc)
+    return set_error(ctx, GBK_ENOMEM, "pr scratch");
   const float kDamp = 0.85f;
   const float init = 1.0f / static_cast<float>(n);
   const float base = (1.0f - kDamp) / static_cast<float>(n);
   EventTimer timer(ctx);
+  fill<float>(ctx, hacc, 0.0f, rt.nheavy + 1);
   pr_init<<<grid_for(n), kBlock, 0, s>>>(g->off, n, init, scores, c0, st);
   for (int it = 0; it < max_iters; it++) {
     const int kt = kt_begin(ctx, GBK_KT_PR_ITER);
+    float *cur = c0, *nxt = c0;
     if (variant == GBK_PR_GS) {
-      pr_pull_iter<true><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->off, n, g->heavy_in,
-                                                 g->n_heavy_in, ctx->tune.heavy_degree, scores, c0,
-                                                 c0, st, partial, base, kDamp);
+      pr_pull_iter<true><<<grid, kBlock, 0, s>>>(rt, g->in_nbr, g->off, scores, c0, c0, st, partial, hacc,
+                                                 base, kDamp);
     } else {
-      float *cur = (it & 1) ? c1 : c0;
-      float *nxt = (it & 1) ? c0 : c1;
-      pr_pull_iter<false><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->off, n, g->heavy_in,
-                                                  g->n_heavy_in, ctx->tune.heavy_degree, scores, cur,
-                                                  nxt, st, partial, base, kDamp);
+      cur = (it & 1) ? c1 : c0;
+      nxt = (it & 1) ? c0 : c1;
+      pr_pull_iter<false><<<grid, kBlock, 0, s>>>(rt, g->in_nbr, g->off, scores, cur, nxt, st, partial,
+                                                  hacc, base, kDamp);
     }
     kt_end(ctx, kt);
-    pr_finish<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), st, errs, it, tol, fixed_work);
+    pr_finish<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), st, errs, it, tol, fixed_work, rt.heavy,
+                                 rt.nheavy, hacc, g->off, scores, nxt, base, kDamp);
   }
   GBK_LAUNCH_CHECK(ctx, "pr_pull_iter");
   const double ms = timer.stop_ms();

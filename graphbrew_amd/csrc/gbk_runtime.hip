@@ -256,6 +256,73 @@ int build_heavy_list(gbk_ctx *ctx, const int64_t *off, int64_t n, int64_t thresh
   return GBK_OK;
 }
 
+// ------------------------------------------------------------------ tiles
+constexpr int64_t kTileRows = 64;
+constexpr int64_t kTileEdges = 2048;
+
+__global__ void light_degrees(const int64_t *off, int64_t n, int64_t threshold, int64_t *deg) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < n) {
+    const int64_t d = off[v + 1] - off[v];
+    deg[v] = d > threshold ? 0 : d;
+  }
+}
+
+// A tile starts every kTileRows rows and wherever the light-edge prefix
+// crosses a multiple of kTileEdges.
+__global__ void tile_flags(const int64_t *lp, int64_t n, int32_t *flags) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < n) flags[v] = (v % kTileRows == 0) || (lp[v] / kTileEdges != lp[v - 1] / kTileEdges);
+}
+
+__global__ void append_sentinel(int32_t *tiles, int64_t at, int64_t n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) tiles[at] = static_cast<int32_t>(n);
+}
+
+__global__ void heavy_chunk_counts(const int64_t *off, const int32_t *heavy, int64_t nheavy, int64_t *cnt) {
+  const int64_t h = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (h < nheavy) {
+    const int64_t d = off[heavy[h] + 1] - off[heavy[h]];
+    cnt[h] = (d + kHeavyChunkEdges - 1) / kHeavyChunkEdges;
+  }
+}
+
+int build_tiles(gbk_ctx *ctx, const int64_t *off, int64_t n, int64_t threshold, const int32_t *heavy,
+                int64_t nheavy, int32_t **tiles, int64_t *ntiles, int64_t **hchunk,
+                std::vector<void *> &owned) {
+  hipStream_t s = ctx->stream;
+  auto *deg = static_cast<int64_t *>(scratch(ctx, kS12, sizeof(int64_t) * (n + 1)));
+  auto *lp = static_cast<int64_t *>(scratch(ctx, kS11, sizeof(int64_t) * (n + 1)));
+  auto *flags = static_cast<int32_t *>(scratch(ctx, kS13, sizeof(int32_t) * (n + 1)));
+  auto *tmp = static_cast<int32_t *>(scratch(ctx, kS10, sizeof(int32_t) * (n + 2)));
+  if (!deg || !lp || !flags || !tmp) return set_error(ctx, GBK_ENOMEM, "tile scratch");
+  int64_t cnt = 0;
+  if (n > 0) {
+    light_degrees<<<grid_for(n), kBlock, 0, s>>>(off, n, threshold, deg);
+    if (int rc = exclusive_scan_i64(ctx, deg, lp, n)) return rc;
+    tile_flags<<<grid_for(n), kBlock, 0, s>>>(lp, n, flags);
+    GBK_LAUNCH_CHECK(ctx, "tile_flags");
+    if (int rc = compact_flags(ctx, flags, n, tmp, &cnt)) return rc;
+  }
+  int32_t *out = nullptr;
+  GBK_HIP(ctx, hipMalloc(&out, sizeof(int32_t) * (cnt + 1)));
+  owned.push_back(out);
+  if (cnt > 0) GBK_HIP(ctx, hipMemcpyAsync(out, tmp, sizeof(int32_t) * cnt, hipMemcpyDeviceToDevice, s));
+  append_sentinel<<<1, 1, 0, s>>>(out, cnt, n);
+  int64_t *pre = nullptr;
+  GBK_HIP(ctx, hipMalloc(&pre, sizeof(int64_t) * (nheavy + 1)));
+  owned.push_back(pre);
+  auto *hc = static_cast<int64_t *>(scratch(ctx, kS12, sizeof(int64_t) * (nheavy + 1)));
+  if (!hc) return set_error(ctx, GBK_ENOMEM, "heavy chunk scratch");
+  if (nheavy > 0) heavy_chunk_counts<<<grid_for(nheavy), kBlock, 0, s>>>(off, heavy, nheavy, hc);
+  if (int rc = exclusive_scan_i64(ctx, hc, pre, nheavy)) return rc;
+  GBK_LAUNCH_CHECK(ctx, "build_tiles");
+  *tiles = out;
+  *ntiles = cnt;
+  *hchunk = pre;
+  return GBK_OK;
+}
+
 }  // namespace gbk
 
 using namespace gbk;
@@ -390,17 +457,27 @@ int gbk_graph_free(gbk_ctx *ctx, gbk_graph *g) {
 
 // Builds heavy lists for both CSRs of a freshly filled device graph.
 int gbk_graph_finalize(gbk_ctx *ctx, gbk_graph *g) {
-  int rc = build_heavy_list(ctx, g->off, g->n, ctx->tune.heavy_degree, &g->heavy_out,
-                            &g->n_heavy_out, &g->max_out_degree, g->owned);
+  g->heavy_deg = ctx->tune.heavy_degree;
+  int rc = build_heavy_list(ctx, g->off, g->n, g->heavy_deg, &g->heavy_out, &g->n_heavy_out,
+                            &g->max_out_degree, g->owned);
+  if (rc) return rc;
+  rc = build_tiles(ctx, g->off, g->n, g->heavy_deg, g->heavy_out, g->n_heavy_out, &g->tiles_out,
+                   &g->ntiles_out, &g->hchunk_out, g->owned);
   if (rc) return rc;
   if (g->directed) {
-    rc = build_heavy_list(ctx, g->in_off, g->n, ctx->tune.heavy_degree, &g->heavy_in,
-                          &g->n_heavy_in, &g->max_in_degree, g->owned);
+    rc = build_heavy_list(ctx, g->in_off, g->n, g->heavy_deg, &g->heavy_in, &g->n_heavy_in,
+                          &g->max_in_degree, g->owned);
+    if (rc) return rc;
+    rc = build_tiles(ctx, g->in_off, g->n, g->heavy_deg, g->heavy_in, g->n_heavy_in, &g->tiles_in,
+                     &g->ntiles_in, &g->hchunk_in, g->owned);
     if (rc) return rc;
   } else {
     g->heavy_in = g->heavy_out;
     g->n_heavy_in = g->n_heavy_out;
     g->max_in_degree = g->max_out_degree;
+    g->tiles_in = g->tiles_out;
+    g->ntiles_in = g->ntiles_out;
+    g->hchunk_in = g->hchunk_out;
   }
   return GBK_OK;
 }

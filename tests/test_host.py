@@ -194,3 +194,45 @@ def test_mapping_dispatch_covers_cheap_orderings():
         assert H.validate_permutation(ids) == 0
     with pytest.raises(ValueError):
         H.mapping(12, g.off)
+
+
+# ---- selector corpus and encoding (host side) ------------------------------
+def test_selector_corpus_is_complete_and_split_by_family():
+    import json
+    from graphbrew_amd import selector as S
+    with open(os.path.join(os.path.dirname(S.MODEL_PATH), "selector_corpus.json")) as f:
+        data = json.load(f)
+    assert len(data) >= 96
+    for g in data:
+        assert len(g["features"]) == 12
+        assert sorted(g["orderings"]) == [str(a) for a in range(8)]
+        for row in g["orderings"].values():
+            assert sorted(row["kernel_ms"]) == sorted(S.KERNELS)
+            assert all(v > 0 for v in row["kernel_ms"].values())
+        assert g["orderings"]["0"]["map_ms"] == 0 and g["orderings"]["0"]["relabel_ms"] == 0
+    train, test = S.split(data)
+    fams = lambda part: {g["spec"]["family"] for g in part}  # noqa: E731
+    assert fams(train) == fams(test) == {"rmat", "sbm", "lattice", "powerlaw"}
+    assert len(test) * 4 == len(data)
+
+
+def test_selector_encoding_and_samples():
+    from graphbrew_amd import selector as S
+    feats = np.arange(12, dtype=np.float64)
+    x = S.encode(feats, 3, "bfs", 2, np.zeros(12), np.ones(12))
+    assert x.shape == (S.IN_DIM,) and S.IN_DIM == 28
+    assert np.array_equal(x[:12], feats)
+    assert x[12 + 3] == 1 and x[12:20].sum() == 1
+    assert x[20 + S.KERNELS.index("bfs")] == 1 and x[20:27].sum() == 1
+    assert x[27] == 0.5
+    row = {"map_ms": 1.0, "relabel_ms": 2.0, "relabel_w_ms": 3.0, "kernel_ms": {k: 4.0 for k in S.KERNELS}}
+    assert S.e2e_ms(row, "pr", 2) == 11.0 and S.e2e_ms(row, "sssp", 1) == 8.0
+
+
+def test_selector_generators_are_seeded():
+    from graphbrew_amd import selector as S
+    for spec in S.corpus(8, min_scale=9, max_scale=10):
+        (u1, v1), n1 = S.make_edges(spec)
+        (u2, v2), n2 = S.make_edges(spec)
+        assert n1 == n2 and np.array_equal(u1, u2) and np.array_equal(v1, v2)
+        assert u1.min() >= 0 and max(u1.max(), v1.max()) < n1 and not np.any(u1 == v1)
