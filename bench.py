@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="first copy MIB MiB device to device (calibrates HBM byte counters)")
+    p.add_argument("--no-selector", action="store_true",
+                   help="skip the learned-order vs identity end-to-end comparison")
     p.add_argument("--single-source", action="store_true",
                    help="run BFS and SSSP one source per call instead of 64-source batches")
     p.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
@@ -225,6 +227,51 @@ def load_traffic(workload, family, single):
     return None if e is None else e.get("bytes_per_launch")
 
 
+def selector_e2e(work, gbk, H):
+    """End-to-end time (mapping on the host + relabel + reuse x kernel, the
+    reference's accounting) of the learned pick, the original order and the
+    frozen rule's deployable pick (DBG) on this workload's graph, for every
+    kernel at reuse 1 and 2.  Runs after the timed region."""
+    from graphbrew_amd import selector as S
+    ctx = work.ctx
+    sel = S.Selector.load()
+    feats = ctx.features(work.g)
+    off = work.g.download()["off"]
+    src_orig = int(H.source_picker(off, 1)[0][0])
+    picks = {k: {r: sel.pick(ctx, feats, k, r) for r in S.REUSE} for k in S.KERNELS}
+    needed = sorted({0, S.FALLBACK} | {a for k in picks for a in picks[k].values()})
+    rows = {}
+    for a in needed:
+        t0 = time.perf_counter()
+        ids = H.mapping(a, off)
+        map_ms = (time.perf_counter() - t0) * 1e3
+        if a == 0:
+            g, gw, rel, relw, map_ms = work.g, work.gw, 0.0, 0.0, 0.0
+        else:
+            g, st = ctx.relabel(work.g, ids)
+            gw, st2 = ctx.relabel(work.gw, ids)
+            rel, relw = st.device_ms, st2.device_ms
+        src = int(ids[src_orig])
+        rows[a] = {"map_ms": map_ms, "relabel_ms": rel, "relabel_w_ms": relw,
+                   "kernel_ms": {k: S.kernel_ms(ctx, k, g, gw, src, repeats=1) for k in S.KERNELS}}
+        if a != 0:
+            g.free()
+            gw.free()
+    out, ratios = {}, []
+    for k in S.KERNELS:
+        out[k] = {}
+        for r in S.REUSE:
+            learned = S.e2e_ms(rows[picks[k][r]], k, r)
+            ident = S.e2e_ms(rows[0], k, r)
+            out[k][f"reuse{r}"] = {"pick": S.ORDER_NAMES[picks[k][r]], "learned_ms": round(learned, 3),
+                                   "identity_ms": round(ident, 3),
+                                   "frozen_rule_dbg_ms": round(S.e2e_ms(rows[S.FALLBACK], k, r), 3)}
+            ratios.append(learned / ident)
+    return {"model_graphs": sel.meta.get("graphs"), "margin": sel.margin,
+            "learned_over_identity_geomean": round(float(np.exp(np.mean(np.log(ratios)))), 4),
+            "kernels": out}
+
+
 def cpu_baseline(work, budget_s):
     """CPU oracle (OpenMP restatement of the reference kernels) on a bounded
     sample of the same relabelled graph: PR 2 fixed iterations, then BFS and
@@ -391,6 +438,8 @@ def main():
         "roofline": roof,
         "wall_s": round(wall, 2),
     }
+    if not args.no_selector:
+        out["selector_e2e"] = selector_e2e(work, gbk, H)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(work, args.cpu_budget)
     print(json.dumps(out), flush=True)
