@@ -8,6 +8,10 @@
 //             integers < 2^53, so the atomic sum is exact in any order).
 //   backward  levels deepest first; delta[u] = sum over successor edges of
 //             sigma[u]/sigma[v] * (1 + delta[v]); scores[u] += delta[u].
+//             Once delta[v] is final its factor (1 + delta[v]) / sigma[v]
+//             (computed in double, kept as float) is all a predecessor
+//             needs, so a successor edge costs one random 4-byte read
+//             instead of a double and a float from two arrays.
 // Scores are divided by their maximum at the end (bc.cc:224-230).
 // Kernels: the forward uses the coalesced frontier expansion; the backward
 // walks each level's edges wave-wise with per-row LDS accumulators (rows
@@ -25,12 +29,11 @@ struct BcCounters {
 };
 
 __global__ void bc_init_source(int64_t n, int32_t source, int32_t *depth, double *sigma,
-                               float *delta, int32_t *queue, BcCounters *c) {
+                               int32_t *queue, BcCounters *c) {
   const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v < n) {
     depth[v] = v == source ? 0 : -1;
     sigma[v] = v == source ? 1.0 : 0.0;
-    delta[v] = 0.0f;
   }
   if (v == 0) {
     queue[0] = source;
@@ -50,6 +53,7 @@ struct BcForwardEdge {
   __device__ void operator()(int64_t u, int64_t k, bool active) {
     bool claimed = false;
     int32_t v = 0;
+    uint32_t bits = 0;
     if (active) {
       v = nbr[k];
       // A plain load sees either a depth fixed in an earlier level (final)
@@ -61,10 +65,23 @@ struct BcForwardEdge {
         claimed = x == -1;
       }
       if (x == -1 || x == d) {
-        bit_set_atomic(succ, k - edge_base);
+        bits = 1u << ((k - edge_base) & 31);
         unsafeAtomicAdd(&sigma[v], sigma[u]);
       }
     }
+    // Successor bits: lanes holding consecutive edges share bitmap words, so
+    // OR the bits over each run of lanes with the same word and let the
+    // run's last lane write it (one atomic per word instead of per edge).
+    const int lane = lane_id();
+    const int64_t word = active ? (k - edge_base) >> 5 : -1 - lane;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t b = __shfl_up(bits, o);
+      const int64_t w = __shfl_up(word, o);
+      if (lane >= o && w == word) bits |= b;
+    }
+    const bool tail = lane == kWave - 1 || __shfl_down(word, 1) != word;
+    if (tail && bits) atomicOr(&succ[word], bits);
     q.push(claimed, v);
   }
 };
@@ -108,10 +125,11 @@ bc_forward_heavy(const int64_t *off, const int32_t *nbr, int32_t *depth, double 
   f.q.flush();
 }
 
-// One level of the dependency accumulation.
+// One level of the dependency accumulation; coef[v] = (1 + delta[v]) /
+// sigma[v] of the deeper levels, written here for this level's vertices.
 __global__ void __launch_bounds__(kBlock)
 bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, int64_t count,
-                  const double *sigma, const uint32_t *succ, float *delta, float *scores,
+                  const double *sigma, const uint32_t *succ, float *coef, float *scores,
                   int64_t heavy_deg) {
   __shared__ float acc[kBlock];
   __shared__ int32_t heavy_rows[kBlock];
@@ -160,8 +178,7 @@ bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, i
       if (act) {
         const int64_t k = ostart + (e - oexcl);
         if (bit_get(succ, k - edge_base)) {
-          const int32_t v = nbr[k];
-          term = static_cast<float>(osig / sigma[v] * (1.0 + delta[v]));
+          term = static_cast<float>(osig * static_cast<double>(coef[nbr[k]]));
         }
       }
       bool tail;
@@ -172,7 +189,7 @@ bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, i
     __syncthreads();
     if (light) {
       const float du = acc[threadIdx.x];
-      delta[u] = du;
+      coef[u] = static_cast<float>((1.0 + du) / su);
       scores[u] += du;
     }
     const int nh = heavy_n;
@@ -182,14 +199,11 @@ bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, i
       const double hsig = sigma[hu];
       float part = 0.0f;
       for (int64_t k = hs + threadIdx.x; k < he; k += blockDim.x) {
-        if (bit_get(succ, k - edge_base)) {
-          const int32_t v = nbr[k];
-          part += static_cast<float>(hsig / sigma[v] * (1.0 + delta[v]));
-        }
+        if (bit_get(succ, k - edge_base)) part += static_cast<float>(hsig * static_cast<double>(coef[nbr[k]]));
       }
       const float tot = block_sum(part, red);
       if (threadIdx.x == 0) {
-        delta[hu] = tot;
+        coef[hu] = static_cast<float>((1.0 + tot) / hsig);
         scores[hu] += tot;
       }
     }
@@ -246,12 +260,12 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
   const int64_t words = (g->m + 31) / 32 + 1;
   auto *depth = static_cast<int32_t *>(scratch(ctx, kS5, sizeof(int32_t) * (n + 1)));
   auto *sigma = static_cast<double *>(scratch(ctx, kS6, sizeof(double) * (n + 1)));
-  auto *delta = static_cast<float *>(scratch(ctx, kS7, sizeof(float) * (n + 1)));
+  auto *coef = static_cast<float *>(scratch(ctx, kS7, sizeof(float) * (n + 1)));
   auto *queue = static_cast<int32_t *>(scratch(ctx, kS8, sizeof(int32_t) * (n + 64)));
   auto *succ = static_cast<uint32_t *>(scratch(ctx, kS9, sizeof(uint32_t) * words));
   auto *c = static_cast<BcCounters *>(scratch(ctx, kS10, sizeof(BcCounters) + 64));
   auto *hc = static_cast<BcCounters *>(pinned(ctx, sizeof(BcCounters)));
-  if (!depth || !sigma || !delta || !queue || !succ || !c || !hc)
+  if (!depth || !sigma || !coef || !queue || !succ || !c || !hc)
     return set_error(ctx, GBK_ENOMEM, "bc scratch");
   auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS11, sizeof(unsigned long long) * kGridCap));
   HeavyQueue hq;
@@ -268,7 +282,7 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
   for (int it = 0; it < num_sources; it++) {
     const int32_t src = sources[it];
     fill<uint32_t>(ctx, succ, 0u, words);
-    bc_init_source<<<grid_for(n), kBlock, 0, s>>>(n, src, depth, sigma, delta, queue, c);
+    bc_init_source<<<grid_for(n), kBlock, 0, s>>>(n, src, depth, sigma, queue, c);
     level.assign({0, 1});
     int32_t d = 0;
     for (;;) {
@@ -295,7 +309,7 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
       const int64_t lb = level[lv], cnt = level[lv + 1] - lb;
       const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
       bc_backward_level<<<expand_grid(cnt), kBlock, 0, s>>>(g->off, g->nbr, queue + lb, cnt, sigma,
-                                                           succ, delta, scores, H);
+                                                           succ, coef, scores, H);
       kt_end(ctx, kt);
     }
     bc_reach_edges<<<ng, kBlock, 0, s>>>(g->off, depth, n, partial);
