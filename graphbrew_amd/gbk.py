@@ -104,6 +104,9 @@ def lib():
         "gbk_bc": (i32, [VP, VP, VP, i32, VP, C.POINTER(BcStats)]),
         "gbk_bfs_batch": (i32, [VP, VP, VP, i32, VP, VP, VP, C.POINTER(BatchStats)]),
         "gbk_graph_features": (i32, [VP, VP, VP, i32]),
+        "gbk_graph_union": (i32, [VP, VP, i32, VP, pp]),
+        "gbk_cc_many": (i32, [VP, VP, VP, i32, i32, VP, C.POINTER(CcStats)]),
+        "gbk_pr_many": (i32, [VP, VP, VP, i32, i32, i32, f64, i32, VP, VP, C.POINTER(PrStats)]),
         "gbk_mlp_create": (i32, [VP, i32, C.c_uint64, pp]),
         "gbk_mlp_destroy": (i32, [VP, VP]),
         "gbk_mlp_num_params": (i64, [VP]),
@@ -333,6 +336,33 @@ class Context:
         self._check(lib().gbk_sssp_batch(self.h, g.handle, _ptr(s), len(s), int(delta), _ptr(dist),
                                          _ptr(reached), _ptr(edges), C.byref(st)))
         return dist, reached, edges, st
+
+    # ------------------------------------------------ many small graphs
+    def graph_union(self, graphs):
+        """One device graph holding the given graphs side by side, and the
+        vertex base of each part (count + 1 entries)."""
+        arr = (VP * len(graphs))(*[g.handle for g in graphs])
+        vbase = np.empty(len(graphs) + 1, np.int64)
+        h = VP()
+        self._check(lib().gbk_graph_union(self.h, arr, len(graphs), _ptr(vbase), C.byref(h)))
+        return self._wrap(h), vbase
+
+    def cc_many(self, g: DeviceGraph, vbase, variant=CC_AFFOREST, download=True):
+        vbase = np.ascontiguousarray(vbase, np.int64)
+        out = np.empty(g.n, np.int32) if download else None
+        st = CcStats()
+        self._check(lib().gbk_cc_many(self.h, g.handle, _ptr(vbase), len(vbase) - 1, variant, _ptr(out),
+                                      C.byref(st)))
+        return out, st
+
+    def pr_many(self, g: DeviceGraph, vbase, variant=PR_GS, max_iters=20, tol=1e-4, fixed=False, download=True):
+        vbase = np.ascontiguousarray(vbase, np.int64)
+        out = np.empty(g.n, np.float32) if download else None
+        iters = np.empty(len(vbase) - 1, np.int32)
+        st = PrStats()
+        self._check(lib().gbk_pr_many(self.h, g.handle, _ptr(vbase), len(vbase) - 1, variant, max_iters, tol,
+                                      int(fixed), _ptr(out), _ptr(iters), C.byref(st)))
+        return out, iters, st
 
     # -------------------------------------------------- selector support
     def features(self, g: DeviceGraph) -> np.ndarray:

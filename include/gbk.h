@@ -198,6 +198,25 @@ int gbk_sssp_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources,
                    int64_t *reached, int64_t *edges_reached,
                    gbk_batch_stats *stats);
 
+/* ---- many small graphs in one launch ------------------------------------ */
+/* No reference counterpart as a call (the reference runs one graph per
+ * process).  gbk_graph_union concatenates graphs (all directed or all
+ * undirected, all weighted or not) into one device graph whose vertex ids
+ * are shifted by each part's base; vbase (host, count + 1 entries) receives
+ * the bases and the total. */
+int gbk_graph_union(gbk_ctx *ctx, const gbk_graph *const *graphs, int count,
+                    int64_t *vbase, gbk_graph **out);
+/* CC of every part at once; comp (host int32[n] of the union) holds each
+ * part's own labels (smallest id of the component within the part). */
+int gbk_cc_many(gbk_ctx *ctx, const gbk_graph *g, const int64_t *vbase, int count,
+                int variant, int32_t *comp, gbk_cc_stats *stats);
+/* PR of every part at once: each part has its own base score, L1 change and
+ * stopping iteration (pr.cc:77-88); scores (host float[n] of the union),
+ * iters (host int32[count]) per part. */
+int gbk_pr_many(gbk_ctx *ctx, const gbk_graph *g, const int64_t *vbase, int count,
+                int variant, int max_iters, double tol, int fixed_work, float *scores,
+                int32_t *iters, gbk_pr_stats *stats);
+
 /* ---- learned ordering selector ------------------------------------------ */
 /* No reference counterpart as a call: the reference's selector computes its
  * features on the CPU (ComputeTier0SampledGraphFeatures, reorder/
