@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="first copy MIB MiB device to device (calibrates HBM byte counters)")
+    p.add_argument("--many-graphs", type=int, default=256,
+                   help="small seeded R-MAT graphs (scales 10-16, ef 8-32) run as one batch; 0 skips")
     p.add_argument("--no-selector", action="store_true",
                    help="skip the learned-order vs identity end-to-end comparison")
     p.add_argument("--single-source", action="store_true",
@@ -272,6 +274,47 @@ def selector_e2e(work, gbk, H):
             "kernels": out}
 
 
+def many_graphs(ctx, gbk, count):
+    """PR (Gauss-Seidel, <=20 iterations, tol 1e-4) and Afforest over
+    'count' seeded R-MAT graphs of uneven size (scales 10-16, edge factor
+    8-32, varied a/b/c), once as one union batch and once one graph per
+    call; device times from the library's hipEvents."""
+    from graphbrew_amd import selector as S
+    rng = np.random.default_rng(20240)
+    graphs, edges = [], 0
+    t0 = time.time()
+    for i in range(count):
+        scale = int(rng.integers(10, 17))
+        ef = int(rng.choice([8, 16, 32]))
+        a = float(rng.uniform(0.45, 0.6))
+        b = c = float(rng.uniform(0.1, (1 - a) / 2 - 0.02))
+        (u, v), n = S.rmat(scale, ef, a, b, c, seed=5000 + i, permute=True)
+        g, _ = ctx.build(u, v, None, n, True)
+        graphs.append(g)
+        edges += g.m
+    gen_s = time.time() - t0
+    ug, vbase = ctx.graph_union(graphs)
+    out = {"graphs": count, "vertices": int(ug.n), "edges_directed": int(ug.m),
+           "generation_s": round(gen_s, 1)}
+    for name, run_many, run_one in (
+            ("pr", lambda: ctx.pr_many(ug, vbase, gbk.PR_GS, 20, 1e-4, False, download=False)[2].device_ms,
+             lambda g: ctx.pr(g, gbk.PR_GS, 20, 1e-4, False, download=False)[1].device_ms),
+            ("cc", lambda: ctx.cc_many(ug, vbase, gbk.CC_AFFOREST, download=False)[1].device_ms,
+             lambda g: ctx.cc(g, gbk.CC_AFFOREST, download=False)[1].device_ms)):
+        run_many()
+        batch_ms = float(np.median([run_many() for _ in range(3)]))
+        t = time.time()
+        one_ms = sum(run_one(g) for g in graphs)
+        one_wall = (time.time() - t) * 1e3
+        out[name] = {"batch_ms": round(batch_ms, 3), "one_by_one_device_ms": round(one_ms, 3),
+                     "one_by_one_wall_ms": round(one_wall, 1),
+                     "speedup_vs_one_by_one_wall": round(one_wall / batch_ms, 1)}
+    for g in graphs:
+        g.free()
+    ug.free()
+    return out
+
+
 def cpu_baseline(work, budget_s):
     """CPU oracle (OpenMP restatement of the reference kernels) on a bounded
     sample of the same relabelled graph: PR 2 fixed iterations, then BFS and
@@ -440,6 +483,8 @@ def main():
     }
     if not args.no_selector:
         out["selector_e2e"] = selector_e2e(work, gbk, H)
+    if args.many_graphs > 0:
+        out["many_graphs"] = many_graphs(ctx, gbk, args.many_graphs)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(work, args.cpu_budget)
     print(json.dumps(out), flush=True)
