@@ -40,6 +40,7 @@ def lib():
             "gbh_mapping_hubsortdbg": (i32, [i64, VP, VP]),
             "gbh_mapping_hubclusterdbg": (i32, [i64, VP, VP]),
             "gbh_mapping_fingerprint": (C.c_uint64, [i64, VP]),
+            "gbh_generate_rmat": (i32, [i32, i64, f64, f64, f64, C.c_uint64, i32, VP, VP]),
             "gbh_verify_bfs": (i32, [i64, VP, VP, VP, VP, C.c_int32, VP]),
             "gbh_verify_cc": (i32, [i64, VP, VP, VP, VP, i32, VP]),
             "gbh_verify_sssp": (i64, [i64, VP, VP, VP, C.c_int32, VP]),
@@ -78,6 +79,13 @@ def generate_el(scale: int, degree: int = 16, uniform: bool = False):
     u = np.empty(m, np.int32)
     v = np.empty(m, np.int32)
     _chk(lib().gbh_generate_el(scale, degree, int(uniform), _p(u), _p(v)), "generate_el")
+    return u, v
+
+
+def generate_rmat(scale: int, edges: int, a: float, b: float, c: float, seed: int, permute: bool):
+    u = np.empty(edges, np.int32)
+    v = np.empty(edges, np.int32)
+    _chk(lib().gbh_generate_rmat(scale, edges, a, b, c, seed, int(permute), _p(u), _p(v)), "generate_rmat")
     return u, v
 
 

@@ -200,9 +200,59 @@ int gbh_mapping_random(int64_t n, int32_t *new_ids) {
   return 0;
 }
 
-// GenerateSortMapping (reorder_basic.h:213-258): stable, degree descending.
+}  // extern "C"
+
+namespace {
+
+// Stable bucket numbering in parallel: vertices get new ids bucket by bucket
+// (bucket B-1 first, down to 0), in id order inside a bucket -- the order a
+// sequential pass produces.  Each thread counts a contiguous id range, a
+// prefix over (bucket, thread) gives every thread its first id per bucket.
+template <class BucketOf>
+void stable_buckets_desc(int64_t n, int64_t buckets, BucketOf bucket_of, int32_t *new_ids) {
+  const int T = std::max(1, omp_get_max_threads());
+  std::vector<int64_t> cnt(static_cast<size_t>(T) * buckets, 0);
+  const int64_t chunk = (n + T - 1) / T;
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    int64_t *c = cnt.data() + static_cast<size_t>(t) * buckets;
+    for (int64_t v = t * chunk; v < std::min(n, (t + 1) * chunk); v++) c[bucket_of(v)]++;
+  }
+  int64_t run = 0;
+  for (int64_t b = buckets - 1; b >= 0; b--)
+    for (int t = 0; t < T; t++) {
+      int64_t &c = cnt[static_cast<size_t>(t) * buckets + b];
+      const int64_t x = c;
+      c = run;
+      run += x;
+    }
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    int64_t *c = cnt.data() + static_cast<size_t>(t) * buckets;
+    for (int64_t v = t * chunk; v < std::min(n, (t + 1) * chunk); v++)
+      new_ids[v] = static_cast<int32_t>(c[bucket_of(v)]++);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// GenerateSortMapping (reorder_basic.h:213-258): stable, degree descending
+// -- a stable counting sort on the degree when the degree range is small
+// enough for per-thread counters, a stable comparison sort otherwise.
 int gbh_mapping_sort(int64_t n, const int64_t *off, int32_t *new_ids) {
   if (n < 0 || (n > 0 && (!off || !new_ids))) return -1;
+  if (n == 0) return 0;
+  int64_t max_deg = 0;
+#pragma omp parallel for reduction(max : max_deg)
+  for (int64_t v = 0; v < n; v++) max_deg = std::max<int64_t>(max_deg, off[v + 1] - off[v]);
+  if ((max_deg + 1) * std::max(1, omp_get_max_threads()) <= (int64_t{1} << 26)) {
+    stable_buckets_desc(n, max_deg + 1, [off](int64_t v) { return off[v + 1] - off[v]; }, new_ids);
+    return 0;
+  }
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [off](int32_t a, int32_t b) {
@@ -227,17 +277,9 @@ int gbh_mapping_dbg(int64_t n, const int64_t *off, int32_t *new_ids) {
     const uint64_t d = static_cast<uint64_t>(off[v + 1] - off[v]);
     int b = 0;
     while (d > limit[b]) b++;
-    return b;
+    return static_cast<int64_t>(b);
   };
-  int64_t count[8] = {0};
-  for (int64_t v = 0; v < n; v++) count[bucket_of(v)]++;
-  int64_t start[8];
-  int64_t run = 0;
-  for (int b = 7; b >= 0; b--) {
-    start[b] = run;
-    run += count[b];
-  }
-  for (int64_t v = 0; v < n; v++) new_ids[v] = static_cast<int32_t>(start[bucket_of(v)]++);
+  stable_buckets_desc(n, 8, bucket_of, new_ids);
   return 0;
 }
 
@@ -504,14 +546,16 @@ std::vector<int32_t> hubs_by_id(int64_t n, const int64_t *off) {
   return hubs;
 }
 
+
 // Hubs first in the given order, then every other vertex in id order
 // (the DBG variants, reorder_hub.h:380-502 and 505-582).
 void hubs_then_rest(int64_t n, const std::vector<int32_t> &hubs, int32_t *new_ids) {
-  std::fill(new_ids, new_ids + n, -1);
+  std::vector<uint8_t> hub(n, 0);
+  for (int32_t v : hubs) hub[v] = 1;
+  // non-hubs keep id order after the hubs: a stable two-bucket numbering
+  stable_buckets_desc(n, 2, [&hub](int64_t v) { return static_cast<int64_t>(hub[v]); }, new_ids);
   int32_t next = 0;
   for (int32_t v : hubs) new_ids[v] = next++;
-  for (int64_t v = 0; v < n; v++)
-    if (new_ids[v] == -1) new_ids[v] = next++;
 }
 
 }  // namespace
@@ -541,3 +585,42 @@ int gbh_mapping_hubclusterdbg(int64_t n, const int64_t *off, int32_t *new_ids) {
 }
 
 }  // extern "C"
+
+// ---- parameterised R-MAT for the selector's corpus ------------------------
+// Not the reference generator (that one is gbh_generate_el): R-MAT after
+// Chakrabarti et al. with quadrant probabilities a, b, c, 1-a-b-c, driven by
+// a counter-based stream (SplitMix64 of seed and edge index) so that edges
+// are generated in parallel and the output does not depend on the thread
+// count; ids are optionally relabelled by a seeded Fisher-Yates permutation.
+extern "C" int gbh_generate_rmat(int scale, int64_t edges, double a, double b, double c, uint64_t seed,
+                                 int permute, int32_t *u, int32_t *v) {
+  if (scale < 1 || scale > 30 || edges < 0 || !u || !v || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
+    return -1;
+  const double ab = a + b, abc = a + b + c;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < edges; e++) {
+    uint64_t state = seed * 0x9E3779B97F4A7C15ULL ^ (static_cast<uint64_t>(e) * 0xD1B54A32D192ED03ULL);
+    uint32_t x = 0, y = 0;
+    for (int bit = 0; bit < scale; bit++) {
+      const double r = static_cast<double>(splitmix_next(state) >> 11) * (1.0 / 9007199254740992.0);
+      const uint32_t down = r >= ab, right = (r >= a && r < ab) || r >= abc;
+      x |= down << bit;
+      y |= right << bit;
+    }
+    u[e] = static_cast<int32_t>(x);
+    v[e] = static_cast<int32_t>(y);
+  }
+  if (permute) {
+    const int64_t n = int64_t{1} << scale;
+    std::vector<int32_t> p(n);
+    std::iota(p.begin(), p.end(), 0);
+    uint64_t state = seed ^ 0x5851F42D4C957F2DULL;
+    for (int64_t left = n; left > 1; --left) std::swap(p[left - 1], p[splitmix_next(state) % left]);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < edges; e++) {
+      u[e] = p[u[e]];
+      v[e] = p[v[e]];
+    }
+  }
+  return 0;
+}

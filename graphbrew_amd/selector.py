@@ -53,18 +53,12 @@ def _finish(u, v, n, rng, permute):
 
 def rmat(scale, ef, a, b, c, seed, permute=True):
     """R-MAT edge list (Chakrabarti et al.): each edge picks one quadrant per
-    bit level with probabilities a, b, c, 1-a-b-c."""
-    rng = np.random.default_rng(seed)
+    bit level with probabilities a, b, c, 1-a-b-c (host generator, seeded,
+    parallel), self loops dropped."""
     n, m = 1 << scale, (1 << scale) * ef
-    u = np.zeros(m, np.int64)
-    v = np.zeros(m, np.int64)
-    for bit in range(scale):
-        r = rng.random(m)
-        right = (r >= a) & (r < a + b) | (r >= a + b + c)
-        down = r >= a + b
-        u |= down.astype(np.int64) << bit
-        v |= right.astype(np.int64) << bit
-    return _finish(u, v, n, rng, permute), n
+    u, v = H.generate_rmat(scale, m, a, b, c, seed, permute)
+    keep = u != v
+    return (u[keep], v[keep]), n
 
 
 def sbm(n, blocks, avg_degree, ratio, seed, permute=False):
@@ -369,17 +363,21 @@ def main(argv=None):
     p.add_argument("--graphs", type=int, default=96)
     p.add_argument("--min-scale", type=int, default=12)
     p.add_argument("--max-scale", type=int, default=15)
-    p.add_argument("--corpus", default=os.path.join(os.path.dirname(MODEL_PATH), "selector_corpus.json"))
+    p.add_argument("--seed0", type=int, default=1000)
+    p.add_argument("--corpus", default=os.path.join(os.path.dirname(MODEL_PATH), "selector_corpus.json"),
+                   help="output of collect; for train / evaluate a comma-separated list to merge")
     p.add_argument("--model", default=MODEL_PATH)
     p.add_argument("--epochs", type=int, default=150)
     a = p.parse_args(argv)
     ctx = gbk.Context(0)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
     if a.cmd == "collect":
-        collect(ctx, corpus(a.graphs, min_scale=a.min_scale, max_scale=a.max_scale), a.corpus, log)
+        collect(ctx, corpus(a.graphs, seed0=a.seed0, min_scale=a.min_scale, max_scale=a.max_scale), a.corpus, log)
         return 0
-    with open(a.corpus) as f:
-        data = json.load(f)
+    data = []
+    for path in a.corpus.split(","):
+        with open(path) as f:
+            data += json.load(f)
     train, test = split(data)
     if a.cmd == "train":
         sel = Selector()

@@ -46,6 +46,13 @@ int gbh_mapping_hubsort(int64_t n, const int64_t *off, int32_t *new_ids);
 int gbh_mapping_hubcluster(int64_t n, const int64_t *off, int32_t *new_ids);
 int gbh_mapping_hubsortdbg(int64_t n, const int64_t *off, int32_t *new_ids);
 int gbh_mapping_hubclusterdbg(int64_t n, const int64_t *off, int32_t *new_ids);
+/* Parameterised R-MAT (quadrant probabilities a, b, c, 1-a-b-c; 2^scale
+ * vertices; 'edges' edges, self loops included) from a counter-based
+ * seeded stream, optionally permuted; for the selector's graph families.
+ * No reference counterpart (the reference's generator has fixed
+ * probabilities and seed, gapbs/generator.h:89-186). */
+int gbh_generate_rmat(int scale, int64_t edges, double a, double b, double c, uint64_t seed,
+                      int permute, int32_t *u, int32_t *v);
 uint64_t gbh_mapping_fingerprint(int64_t n, const int32_t *new_ids);
 /* 0 ok, -1 out-of-range id, -2 duplicate id. */
 int gbh_validate_permutation(int64_t n, const int32_t *ids);
