@@ -176,6 +176,11 @@ class Workload:
             _, _, edges, st = ctx.sssp_batch(gw, self.srcs[:a.sssp_sources], a.batch_delta, download=False)
             add("sssp", st.device_ms, st.edges_reached, sum(bytes_sssp(n, int(e)) for e in edges),
                 len(edges))
+            scan = rec.setdefault("_sssp_scan", [0, 0, 0, 0])
+            scan[0] += 1
+            scan[1] += st.rounds
+            scan[2] += st.rows_scanned
+            scan[3] += st.edges_scanned
         if "bc" in self.kernels:
             _, st = ctx.bc(g, self.srcs[:a.bc_sources], download=False)
             add("bc", st.device_ms, 2 * st.edges_reached, bytes_bc(n, st.edges_reached) , a.bc_sources)
@@ -439,6 +444,7 @@ def main():
     n, m = work.g.n, work.g.m
     kern = {}
     total_edges = 0
+    scan = rec.pop("_sssp_scan", None)
     for name, (ms, edges, nbytes, calls) in rec.items():
         secs = ms / 1e3
         total_edges += edges
@@ -446,6 +452,12 @@ def main():
                       "gbps": nbytes / secs / 1e9 if secs else 0.0,
                       "hbm_frac": nbytes / secs / HBM_PEAK if secs else 0.0,
                       "calls_per_step": calls / args.steps}
+    if scan and "sssp" in kern:
+        # work the batch actually did: queued rows relaxed and their edges
+        # walked (each edge once for all sources of the batch)
+        kern["sssp"].update({"rounds_per_call": scan[1] / scan[0], "rows_scanned_per_call": scan[2] / scan[0],
+                             "edges_scanned_per_call": scan[3] / scan[0],
+                             "edges_scanned_per_us": scan[3] / (rec["sssp"][0] * 1e3)})
     value = total_edges / (total_ms / 1e3) / 1e9
     workload = (f"{'uniform' if args.uniform else 'kronecker'}-scale{args.scale}-ef{args.degree}"
                 f" relabel+pr+bfs{args.bfs_sources}+cc+cc_sv+sssp{args.sssp_sources}+bc{args.bc_sources}")

@@ -42,11 +42,18 @@ constexpr int kBatchMax = 64;
 constexpr int32_t kBatchInf = INT32_MAX / 2;  // sssp.cc:67
 constexpr int64_t kBatchHeavy = 256;          // rows above this are chunked over groups
 constexpr int64_t kBatchChunk = 256;
-constexpr int kBatchUnroll = 4;
+constexpr int kBatchUnroll = 8;
 
 inline int batch_width(int k) { return k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64; }
 
 // ------------------------------------------------------------------ shared
+// A wave-uniform 64-bit value moved to scalar registers.
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v & 0xffffffffll)));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v >> 32)));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 template <int Kp>
 __device__ __forceinline__ int32_t group_min(int32_t v) {
 #pragma unroll
@@ -317,6 +324,8 @@ struct BatchState {
   long long next_bucket;
   long long relax_rounds;
   long long buckets;
+  unsigned long long rows_scanned;   // queue entries relaxed, over all rounds
+  unsigned long long edges_scanned;  // their out-edges walked (each for every source)
   int cur, phase, done, overflow;
 };
 
@@ -328,6 +337,8 @@ __global__ void sb_init_state(BatchState *st, long long first) {
     st->next_bucket = 0;
     st->relax_rounds = 0;
     st->buckets = 0;
+    st->rows_scanned = 0;
+    st->edges_scanned = 0;
     st->cur = 0;
     st->phase = 0;
     st->done = 0;
@@ -387,6 +398,10 @@ struct GroupRelax {
   // Relaxes edges [s, s + len) of a row for every source of the group;
   // len is group-uniform, the trip count wave-uniform.
   __device__ void walk(int64_t s, int64_t len, int64_t du, bool act) {
+    if constexpr (Kp == kWave) {
+      walk_wave(s, len, du, act);
+      return;
+    }
     const int64_t trips = wave_max(len);
     for (int64_t j0 = 0; j0 < trips; j0 += kBatchUnroll) {
       int32_t x[kBatchUnroll], w[kBatchUnroll], dv[kBatchUnroll];
@@ -405,12 +420,16 @@ struct GroupRelax {
         bool here = false;
         int32_t fut = INT32_MAX;
         const int64_t cand = du + w[j];
-        if (valid[j] && act && cand < dv[j]) {
-          const int32_t old = atomicMin(&dist[static_cast<int64_t>(x[j]) * Kp + k], static_cast<int32_t>(cand));
-          if (cand < old) {
-            if (cand < upper) here = true;
-            else fut = static_cast<int32_t>(cand / delta);
-          }
+        // Judged against the loaded distance (never below the current one),
+        // so every real improvement queues x; a lane that lost a race to a
+        // lower value queues x needlessly, which costs a visit, not
+        // correctness.  The atomic's result is not waited for.
+        const bool imp = valid[j] && act && cand < dv[j];
+        if (__ballot(imp) == 0) continue;  // nothing improved in the wave
+        if (imp) {
+          atomicMin(&dist[static_cast<int64_t>(x[j]) * Kp + k], static_cast<int32_t>(cand));
+          if (cand < upper) here = true;
+          else fut = static_cast<int32_t>(cand / delta);
         }
         const bool anyh = group_any<Kp>(here);
         const int32_t f = group_min<Kp>(fut);
@@ -420,6 +439,54 @@ struct GroupRelax {
           if (f != INT32_MAX && f < pend[x[j]]) atomicMin(&pend[x[j]], f);
         }
         nearq.push(push, x[j]);
+      }
+    }
+  }
+
+  // Kp == 64: the wave is one group, so the row and its edge range are
+  // wave-uniform.  Each 64-edge slice of neighbour ids and weights comes in
+  // with one coalesced load and is broadcast edge by edge (readlane); the
+  // distance rows of kBatchUnroll neighbours are in flight together; the
+  // cross-lane reductions and the leader's atomics run only for edges on
+  // which some lane improved.
+  __device__ void walk_wave(int64_t s, int64_t len, int64_t du, bool act) {
+    s = uniform64(s);
+    len = uniform64(len);
+    for (int64_t c0 = 0; c0 < len; c0 += kWave) {
+      const int cnt = static_cast<int>(len - c0 < kWave ? len - c0 : kWave);
+      const bool mine = k < cnt;
+      const int32_t vx = mine ? nbr[s + c0 + k] : 0;
+      const int32_t vw = mine ? wgt[s + c0 + k] : 0;
+      for (int j0 = 0; j0 < cnt; j0 += kBatchUnroll) {
+        int32_t x[kBatchUnroll], w[kBatchUnroll], dv[kBatchUnroll];
+#pragma unroll
+        for (int j = 0; j < kBatchUnroll; j++) {
+          const int at = j0 + j < kWave ? j0 + j : kWave - 1;  // lanes past cnt hold vertex 0
+          x[j] = __builtin_amdgcn_readlane(vx, at);
+          w[j] = __builtin_amdgcn_readlane(vw, at);
+        }
+#pragma unroll
+        for (int j = 0; j < kBatchUnroll; j++) dv[j] = dist[static_cast<int64_t>(x[j]) * Kp + k];
+#pragma unroll
+        for (int j = 0; j < kBatchUnroll; j++) {
+          const int64_t cand = du + w[j];
+          // judged against the loaded distance, as in walk()
+          const bool imp = j0 + j < cnt && act && cand < dv[j];
+          const uint64_t im = __ballot(imp);
+          if (im == 0) continue;
+          if (imp) atomicMin(&dist[static_cast<int64_t>(x[j]) * Kp + k], static_cast<int32_t>(cand));
+          const bool here = imp && cand < upper;
+          const uint64_t hm = __ballot(here);
+          if (hm != im) {  // some improvement lies in a later bucket
+            const int32_t f = group_min<Kp>(imp && !here ? static_cast<int32_t>(cand / delta) : INT32_MAX);
+            if (leader && f < pend[x[j]]) atomicMin(&pend[x[j]], f);
+          }
+          if (hm) {
+            bool push = false;
+            if (leader) push = atomicExch(&vtag[x[j]], tag) != tag;
+            nearq.push(push, x[j]);
+          }
+        }
       }
     }
   }
@@ -442,6 +509,7 @@ sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *di
   f.setup(seg, nbr, wgt, dist, pend, vtag, near0, near1, cap, delta, st);
   const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  unsigned long long walked = 0;
   for (int64_t base = gw * G; base < count; base += waves * G) {
     const int64_t i = base + grp;
     const bool valid = i < count;
@@ -466,8 +534,11 @@ sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *di
     bool act;
     f.row_lane(row, valid, !heavy, du, act);
     f.walk(s, heavy ? 0 : len, du, act);
+    if (f.leader && !heavy) walked += static_cast<unsigned long long>(len);
   }
   f.nearq.flush();
+  walked = wave_sum(walked);
+  if (lane_id() == 0 && walked) atomicAdd(&st->edges_scanned, walked);
 }
 
 template <int Kp>
@@ -487,6 +558,7 @@ sb_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *p
   const int64_t total = hq.prefix[rows];
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  unsigned long long walked = 0;
   for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
     const int64_t c = c0 + grp;
     const bool valid = c < total;
@@ -507,14 +579,18 @@ sb_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *p
     bool act;
     f.row_lane(row, valid, first == 0, du, act);
     f.walk(start + first, valid ? last - first : 0, du, act);
+    if (f.leader && valid) walked += static_cast<unsigned long long>(last - first);
   }
   f.nearq.flush();
+  walked = wave_sum(walked);
+  if (lane_id() == 0 && walked) atomicAdd(&st->edges_scanned, walked);
 }
 
 __global__ void sb_relax_done(BatchState *st, unsigned long long *heavy_count) {
   if (threadIdx.x != 0 || st->done || st->phase != 0) return;
   *heavy_count = 0;
   const int cur = st->cur;
+  st->rows_scanned += st->near_cnt[cur];
   st->near_cnt[cur] = 0;
   st->cur = 1 - cur;
   st->relax_rounds += 1;
@@ -795,6 +871,8 @@ int sssp_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K
   acc->device_ms += ms;
   acc->rounds += fin.relax_rounds;
   acc->buckets += fin.buckets + 1;
+  acc->rows_scanned += static_cast<int64_t>(fin.rows_scanned);
+  acc->edges_scanned += static_cast<int64_t>(fin.edges_scanned);
   acc->edges_reached += etot;
   if (dist_host) {
     GBK_HIP(ctx, hipMemcpyAsync(dist_host + first * n, out, sizeof(int32_t) * n * K, hipMemcpyDeviceToHost, s));

@@ -58,7 +58,7 @@ class BcStats(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("rounds", C.c_int64), ("buckets", C.c_int64),
-                ("edges_reached", C.c_int64)]
+                ("edges_reached", C.c_int64), ("rows_scanned", C.c_int64), ("edges_scanned", C.c_int64)]
 
 
 class RelabelStats(C.Structure):

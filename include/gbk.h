@@ -184,6 +184,8 @@ typedef struct {
   int64_t rounds;         /* BFS levels or SSSP relaxation rounds, summed */
   int64_t buckets;        /* SSSP buckets opened (0 for BFS) */
   int64_t edges_reached;  /* sum over sources of the out-degrees of reached vertices */
+  int64_t rows_scanned;   /* SSSP: queued rows relaxed over all rounds (0 for BFS) */
+  int64_t edges_scanned;  /* SSSP: their out-edges walked, each once for all sources */
 } gbk_batch_stats;
 
 /* depth: host int32[num_sources * n] (source-major, -1 unreached) or NULL;
