@@ -57,7 +57,8 @@ def parse():
     p.add_argument("--no-selector", action="store_true",
                    help="skip the learned-order vs identity end-to-end comparison")
     p.add_argument("--single-source", action="store_true",
-                   help="run BFS and SSSP one source per call instead of 64-source batches")
+                   help="run BFS and SSSP one source per call and BC one source per traversal, "
+                        "instead of source batches")
     p.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
                    help="library tunable to set before the run (repeatable)")
     p.add_argument("--autotune", action="store_true",
@@ -181,9 +182,15 @@ class Workload:
             scan[1] += st.rounds
             scan[2] += st.rows_scanned
             scan[3] += st.edges_scanned
-        if "bc" in self.kernels:
+        if "bc" in self.kernels and a.single_source:
             _, st = ctx.bc(g, self.srcs[:a.bc_sources], download=False)
-            add("bc", st.device_ms, 2 * st.edges_reached, bytes_bc(n, st.edges_reached) , a.bc_sources)
+            add("bc", st.device_ms, 2 * st.edges_reached, bytes_bc(n, st.edges_reached), a.bc_sources)
+        elif "bc" in self.kernels:
+            k = len(self.srcs[:a.bc_sources])
+            _, st = ctx.bc_batch(g, self.srcs[:a.bc_sources], download=False)
+            # per-source figure summed over the sources (linear in m_r)
+            add("bc", st.device_ms, 2 * st.edges_reached,
+                bytes_bc(n, st.edges_reached) + (k - 1) * bytes_bc(n, 0), k)
 
 
 # Kernels followed one launch at a time (HIP events recorded by the library
@@ -200,6 +207,8 @@ def kt_table(gbk, single):
                                         "(one relaxation round of a 64-source batch)")
         t["bfs"] = (gbk.KT_BFS_STEP, "msbfs_front_queue+msbfs_td+heavy_prefix+msbfs_td_heavy | msbfs_bu<8> "
                                      "(one level of a 64-source batch)")
+        t["bc"] = (gbk.KT_BC_LEVEL, "bcb_forward+heavy_prefix+bcb_forward_heavy | bcb_backward+heavy_prefix+"
+                                    "bcb_backward_heavy+bcb_backward_finish (one level of a source batch)")
     return t
 
 

@@ -666,6 +666,367 @@ __global__ void sb_split_done(BatchState *st) {
   st->phase = 0;
 }
 
+// ------------------------------------------------------------------ BC
+// Brandes from up to 64 sources at once (bc.cc:71-231 per source).  Lane k
+// of a group holds source k; depth, path count and the backward coefficient
+// of a vertex are stored for all sources together (vertex-major), so one
+// random access of a neighbour serves the whole group.  The union frontier
+// of level d lists every vertex that some source reaches at depth d (once:
+// a per-vertex level tag de-duplicates the pushes), and a row's lanes act
+// only where the vertex is at depth d for their source.  A lane's successor
+// test is depth[v] == depth[u] + 1, the relation the reference's successor
+// bitmap records (bc.cc:104-112), so no per-source bitmap is kept.
+struct BcbCounters {
+  unsigned long long tail;
+  unsigned long long edges_scanned;
+  int overflow;
+  int pad;
+};
+
+template <int Kp>
+__device__ __forceinline__ double group_sum_d(double v) {
+#pragma unroll
+  for (int o = Kp / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Calls f(x, on) on batches of kBatchUnroll edges of [s, s + len): for
+// Kp == 64 the row is wave-uniform and 64 ids come in with one coalesced
+// load (broadcast by readlane), otherwise every group walks its own row.
+template <int Kp, class F>
+__device__ void bcb_edges(const int32_t *nbr, int64_t s, int64_t len, F &f) {
+  int32_t x[kBatchUnroll];
+  bool on[kBatchUnroll];
+  if constexpr (Kp == kWave) {
+    s = uniform64(s);
+    len = uniform64(len);
+    const int lane = lane_id();
+    for (int64_t c0 = 0; c0 < len; c0 += kWave) {
+      const int cnt = static_cast<int>(len - c0 < kWave ? len - c0 : kWave);
+      const int32_t vx = lane < cnt ? nbr[s + c0 + lane] : 0;
+      for (int j0 = 0; j0 < cnt; j0 += kBatchUnroll) {
+#pragma unroll
+        for (int j = 0; j < kBatchUnroll; j++) {
+          const int at = j0 + j < kWave ? j0 + j : kWave - 1;  // lanes past cnt hold vertex 0
+          x[j] = __builtin_amdgcn_readlane(vx, at);
+          on[j] = j0 + j < cnt;
+        }
+        f(x, on);
+      }
+    }
+  } else {
+    const int64_t trips = wave_max(len);
+    for (int64_t j0 = 0; j0 < trips; j0 += kBatchUnroll) {
+#pragma unroll
+      for (int j = 0; j < kBatchUnroll; j++) {
+        on[j] = j0 + j < len;
+        x[j] = on[j] ? nbr[s + j0 + j] : 0;
+      }
+      f(x, on);
+    }
+  }
+}
+
+template <int Kp>
+struct BcbForward {
+  int32_t *depth;
+  double *sigma;
+  uint32_t *ltag;
+  WaveQueue q;
+  int32_t d;
+  uint32_t tag;
+  double su;
+  bool act;
+  int k;
+  bool leader;
+  __device__ void operator()(const int32_t (&x)[kBatchUnroll], const bool (&on)[kBatchUnroll]) {
+    int32_t dv[kBatchUnroll];
+#pragma unroll
+    for (int j = 0; j < kBatchUnroll; j++) dv[j] = depth[static_cast<int64_t>(x[j]) * Kp + k];
+#pragma unroll
+    for (int j = 0; j < kBatchUnroll; j++) {
+      const bool a = on[j] && act && (dv[j] == -1 || dv[j] == d + 1);
+      if (__ballot(a) == 0) continue;
+      bool claimed = false;
+      if (a) {
+        int32_t dd = dv[j];
+        if (dd == -1) {
+          const int32_t old = atomicCAS(&depth[static_cast<int64_t>(x[j]) * Kp + k], -1, d + 1);
+          claimed = old == -1;
+          dd = claimed ? d + 1 : old;
+        }
+        // path counts are integers below 2^53: the double sum is exact in any order
+        if (dd == d + 1) unsafeAtomicAdd(&sigma[static_cast<int64_t>(x[j]) * Kp + k], su);
+      }
+      const bool anyc = group_any<Kp>(claimed);
+      bool push = false;
+      if (leader && on[j] && anyc) push = atomicExch(&ltag[x[j]], tag) != tag;
+      q.push(push, x[j]);
+    }
+  }
+};
+
+template <int Kp>
+struct BcbBackward {
+  const int32_t *depth;
+  const float *coef;
+  int32_t d;
+  bool act;
+  int k;
+  double acc;
+  __device__ void operator()(const int32_t (&x)[kBatchUnroll], const bool (&on)[kBatchUnroll]) {
+    int32_t dv[kBatchUnroll];
+    float cf[kBatchUnroll];
+#pragma unroll
+    for (int j = 0; j < kBatchUnroll; j++) {
+      dv[j] = depth[static_cast<int64_t>(x[j]) * Kp + k];
+      cf[j] = coef[static_cast<int64_t>(x[j]) * Kp + k];
+    }
+#pragma unroll
+    for (int j = 0; j < kBatchUnroll; j++)
+      if (on[j] && act && dv[j] == d + 1) acc += static_cast<double>(cf[j]);
+  }
+};
+
+__global__ void bcb_init_counters(BcbCounters *c, long long first) {
+  if (threadIdx.x == 0) {
+    c->tail = static_cast<unsigned long long>(first);
+    c->edges_scanned = 0;
+    c->overflow = 0;
+  }
+}
+
+__global__ void bcb_sources(const int32_t *src, int K, int Kp, int32_t *depth, double *sigma) {
+  const int j = threadIdx.x;
+  if (j < K) {
+    depth[static_cast<int64_t>(src[j]) * Kp + j] = 0;
+    sigma[static_cast<int64_t>(src[j]) * Kp + j] = 1.0;
+  }
+}
+
+__device__ __forceinline__ void bcb_queue_heavy(const HeavyQueue &hq, int32_t row, int64_t s, int64_t len) {
+  const unsigned long long at = atomicAdd(hq.count, 1ull);
+  if (static_cast<int64_t>(at) < hq.cap) {
+    hq.row[at] = row;
+    hq.start[at] = s;
+    hq.len[at] = len;
+  } else {
+    *hq.overflow = 1;
+  }
+}
+
+// Heavy chunk c of the queue: its row and edge range (binary search in the
+// chunk prefix).
+__device__ __forceinline__ void bcb_chunk(const HeavyQueue &hq, int64_t rows, int64_t c, int64_t &h,
+                                          int64_t &first, int64_t &len) {
+  int64_t lo = 0, hi = rows;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (hq.prefix[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  h = lo;
+  first = (c - hq.prefix[lo]) * hq.chunk;
+  const int64_t rest = hq.len[lo] - first;
+  len = rest < hq.chunk ? rest : hq.chunk;
+}
+
+// Forward level d over the union frontier list[0, count); rows above
+// kBatchHeavy go to the chunked kernel.
+template <int Kp>
+__global__ void __launch_bounds__(kBlock)
+bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *ltag,
+            const int32_t *list, int64_t count, int32_t *queue, int64_t cap, int32_t d, BcbCounters *c,
+            HeavyQueue hq) {
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  constexpr int G = kWave / Kp;
+  const int grp = lane_id() / Kp;
+  BcbForward<Kp> f;
+  f.depth = depth;
+  f.sigma = sigma;
+  f.ltag = ltag;
+  f.q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
+  f.d = d;
+  f.tag = static_cast<uint32_t>(d) + 2;
+  f.k = lane_id() % Kp;
+  f.leader = f.k == 0;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  unsigned long long walked = 0;
+  for (int64_t base = gw * G; base < count; base += waves * G) {
+    const int64_t i = base + grp;
+    const bool valid = i < count;
+    int64_t row = 0, s = 0, len = 0;
+    if (valid) {
+      row = list[i];
+      s = off[row];
+      len = off[row + 1] - s;
+    }
+    f.act = valid && depth[row * Kp + f.k] == d;
+    f.su = f.act ? sigma[row * Kp + f.k] : 0.0;
+    const bool heavy = len > kBatchHeavy;
+    if (heavy && f.leader) bcb_queue_heavy(hq, static_cast<int32_t>(row), s, len);
+    bcb_edges<Kp>(nbr, s, heavy ? 0 : len, f);
+    if (f.leader && !heavy) walked += static_cast<unsigned long long>(len);
+  }
+  f.q.flush();
+  walked = wave_sum(walked);
+  if (lane_id() == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+}
+
+template <int Kp>
+__global__ void __launch_bounds__(kBlock)
+bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *ltag, int32_t *queue,
+                  int64_t cap, int32_t d, BcbCounters *c, HeavyQueue hq) {
+  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  if (rows == 0) return;
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  constexpr int G = kWave / Kp;
+  const int grp = lane_id() / Kp;
+  BcbForward<Kp> f;
+  f.depth = depth;
+  f.sigma = sigma;
+  f.ltag = ltag;
+  f.q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
+  f.d = d;
+  f.tag = static_cast<uint32_t>(d) + 2;
+  f.k = lane_id() % Kp;
+  f.leader = f.k == 0;
+  const int64_t total = hq.prefix[rows];
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
+  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  unsigned long long walked = 0;
+  for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
+    const int64_t cc = c0 + grp;
+    const bool valid = cc < total;
+    int64_t h = 0, first = 0, len = 0, row = 0;
+    if (valid) {
+      bcb_chunk(hq, rows, cc, h, first, len);
+      row = hq.row[h];
+    }
+    f.act = valid && depth[row * Kp + f.k] == d;
+    f.su = f.act ? sigma[row * Kp + f.k] : 0.0;
+    bcb_edges<Kp>(nbr, valid ? hq.start[h] + first : 0, len, f);
+    if (f.leader) walked += static_cast<unsigned long long>(len);
+  }
+  f.q.flush();
+  walked = wave_sum(walked);
+  if (lane_id() == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+}
+
+// Backward level d: delta of a lane = sigma[u] * sum of coef[v] over its
+// successors; coef[u] = (1 + delta) / sigma[u] for the level above, and the
+// row's score gains the deltas of all its lanes at this depth.
+template <int Kp>
+__global__ void __launch_bounds__(kBlock)
+bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const double *sigma, float *coef,
+             float *scores, const int32_t *list, int64_t count, int32_t d, BcbCounters *c, HeavyQueue hq) {
+  constexpr int G = kWave / Kp;
+  const int grp = lane_id() / Kp;
+  BcbBackward<Kp> f;
+  f.depth = depth;
+  f.coef = coef;
+  f.d = d;
+  f.k = lane_id() % Kp;
+  const bool leader = f.k == 0;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  unsigned long long walked = 0;
+  for (int64_t base = gw * G; base < count; base += waves * G) {
+    const int64_t i = base + grp;
+    const bool valid = i < count;
+    int64_t row = 0, s = 0, len = 0;
+    if (valid) {
+      row = list[i];
+      s = off[row];
+      len = off[row + 1] - s;
+    }
+    f.act = valid && depth[row * Kp + f.k] == d;
+    const bool heavy = len > kBatchHeavy;
+    if (heavy && leader) bcb_queue_heavy(hq, static_cast<int32_t>(row), s, len);
+    f.acc = 0.0;
+    bcb_edges<Kp>(nbr, s, heavy ? 0 : len, f);
+    double delta = 0.0;
+    if (f.act && !heavy) {
+      const double su = sigma[row * Kp + f.k];
+      delta = su * f.acc;
+      coef[row * Kp + f.k] = static_cast<float>((1.0 + delta) / su);
+    }
+    const double tot = group_sum_d<Kp>(delta);
+    if (leader && valid && !heavy && tot != 0.0) scores[row] += static_cast<float>(tot);
+    if (leader && !heavy) walked += static_cast<unsigned long long>(len);
+  }
+  walked = wave_sum(walked);
+  if (lane_id() == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+}
+
+// Heavy rows of a backward level: hpart[chunk][k] = lane k's coefficient
+// sum over the chunk's successors.
+template <int Kp>
+__global__ void __launch_bounds__(kBlock)
+bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const float *coef, int32_t d, BcbCounters *c,
+                   HeavyQueue hq, double *hpart) {
+  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  if (rows == 0) return;
+  constexpr int G = kWave / Kp;
+  const int grp = lane_id() / Kp;
+  BcbBackward<Kp> f;
+  f.depth = depth;
+  f.coef = coef;
+  f.d = d;
+  f.k = lane_id() % Kp;
+  const int64_t total = hq.prefix[rows];
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
+  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  unsigned long long walked = 0;
+  for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
+    const int64_t cc = c0 + grp;
+    const bool valid = cc < total;
+    int64_t h = 0, first = 0, len = 0, row = 0;
+    if (valid) {
+      bcb_chunk(hq, rows, cc, h, first, len);
+      row = hq.row[h];
+    }
+    f.act = valid && depth[row * Kp + f.k] == d;
+    f.acc = 0.0;
+    bcb_edges<Kp>(nbr, valid ? hq.start[h] + first : 0, len, f);
+    if (valid) hpart[cc * Kp + f.k] = f.acc;
+    if (f.k == 0) walked += static_cast<unsigned long long>(len);
+  }
+  walked = wave_sum(walked);
+  if (lane_id() == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+}
+
+// One group per heavy row: the chunks' sums added in chunk order.
+template <int Kp>
+__global__ void __launch_bounds__(kBlock)
+bcb_backward_finish(const int32_t *depth, const double *sigma, float *coef, float *scores, int32_t d,
+                    HeavyQueue hq, const double *hpart) {
+  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  constexpr int G = kWave / Kp;
+  const int k = lane_id() % Kp;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
+  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  for (int64_t base = gw * G; base < rows; base += stride) {
+    const int64_t h = base + lane_id() / Kp;
+    const bool valid = h < rows;
+    double delta = 0.0;
+    int32_t row = 0;
+    if (valid) {
+      row = hq.row[h];
+      if (depth[static_cast<int64_t>(row) * Kp + k] == d) {
+        double tot = 0.0;
+        for (int64_t cc = hq.prefix[h]; cc < hq.prefix[h + 1]; cc++) tot += hpart[cc * Kp + k];
+        const double su = sigma[static_cast<int64_t>(row) * Kp + k];
+        delta = su * tot;
+        coef[static_cast<int64_t>(row) * Kp + k] = static_cast<float>((1.0 + delta) / su);
+      }
+    }
+    const double tot = group_sum_d<Kp>(delta);
+    if (k == 0 && valid && tot != 0.0) scores[row] += static_cast<float>(tot);
+  }
+}
+
 template <int Kp>
 static void launch_sb_relax(gbk_ctx *ctx, const gbk_graph *g, int32_t *dist, int32_t *pend, uint32_t *vtag,
                             int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, BatchState *st,
@@ -881,6 +1242,99 @@ int sssp_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K
   return GBK_OK;
 }
 
+// Brandes for one chunk of K <= 64 sources; deltas are added into 'scores'
+// (device, not normalised).
+template <int Kp>
+int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, float *scores,
+                 gbk_batch_stats *acc) {
+  const int64_t n = g->n;
+  hipStream_t s = ctx->stream;
+  // every vertex enters at most one level per distinct depth of its lanes
+  const int64_t cap = n * Kp + 64;
+  auto *depth = static_cast<int32_t *>(scratch(ctx, kS0, sizeof(int32_t) * n * Kp + 64));
+  auto *sigma = static_cast<double *>(scratch(ctx, kS1, sizeof(double) * n * Kp + 64));
+  auto *coef = static_cast<float *>(scratch(ctx, kS2, sizeof(float) * n * Kp + 64));
+  auto *queue = static_cast<int32_t *>(scratch(ctx, kS3, sizeof(int32_t) * cap));
+  auto *c = static_cast<BcbCounters *>(scratch(ctx, kS5, sizeof(BcbCounters) + 64));
+  auto *ltag = static_cast<uint32_t *>(scratch(ctx, kS8, sizeof(uint32_t) * (n + 1)));
+  auto *hc = static_cast<BcbCounters *>(pinned(ctx, sizeof(BcbCounters)));
+  if (!depth || !sigma || !coef || !queue || !c || !ltag || !hc) return set_error(ctx, GBK_ENOMEM, "bc batch scratch");
+  HeavyQueue hq;
+  {
+    const int64_t H = ctx->tune.heavy_degree;
+    ctx->tune.heavy_degree = kBatchHeavy;  // rows above kBatchHeavy are chunked
+    const bool ok = make_heavy_queue(ctx, g, kS7, &hq);
+    ctx->tune.heavy_degree = H;
+    if (!ok) return set_error(ctx, GBK_ENOMEM, "bc batch heavy queue");
+  }
+  hq.overflow = &c->overflow;
+  hq.chunk = kBatchChunk;
+  auto *hpart = static_cast<double *>(scratch(ctx, kS9, sizeof(double) * (g->m / kBatchChunk + hq.cap + 1) * Kp));
+  if (!hpart) return set_error(ctx, GBK_ENOMEM, "bc batch heavy partials");
+  std::vector<int32_t> uniq(src, src + K);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  int32_t *dsrc = nullptr;
+  if (int rc = upload_sources(ctx, src, K, &dsrc)) return rc;
+  const unsigned grid = static_cast<unsigned>(ctx->tune.relax_grid);
+  GBK_HIP(ctx, hipMemcpyAsync(queue, uniq.data(), sizeof(int32_t) * uniq.size(), hipMemcpyHostToDevice, s));
+  fill<int32_t>(ctx, depth, -1, n * Kp);
+  fill<double>(ctx, sigma, 0.0, n * Kp);
+  fill<uint32_t>(ctx, ltag, 0u, n);
+  bcb_init_counters<<<1, 1, 0, s>>>(c, static_cast<long long>(uniq.size()));
+  bcb_sources<<<1, kBatchMax, 0, s>>>(dsrc, K, Kp, depth, sigma);
+  std::vector<int64_t> level = {0, static_cast<int64_t>(uniq.size())};
+  for (int32_t d = 0;; d++) {
+    const int64_t lb = level[level.size() - 2], le = level.back();
+    fill<unsigned long long>(ctx, hq.count, 0ull, 1);
+    const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
+    bcb_forward<Kp><<<grid, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, ltag, queue + lb, le - lb, queue, cap,
+                                            d, c, hq);
+    heavy_prefix<<<1, 1024, 0, s>>>(hq);
+    bcb_forward_heavy<Kp><<<grid, kBlock, 0, s>>>(g->nbr, depth, sigma, ltag, queue, cap, d, c, hq);
+    kt_end(ctx, kt);
+    GBK_LAUNCH_CHECK(ctx, "bc batch forward");
+    GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcbCounters), hipMemcpyDeviceToHost, s));
+    GBK_HIP(ctx, hipStreamSynchronize(s));
+    if (hc->overflow || static_cast<int64_t>(hc->tail) > cap) return set_error(ctx, GBK_EHIP, "bc batch: queue overflow");
+    const int64_t tail = static_cast<int64_t>(hc->tail);
+    if (tail == le) break;
+    level.push_back(tail);
+  }
+  const int levels = static_cast<int>(level.size()) - 1;
+  for (int32_t d = levels - 1; d >= 0; d--) {
+    const int64_t lb = level[d], cnt = level[d + 1] - lb;
+    fill<unsigned long long>(ctx, hq.count, 0ull, 1);
+    const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
+    bcb_backward<Kp><<<grid, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, coef, scores, queue + lb, cnt, d, c, hq);
+    heavy_prefix<<<1, 1024, 0, s>>>(hq);
+    bcb_backward_heavy<Kp><<<grid, kBlock, 0, s>>>(g->nbr, depth, coef, d, c, hq, hpart);
+    bcb_backward_finish<Kp><<<64, kBlock, 0, s>>>(depth, sigma, coef, scores, d, hq, hpart);
+    kt_end(ctx, kt);
+  }
+  GBK_LAUNCH_CHECK(ctx, "bc batch backward");
+  int64_t etot = 0;
+  if (int rc = collect(ctx, g, depth, Kp, K, -1, nullptr, nullptr, nullptr, 0, &etot)) return rc;
+  GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcbCounters), hipMemcpyDeviceToHost, s));
+  GBK_HIP(ctx, hipStreamSynchronize(s));
+  if (hc->overflow) return set_error(ctx, GBK_EHIP, "bc batch: heavy queue overflow");
+  acc->rounds += levels;
+  acc->edges_reached += etot;
+  acc->rows_scanned += level.back();
+  acc->edges_scanned += static_cast<int64_t>(hc->edges_scanned);
+  return GBK_OK;
+}
+
+int bc_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, float *scores,
+                   gbk_batch_stats *acc) {
+  switch (batch_width(K)) {
+    case 8: return bc_batch_run<8>(ctx, g, src, K, scores, acc);
+    case 16: return bc_batch_run<16>(ctx, g, src, K, scores, acc);
+    case 32: return bc_batch_run<32>(ctx, g, src, K, scores, acc);
+    default: return bc_batch_run<64>(ctx, g, src, K, scores, acc);
+  }
+}
+
 }  // namespace
 
 extern "C" int gbk_bfs_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int num_sources,
@@ -907,6 +1361,34 @@ extern "C" int gbk_sssp_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *s
     const int K = std::min(kBatchMax, num_sources - first);
     if (int rc = sssp_batch_chunk(ctx, g, sources + first, K, delta, dist, reached, edges_reached, first, &acc))
       return rc;
+  }
+  if (stats) *stats = acc;
+  return GBK_OK;
+}
+
+extern "C" int gbk_bc_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int num_sources,
+                            float *scores_out, gbk_batch_stats *stats) {
+  if (int rc = check_sources(ctx, g, sources, num_sources)) return rc;
+  const int64_t n = g->n;
+  hipStream_t s = ctx->stream;
+  auto *scores = static_cast<float *>(scratch(ctx, kS10, sizeof(float) * (n + 1)));
+  auto *pmax = static_cast<unsigned int *>(scratch(ctx, kS11, sizeof(unsigned int) * kGridCap));
+  if (!scores || !pmax) return set_error(ctx, GBK_ENOMEM, "bc batch scores");
+  gbk_batch_stats acc{};
+  EventTimer timer(ctx);
+  fill<float>(ctx, scores, 0.0f, n);
+  for (int first = 0; first < num_sources; first += kBatchMax) {
+    const int K = std::min(kBatchMax, num_sources - first);
+    if (int rc = bc_batch_chunk(ctx, g, sources + first, K, scores, &acc)) return rc;
+  }
+  const unsigned ng = expand_grid(n);
+  bc_max<<<ng, kBlock, 0, s>>>(scores, n, pmax);
+  bc_normalize<<<ng, kBlock, 0, s>>>(scores, n, pmax, static_cast<int>(ng));
+  GBK_LAUNCH_CHECK(ctx, "bc batch normalize");
+  acc.device_ms = timer.stop_ms();
+  if (scores_out) {
+    GBK_HIP(ctx, hipMemcpyAsync(scores_out, scores, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    GBK_HIP(ctx, hipStreamSynchronize(s));
   }
   if (stats) *stats = acc;
   return GBK_OK;

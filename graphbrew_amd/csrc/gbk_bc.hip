@@ -14,8 +14,10 @@
 //             instead of a double and a float from two arrays.
 // Scores are divided by their maximum at the end (bc.cc:224-230).
 // Kernels: the forward uses the coalesced frontier expansion; the backward
-// walks each level's edges wave-wise with per-row LDS accumulators (rows
-// above the heavy threshold are reduced by a whole workgroup).
+// walks each level's edges wave-wise with per-row LDS accumulators; rows
+// above the heavy threshold are cut into chunks spread over every wave of
+// the grid, whose double partial sums a finish kernel adds per row in chunk
+// order (the same sum whatever the schedule).
 #include "gbk_expand.h"
 
 namespace gbk {
@@ -127,19 +129,16 @@ bc_forward_heavy(const int64_t *off, const int32_t *nbr, int32_t *depth, double 
 
 // One level of the dependency accumulation; coef[v] = (1 + delta[v]) /
 // sigma[v] of the deeper levels, written here for this level's vertices.
+// Rows above heavy_deg are queued for bc_backward_heavy.
 __global__ void __launch_bounds__(kBlock)
 bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, int64_t count,
                   const double *sigma, const uint32_t *succ, float *coef, float *scores,
-                  int64_t heavy_deg) {
+                  int64_t heavy_deg, HeavyQueue hq) {
   __shared__ float acc[kBlock];
-  __shared__ int32_t heavy_rows[kBlock];
-  __shared__ int heavy_n;
-  __shared__ float red[kWavesPerBlock];
   const int64_t edge_base = off[0];
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
   for (int64_t tile = blockIdx.x; tile * kBlock < count; tile += gridDim.x) {
-    if (threadIdx.x == 0) heavy_n = 0;
     acc[threadIdx.x] = 0.0f;
     __syncthreads();
     const int64_t i = tile * kBlock + threadIdx.x;
@@ -151,7 +150,14 @@ bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, i
       start = off[u];
       len = off[u + 1] - start;
       if (len > heavy_deg) {
-        heavy_rows[atomicAdd(&heavy_n, 1)] = u;
+        const unsigned long long at = atomicAdd(hq.count, 1ull);
+        if (static_cast<int64_t>(at) < hq.cap) {
+          hq.row[at] = u;
+          hq.start[at] = start;
+          hq.len[at] = len;
+        } else {
+          *hq.overflow = 1;
+        }
         len = 0;
       } else {
         light = true;
@@ -192,22 +198,64 @@ bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, i
       coef[u] = static_cast<float>((1.0 + du) / su);
       scores[u] += du;
     }
-    const int nh = heavy_n;
-    for (int h = 0; h < nh; h++) {
-      const int32_t hu = heavy_rows[h];
-      const int64_t hs = off[hu], he = off[hu + 1];
-      const double hsig = sigma[hu];
-      float part = 0.0f;
-      for (int64_t k = hs + threadIdx.x; k < he; k += blockDim.x) {
-        if (bit_get(succ, k - edge_base)) part += static_cast<float>(hsig * static_cast<double>(coef[nbr[k]]));
-      }
-      const float tot = block_sum(part, red);
-      if (threadIdx.x == 0) {
-        coef[hu] = static_cast<float>((1.0 + tot) / hsig);
-        scores[hu] += tot;
-      }
-    }
     __syncthreads();
+  }
+}
+
+// Chunks of the level's heavy rows, one wave each over the whole grid:
+// hpart[c] = sum over the successor edges of chunk c of sigma[u] * coef[v].
+__global__ void __launch_bounds__(kBlock)
+bc_backward_heavy(const int64_t *off, const int32_t *nbr, const double *sigma, const uint32_t *succ,
+                  const float *coef, HeavyQueue hq, double *hpart) {
+  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  if (rows == 0) return;
+  const int64_t edge_base = off[0];
+  const int64_t total = hq.prefix[rows];
+  const int lane = lane_id();
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t c = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave; c < total;
+       c += waves) {
+    int64_t lo = 0, hi = rows;  // last row with prefix[row] <= c
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (hq.prefix[mid] <= c) lo = mid;
+      else hi = mid;
+    }
+    const int64_t first = hq.start[lo] + (c - hq.prefix[lo]) * hq.chunk;
+    const int64_t rest = hq.start[lo] + hq.len[lo] - first;
+    const int64_t end = first + (rest < hq.chunk ? rest : hq.chunk);
+    const double su = sigma[hq.row[lo]];
+    double part = 0.0;
+    constexpr int U = 4;
+    for (int64_t k0 = first; k0 < end; k0 += U * kWave) {
+      int32_t v[U];
+      bool on[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int64_t k = k0 + j * kWave + lane;
+        on[j] = k < end && bit_get(succ, k - edge_base);
+        v[j] = on[j] ? nbr[k] : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++)
+        if (on[j]) part += su * static_cast<double>(coef[v[j]]);
+    }
+    part = wave_sum(part);
+    if (lane == 0) hpart[c] = part;
+  }
+}
+
+// One lane per heavy row: its chunks' partials added in order.
+__global__ void bc_backward_heavy_finish(HeavyQueue hq, const double *sigma, const double *hpart,
+                                         float *coef, float *scores) {
+  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t h = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; h < rows; h += stride) {
+    double tot = 0.0;
+    for (int64_t c = hq.prefix[h]; c < hq.prefix[h + 1]; c++) tot += hpart[c];
+    const int32_t u = hq.row[h];
+    coef[u] = static_cast<float>((1.0 + tot) / sigma[u]);
+    scores[u] += static_cast<float>(tot);
   }
 }
 
@@ -218,35 +266,6 @@ __global__ void bc_reach_edges(const int64_t *off, const int32_t *depth, int64_t
   for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride)
     if (depth[v] >= 0) e += static_cast<unsigned long long>(off[v + 1] - off[v]);
   block_partial(e, partial);
-}
-
-// Max over non-negative floats via their bit patterns (ordered like the values).
-__global__ void bc_max(const float *scores, int64_t n, unsigned int *partial) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  unsigned int x = 0;
-  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride)
-    x = max(x, __float_as_uint(scores[v]));
-  x = wave_max(x);
-  __shared__ unsigned int red[kWavesPerBlock];
-  if (lane_id() == 0) red[threadIdx.x / kWave] = x;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < kWavesPerBlock; i++) x = max(x, red[i]);
-    partial[blockIdx.x] = x;
-  }
-}
-
-__global__ void bc_normalize(float *scores, int64_t n, const unsigned int *partial, int count) {
-  __shared__ float biggest;
-  if (threadIdx.x == 0) {
-    unsigned int x = 0;
-    for (int i = 0; i < count; i++) x = max(x, partial[i]);
-    biggest = __uint_as_float(x);
-  }
-  __syncthreads();
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride)
-    scores[v] = scores[v] / biggest;
 }
 
 }  // namespace gbk
@@ -271,6 +290,9 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
   HeavyQueue hq;
   if (!partial || !make_heavy_queue(ctx, g, kS12, &hq)) return set_error(ctx, GBK_ENOMEM, "bc partials");
   hq.overflow = &c->overflow;
+  // chunks of one level's heavy rows: at most m / chunk + one per row
+  auto *hpart = static_cast<double *>(scratch(ctx, kS13, sizeof(double) * (g->m / hq.chunk + hq.cap + 1)));
+  if (!hpart) return set_error(ctx, GBK_ENOMEM, "bc heavy partials");
   const int64_t H = ctx->tune.heavy_degree;
   const int64_t cap = n + 64;
   const unsigned ng = expand_grid(n);
@@ -307,11 +329,19 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
     max_depth = std::max<int>(max_depth, static_cast<int>(level.size()) - 2);
     for (int lv = static_cast<int>(level.size()) - 2; lv >= 0; lv--) {
       const int64_t lb = level[lv], cnt = level[lv + 1] - lb;
+      fill<unsigned long long>(ctx, hq.count, 0ull, 1);
       const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
       bc_backward_level<<<expand_grid(cnt), kBlock, 0, s>>>(g->off, g->nbr, queue + lb, cnt, sigma,
-                                                           succ, coef, scores, H);
+                                                           succ, coef, scores, H, hq);
+      heavy_prefix<<<1, 1024, 0, s>>>(hq);
+      bc_backward_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->off, g->nbr, sigma, succ, coef, hq, hpart);
+      bc_backward_heavy_finish<<<4, kBlock, 0, s>>>(hq, sigma, hpart, coef, scores);
       kt_end(ctx, kt);
     }
+    GBK_LAUNCH_CHECK(ctx, "bc_backward_heavy");
+    GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcCounters), hipMemcpyDeviceToHost, s));
+    GBK_HIP(ctx, hipStreamSynchronize(s));
+    if (hc->overflow) return set_error(ctx, GBK_EHIP, "bc: heavy queue overflow");
     bc_reach_edges<<<ng, kBlock, 0, s>>>(g->off, depth, n, partial);
     sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial, static_cast<int>(ng), &c->reach, 1);
     GBK_LAUNCH_CHECK(ctx, "bc_backward_level");

@@ -549,4 +549,34 @@ __device__ __forceinline__ bool last_block_done(unsigned int *counter) {
   return is_last && threadIdx.x == 0;
 }
 
+// BC normalisation (bc.cc:224-230): max over non-negative floats via their
+// bit patterns (ordered like the values), then every score divided by it.
+static __global__ void bc_max(const float *scores, int64_t n, unsigned int *partial) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  unsigned int x = 0;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride)
+    x = max(x, __float_as_uint(scores[v]));
+  x = wave_max(x);
+  __shared__ unsigned int red[kWavesPerBlock];
+  if (lane_id() == 0) red[threadIdx.x / kWave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < kWavesPerBlock; i++) x = max(x, red[i]);
+    partial[blockIdx.x] = x;
+  }
+}
+
+static __global__ void bc_normalize(float *scores, int64_t n, const unsigned int *partial, int count) {
+  __shared__ float biggest;
+  if (threadIdx.x == 0) {
+    unsigned int x = 0;
+    for (int i = 0; i < count; i++) x = max(x, partial[i]);
+    biggest = __uint_as_float(x);
+  }
+  __syncthreads();
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride)
+    scores[v] = scores[v] / biggest;
+}
+
 }  // namespace gbk

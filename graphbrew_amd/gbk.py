@@ -116,6 +116,7 @@ def lib():
         "gbk_mlp_grad": (i32, [VP, VP, VP, VP, i64, VP, C.POINTER(f64)]),
         "gbk_mlp_train": (i32, [VP, VP, VP, VP, i64, i32, i32, C.c_float, C.c_uint64, VP]),
         "gbk_sssp_batch": (i32, [VP, VP, VP, i32, C.c_int32, VP, VP, VP, C.POINTER(BatchStats)]),
+        "gbk_bc_batch": (i32, [VP, VP, VP, i32, VP, C.POINTER(BatchStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name, None)
@@ -336,6 +337,16 @@ class Context:
         self._check(lib().gbk_sssp_batch(self.h, g.handle, _ptr(s), len(s), int(delta), _ptr(dist),
                                          _ptr(reached), _ptr(edges), C.byref(st)))
         return dist, reached, edges, st
+
+    def bc_batch(self, g: DeviceGraph, sources: Sequence[int], download=True):
+        """Brandes scores over all sources (normalised by the maximum, as
+        bc()), 64 sources per traversal; stats.rounds counts BFS levels and
+        stats.edges_reached the per-source reached out-edges (TEPS = 2x)."""
+        s = np.ascontiguousarray(sources, np.int32)
+        out = np.empty(g.n, np.float32) if download else None
+        st = BatchStats()
+        self._check(lib().gbk_bc_batch(self.h, g.handle, _ptr(s), len(s), _ptr(out), C.byref(st)))
+        return out, st
 
     # ------------------------------------------------ many small graphs
     def graph_union(self, graphs):

@@ -184,8 +184,8 @@ typedef struct {
   int64_t rounds;         /* BFS levels or SSSP relaxation rounds, summed */
   int64_t buckets;        /* SSSP buckets opened (0 for BFS) */
   int64_t edges_reached;  /* sum over sources of the out-degrees of reached vertices */
-  int64_t rows_scanned;   /* SSSP: queued rows relaxed over all rounds (0 for BFS) */
-  int64_t edges_scanned;  /* SSSP: their out-edges walked, each once for all sources */
+  int64_t rows_scanned;   /* SSSP: queued rows relaxed over all rounds; BC: frontier entries (0 for BFS) */
+  int64_t edges_scanned;  /* SSSP / BC: out-edges walked, each once for all sources of a chunk */
 } gbk_batch_stats;
 
 /* depth: host int32[num_sources * n] (source-major, -1 unreached) or NULL;
@@ -199,6 +199,14 @@ int gbk_sssp_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources,
                    int num_sources, int32_t delta, int32_t *dist,
                    int64_t *reached, int64_t *edges_reached,
                    gbk_batch_stats *stats);
+/* Brandes BC over all sources (bc.cc:134-231 runs them one per iteration of
+ * its source loop): up to 64 sources share each forward level and each
+ * backward level; scores (host float[n] or NULL) are summed over the sources
+ * and divided by their maximum as gbk_bc's, equal to them within float
+ * rounding.  stats: rounds = BFS levels, rows_scanned = union-frontier
+ * entries, edges_scanned = edges walked forward and backward. */
+int gbk_bc_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources,
+                 int num_sources, float *scores, gbk_batch_stats *stats);
 
 /* ---- many small graphs in one launch ------------------------------------ */
 /* No reference counterpart as a call (the reference runs one graph per

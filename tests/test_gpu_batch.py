@@ -130,6 +130,48 @@ def test_sssp_batch_equals_single_source_counts(ctx):
     dg.free()
 
 
+BC_GRAPHS = {
+    "kron13": lambda: O.kron(13),
+    "uniform12": lambda: O.kron(12, uniform=True),
+    "star3000": lambda: _star(3000),
+    "path": lambda: _fixture("path.el"),
+    "disconnected": lambda: _fixture("disconnected.el"),
+    "directed": lambda: _directed(),
+}
+
+
+@pytest.mark.parametrize("name", list(BC_GRAPHS))
+@pytest.mark.parametrize("k", [1, 4, 17, 64, 70])
+def test_bc_batch_matches_brandes(ctx, name, k):
+    """Scores over all sources against the serial Brandes oracle (1e-4
+    relative and the reference's BCScoreWithinTolerance), the per-source
+    reached out-edges against the oracle's BFS."""
+    g = BC_GRAPHS[name]()
+    dg = ctx.upload_graph(g)
+    srcs = _sources(g, k, seed=200 + k)
+    srcs[-1] = srcs[0]  # a repeated source counts twice, as in the reference's loop
+    scores, st = ctx.bc_batch(dg, srcs)
+    ref = O.bc_serial(g, srcs)
+    bad = [i for i in range(g.n) if not O.bc_within_tolerance(float(ref[i]), float(scores[i]))]
+    assert not bad, (name, k, bad[:5])
+    np.testing.assert_allclose(scores, ref, rtol=1e-4, atol=1e-6)
+    deg = np.diff(g.off)
+    want = sum(int(deg[O.bfs_depths(g, int(s)) >= 0].sum()) for s in srcs)
+    assert st.edges_reached == want
+    assert st.rounds >= len(srcs) // 64
+    dg.free()
+
+
+def test_bc_batch_equals_one_source_runs(ctx):
+    g = O.kron(14)
+    dg = ctx.upload_graph(g)
+    srcs = O.source_picker(g, 8)[1]
+    batch, _ = ctx.bc_batch(dg, srcs)
+    single, _ = ctx.bc(dg, srcs)
+    np.testing.assert_allclose(batch, single, rtol=1e-4, atol=1e-6)
+    dg.free()
+
+
 def test_batch_rejects_bad_input(ctx):
     g = O.kron(8, weighted=True)
     dg = ctx.upload_graph(g)
@@ -139,6 +181,8 @@ def test_batch_rejects_bad_input(ctx):
         ctx.sssp_batch(dg, [-1], 4)
     with pytest.raises(gbk.GbkError):
         ctx.sssp_batch(dg, [0], 0)
+    with pytest.raises(gbk.GbkError):
+        ctx.bc_batch(dg, [g.n])
     gu = ctx.upload_graph(O.kron(8))
     with pytest.raises(gbk.GbkError):
         ctx.sssp_batch(gu, [0], 4)

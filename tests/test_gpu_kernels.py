@@ -171,6 +171,22 @@ def test_bc_parity(ctx, name):
     np.testing.assert_allclose(scores, ref, rtol=1e-4, atol=1e-6)
 
 
+def test_bc_parity_with_many_heavy_rows(ctx):
+    """A heavy threshold of 64 sends most rows of every level through the
+    chunked heavy path of both passes."""
+    g = O.kron(13)
+    dg = ctx.upload_graph(g)
+    _, srcs = O.source_picker(g, 4)
+    old = ctx.get_param("heavy_degree")
+    ctx.set_param("heavy_degree", 64)
+    try:
+        scores, _ = ctx.bc(dg, srcs)
+    finally:
+        ctx.set_param("heavy_degree", old)
+    ref = O.bc_serial(g, srcs)
+    np.testing.assert_allclose(scores, ref, rtol=1e-4, atol=1e-6)
+
+
 def test_kernels_after_relabel_are_ordering_invariant(ctx):
     """Distances / components / reachability of original ids survive any
     relabelling (sssp.cc:92-127 fingerprints; bench/tests/test_relabel_weights.cc:167-206)."""
