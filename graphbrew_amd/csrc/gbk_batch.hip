@@ -1190,17 +1190,9 @@ int sssp_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K
   if (int rc = upload_sources(ctx, src, K, &dsrc)) return rc;
   const unsigned grid = static_cast<unsigned>(ctx->tune.relax_grid);
   const unsigned scan_grid = expand_grid(n);
-  EventTimer timer(ctx);
-  GBK_HIP(ctx, hipMemcpyAsync(near0, uniq.data(), sizeof(int32_t) * uniq.size(), hipMemcpyHostToDevice, s));
-  fill<int32_t>(ctx, dist, kBatchInf, n * Kp);
-  fill<int32_t>(ctx, pend, INT32_MAX, n);
-  fill<uint32_t>(ctx, vtag, 0u, n);
-  sb_sources<<<1, kBatchMax, 0, s>>>(dsrc, K, Kp, dist);
-  sb_init_state<<<1, 1, 0, s>>>(st, static_cast<long long>(uniq.size()));
-  int batch = 2;
-  int64_t launched = 0;
-  for (;;) {
-    for (int j = 0; j < batch; j++) {
+  // one step: relaxation round, then the bucket bookkeeping (each kernel
+  // returns at once unless the state's phase is its own)
+  auto step = [&]() {
       const int kt = kt_begin(ctx, GBK_KT_SSSP_RELAX);
       switch (Kp) {
         case 8: launch_sb_relax<8>(ctx, g, dist, pend, vtag, near0, near1, cap, delta, st, hq, grid); break;
@@ -1214,7 +1206,27 @@ int sssp_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K
       sb_min_done<<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), st);
       sb_split<<<scan_grid, kBlock, 0, s>>>(pend, n, near0, near1, cap, st);
       sb_split_done<<<1, kWave, 0, s>>>(st);
-    }
+  };
+  // steps are queued blindly in blocks of kStepsPerGraph (a captured graph)
+  // and the host reads the state after 4, 8, ... 64 of them
+  constexpr int kStepsPerGraph = 4;
+  auto block = [&]() {
+    for (int j = 0; j < kStepsPerGraph; j++) step();
+  };
+  LoopGraph loop(ctx);
+  if (int rc = loop.capture(block)) return rc;
+  EventTimer timer(ctx);
+  GBK_HIP(ctx, hipMemcpyAsync(near0, uniq.data(), sizeof(int32_t) * uniq.size(), hipMemcpyHostToDevice, s));
+  fill<int32_t>(ctx, dist, kBatchInf, n * Kp);
+  fill<int32_t>(ctx, pend, INT32_MAX, n);
+  fill<uint32_t>(ctx, vtag, 0u, n);
+  sb_sources<<<1, kBatchMax, 0, s>>>(dsrc, K, Kp, dist);
+  sb_init_state<<<1, 1, 0, s>>>(st, static_cast<long long>(uniq.size()));
+  int batch = kStepsPerGraph;
+  int64_t launched = 0;
+  for (;;) {
+    for (int j = 0; j < batch; j += kStepsPerGraph)
+      if (int rc = loop.run(block)) return rc;
     launched += batch;
     GBK_LAUNCH_CHECK(ctx, "sssp batch step");
     GBK_HIP(ctx, hipMemcpyAsync(hs, st, sizeof(BatchState), hipMemcpyDeviceToHost, s));

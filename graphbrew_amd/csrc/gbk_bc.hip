@@ -82,7 +82,10 @@ struct BcForwardEdge {
       const int64_t w = __shfl_up(word, o);
       if (lane >= o && w == word) bits |= b;
     }
-    const bool tail = lane == kWave - 1 || __shfl_down(word, 1) != word;
+    // The shuffle runs on every lane: inside the short-circuit it would run
+    // only on lanes 0..62, and a lane reading a disabled lane gets 0.
+    const int64_t next = __shfl_down(word, 1);
+    const bool tail = lane == kWave - 1 || next != word;
     if (tail && bits) atomicOr(&succ[word], bits);
     q.push(claimed, v);
   }

@@ -35,6 +35,7 @@ struct Tuning {
   int bfs_beta = 18;
   int xcd_swizzle = 1;          // contiguous vertex ranges per XCD
   int relax_grid = 1024;        // workgroups of a frontier (queue) launch
+  int graphs = 1;               // iteration loops replayed as captured hipGraphs
 };
 constexpr int kGridCapRuntime = 2048;  // upper bound of relax_grid
 constexpr int64_t kHeavyChunkEdges = 1024;  // heavy-row chunk of the all-rows kernels
@@ -251,3 +252,53 @@ constexpr int kKtPartials = 2048;  // partial slots per counter in kt_dev
     hipError_t _e = hipGetLastError();                                  \
     if (_e != hipSuccess) return gbk::check_hip((ctx), _e, what);       \
   } while (0)
+
+namespace gbk {
+
+// A block of loop iterations recorded once per call as a hipGraph (stream
+// capture of the kernels 'body' launches on ctx->stream) and replayed.
+// Callers capture before their timer starts, so the device time covers the
+// replays only.  With capture off (parameter "graphs" = 0) or per-launch
+// timing on (its events stay outside graphs), run() launches the body
+// eagerly.  A body launches kernels only: no memset nodes, no host
+// synchronisation.
+class LoopGraph {
+ public:
+  explicit LoopGraph(gbk_ctx *ctx) : ctx_(ctx) {}
+  LoopGraph(const LoopGraph &) = delete;
+  LoopGraph &operator=(const LoopGraph &) = delete;
+  ~LoopGraph() {
+    if (exec_) hipGraphExecDestroy(exec_);
+  }
+  template <class Body>
+  int capture(Body &&body) {
+    if (!ctx_->tune.graphs || ctx_->ktime != 0) return 0;
+    hipGraph_t graph = nullptr;
+    GBK_HIP(ctx_, hipStreamBeginCapture(ctx_->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    const hipError_t e = hipStreamEndCapture(ctx_->stream, &graph);
+    if (e != hipSuccess) return check_hip(ctx_, e, "loop graph capture");
+    const hipError_t e2 = hipGraphInstantiate(&exec_, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e2 != hipSuccess) {
+      exec_ = nullptr;
+      return check_hip(ctx_, e2, "loop graph instantiate");
+    }
+    return 0;
+  }
+  template <class Body>
+  int run(Body &&body) {
+    if (!exec_) {
+      body();
+      return 0;
+    }
+    GBK_HIP(ctx_, hipGraphLaunch(exec_, ctx_->stream));
+    return 0;
+  }
+
+ private:
+  gbk_ctx *ctx_;
+  hipGraphExec_t exec_ = nullptr;
+};
+
+}  // namespace gbk

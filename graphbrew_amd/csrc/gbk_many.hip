@@ -334,10 +334,9 @@ extern "C" int gbk_pr_many(gbk_ctx *ctx, const gbk_graph *g, const int64_t *vbas
   auto *gdone = reinterpret_cast<int *>(ginit + count + 1);
   auto *giters = gdone + count + 1;
   const float kDamp = 0.85f;
-  EventTimer timer(ctx);
-  fill<float>(ctx, hacc, 0.0f, rt.nheavy + 1);
-  many_graph_init<<<grid_for(count), kBlock, 0, s>>>(dvb, count, kDamp, gbase, ginit, gerr, gdone, giters, st);
-  many_init<<<grid_for(n), kBlock, 0, s>>>(g->off, gid, ginit, n, scores, c0);
+  // all iterations are queued (one captured graph); finished graphs and a
+  // finished batch make the kernels return at once
+  auto iterations = [&]() {
   for (int it = 0; it < max_iters; it++) {
     float *cur = c0, *nxt = c0;
     if (variant == GBK_PR_GS) {
@@ -356,6 +355,14 @@ extern "C" int gbk_pr_many(gbk_ctx *ctx, const gbk_graph *g, const int64_t *vbas
     many_finish<<<1, 1024, 0, s>>>(rt.heavy, rt.nheavy, hacc, gid, gbase, gdone, giters, gerr, count, it, tol,
                                    fixed_work, g->off, scores, nxt, kDamp, st);
   }
+  };
+  LoopGraph loop(ctx);
+  if (int rc = loop.capture(iterations)) return rc;
+  EventTimer timer(ctx);
+  fill<float>(ctx, hacc, 0.0f, rt.nheavy + 1);
+  many_graph_init<<<grid_for(count), kBlock, 0, s>>>(dvb, count, kDamp, gbase, ginit, gerr, gdone, giters, st);
+  many_init<<<grid_for(n), kBlock, 0, s>>>(g->off, gid, ginit, n, scores, c0);
+  if (int rc = loop.run(iterations)) return rc;
   GBK_LAUNCH_CHECK(ctx, "pr many");
   const double ms = timer.stop_ms();
   ManyState h{};

@@ -184,9 +184,9 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   const float kDamp = 0.85f;
   const float init = 1.0f / static_cast<float>(n);
   const float base = (1.0f - kDamp) / static_cast<float>(n);
-  EventTimer timer(ctx);
-  fill<float>(ctx, hacc, 0.0f, rt.nheavy + 1);
-  pr_init<<<grid_for(n), kBlock, 0, s>>>(g->off, n, init, scores, c0, st);
+  // all iterations are queued (one captured graph); after convergence the
+  // kernels return at once on the state's done flag
+  auto iterations = [&]() {
   for (int it = 0; it < max_iters; it++) {
     const int kt = kt_begin(ctx, GBK_KT_PR_ITER);
     float *cur = c0, *nxt = c0;
@@ -203,6 +203,13 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
     pr_finish<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), st, errs, it, tol, fixed_work, rt.heavy,
                                  rt.nheavy, hacc, g->off, scores, nxt, base, kDamp);
   }
+  };
+  LoopGraph loop(ctx);
+  if (int rc = loop.capture(iterations)) return rc;
+  EventTimer timer(ctx);
+  fill<float>(ctx, hacc, 0.0f, rt.nheavy + 1);
+  pr_init<<<grid_for(n), kBlock, 0, s>>>(g->off, n, init, scores, c0, st);
+  if (int rc = loop.run(iterations)) return rc;
   GBK_LAUNCH_CHECK(ctx, "pr_pull_iter");
   const double ms = timer.stop_ms();
   PrState h{};

@@ -386,6 +386,7 @@ int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value) {
   else if (k == "xcd_swizzle" && (value == 0 || value == 1)) t.xcd_swizzle = static_cast<int>(value);
   else if (k == "relax_grid" && value >= 64 && value <= kGridCapRuntime && value % 8 == 0)
     t.relax_grid = static_cast<int>(value);
+  else if (k == "graphs" && (value == 0 || value == 1)) t.graphs = static_cast<int>(value);
   else if (k == "ktime" && value >= GBK_KT_NONE && value <= GBK_KT_LAST) {
     if (value != GBK_KT_NONE && !ctx->kt_dev) {
       const size_t bytes = sizeof(unsigned long long) * (2 + 2 * kKtPartials);
@@ -411,6 +412,7 @@ int gbk_ctx_get_param(gbk_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "bfs_beta") *value = t.bfs_beta;
   else if (k == "xcd_swizzle") *value = t.xcd_swizzle;
   else if (k == "relax_grid") *value = t.relax_grid;
+  else if (k == "graphs") *value = t.graphs;
   else if (k == "ktime") *value = ctx->ktime;
   else return set_error(ctx, GBK_EINVAL, "unknown parameter: " + k);
   return GBK_OK;

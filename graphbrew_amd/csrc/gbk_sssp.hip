@@ -324,12 +324,9 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   const bool count = ctx->ktime == GBK_KT_SSSP_RELAX && ctx->kt_dev;
   const unsigned count_grid = static_cast<unsigned>(kKtPartials < 256 ? kKtPartials : 256);
   const unsigned scan_grid = expand_grid(n);
-  EventTimer timer(ctx);
-  sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, word, near0, st);
-  int batch = 2;
-  int64_t launched = 0;
-  for (;;) {
-    for (int j = 0; j < batch; j++) {
+  // one bucket step: relaxation round, then the bucket bookkeeping (each
+  // kernel returns at once unless the state's phase is its own)
+  auto step = [&]() {
       if (count) {
         unsigned long long *part = ctx->kt_dev + 2;
         sssp_count_round<<<count_grid, kBlock, 0, s>>>(g->off, word, near0, near1, delta, st, part);
@@ -349,7 +346,22 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
       sssp_min_done<<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), st);
       sssp_split<<<scan_grid, kBlock, 0, s>>>(word, n, near0, near1, cap, delta, st);
       sssp_split_done<<<1, kWave, 0, s>>>(st);
-    }
+  };
+  // steps are queued blindly in blocks of kStepsPerGraph (a captured graph)
+  // and the host reads the state after 4, 8, ... 64 of them
+  constexpr int kStepsPerGraph = 4;
+  auto block = [&]() {
+    for (int j = 0; j < kStepsPerGraph; j++) step();
+  };
+  LoopGraph loop(ctx);
+  if (int rc = loop.capture(block)) return rc;
+  EventTimer timer(ctx);
+  sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, word, near0, st);
+  int batch = kStepsPerGraph;
+  int64_t launched = 0;
+  for (;;) {
+    for (int j = 0; j < batch; j += kStepsPerGraph)
+      if (int rc = loop.run(block)) return rc;
     launched += batch;
     GBK_LAUNCH_CHECK(ctx, "sssp step");
     GBK_HIP(ctx, hipMemcpyAsync(hs, st, sizeof(SsspState), hipMemcpyDeviceToHost, s));

@@ -103,8 +103,10 @@ const char *gbk_last_error(const gbk_ctx *ctx);
 void *gbk_ctx_stream(gbk_ctx *ctx);
 int gbk_ctx_sync(gbk_ctx *ctx);
 /* Tunables: "heavy_degree", "bu_group", "bfs_alpha", "bfs_beta",
- * "xcd_swizzle", "relax_grid"; and "ktime" (a gbk_ktime_kernel) selecting
- * the kernel whose launches are timed one by one with HIP events.  Returns
+ * "xcd_swizzle", "relax_grid", "graphs" (1: SSSP rounds and PR iterations
+ * are captured once per call as a hipGraph and replayed; 0: launched one by
+ * one); and "ktime" (a gbk_ktime_kernel) selecting the kernel whose
+ * launches are timed one by one with HIP events (loops then run eagerly).  Returns
  * GBK_EINVAL for an unknown name or bad value.  No reference counterpart:
  * the reference's kernels have no launch shape to tune. */
 int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value);

@@ -1,0 +1,99 @@
+"""Iteration loops replayed as captured hipGraphs (GPU).
+
+SSSP rounds (one source and batches) and PR iterations (one graph and
+many) are recorded once per call and replayed; every replayed run must
+equal an eager run of the same call (distances bit-exact, PR within float
+rounding of the eager scores and with the same iteration count), with
+eager work and synchronisation between the replays."""
+import numpy as np
+import pytest
+
+import oracle as O
+from graphbrew_amd import gbk
+
+pytestmark = pytest.mark.gpu
+
+
+def _mode(ctx, on):
+    ctx.set_param("graphs", 1 if on else 0)
+
+
+@pytest.fixture()
+def graphs_on(ctx):
+    old = ctx.get_param("graphs")
+    yield
+    ctx.set_param("graphs", old)
+
+
+def test_graphs_param_values(ctx, graphs_on):
+    assert ctx.get_param("graphs") in (0, 1)
+    with pytest.raises(gbk.GbkError):
+        ctx.set_param("graphs", 2)
+
+
+@pytest.mark.parametrize("delta", [1, 64, 100000])
+def test_sssp_replays_equal_eager(ctx, graphs_on, delta):
+    g = O.kron(13, weighted=True)
+    dg = ctx.upload_graph(g)
+    srcs = O.source_picker(g, 3)[1]
+    for s in srcs:
+        ref = O.dijkstra(g, int(s))
+        for rep in range(3):
+            _mode(ctx, False)
+            eager, st0 = ctx.sssp(dg, int(s), delta)
+            _mode(ctx, True)
+            replay, st1 = ctx.sssp(dg, int(s), delta)
+            assert np.array_equal(eager, ref) and np.array_equal(replay, ref), (delta, int(s), rep)
+            assert st1.reached == st0.reached
+    dg.free()
+
+
+def test_sssp_batch_replays_equal_eager(ctx, graphs_on):
+    g = O.kron(13, weighted=True)
+    dg = ctx.upload_graph(g)
+    srcs = O.source_picker(g, 40)[1]
+    refs = [O.dijkstra(g, int(s)) for s in srcs]
+    for rep in range(3):
+        _mode(ctx, True)
+        dist, _, _, _ = ctx.sssp_batch(dg, srcs, 256)
+        _mode(ctx, False)
+        eager, _, _, _ = ctx.sssp_batch(dg, srcs, 256)
+        for j in range(len(srcs)):
+            assert np.array_equal(dist[j], refs[j]) and np.array_equal(eager[j], refs[j]), (rep, j)
+    dg.free()
+
+
+def test_pr_replays_equal_eager(ctx, graphs_on):
+    g = O.kron(13)
+    dg = ctx.upload_graph(g)
+    _mode(ctx, False)
+    eager, st0 = ctx.pr(dg, gbk.PR_SPMV, max_iters=20, tol=1e-4)
+    for rep in range(3):
+        _mode(ctx, True)
+        s, st = ctx.pr(dg, gbk.PR_SPMV, max_iters=20, tol=1e-4)
+        assert st.iterations == st0.iterations, rep
+        np.testing.assert_allclose(s, eager, rtol=1e-5, atol=1e-10)
+        _mode(ctx, False)
+        ctx.cc(dg, gbk.CC_SV)  # eager work between the replays
+        gs, _ = ctx.pr(dg, gbk.PR_GS, max_iters=200, tol=1e-9)
+        np.testing.assert_allclose(gs, O.pr_fp64(g), rtol=1e-4, atol=1e-10)
+        _mode(ctx, True)
+        gs, _ = ctx.pr(dg, gbk.PR_GS, max_iters=200, tol=1e-9)
+        np.testing.assert_allclose(gs, O.pr_fp64(g), rtol=1e-4, atol=1e-10)
+    dg.free()
+
+
+def test_pr_many_replays_equal_eager(ctx, graphs_on):
+    parts = [O.kron(s) for s in (7, 9, 10, 8)]
+    dgs = [ctx.upload_graph(p) for p in parts]
+    ug, vbase = ctx.graph_union(dgs)
+    _mode(ctx, False)
+    eager, it0, _ = ctx.pr_many(ug, vbase, gbk.PR_SPMV, max_iters=20, tol=1e-4)
+    for rep in range(3):
+        _mode(ctx, True)
+        s, it, _ = ctx.pr_many(ug, vbase, gbk.PR_SPMV, max_iters=20, tol=1e-4)
+        assert np.array_equal(it, it0), rep
+        np.testing.assert_allclose(s, eager, rtol=1e-5, atol=1e-10)
+    for d in dgs:
+        d.free()
+    ug.free()

@@ -187,6 +187,26 @@ def test_bc_parity_with_many_heavy_rows(ctx):
     np.testing.assert_allclose(scores, ref, rtol=1e-4, atol=1e-6)
 
 
+def test_bc_repeated_runs_all_match(ctx):
+    """Regression: the forward's successor-bit aggregation read the next
+    lane's word with a shuffle that only lanes 0..62 executed, so a run of
+    bitmap word 0 ending at lane 62 was never written (a few percent of
+    runs lost one successor edge).  Thirty runs with fresh uploads and
+    other calls in between must all match."""
+    g = O.kron(13)
+    _, srcs = O.source_picker(g, 4)
+    ref = O.bc_serial(g, srcs)
+    tiny = _fixture("tiny.el")
+    for rep in range(30):
+        dt = ctx.upload_graph(tiny)
+        ctx.bc(dt, [0, 1])
+        dt.free()
+        dg = ctx.upload_graph(g)
+        scores, _ = ctx.bc(dg, srcs)
+        np.testing.assert_allclose(scores, ref, rtol=1e-4, atol=1e-6, err_msg=f"run {rep}")
+        dg.free()
+
+
 def test_kernels_after_relabel_are_ordering_invariant(ctx):
     """Distances / components / reachability of original ids survive any
     relabelling (sssp.cc:92-127 fingerprints; bench/tests/test_relabel_weights.cc:167-206)."""
