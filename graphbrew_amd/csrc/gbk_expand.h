@@ -471,14 +471,17 @@ struct WaveQueue {
   int32_t *out;                 // global queue
   int64_t cap;                  // capacity of 'out'
   int *overflow;
+  int segcap;                   // entries of the LDS segment (<= kSeg)
 
-  __device__ void init(int32_t *seg, unsigned long long *t, int32_t *o, int64_t c, int *ovf) {
+  __device__ void init(int32_t *seg, unsigned long long *t, int32_t *o, int64_t c, int *ovf,
+                       int seg_entries = kSeg) {
     lds = seg;
     cnt = 0;
     tail = t;
     out = o;
     cap = c;
     overflow = ovf;
+    segcap = seg_entries;
   }
   __device__ void flush() {
     if (cnt == 0) return;
@@ -499,7 +502,7 @@ struct WaveQueue {
     const uint64_t mask = __ballot(pred);
     if (mask == 0) return;
     const int k = __popcll(mask);
-    if (cnt + k > kSeg) flush();
+    if (cnt + k > segcap) flush();
     if (pred) lds[cnt + lanemask_lt_count(mask)] = v;
     cnt += k;
   }

@@ -11,6 +11,9 @@
 // neighbour array is read once and the relabelled array written once:
 //   deg <= 64    a group of 8/16/32/64 lanes per row, one entry per lane,
 //                bitonic network over shuffles (registers only)
+//   deg <= 512   one wave per row, 2/4/8 entries per lane in registers,
+//                bitonic network (in-lane strides in registers, the others
+//                over shuffles; no LDS, no barriers)
 //   deg <= 2048  one row per workgroup, bitonic sort in LDS
 //   larger       2048-element chunks sorted in LDS, then merge-path passes
 //                over a compact staging buffer, then copied into place.
@@ -46,26 +49,31 @@ __global__ void relabel_invert(const int64_t *off, const int32_t *new_ids, const
   org_new[r] = org_old[u];
 }
 
-// Degree classes: <=8, <=16, <=32, <=64 (lane-group register sorts),
-// <=2048 (workgroup LDS sort), larger (chunk sort + merge).  Empty rows need
-// no work.  Each wave stages its class lists in LDS and flushes in batches.
-constexpr int kClasses = 6;
+// Degree classes: <=8, <=16, <=32, <=64 (lane-group register sorts), <=128,
+// <=256, <=512 (wave register sorts), <=2048 (workgroup LDS sort), larger
+// (chunk sort + merge).  Empty rows need no work.  Each wave stages its class
+// lists in LDS and flushes in batches.
+constexpr int kClasses = 9;
+constexpr int kClassWindow = 7;  // the LDS sort class
+constexpr int kClassBig = 8;
+constexpr int kBinSeg = 256;     // staging entries per class and wave
+struct RelabelLists {
+  int32_t *l[kClasses];
+};
 __global__ void __launch_bounds__(kBlock)
-relabel_bin_rows(const int64_t *deg_new, int64_t n, int32_t *l0, int32_t *l1, int32_t *l2,
-                 int32_t *l3, int32_t *l4, int32_t *l5, unsigned long long *counts,
+relabel_bin_rows(const int64_t *deg_new, int64_t n, RelabelLists lists, unsigned long long *counts,
                  int *overflow) {
-  __shared__ int32_t seg[kClasses * kWavesPerBlock * kSeg];
+  __shared__ int32_t seg[kClasses * kWavesPerBlock * kBinSeg];
   const int w = threadIdx.x / kWave;
-  int32_t *lists[kClasses] = {l0, l1, l2, l3, l4, l5};
   WaveQueue q[kClasses];
   for (int c = 0; c < kClasses; c++)
-    q[c].init(seg + (c * kWavesPerBlock + w) * kSeg, &counts[c], lists[c], n + 1, overflow);
+    q[c].init(seg + (c * kWavesPerBlock + w) * kBinSeg, &counts[c], lists.l[c], n + 1, overflow, kBinSeg);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
     const int64_t r = base + threadIdx.x;
     const int64_t d = r < n ? deg_new[r] : 0;
     const int cls = d == 0 ? -1 : d <= 8 ? 0 : d <= 16 ? 1 : d <= 32 ? 2 : d <= 64 ? 3
-                  : d <= kSortCap ? 4 : 5;
+                  : d <= 128 ? 4 : d <= 256 ? 5 : d <= 512 ? 6 : d <= kSortCap ? 7 : 8;
     for (int c = 0; c < kClasses; c++) q[c].push(cls == c, static_cast<int32_t>(r));
   }
   for (int c = 0; c < kClasses; c++) q[c].flush();
@@ -114,6 +122,90 @@ relabel_group_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const
       wgt_out[dst + glane] = static_cast<int32_t>(static_cast<uint32_t>(key) ^ 0x80000000u);
     } else {
       nbr_out[dst + glane] = static_cast<int32_t>(key);
+    }
+  }
+}
+
+// Rows of at most 64*E entries: one wave per row, E entries per lane in
+// registers.  Entry i sits in lane i % 64, register i / 64 (so loads and
+// stores are coalesced); the bitonic network's strides below 64 pair lanes
+// (shuffles), larger strides pair registers of one lane.  Keys compare as
+// (v, w) through the composite integer of RowKey.
+template <bool W>
+struct RowKey {
+  using T = typename std::conditional<W, uint64_t, uint32_t>::type;
+  __device__ static T make(int32_t v, int32_t w) {
+    if constexpr (W)
+      return (static_cast<uint64_t>(static_cast<uint32_t>(v)) << 32) |
+             (static_cast<uint32_t>(w) ^ 0x80000000u);
+    else
+      return static_cast<uint32_t>(v);
+  }
+};
+
+template <bool W, int E>
+__global__ void __launch_bounds__(kBlock)
+relabel_wave_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const int64_t *off,
+                  const int32_t *nbr, const int32_t *wgt, const int32_t *new_ids,
+                  const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out) {
+  using K = typename RowKey<W>::T;
+  constexpr int N = kWave * E;
+  const int lane = lane_id();
+  const int64_t ri = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (ri >= nrows) return;  // wave-uniform
+  const int32_t r = rows[ri];
+  const int32_t u = inv[r];
+  const int64_t src = off[u];
+  const int64_t len = off[u + 1] - src;
+  const int64_t dst = new_off[r];
+  K key[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int64_t i = static_cast<int64_t>(e) * kWave + lane;
+    key[e] = i < len ? RowKey<W>::make(new_ids[nbr[src + i]], W ? wgt[src + i] : 0)
+                     : static_cast<K>(~static_cast<K>(0));
+  }
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < kWave) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const K other = __shfl_xor(key[e], j);
+          const int i = e * kWave + lane;
+          const bool up = (i & k) == 0;
+          const bool lower = (i & j) == 0;
+          key[e] = (lower == up) ? (other < key[e] ? other : key[e]) : (other > key[e] ? other : key[e]);
+        }
+      } else {
+        const int jj = j / kWave;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          if ((e & jj) == 0) {
+            const int e2 = e | jj;
+            const int i = e * kWave + lane;
+            const bool up = (i & k) == 0;
+            const K a = key[e], b = key[e2];
+            if ((a > b) == up) {
+              key[e] = b;
+              key[e2] = a;
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int64_t i = static_cast<int64_t>(e) * kWave + lane;
+    if (i < len) {
+      if constexpr (W) {
+        nbr_out[dst + i] = static_cast<int32_t>(key[e] >> 32);
+        wgt_out[dst + i] = static_cast<int32_t>(static_cast<uint32_t>(key[e]) ^ 0x80000000u);
+      } else {
+        nbr_out[dst + i] = static_cast<int32_t>(key[e]);
+      }
     }
   }
 }
@@ -348,18 +440,24 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
   if (!degrees_ready) return set_error(ctx, GBK_EINVAL, "relabel: degrees not computed");
   int rc = exclusive_scan_i64(ctx, deg_new, off_out, n);
   if (rc) return rc;
-  int32_t *lists[kClasses];
-  const int slots[kClasses] = {kS4, kS5, kS6, kS7, kS12, kS13};
-  for (int c = 0; c < kClasses; c++) {
-    lists[c] = static_cast<int32_t *>(scratch(ctx, slots[c], sizeof(int32_t) * (n + 1)));
-    if (!lists[c]) return set_error(ctx, GBK_ENOMEM, "relabel bins");
+  RelabelLists bins;
+  int32_t *const *lists = bins.l;
+  const int slots[6] = {kS4, kS5, kS6, kS7, kS12, kS13};
+  for (int c = 0; c < 6; c++) {
+    bins.l[c] = static_cast<int32_t *>(scratch(ctx, slots[c], sizeof(int32_t) * (n + 1)));
+    if (!bins.l[c]) return set_error(ctx, GBK_ENOMEM, "relabel bins");
+  }
+  {
+    // classes 6..8 share one slot, n + 1 entries each
+    auto *rest = static_cast<int32_t *>(scratch(ctx, kS14, sizeof(int32_t) * 3 * (n + 1)));
+    if (!rest) return set_error(ctx, GBK_ENOMEM, "relabel bins");
+    for (int c = 6; c < kClasses; c++) bins.l[c] = rest + (c - 6) * (n + 1);
   }
   auto *counts_d = static_cast<unsigned long long *>(scratch(ctx, kS8, 128));
   if (!counts_d) return set_error(ctx, GBK_ENOMEM, "relabel counts");
   fill<unsigned long long>(ctx, counts_d, 0ull, kClasses + 1);
   if (n > 0)
-    relabel_bin_rows<<<expand_grid(n), kBlock, 0, s>>>(deg_new, n, lists[0], lists[1], lists[2],
-                                                       lists[3], lists[4], lists[5], counts_d,
+    relabel_bin_rows<<<expand_grid(n), kBlock, 0, s>>>(deg_new, n, bins, counts_d,
                                                        reinterpret_cast<int *>(counts_d + kClasses));
   GBK_LAUNCH_CHECK(ctx, "relabel_bin_rows");
   unsigned long long counts[kClasses + 1];
@@ -380,14 +478,27 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
     }
     GBK_LAUNCH_CHECK(ctx, "relabel_group_sort");
   }
-  if (counts[4] > 0) {
-    const int64_t cnt = static_cast<int64_t>(counts[4]);
-    relabel_window_sort<W><<<cnt, kBlock, 0, s>>>(lists[4], cnt, 1, inv, off, nbr, wgt, new_ids_d,
+  // rows of <= 512 entries: one wave each, in registers
+  for (int c = 4; c <= 6; c++) {
+    const int64_t cnt = static_cast<int64_t>(counts[c]);
+    if (cnt == 0) continue;
+    const unsigned blocks = static_cast<unsigned>((cnt + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (c == 4)
+      relabel_wave_sort<W, 2><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out);
+    else if (c == 5)
+      relabel_wave_sort<W, 4><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out);
+    else
+      relabel_wave_sort<W, 8><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out);
+    GBK_LAUNCH_CHECK(ctx, "relabel_wave_sort");
+  }
+  if (counts[kClassWindow] > 0) {
+    const int64_t cnt = static_cast<int64_t>(counts[kClassWindow]);
+    relabel_window_sort<W><<<cnt, kBlock, 0, s>>>(lists[kClassWindow], cnt, 1, inv, off, nbr, wgt, new_ids_d,
                                                    off_out, nbr_out, wgt_out);
     GBK_LAUNCH_CHECK(ctx, "relabel_window_sort");
   }
-  const int nbig = static_cast<int>(counts[5]);
-  int32_t *big_list = lists[5];
+  const int nbig = static_cast<int>(counts[kClassBig]);
+  int32_t *big_list = lists[kClassBig];
   if (nbig == 0) return GBK_OK;
   // Big rows: host builds the staging layout and chunk table (few rows).
   std::vector<int32_t> big(nbig);

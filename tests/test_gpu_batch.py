@@ -130,6 +130,22 @@ def test_sssp_batch_equals_single_source_counts(ctx):
     dg.free()
 
 
+def test_sssp_batch_long_weighted_path(ctx):
+    """64 sources on a weighted path: distances far above 65535, hundreds
+    of buckets and rounds; every source equals Dijkstra."""
+    n = 1500
+    u = np.arange(n - 1, dtype=np.int32)
+    g = O.make_graph(u, u + 1, np.full(n - 1, 200, np.int32))
+    dg = ctx.upload_graph(g)
+    srcs = np.arange(0, 64 * 23, 23, dtype=np.int32)
+    dist, reached, edges, st = ctx.sssp_batch(dg, srcs, 1024)
+    assert dist.max() > 65535
+    for j, s in enumerate(srcs):
+        assert np.array_equal(dist[j], O.dijkstra(g, int(s))), j
+    _check_counts(g, dist, O.K_DIST_INF, reached, edges)
+    dg.free()
+
+
 BC_GRAPHS = {
     "kron13": lambda: O.kron(13),
     "uniform12": lambda: O.kron(12, uniform=True),

@@ -41,7 +41,25 @@ def _assert_same_graph(dg, ref):
     assert np.array_equal(got["org_ids"], ref.org_ids)
 
 
+def _class_rows(weighted=True):
+    """Directed graph whose rows sit exactly at the sort classes' edges (64 |
+    65, 128 | 129, 256 | 257, 512 | 513, 2048 | 2049 entries)."""
+    rng = np.random.default_rng(11)
+    n = 8192
+    sizes = [64, 65, 127, 128, 129, 255, 256, 257, 511, 512, 513, 1000, 2048, 2049, 6000]
+    us, vs = [], []
+    for i, k in enumerate(sizes * 4):
+        us.append(np.full(k, i * 7, np.int32))
+        vs.append(rng.choice(n, k, replace=False).astype(np.int32))
+    u = np.concatenate(us)
+    v = np.concatenate(vs)
+    w = rng.integers(1, 256, len(u)).astype(np.int32) if weighted else None
+    return O.make_graph(u, v, w, symmetrize=False, n=n)
+
+
 GRAPHS = {
+    "classes_w": lambda: _class_rows(),
+    "classes": lambda: _class_rows(False),
     "kron12": lambda: O.kron(12),
     "kron14w": lambda: O.kron(14, weighted=True),
     "uniform12": lambda: O.kron(12, uniform=True),
