@@ -42,7 +42,8 @@ constexpr int kBatchMax = 64;
 constexpr int32_t kBatchInf = INT32_MAX / 2;  // sssp.cc:67
 constexpr int64_t kBatchHeavy = 256;          // rows above this are chunked over groups
 constexpr int64_t kBatchChunk = 256;
-constexpr int kBatchUnroll = 8;
+constexpr int kBatchUnroll = 8;   // edges in flight per trip (BC batch)
+constexpr int kRelaxUnroll = 4;   // edges in flight per trip (SSSP batch: fewer registers, more waves)
 
 inline int batch_width(int k) { return k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64; }
 
@@ -403,20 +404,20 @@ struct GroupRelax {
       return;
     }
     const int64_t trips = wave_max(len);
-    for (int64_t j0 = 0; j0 < trips; j0 += kBatchUnroll) {
-      int32_t x[kBatchUnroll], w[kBatchUnroll], dv[kBatchUnroll];
-      bool valid[kBatchUnroll];
+    for (int64_t j0 = 0; j0 < trips; j0 += kRelaxUnroll) {
+      int32_t x[kRelaxUnroll], w[kRelaxUnroll], dv[kRelaxUnroll];
+      bool valid[kRelaxUnroll];
 #pragma unroll
-      for (int j = 0; j < kBatchUnroll; j++) {
+      for (int j = 0; j < kRelaxUnroll; j++) {
         valid[j] = j0 + j < len;
         const int64_t e = valid[j] ? s + j0 + j : 0;
         x[j] = nbr[e];
         w[j] = wgt[e];
       }
 #pragma unroll
-      for (int j = 0; j < kBatchUnroll; j++) dv[j] = dist[static_cast<int64_t>(x[j]) * Kp + k];
+      for (int j = 0; j < kRelaxUnroll; j++) dv[j] = dist[static_cast<int64_t>(x[j]) * Kp + k];
 #pragma unroll
-      for (int j = 0; j < kBatchUnroll; j++) {
+      for (int j = 0; j < kRelaxUnroll; j++) {
         bool here = false;
         int32_t fut = INT32_MAX;
         const int64_t cand = du + w[j];
@@ -446,7 +447,7 @@ struct GroupRelax {
   // Kp == 64: the wave is one group, so the row and its edge range are
   // wave-uniform.  Each 64-edge slice of neighbour ids and weights comes in
   // with one coalesced load and is broadcast edge by edge (readlane); the
-  // distance rows of kBatchUnroll neighbours are in flight together; the
+  // distance rows of kRelaxUnroll neighbours are in flight together; the
   // cross-lane reductions and the leader's atomics run only for edges on
   // which some lane improved.
   __device__ void walk_wave(int64_t s, int64_t len, int64_t du, bool act) {
@@ -457,18 +458,18 @@ struct GroupRelax {
       const bool mine = k < cnt;
       const int32_t vx = mine ? nbr[s + c0 + k] : 0;
       const int32_t vw = mine ? wgt[s + c0 + k] : 0;
-      for (int j0 = 0; j0 < cnt; j0 += kBatchUnroll) {
-        int32_t x[kBatchUnroll], w[kBatchUnroll], dv[kBatchUnroll];
+      for (int j0 = 0; j0 < cnt; j0 += kRelaxUnroll) {
+        int32_t x[kRelaxUnroll], w[kRelaxUnroll], dv[kRelaxUnroll];
 #pragma unroll
-        for (int j = 0; j < kBatchUnroll; j++) {
+        for (int j = 0; j < kRelaxUnroll; j++) {
           const int at = j0 + j < kWave ? j0 + j : kWave - 1;  // lanes past cnt hold vertex 0
           x[j] = __builtin_amdgcn_readlane(vx, at);
           w[j] = __builtin_amdgcn_readlane(vw, at);
         }
 #pragma unroll
-        for (int j = 0; j < kBatchUnroll; j++) dv[j] = dist[static_cast<int64_t>(x[j]) * Kp + k];
+        for (int j = 0; j < kRelaxUnroll; j++) dv[j] = dist[static_cast<int64_t>(x[j]) * Kp + k];
 #pragma unroll
-        for (int j = 0; j < kBatchUnroll; j++) {
+        for (int j = 0; j < kRelaxUnroll; j++) {
           const int64_t cand = du + w[j];
           // judged against the loaded distance, as in walk()
           const bool imp = j0 + j < cnt && act && cand < dv[j];
@@ -493,7 +494,7 @@ struct GroupRelax {
 };
 
 template <int Kp>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, 8)
 sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *pend,
          uint32_t *vtag, int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, BatchState *st,
          HeavyQueue hq) {
@@ -542,7 +543,7 @@ sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *di
 }
 
 template <int Kp>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, 8)
 sb_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *pend, uint32_t *vtag,
                int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, BatchState *st,
                HeavyQueue hq) {
@@ -1027,6 +1028,16 @@ bcb_backward_finish(const int32_t *depth, const double *sigma, float *coef, floa
   }
 }
 
+// The relaxation launches fill the GPU with exactly the workgroups it keeps
+// resident (grid-stride loops; a second round of workgroups would leave a
+// tail).
+template <int Kp>
+static unsigned sb_grid(gbk_ctx *ctx) {
+  const unsigned a = resident_grid(ctx, reinterpret_cast<const void *>(&sb_relax<Kp>));
+  const unsigned b = resident_grid(ctx, reinterpret_cast<const void *>(&sb_relax_heavy<Kp>));
+  return a < b ? a : b;
+}
+
 template <int Kp>
 static void launch_sb_relax(gbk_ctx *ctx, const gbk_graph *g, int32_t *dist, int32_t *pend, uint32_t *vtag,
                             int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, BatchState *st,
@@ -1188,7 +1199,13 @@ int sssp_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K
   uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
   int32_t *dsrc = nullptr;
   if (int rc = upload_sources(ctx, src, K, &dsrc)) return rc;
-  const unsigned grid = static_cast<unsigned>(ctx->tune.relax_grid);
+  unsigned grid = 0;
+  switch (Kp) {
+    case 8: grid = sb_grid<8>(ctx); break;
+    case 16: grid = sb_grid<16>(ctx); break;
+    case 32: grid = sb_grid<32>(ctx); break;
+    default: grid = sb_grid<64>(ctx); break;
+  }
   const unsigned scan_grid = expand_grid(n);
   // one step: relaxation round, then the bucket bookkeeping (each kernel
   // returns at once unless the state's phase is its own)
@@ -1288,7 +1305,11 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
   uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
   int32_t *dsrc = nullptr;
   if (int rc = upload_sources(ctx, src, K, &dsrc)) return rc;
-  const unsigned grid = static_cast<unsigned>(ctx->tune.relax_grid);
+  // forward and backward launches fill the GPU with resident workgroups
+  const unsigned grid_f = std::min(resident_grid(ctx, reinterpret_cast<const void *>(&bcb_forward<Kp>)),
+                                   resident_grid(ctx, reinterpret_cast<const void *>(&bcb_forward_heavy<Kp>)));
+  const unsigned grid_b = std::min(resident_grid(ctx, reinterpret_cast<const void *>(&bcb_backward<Kp>)),
+                                   resident_grid(ctx, reinterpret_cast<const void *>(&bcb_backward_heavy<Kp>)));
   GBK_HIP(ctx, hipMemcpyAsync(queue, uniq.data(), sizeof(int32_t) * uniq.size(), hipMemcpyHostToDevice, s));
   fill<int32_t>(ctx, depth, -1, n * Kp);
   fill<double>(ctx, sigma, 0.0, n * Kp);
@@ -1300,10 +1321,10 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
     const int64_t lb = level[level.size() - 2], le = level.back();
     fill<unsigned long long>(ctx, hq.count, 0ull, 1);
     const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
-    bcb_forward<Kp><<<grid, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, ltag, queue + lb, le - lb, queue, cap,
-                                            d, c, hq);
+    bcb_forward<Kp><<<grid_f, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, ltag, queue + lb, le - lb, queue, cap,
+                                              d, c, hq);
     heavy_prefix<<<1, 1024, 0, s>>>(hq);
-    bcb_forward_heavy<Kp><<<grid, kBlock, 0, s>>>(g->nbr, depth, sigma, ltag, queue, cap, d, c, hq);
+    bcb_forward_heavy<Kp><<<grid_f, kBlock, 0, s>>>(g->nbr, depth, sigma, ltag, queue, cap, d, c, hq);
     kt_end(ctx, kt);
     GBK_LAUNCH_CHECK(ctx, "bc batch forward");
     GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcbCounters), hipMemcpyDeviceToHost, s));
@@ -1318,9 +1339,9 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
     const int64_t lb = level[d], cnt = level[d + 1] - lb;
     fill<unsigned long long>(ctx, hq.count, 0ull, 1);
     const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
-    bcb_backward<Kp><<<grid, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, coef, scores, queue + lb, cnt, d, c, hq);
+    bcb_backward<Kp><<<grid_b, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, coef, scores, queue + lb, cnt, d, c, hq);
     heavy_prefix<<<1, 1024, 0, s>>>(hq);
-    bcb_backward_heavy<Kp><<<grid, kBlock, 0, s>>>(g->nbr, depth, coef, d, c, hq, hpart);
+    bcb_backward_heavy<Kp><<<grid_b, kBlock, 0, s>>>(g->nbr, depth, coef, d, c, hq, hpart);
     bcb_backward_finish<Kp><<<64, kBlock, 0, s>>>(depth, sigma, coef, scores, d, hq, hpart);
     kt_end(ctx, kt);
   }

@@ -12,6 +12,7 @@
 // except where a frontier decision needs a count on the host.
 #pragma once
 
+#include <unordered_map>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -36,6 +37,7 @@ struct Tuning {
   int xcd_swizzle = 1;          // contiguous vertex ranges per XCD
   int relax_grid = 1024;        // workgroups of a frontier (queue) launch
   int graphs = 1;               // iteration loops replayed as captured hipGraphs
+  int batch_grid = 0;           // workgroups of a source batch's launches (0: as many as stay resident)
 };
 constexpr int kGridCapRuntime = 2048;  // upper bound of relax_grid
 constexpr int64_t kHeavyChunkEdges = 1024;  // heavy-row chunk of the all-rows kernels
@@ -189,6 +191,9 @@ struct gbk_ctx {
   double kt_ms = 0;
   int64_t kt_launches = 0;
   unsigned long long *kt_dev = nullptr;
+  // resident workgroups per kernel (occupancy x compute units), cached
+  int cu_count = 0;
+  std::unordered_map<const void *, unsigned> resident;
 };
 
 namespace gbk {
@@ -199,6 +204,11 @@ enum SlotId {
 };
 
 int set_error(gbk_ctx *ctx, int code, const std::string &msg);
+// Workgroups of kBlock lanes of 'kernel' that the device keeps resident at
+// once (its occupancy times the compute units), or the "batch_grid"
+// parameter when set: the grid of a grid-stride launch that fills the GPU
+// exactly, with no second round of workgroups.
+unsigned resident_grid(gbk_ctx *ctx, const void *kernel);
 void *scratch(gbk_ctx *ctx, int slot, size_t bytes);   // nullptr on failure
 void *pinned(gbk_ctx *ctx, size_t bytes);
 int check_hip(gbk_ctx *ctx, hipError_t e, const char *what);

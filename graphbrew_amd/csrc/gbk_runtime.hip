@@ -38,6 +38,24 @@ void *scratch(gbk_ctx *ctx, int slot, size_t bytes) {
   return s.p;
 }
 
+unsigned resident_grid(gbk_ctx *ctx, const void *kernel) {
+  if (ctx->tune.batch_grid > 0) return static_cast<unsigned>(ctx->tune.batch_grid);
+  auto it = ctx->resident.find(kernel);
+  if (it != ctx->resident.end()) return it->second;
+  if (ctx->cu_count == 0) {
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cu < 1)
+      cu = 256;
+    ctx->cu_count = cu;
+  }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1)
+    per_cu = 4;
+  const unsigned grid = static_cast<unsigned>(per_cu) * static_cast<unsigned>(ctx->cu_count);
+  ctx->resident[kernel] = grid;
+  return grid;
+}
+
 void *pinned(gbk_ctx *ctx, size_t bytes) {
   if (ctx->pinned_bytes >= bytes) return ctx->pinned;
   if (ctx->pinned) hipHostFree(ctx->pinned);
@@ -387,6 +405,7 @@ int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value) {
   else if (k == "relax_grid" && value >= 64 && value <= kGridCapRuntime && value % 8 == 0)
     t.relax_grid = static_cast<int>(value);
   else if (k == "graphs" && (value == 0 || value == 1)) t.graphs = static_cast<int>(value);
+  else if (k == "batch_grid" && value >= 0 && value <= 16384 && value % 8 == 0) t.batch_grid = static_cast<int>(value);
   else if (k == "ktime" && value >= GBK_KT_NONE && value <= GBK_KT_LAST) {
     if (value != GBK_KT_NONE && !ctx->kt_dev) {
       const size_t bytes = sizeof(unsigned long long) * (2 + 2 * kKtPartials);
@@ -413,6 +432,7 @@ int gbk_ctx_get_param(gbk_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "xcd_swizzle") *value = t.xcd_swizzle;
   else if (k == "relax_grid") *value = t.relax_grid;
   else if (k == "graphs") *value = t.graphs;
+  else if (k == "batch_grid") *value = t.batch_grid;
   else if (k == "ktime") *value = ctx->ktime;
   else return set_error(ctx, GBK_EINVAL, "unknown parameter: " + k);
   return GBK_OK;
