@@ -42,7 +42,11 @@ constexpr int kBatchMax = 64;
 constexpr int32_t kBatchInf = INT32_MAX / 2;  // sssp.cc:67
 constexpr int64_t kBatchHeavy = 256;          // rows above this are chunked over groups
 constexpr int64_t kBatchChunk = 256;
-constexpr int kBatchUnroll = 8;   // edges in flight per trip (BC batch)
+// BC batch: edges in flight per trip, and the waves per SIMD its kernels'
+// registers are capped for (64 lanes: fewer registers, more waves measured
+// faster; narrower groups keep their registers).
+template <int Kp> constexpr int kBcbUnroll = Kp == 64 ? 4 : 8;
+template <int Kp> constexpr int kBcbWaves = Kp == 64 ? 8 : 1;
 constexpr int kRelaxUnroll = 4;   // edges in flight per trip (SSSP batch: fewer registers, more waves)
 
 inline int batch_width(int k) { return k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64; }
@@ -691,13 +695,14 @@ __device__ __forceinline__ double group_sum_d(double v) {
   return v;
 }
 
-// Calls f(x, on) on batches of kBatchUnroll edges of [s, s + len): for
+// Calls f(x, on) on batches of kBcbUnroll<Kp> edges of [s, s + len): for
 // Kp == 64 the row is wave-uniform and 64 ids come in with one coalesced
 // load (broadcast by readlane), otherwise every group walks its own row.
 template <int Kp, class F>
 __device__ void bcb_edges(const int32_t *nbr, int64_t s, int64_t len, F &f) {
-  int32_t x[kBatchUnroll];
-  bool on[kBatchUnroll];
+  constexpr int U = kBcbUnroll<Kp>;
+  int32_t x[U];
+  bool on[U];
   if constexpr (Kp == kWave) {
     s = uniform64(s);
     len = uniform64(len);
@@ -705,9 +710,9 @@ __device__ void bcb_edges(const int32_t *nbr, int64_t s, int64_t len, F &f) {
     for (int64_t c0 = 0; c0 < len; c0 += kWave) {
       const int cnt = static_cast<int>(len - c0 < kWave ? len - c0 : kWave);
       const int32_t vx = lane < cnt ? nbr[s + c0 + lane] : 0;
-      for (int j0 = 0; j0 < cnt; j0 += kBatchUnroll) {
+      for (int j0 = 0; j0 < cnt; j0 += U) {
 #pragma unroll
-        for (int j = 0; j < kBatchUnroll; j++) {
+        for (int j = 0; j < U; j++) {
           const int at = j0 + j < kWave ? j0 + j : kWave - 1;  // lanes past cnt hold vertex 0
           x[j] = __builtin_amdgcn_readlane(vx, at);
           on[j] = j0 + j < cnt;
@@ -717,9 +722,9 @@ __device__ void bcb_edges(const int32_t *nbr, int64_t s, int64_t len, F &f) {
     }
   } else {
     const int64_t trips = wave_max(len);
-    for (int64_t j0 = 0; j0 < trips; j0 += kBatchUnroll) {
+    for (int64_t j0 = 0; j0 < trips; j0 += U) {
 #pragma unroll
-      for (int j = 0; j < kBatchUnroll; j++) {
+      for (int j = 0; j < U; j++) {
         on[j] = j0 + j < len;
         x[j] = on[j] ? nbr[s + j0 + j] : 0;
       }
@@ -740,12 +745,13 @@ struct BcbForward {
   bool act;
   int k;
   bool leader;
-  __device__ void operator()(const int32_t (&x)[kBatchUnroll], const bool (&on)[kBatchUnroll]) {
-    int32_t dv[kBatchUnroll];
+  static constexpr int U = kBcbUnroll<Kp>;
+  __device__ void operator()(const int32_t (&x)[U], const bool (&on)[U]) {
+    int32_t dv[U];
 #pragma unroll
-    for (int j = 0; j < kBatchUnroll; j++) dv[j] = depth[static_cast<int64_t>(x[j]) * Kp + k];
+    for (int j = 0; j < U; j++) dv[j] = depth[static_cast<int64_t>(x[j]) * Kp + k];
 #pragma unroll
-    for (int j = 0; j < kBatchUnroll; j++) {
+    for (int j = 0; j < U; j++) {
       const bool a = on[j] && act && (dv[j] == -1 || dv[j] == d + 1);
       if (__ballot(a) == 0) continue;
       bool claimed = false;
@@ -775,16 +781,17 @@ struct BcbBackward {
   bool act;
   int k;
   double acc;
-  __device__ void operator()(const int32_t (&x)[kBatchUnroll], const bool (&on)[kBatchUnroll]) {
-    int32_t dv[kBatchUnroll];
-    float cf[kBatchUnroll];
+  static constexpr int U = kBcbUnroll<Kp>;
+  __device__ void operator()(const int32_t (&x)[U], const bool (&on)[U]) {
+    int32_t dv[U];
+    float cf[U];
 #pragma unroll
-    for (int j = 0; j < kBatchUnroll; j++) {
+    for (int j = 0; j < U; j++) {
       dv[j] = depth[static_cast<int64_t>(x[j]) * Kp + k];
       cf[j] = coef[static_cast<int64_t>(x[j]) * Kp + k];
     }
 #pragma unroll
-    for (int j = 0; j < kBatchUnroll; j++)
+    for (int j = 0; j < U; j++)
       if (on[j] && act && dv[j] == d + 1) acc += static_cast<double>(cf[j]);
   }
 };
@@ -835,7 +842,7 @@ __device__ __forceinline__ void bcb_chunk(const HeavyQueue &hq, int64_t rows, in
 // Forward level d over the union frontier list[0, count); rows above
 // kBatchHeavy go to the chunked kernel.
 template <int Kp>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
 bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *ltag,
             const int32_t *list, int64_t count, int32_t *queue, int64_t cap, int32_t d, BcbCounters *c,
             HeavyQueue hq) {
@@ -876,7 +883,7 @@ bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigm
 }
 
 template <int Kp>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
 bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *ltag, int32_t *queue,
                   int64_t cap, int32_t d, BcbCounters *c, HeavyQueue hq) {
   const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
@@ -919,7 +926,7 @@ bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *l
 // successors; coef[u] = (1 + delta) / sigma[u] for the level above, and the
 // row's score gains the deltas of all its lanes at this depth.
 template <int Kp>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
 bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const double *sigma, float *coef,
              float *scores, const int32_t *list, int64_t count, int32_t d, BcbCounters *c, HeavyQueue hq) {
   constexpr int G = kWave / Kp;
@@ -964,7 +971,7 @@ bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const
 // Heavy rows of a backward level: hpart[chunk][k] = lane k's coefficient
 // sum over the chunk's successors.
 template <int Kp>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
 bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const float *coef, int32_t d, BcbCounters *c,
                    HeavyQueue hq, double *hpart) {
   const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
