@@ -409,9 +409,10 @@ relabel_merge_pass(const K *src, K *dst, const int64_t *stage_off, int nbig, int
 }
 
 template <bool W>
-__global__ void relabel_big_scatter(const typename BigKey<W>::T *stage, const int64_t *stage_off,
-                                    const int32_t *big_rows, int nbig, int64_t total,
-                                    const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out) {
+__global__ void relabel_big_scatter(const typename BigKey<W>::T *stage0, const typename BigKey<W>::T *stage1,
+                                    const int64_t *stage_off, const int32_t *big_rows, int nbig,
+                                    int64_t total, const int64_t *new_off, int32_t *nbr_out,
+                                    int32_t *wgt_out) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= total) return;
   int lo = 0, hi = nbig;
@@ -420,9 +421,16 @@ __global__ void relabel_big_scatter(const typename BigKey<W>::T *stage, const in
     if (stage_off[mid] <= t) lo = mid;
     else hi = mid;
   }
+  // a row of length L took one merge pass per width kSortCap * 2^j below L,
+  // ping-ponging from stage0, so its sorted data ends in the buffer of that
+  // count's parity
+  const int64_t len = stage_off[lo + 1] - stage_off[lo];
+  int passes = 0;
+  for (int64_t w = kSortCap; w < len; w *= 2) passes++;
+  const typename BigKey<W>::T key = (passes & 1) ? stage1[t] : stage0[t];
   const int64_t dst = new_off[big_rows[lo]] + (t - stage_off[lo]);
-  nbr_out[dst] = BigKey<W>::v(stage[t]);
-  if (W) wgt_out[dst] = BigKey<W>::w(stage[t]);
+  nbr_out[dst] = BigKey<W>::v(key);
+  if (W) wgt_out[dst] = BigKey<W>::w(key);
 }
 
 __global__ void gather_i64(const int64_t *src, const int32_t *idx, int64_t n, int64_t *out) {
@@ -505,16 +513,31 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
   std::vector<int64_t> bdeg(nbig);
   GBK_HIP(ctx, hipMemcpyAsync(big.data(), big_list, sizeof(int32_t) * nbig, hipMemcpyDeviceToHost, s));
   GBK_HIP(ctx, hipStreamSynchronize(s));
-  std::sort(big.begin(), big.end());
   {
     // degrees of the big rows: deg_new[row], gathered on the device
     auto *tmp = static_cast<int64_t *>(scratch(ctx, kS9, sizeof(int64_t) * nbig));
     if (!tmp) return set_error(ctx, GBK_ENOMEM, "relabel big degrees");
-    GBK_HIP(ctx, hipMemcpyAsync(big_list, big.data(), sizeof(int32_t) * nbig, hipMemcpyHostToDevice, s));
     gather_i64<<<grid_for(nbig), kBlock, 0, s>>>(deg_new, big_list, nbig, tmp);
     GBK_LAUNCH_CHECK(ctx, "gather_i64");
     GBK_HIP(ctx, hipMemcpyAsync(bdeg.data(), tmp, sizeof(int64_t) * nbig, hipMemcpyDeviceToHost, s));
     GBK_HIP(ctx, hipStreamSynchronize(s));
+  }
+  {
+    // longest rows first: a merge pass of width w then covers a prefix of
+    // the rows (those longer than w) instead of copying every big row again
+    std::vector<int> order(nbig);
+    for (int i = 0; i < nbig; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+      return bdeg[a] != bdeg[b] ? bdeg[a] > bdeg[b] : big[a] < big[b];
+    });
+    std::vector<int32_t> b2(nbig);
+    std::vector<int64_t> d2(nbig);
+    for (int i = 0; i < nbig; i++) {
+      b2[i] = big[order[i]];
+      d2[i] = bdeg[order[i]];
+    }
+    big.swap(b2);
+    bdeg.swap(d2);
   }
   std::vector<int64_t> soff(nbig + 1, 0);
   std::vector<int32_t> crow;
@@ -549,13 +572,16 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
       big_d, soff_d, crow_d, cstart_d, inv, off, nbr, wgt, new_ids_d, st0);
   GBK_LAUNCH_CHECK(ctx, "relabel_big_chunks");
   K *src = st0, *dst = st1;
+  int active = nbig;  // rows longer than the current width: a prefix
   for (int64_t width = kSortCap; width < maxdeg; width *= 2) {
-    const int64_t threads = (total + kMergeItems - 1) / kMergeItems;
-    relabel_merge_pass<K><<<grid_for(threads), kBlock, 0, s>>>(src, dst, soff_d, nbig, total, width);
+    while (active > 0 && bdeg[active - 1] <= width) active--;
+    const int64_t span = soff[active];
+    const int64_t threads = (span + kMergeItems - 1) / kMergeItems;
+    relabel_merge_pass<K><<<grid_for(threads), kBlock, 0, s>>>(src, dst, soff_d, active, span, width);
     GBK_LAUNCH_CHECK(ctx, "relabel_merge_pass");
     std::swap(src, dst);
   }
-  relabel_big_scatter<W><<<grid_for(total), kBlock, 0, s>>>(src, soff_d, big_d, nbig, total, off_out,
+  relabel_big_scatter<W><<<grid_for(total), kBlock, 0, s>>>(st0, st1, soff_d, big_d, nbig, total, off_out,
                                                             nbr_out, wgt_out);
   GBK_LAUNCH_CHECK(ctx, "relabel_big_scatter");
   GBK_HIP(ctx, hipStreamSynchronize(s));  // staging buffers are reused by the next CSR
