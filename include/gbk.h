@@ -105,7 +105,9 @@ int gbk_ctx_sync(gbk_ctx *ctx);
 /* Tunables: "heavy_degree", "bu_group", "bfs_alpha", "bfs_beta",
  * "xcd_swizzle", "relax_grid", "graphs" (1: SSSP rounds and PR iterations
  * are captured once per call as a hipGraph and replayed; 0: launched one by
- * one); and "ktime" (a gbk_ktime_kernel) selecting the kernel whose
+ * one); "batch_grid" (workgroups of the source-batch relax launches: 0 launches
+ * as many as the GPU keeps resident, else 8..16384 in multiples of 8); and
+ * "ktime" (a gbk_ktime_kernel) selecting the kernel whose
  * launches are timed one by one with HIP events (loops then run eagerly).  Returns
  * GBK_EINVAL for an unknown name or bad value.  No reference counterpart:
  * the reference's kernels have no launch shape to tune. */
