@@ -38,6 +38,9 @@ struct Tuning {
   int relax_grid = 1024;        // workgroups of a frontier (queue) launch
   int graphs = 1;               // iteration loops replayed as captured hipGraphs
   int batch_grid = 0;           // workgroups of a source batch's launches (0: as many as stay resident)
+  int pr_blocks = 0;            // row blocks of a Gauss-Seidel PageRank sweep (0: by size)
+  int pr_unroll = 0;            // 64-edge chunks a PageRank wave has in flight (1, 2, 4; 0: 4 for
+                                // Gauss-Seidel blocks, 1 for Jacobi's one launch)
 };
 constexpr int kGridCapRuntime = 2048;  // upper bound of relax_grid
 constexpr int64_t kHeavyChunkEdges = 1024;  // heavy-row chunk of the all-rows kernels
@@ -278,7 +281,7 @@ class LoopGraph {
   LoopGraph(const LoopGraph &) = delete;
   LoopGraph &operator=(const LoopGraph &) = delete;
   ~LoopGraph() {
-    if (exec_) hipGraphExecDestroy(exec_);
+    if (exec_) (void)hipGraphExecDestroy(exec_);
   }
   template <class Body>
   int capture(Body &&body) {
@@ -289,7 +292,7 @@ class LoopGraph {
     const hipError_t e = hipStreamEndCapture(ctx_->stream, &graph);
     if (e != hipSuccess) return check_hip(ctx_, e, "loop graph capture");
     const hipError_t e2 = hipGraphInstantiate(&exec_, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (e2 != hipSuccess) {
       exec_ = nullptr;
       return check_hip(ctx_, e2, "loop graph instantiate");

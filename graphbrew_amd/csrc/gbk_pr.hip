@@ -1,27 +1,44 @@
 // gbk_pr.hip -- PageRank pull on gfx950.
 //
-// PageRankPullGS (bench/src/pr.cc:40-94): Gauss-Seidel, contributions are
-// updated in place so later vertices of the same sweep may see new values
-// (the reference's relaxed atomics allow exactly this).
-// PageRankPull (bench/src/pr_spmv.cc:40-83): Jacobi; the contribution pass
-// of the next iteration is fused into this iteration's store (contrib_next =
-// score / out_degree), which is the same float the reference recomputes.
+// PageRankPullGS (bench/src/pr.cc:40-94): Gauss-Seidel.  The reference
+// updates contributions in place, so a row may read neighbours' values of
+// this sweep or the last one (its relaxed atomics allow either).  On the
+// GPU, thousands of waves updating in place in a different order every
+// sweep keep only the contraction bound (the L1 change shrinks ~0.82x a
+// sweep at Kronecker scale 20, so 20 sweeps do not reach 1e-4 where the
+// reference needs 7).  Here a sweep is a block Gauss-Seidel in a fixed
+// order instead: rows are cut into tile-aligned blocks (pr_blocks, or by
+// default one per ~4096 tiles, at most 8), one launch each,
+// and a row of block b reads the new contributions of the
+// rows before block b and the previous sweep's for the rest -- no races,
+// the same answer every run.  The gathers read one buffer: a block's new
+// contributions go to a side buffer and a small close launch copies them
+// in before the next block (selecting between two buffers per edge costs
+// 20%).  A graph too small to fill the GPU from a block's launch runs one
+// launch that updates contributions in place, as the reference does: on
+// such graphs it converges like the reference's sweep.
+// PageRankPull (bench/src/pr_spmv.cc:40-83): Jacobi, the same kernel with
+// one block that reads only the previous sweep; contrib_next = score /
+// out_degree is the float the reference recomputes.
 //
-// One launch per iteration plus a one-workgroup finish launch.  The pull
-// launch works on the graph's fixed, edge-balanced units: heavy in-rows (more
-// than the graph's heavy threshold) in 1024-edge chunks, each summed by a
-// workgroup and added into the row's accumulator, then tiles of at most 64
-// rows and about 2048 edges, one per wave.  A tile's in-edges are contiguous
-// in the CSR, so the wave streams them in 64-edge chunks with coalesced
-// neighbour reads, maps each edge to its owner lane by a 6-step shuffle
-// search over the tile's edge prefix, gathers contrib[v] and accumulates
-// into the owner's LDS slot.  However the rows are ordered (degree-sorted
-// orders put all hubs side by side), no unit is larger than a chunk or a
-// tile.  The finish launch applies the heavy rows, adds the per-workgroup L1
-// changes, decides convergence and raises a device flag that makes the
-// remaining queued launches return at once -- no host synchronisation
-// inside the iteration loop and no same-address atomics beyond one per
-// heavy chunk.
+// A block's launch works on the graph's fixed, edge-balanced units: heavy
+// in-rows (more than the graph's heavy threshold) in 1024-edge chunks, each
+// summed by a workgroup and added into the row's accumulator, then tiles of
+// at most 64 rows and about 2048 edges, one per wave.  A tile's in-edges are
+// contiguous in the CSR, so the wave streams them in 64-edge chunks with
+// coalesced neighbour reads, maps each edge to its owner lane by a 6-step
+// shuffle search over the tile's edge prefix, gathers the contribution and
+// accumulates into the owner's LDS slot.  A small close launch then
+// finishes the block's heavy rows (and, for Gauss-Seidel, publishes the
+// block's contributions); finishing a heavy row in the workgroup that adds
+// its last chunk would need an agent-scope acquire/release per chunk, which
+// writes back and invalidates the L2 of a multi-XCD GPU (2.4x slower).  A
+// one-workgroup finish launch adds the per-workgroup L1 changes, decides
+// convergence and raises a device flag that makes the remaining queued
+// launches return at once -- no host synchronisation inside the iteration
+// loop.
+#include <algorithm>
+
 #include "gbk_expand.h"
 
 namespace gbk {
@@ -50,29 +67,58 @@ __global__ void pr_init(const int64_t *out_off, int64_t n, float init, float *sc
   }
 }
 
-// One workgroup: applies the heavy rows (their sums were accumulated chunk
-// by chunk into hacc by the pull launch), adds their L1 change to the
-// per-workgroup partials of iteration 'iter' and decides convergence
-// (pr.cc:77-88).
-__global__ void pr_finish(const double *partial, int count, PrState *st, double *errs, int iter,
-                          double tol, int fixed, const int32_t *heavy, int64_t nheavy, float *hacc,
-                          const int64_t *out_off, float *scores, float *contrib_next, float base,
-                          float damp) {
+// Units of one block of rows: its tiles and its heavy rows' chunks.
+struct PrBlock {
+  int64_t row_lo;             // first row (rows below it are earlier blocks)
+  int64_t tile_lo, tile_hi;
+  int64_t chunk_lo, chunk_hi;
+  int64_t heavy_lo, heavy_hi;  // the block's entries of the heavy list
+};
+
+__device__ __forceinline__ int64_t lower_index(const int32_t *a, int64_t len, int64_t key) {
+  int64_t lo = 0, hi = len;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Block b starts at the first tile at or after row b*n/B; its heavy rows are
+// the heavy-list entries in its row range (the list is ascending).  Also
+// clears the heavy rows' accumulators.
+__global__ void pr_blocks_init(RowTiles rt, int64_t n, int B, PrBlock *blk, float *hacc) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < rt.nheavy) hacc[t] = 0.0f;
+  if (t < B) {
+    auto tile_at = [&](int64_t k) -> int64_t {
+      if (k == 0) return 0;
+      if (k == B) return rt.ntiles;
+      return lower_index(rt.tiles, rt.ntiles, n * k / B);
+    };
+    PrBlock pb;
+    pb.tile_lo = tile_at(t);
+    pb.tile_hi = tile_at(t + 1);
+    pb.row_lo = rt.tiles[pb.tile_lo];
+    const int64_t h0 = lower_index(rt.heavy, rt.nheavy, pb.row_lo);
+    const int64_t h1 = lower_index(rt.heavy, rt.nheavy, rt.tiles[pb.tile_hi]);
+    pb.chunk_lo = rt.nheavy ? rt.hprefix[h0] : 0;
+    pb.chunk_hi = rt.nheavy ? rt.hprefix[h1] : 0;
+    pb.heavy_lo = h0;
+    pb.heavy_hi = h1;
+    blk[t] = pb;
+  }
+}
+
+// One workgroup: adds the per-workgroup L1 changes of iteration 'iter' and
+// decides convergence (pr.cc:77-88).
+__global__ void pr_finish(const double *partial, int count, PrState *st, double *errs, int iter, double tol,
+                          int fixed) {
   if (*reinterpret_cast<volatile int *>(&st->done)) return;
   __shared__ double red[1024 / kWave];
   double s = 0;
   for (int i = threadIdx.x; i < count; i += blockDim.x) s += partial[i];
-  for (int64_t h = threadIdx.x; h < nheavy; h += blockDim.x) {
-    const int32_t u = heavy[h];
-    const float sum = hacc[h];
-    hacc[h] = 0.0f;
-    const float old = scores[u];
-    const float nw = base + damp * sum;
-    scores[u] = nw;
-    s += fabs(static_cast<double>(nw - old));
-    const int64_t d = out_off[u + 1] - out_off[u];
-    contrib_next[u] = d ? nw / static_cast<float>(d) : 0.0f;
-  }
   s = wave_sum(s);
   if (lane_id() == 0) red[threadIdx.x / kWave] = s;
   __syncthreads();
@@ -87,32 +133,53 @@ __global__ void pr_finish(const double *partial, int count, PrState *st, double 
   }
 }
 
-// Heavy in-rows chunk by chunk (workgroup sums added into hacc), then light
-// rows tile by tile (one wave per tile of at most 64 rows, the owner of each
-// edge found by a shuffle search over the tile's edge prefix).
-template <bool GS>
+// New score and contribution of row u from its in-sum; returns |change|.
+__device__ __forceinline__ double pr_store(int64_t u, float sum, const int64_t *out_off, float *scores,
+                                           float *contrib_next, float base, float damp) {
+  const float old = scores[u];
+  const float nw = base + damp * sum;
+  scores[u] = nw;
+  const int64_t d = out_off[u + 1] - out_off[u];
+  contrib_next[u] = d ? nw / static_cast<float>(d) : 0.0f;
+  return fabs(static_cast<double>(nw - old));
+}
+
+// One block of rows: heavy chunks first (summed into hacc), then light rows
+// tile by tile (one wave per tile of at most 64 rows, the owner of each edge
+// found by a shuffle search over the tile's edge prefix).  Contributions are
+// read from contrib_cur and the light rows' new ones written to
+// contrib_next.
+template <int U>
 __global__ void __launch_bounds__(kBlock)
-pr_pull_iter(RowTiles rt, const int32_t *in_nbr, const int64_t *out_off, float *scores,
-             const float *contrib_cur, float *contrib_next, const PrState *st, double *partial,
-             float *hacc, float base, float damp) {
+pr_pull_block(RowTiles rt, const PrBlock *blocks, int b, const int32_t *in_nbr,
+              const int64_t *out_off, float *scores, const float *contrib_cur, float *contrib_next,
+              const PrState *st, double *partial, float *hacc, float base, float damp) {
   if (*reinterpret_cast<const volatile int *>(&st->done)) return;
   __shared__ float acc[kBlock];
   __shared__ float hred[kWavesPerBlock];
+  const PrBlock blk = blocks[b];
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
-  const int64_t chunks = rt.nheavy ? rt.hprefix[rt.nheavy] : 0;
-  for (int64_t gc = blockIdx.x; gc < chunks; gc += gridDim.x) {
+  double err = 0;
+  for (int64_t gc = blk.chunk_lo + blockIdx.x; gc < blk.chunk_hi; gc += gridDim.x) {
     int64_t h, first, last;
     heavy_chunk_of(rt, gc, h, first, last);
-    const int64_t s = rt.off[rt.heavy[h]];
+    const int32_t *row = in_nbr + rt.off[rt.heavy[h]];
     float part = 0;
-    for (int64_t k = first + threadIdx.x; k < last; k += blockDim.x) part += contrib_cur[in_nbr[s + k]];
+    // U neighbour loads, then U gathers, in flight per thread
+    for (int64_t k = first + threadIdx.x; k < last; k += U * blockDim.x) {
+      int32_t v[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) v[j] = k + j * blockDim.x < last ? row[k + j * blockDim.x] : -1;
+#pragma unroll
+      for (int j = 0; j < U; j++) part += v[j] >= 0 ? contrib_cur[v[j]] : 0.0f;
+    }
     const float sum = block_sum(part, hred);
     if (threadIdx.x == 0) atomicAdd(&hacc[h], sum);
   }
-  double err = 0;
   const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
-  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wv; t < rt.ntiles; t += waves) {
+  for (int64_t t = blk.tile_lo + static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wv; t < blk.tile_hi;
+       t += waves) {
     const int64_t u = rt.tiles[t] + lane;
     const bool valid = u < rt.tiles[t + 1];
     int64_t start = 0, len = 0;
@@ -126,37 +193,68 @@ pr_pull_iter(RowTiles rt, const int32_t *in_nbr, const int64_t *out_off, float *
     const int64_t incl = wave_incl_scan(len);
     const int64_t excl = incl - len;
     const int64_t total = __shfl(incl, kWave - 1);
-    for (int64_t c = 0; c < total; c += kWave) {
-      const int64_t e = c + lane;
-      int lo = 0;
+    for (int64_t c = 0; c < total; c += U * kWave) {
+      int owner[U];
+      int32_t v[U];
 #pragma unroll
-      for (int step = 32; step >= 1; step >>= 1) {
-        const int probe = lo + step;
-        const int64_t x = __shfl(excl, probe & (kWave - 1));
-        if (probe < kWave && x <= e) lo = probe;
+      for (int j = 0; j < U; j++) {
+        const int64_t e = c + j * kWave + lane;
+        int lo = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+          const int probe = lo + step;
+          const int64_t x = __shfl(excl, probe & (kWave - 1));
+          if (probe < kWave && x <= e) lo = probe;
+        }
+        const int64_t ostart = __shfl(start, lo);
+        const int64_t oexcl = __shfl(excl, lo);
+        const bool act = e < total;
+        owner[j] = act ? lo : kWave;
+        v[j] = act ? in_nbr[ostart + (e - oexcl)] : -1;
       }
-      const int64_t ostart = __shfl(start, lo);
-      const int64_t oexcl = __shfl(excl, lo);
-      const bool act = e < total;
-      const float val = act ? contrib_cur[in_nbr[ostart + (e - oexcl)]] : 0.0f;
-      bool tail;
-      const float run = seg_run_sum(val, act ? lo : kWave, tail);
-      // one writer per owner per chunk; chunks are sequential in this wave
-      if (act && tail) acc[wv * kWave + lo] += run;
+      float val[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) val[j] = v[j] >= 0 ? contrib_cur[v[j]] : 0.0f;
+      // one writer per owner per chunk; chunks apply in order in this wave
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        bool tail;
+        const float run = seg_run_sum(val[j], owner[j], tail);
+        if (owner[j] < kWave && tail) acc[wv * kWave + owner[j]] += run;
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     __builtin_amdgcn_wave_barrier();
-    if (light) {
-      const float old = scores[u];
-      const float nw = base + damp * acc[threadIdx.x];
-      scores[u] = nw;
-      err += fabs(static_cast<double>(nw - old));
-      const int64_t d = out_off[u + 1] - out_off[u];
-      contrib_next[u] = d ? nw / static_cast<float>(d) : 0.0f;
-    }
+    if (light) err += pr_store(u, acc[threadIdx.x], out_off, scores, contrib_next, base, damp);
     __builtin_amdgcn_wave_barrier();
   }
-  block_partial(err, partial);
+  block_partial(err, partial + static_cast<int64_t>(b) * gridDim.x);
+}
+
+// Closes block b: finishes its heavy rows from their accumulators (cleared
+// for the next sweep), writing their contributions to 'contrib'; with a
+// side buffer (Gauss-Seidel) also copies the block's light rows' new
+// contributions from it into 'contrib'.  L1 changes into the partials.
+__global__ void __launch_bounds__(kBlock)
+pr_block_close(RowTiles rt, const PrBlock *blocks, int b, int64_t row_hi, const int64_t *out_off, float *scores,
+               float *contrib, const float *side, const PrState *st, double *partial, float *hacc, float base,
+               float damp) {
+  if (*reinterpret_cast<const volatile int *>(&st->done)) return;
+  const PrBlock blk = blocks[b];
+  const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nth = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  double err = 0;
+  for (int64_t h = blk.heavy_lo + tid; h < blk.heavy_hi; h += nth) {
+    const float sum = hacc[h];
+    hacc[h] = 0.0f;
+    err += pr_store(rt.heavy[h], sum, out_off, scores, contrib, base, damp);
+  }
+  if (side) {
+    const int64_t hi = row_hi < 0 ? blocks[b + 1].row_lo : row_hi;
+    for (int64_t u = blk.row_lo + tid; u < hi; u += nth)
+      if (rt.off[u + 1] - rt.off[u] <= rt.heavy_deg) contrib[u] = side[u];
+  }
+  block_partial(err, partial + static_cast<int64_t>(b) * gridDim.x);
 }
 
 }  // namespace gbk
@@ -171,15 +269,36 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   if (n == 0) return set_error(ctx, GBK_EINVAL, "pr: empty graph");
   hipStream_t s = ctx->stream;
   const RowTiles rt = row_tiles(g, true);
-  const unsigned grid = rows_grid(rt);
+  const bool gs = variant == GBK_PR_GS;
+  // Gauss-Seidel blocks: one per ~4096 tiles (a block's launch then keeps
+  // the GPU's waves busy), at most 8; a small graph's one block is Jacobi
+  int64_t want = ctx->tune.pr_blocks ? ctx->tune.pr_blocks : std::min<int64_t>(8, rt.ntiles / 4096);
+  const int B = gs ? static_cast<int>(std::max<int64_t>(1, std::min(want, std::max<int64_t>(rt.ntiles, 1)))) : 1;
+  // a block's launch covers about 1/B of the tiles, about one per wave;
+  // Jacobi's one launch all of them
+  const int64_t block_units = (rt.ntiles / B) / kWavesPerBlock + 1;
+  const unsigned grid =
+      gs ? static_cast<unsigned>(std::min<int64_t>(resident_grid(ctx, reinterpret_cast<const void *>(pr_pull_block<4>)),
+                                                   std::max<int64_t>(block_units, 64)))
+         : rows_grid(rt);
+  // the close launch copies about n/B contributions (Gauss-Seidel) or
+  // finishes the heavy rows
+  const bool publish = gs && B > 1;
+  const unsigned cgrid = publish ? static_cast<unsigned>(std::min<int64_t>(grid_for(n / B + 1), 1024)) : 16;
+  const bool close = publish || rt.nheavy;
+  // block launches are short and latency-bound: more loads in flight pay;
+  // Jacobi's one launch is bound by the random-line rate and loses 7%
+  const int unroll = ctx->tune.pr_unroll ? ctx->tune.pr_unroll : (publish ? 4 : 1);
+  const size_t nparts = static_cast<size_t>(B) * (grid + (close ? cgrid : 0));
   auto *scores = static_cast<float *>(scratch(ctx, kS0, sizeof(float) * n));
   auto *c0 = static_cast<float *>(scratch(ctx, kS1, sizeof(float) * n));
   auto *c1 = static_cast<float *>(scratch(ctx, kS2, sizeof(float) * n));
   auto *st = static_cast<PrState *>(scratch(ctx, kS3, sizeof(PrState)));
   auto *errs = static_cast<double *>(scratch(ctx, kS4, sizeof(double) * max_iters));
-  auto *partial = static_cast<double *>(scratch(ctx, kS5, sizeof(double) * grid));
+  auto *partial = static_cast<double *>(scratch(ctx, kS5, sizeof(double) * nparts));
   auto *hacc = static_cast<float *>(scratch(ctx, kS6, sizeof(float) * (rt.nheavy + 1)));
-  if (!scores || !c0 || !c1 || !st || !errs || !partial || !hacc)
+  auto *blocks = static_cast<PrBlock *>(scratch(ctx, kS8, sizeof(PrBlock) * B));
+  if (!scores || !c0 || !c1 || !st || !errs || !partial || !hacc || !blocks)
     return set_error(ctx, GBK_ENOMEM, "pr scratch");
   const float kDamp = 0.85f;
   const float init = 1.0f / static_cast<float>(n);
@@ -187,30 +306,42 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   // all iterations are queued (one captured graph); after convergence the
   // kernels return at once on the state's done flag
   auto iterations = [&]() {
-  for (int it = 0; it < max_iters; it++) {
-    const int kt = kt_begin(ctx, GBK_KT_PR_ITER);
-    float *cur = c0, *nxt = c0;
-    if (variant == GBK_PR_GS) {
-      pr_pull_iter<true><<<grid, kBlock, 0, s>>>(rt, g->in_nbr, g->off, scores, c0, c0, st, partial, hacc,
-                                                 base, kDamp);
-    } else {
-      cur = (it & 1) ? c1 : c0;
-      nxt = (it & 1) ? c0 : c1;
-      pr_pull_iter<false><<<grid, kBlock, 0, s>>>(rt, g->in_nbr, g->off, scores, cur, nxt, st, partial,
-                                                  hacc, base, kDamp);
+    for (int it = 0; it < max_iters; it++) {
+      const int kt = kt_begin(ctx, GBK_KT_PR_ITER);
+      // Jacobi: ping-pong buffers; Gauss-Seidel in blocks: c0 read and
+      // published into, c1 the side buffer; in one launch: c0 in place
+      float *cur = gs ? c0 : ((it & 1) ? c1 : c0);
+      float *nxt = gs ? (B > 1 ? c1 : c0) : ((it & 1) ? c0 : c1);
+      for (int b = 0; b < B; b++) {
+        switch (unroll) {
+          case 1:
+            pr_pull_block<1><<<grid, kBlock, 0, s>>>(rt, blocks, b, g->in_nbr, g->off, scores, cur, nxt, st, partial,
+                                                     hacc, base, kDamp);
+            break;
+          case 2:
+            pr_pull_block<2><<<grid, kBlock, 0, s>>>(rt, blocks, b, g->in_nbr, g->off, scores, cur, nxt, st, partial,
+                                                     hacc, base, kDamp);
+            break;
+          default:
+            pr_pull_block<4><<<grid, kBlock, 0, s>>>(rt, blocks, b, g->in_nbr, g->off, scores, cur, nxt, st, partial,
+                                                     hacc, base, kDamp);
+        }
+        if (close)
+          pr_block_close<<<cgrid, kBlock, 0, s>>>(rt, blocks, b, b + 1 == B ? n : -1, g->off, scores,
+                                                  publish ? cur : nxt, publish ? nxt : nullptr, st,
+                                                  partial + static_cast<size_t>(B) * grid, hacc, base, kDamp);
+      }
+      kt_end(ctx, kt);
+      pr_finish<<<1, 1024, 0, s>>>(partial, static_cast<int>(nparts), st, errs, it, tol, fixed_work);
     }
-    kt_end(ctx, kt);
-    pr_finish<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), st, errs, it, tol, fixed_work, rt.heavy,
-                                 rt.nheavy, hacc, g->off, scores, nxt, base, kDamp);
-  }
   };
   LoopGraph loop(ctx);
   if (int rc = loop.capture(iterations)) return rc;
   EventTimer timer(ctx);
-  fill<float>(ctx, hacc, 0.0f, rt.nheavy + 1);
+  pr_blocks_init<<<grid_for(std::max<int64_t>(rt.nheavy, B)), kBlock, 0, s>>>(rt, n, B, blocks, hacc);
   pr_init<<<grid_for(n), kBlock, 0, s>>>(g->off, n, init, scores, c0, st);
   if (int rc = loop.run(iterations)) return rc;
-  GBK_LAUNCH_CHECK(ctx, "pr_pull_iter");
+  GBK_LAUNCH_CHECK(ctx, "pr_pull_block");
   const double ms = timer.stop_ms();
   PrState h{};
   GBK_HIP(ctx, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));

@@ -106,7 +106,11 @@ int gbk_ctx_sync(gbk_ctx *ctx);
  * "xcd_swizzle", "relax_grid", "graphs" (1: SSSP rounds and PR iterations
  * are captured once per call as a hipGraph and replayed; 0: launched one by
  * one); "batch_grid" (workgroups of the source-batch relax launches: 0 launches
- * as many as the GPU keeps resident, else 8..16384 in multiples of 8); and
+ * as many as the GPU keeps resident, else 8..16384 in multiples of 8);
+ * "pr_blocks" (row blocks of a Gauss-Seidel PageRank sweep, run in order: 0
+ * picks one per ~4096 row tiles, at most 8; 1 is one launch updating in
+ * place); "pr_unroll" (64-edge chunks a PageRank wave keeps in flight: 1, 2,
+ * 4, or 0 for 4 in block sweeps and 1 otherwise); and
  * "ktime" (a gbk_ktime_kernel) selecting the kernel whose
  * launches are timed one by one with HIP events (loops then run eagerly).  Returns
  * GBK_EINVAL for an unknown name or bad value.  No reference counterpart:

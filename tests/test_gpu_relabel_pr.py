@@ -173,3 +173,148 @@ def test_pr_heavy_rows_and_directed(ctx):
     s, _ = ctx.pr(ddg, gbk.PR_SPMV, max_iters=8, tol=0, fixed=True)
     ref, _, _ = O.pr_spmv(gd, max_iters=8, tol=0, fixed=True)
     np.testing.assert_allclose(s, ref, rtol=1e-4, atol=1e-9)
+
+
+# --------------------------------------------- Gauss-Seidel in row blocks
+# The Gauss-Seidel sweep runs in pr_blocks tile-aligned row blocks, in order:
+# a row of block b reads the new contributions of the rows before block b
+# and the previous sweep's for the rest (one block: contributions updated in
+# place, as the reference does; held to the verifier and the fixed point
+# above).  Tiles start every 64 rows and
+# wherever the prefix of light in-degrees (rows of at most 2048 in-edges)
+# crosses a multiple of 2048; block b starts at the first tile start at or
+# after row b*n/B.
+def _heavy_in_rows():
+    """Directed graph whose in-rows sit at and around the heavy threshold and
+    its chunk size, some far above it."""
+    rng = np.random.default_rng(5)
+    n = 20000
+    sizes = [1, 64, 1023, 1024, 1025, 2047, 2048, 2049, 4096, 4097, 9000, 15000]
+    us, vs = [], []
+    for i, k in enumerate(sizes * 3):
+        vs.append(np.full(k, (i * 613) % n, np.int32))
+        us.append(rng.choice(n, k, replace=False).astype(np.int32))
+    return O.make_graph(np.concatenate(us), np.concatenate(vs), symmetrize=False, n=n)
+
+
+def _in_csr(g):
+    return (g.in_off, g.in_nbr) if g.directed else (g.off, g.nbr)
+
+
+def _block_starts(g, blocks, heavy=2048):
+    in_off, _ = _in_csr(g)
+    deg = np.diff(in_off)
+    light = np.where(deg > heavy, 0, deg)
+    lp = np.concatenate([[0], np.cumsum(light)[:-1]])
+    v = np.arange(g.n)
+    flag = (v % 64 == 0)
+    flag[1:] |= (lp[1:] // 2048) != (lp[:-1] // 2048)
+    tiles = np.flatnonzero(flag)
+    starts = [0]
+    for b in range(1, blocks):
+        k = np.searchsorted(tiles, g.n * b // blocks)
+        starts.append(int(tiles[k]) if k < len(tiles) else g.n)
+    return np.array(starts + [g.n])
+
+
+def _block_gs_fp64(g, blocks, iters):
+    """The sweep above in float64 (pr.cc:40-94 with the block read rule)."""
+    n = g.n
+    in_off, in_nbr = _in_csr(g)
+    out_deg = np.diff(g.off).astype(np.float64)
+    rows = np.repeat(np.arange(n), np.diff(in_off))
+    starts = _block_starts(g, blocks)
+    s = np.full(n, 1.0 / n)
+    c = np.where(out_deg > 0, s / np.maximum(out_deg, 1), 0.0)
+    for _ in range(iters):
+        new = c.copy()
+        for lo, hi in zip(starts[:-1], starts[1:]):
+            if hi <= lo:
+                continue
+            e0, e1 = in_off[lo], in_off[hi]
+            src = in_nbr[e0:e1]
+            val = np.where(src < lo, new[src], c[src])
+            tot = np.bincount(rows[e0:e1] - lo, weights=val, minlength=hi - lo)
+            s[lo:hi] = 0.15 / n + 0.85 * tot
+            new[lo:hi] = np.where(out_deg[lo:hi] > 0, s[lo:hi] / np.maximum(out_deg[lo:hi], 1), 0.0)
+        c = new
+    return s
+
+
+def _jacobi_fp64(g, iters):
+    """PageRank-Jacobi (pr_spmv.cc:40-83) in float64."""
+    return _block_gs_fp64(g, 1, iters)
+
+
+@pytest.fixture
+def pr_blocks(ctx):
+    def use(count):
+        ctx.set_param("pr_blocks", count)
+    yield use
+    ctx.set_param("pr_blocks", 0)
+
+
+BLOCK_GRAPHS = {
+    "kron13": lambda: O.kron(13),
+    "kron12_sorted": lambda: O.relabel(O.kron(12), O.mapping_sort(O.kron(12))),
+    "star9000": lambda: _star(9000),
+    "heavy_in": _heavy_in_rows,
+}
+
+
+@pytest.mark.parametrize("name", list(BLOCK_GRAPHS))
+@pytest.mark.parametrize("blocks", [2, 3, 16, 64])
+def test_pr_gs_block_sweep_matches_fp64(ctx, pr_blocks, name, blocks):
+    """One and three sweeps equal the float64 block sweep to 1e-5 (a row
+    reading the wrong buffer is off by far more after one sweep)."""
+    g = BLOCK_GRAPHS[name]()
+    dg = ctx.upload_graph(g)
+    pr_blocks(blocks)
+    for iters in (1, 3):
+        s, st = ctx.pr(dg, gbk.PR_GS, max_iters=iters, tol=0.0, fixed=True)
+        assert st.iterations == iters
+        np.testing.assert_allclose(s, _block_gs_fp64(g, blocks, iters), rtol=1e-5, atol=1e-12)
+
+
+def test_block_gs_reference_model_pins():
+    """The float64 model itself: one block is the oracle's Jacobi, and more
+    blocks are not (so the tests above tell the read rules apart)."""
+    g = O.kron(12)
+    ref, _, _ = O.pr_spmv(g, max_iters=3, tol=0.0, fixed=True)
+    np.testing.assert_allclose(_jacobi_fp64(g, 3), ref, rtol=1e-5)
+    assert np.max(np.abs(_block_gs_fp64(g, 16, 1) / _jacobi_fp64(g, 1) - 1)) > 1e-2
+
+
+@pytest.mark.parametrize("blocks", [4, 16, 64])
+def test_pr_gs_blocks_converge(ctx, pr_blocks, blocks):
+    """The fixed-order sweep converges in about the oracle's Gauss-Seidel
+    sweeps (Kronecker scale 15: 13 in the float64 model for 4-256 blocks,
+    the one-thread oracle 13; Gauss-Seidel is not faster than Jacobi at
+    this size, as the reference's is not)."""
+    g = O.kron(15)
+    dg = ctx.upload_graph(g)
+    pr_blocks(blocks)
+    s, st = ctx.pr(dg, gbk.PR_GS, max_iters=20, tol=1e-4)
+    _, it_ref, _ = O.pr_gs(g, max_iters=20, tol=1e-4)
+    print(f"block GS x{blocks}: iters {st.iterations} (oracle GS {it_ref})")
+    assert st.final_error < 1e-4 and st.iterations <= it_ref + 2
+    assert O.pr_residual(g, s) < 1e-4               # PRVerifier (pr.cc:117-134)
+
+
+def test_pr_heavy_rows_finished_in_launch(ctx):
+    """A low heavy threshold (read when the graph is built) makes most rows
+    heavy: each is finished by the workgroup adding its last chunk."""
+    g = O.kron(13)
+    ctx.set_param("heavy_degree", 64)
+    try:
+        dg = ctx.upload_graph(g)
+    finally:
+        ctx.set_param("heavy_degree", 2048)
+    s, _ = ctx.pr(dg, gbk.PR_SPMV, max_iters=6, tol=0.0, fixed=True)
+    ref, _, _ = O.pr_spmv(g, max_iters=6, tol=0.0, fixed=True)
+    np.testing.assert_allclose(s, ref, rtol=1e-4, atol=1e-9)
+    s, _ = ctx.pr(dg, gbk.PR_GS, max_iters=3, tol=0.0, fixed=True)
+    # tiles of a 64-threshold graph differ from the default's: compare the
+    # fixed point instead
+    s2, _ = ctx.pr(dg, gbk.PR_GS, max_iters=200, tol=1e-9)
+    np.testing.assert_allclose(s2, O.pr_fp64(g), rtol=1e-4, atol=1e-10)
