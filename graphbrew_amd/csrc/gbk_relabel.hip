@@ -29,7 +29,7 @@ int gbk_graph_finalize(gbk_ctx *ctx, gbk_graph *g);
 
 namespace gbk {
 
-constexpr int kSortCap = 2048;  // elements per LDS sort (keys 16 KiB + weights 8 KiB)
+constexpr int kSortCap = 2048;  // elements one wave sorts in registers (32 a lane)
 
 __global__ void relabel_invert(const int64_t *off, const int32_t *new_ids, const int32_t *org_old,
                                int64_t n, int32_t *inv, int64_t *deg_new, int32_t *org_new,
@@ -50,12 +50,11 @@ __global__ void relabel_invert(const int64_t *off, const int32_t *new_ids, const
 }
 
 // Degree classes: <=8, <=16, <=32, <=64 (lane-group register sorts), <=128,
-// <=256, <=512 (wave register sorts), <=2048 (workgroup LDS sort), larger
-// (chunk sort + merge).  Empty rows need no work.  Each wave stages its class
-// lists in LDS and flushes in batches.
-constexpr int kClasses = 9;
-constexpr int kClassWindow = 7;  // the LDS sort class
-constexpr int kClassBig = 8;
+// <=256, <=512, <=1024, <=2048 (wave register sorts), larger (chunk sort +
+// merge).  Empty rows need no work.  Each wave stages its class lists in LDS
+// and flushes in batches.
+constexpr int kClasses = 10;
+constexpr int kClassBig = 9;
 constexpr int kBinSeg = 256;     // staging entries per class and wave
 struct RelabelLists {
   int32_t *l[kClasses];
@@ -73,7 +72,7 @@ relabel_bin_rows(const int64_t *deg_new, int64_t n, RelabelLists lists, unsigned
     const int64_t r = base + threadIdx.x;
     const int64_t d = r < n ? deg_new[r] : 0;
     const int cls = d == 0 ? -1 : d <= 8 ? 0 : d <= 16 ? 1 : d <= 32 ? 2 : d <= 64 ? 3
-                  : d <= 128 ? 4 : d <= 256 ? 5 : d <= 512 ? 6 : d <= kSortCap ? 7 : 8;
+                  : d <= 128 ? 4 : d <= 256 ? 5 : d <= 512 ? 6 : d <= 1024 ? 7 : d <= kSortCap ? 8 : 9;
     for (int c = 0; c < kClasses; c++) q[c].push(cls == c, static_cast<int32_t>(r));
   }
   for (int c = 0; c < kClasses; c++) q[c].flush();
@@ -143,27 +142,18 @@ struct RowKey {
   }
 };
 
-template <bool W, int E>
-__global__ void __launch_bounds__(kBlock)
-relabel_wave_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const int64_t *off,
-                  const int32_t *nbr, const int32_t *wgt, const int32_t *new_ids,
-                  const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out) {
+// One wave sorts up to 64*E keys: load(i) gives key i (i < len), store(i,
+// key) takes the i-th smallest.  Padding keys are all ones.
+template <bool W, int E, class Load, class Store>
+__device__ __forceinline__ void wave_sort(int64_t len, Load load, Store store) {
   using K = typename RowKey<W>::T;
   constexpr int N = kWave * E;
   const int lane = lane_id();
-  const int64_t ri = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  if (ri >= nrows) return;  // wave-uniform
-  const int32_t r = rows[ri];
-  const int32_t u = inv[r];
-  const int64_t src = off[u];
-  const int64_t len = off[u + 1] - src;
-  const int64_t dst = new_off[r];
   K key[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int64_t i = static_cast<int64_t>(e) * kWave + lane;
-    key[e] = i < len ? RowKey<W>::make(new_ids[nbr[src + i]], W ? wgt[src + i] : 0)
-                     : static_cast<K>(~static_cast<K>(0));
+    key[e] = i < len ? load(i) : static_cast<K>(~static_cast<K>(0));
   }
 #pragma unroll
   for (int k = 2; k <= N; k <<= 1) {
@@ -199,111 +189,33 @@ relabel_wave_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const 
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int64_t i = static_cast<int64_t>(e) * kWave + lane;
-    if (i < len) {
-      if constexpr (W) {
-        nbr_out[dst + i] = static_cast<int32_t>(key[e] >> 32);
-        wgt_out[dst + i] = static_cast<int32_t>(static_cast<uint32_t>(key[e]) ^ 0x80000000u);
-      } else {
-        nbr_out[dst + i] = static_cast<int32_t>(key[e]);
-      }
-    }
+    if (i < len) store(i, key[e]);
   }
 }
 
-template <bool W>
-struct SortItem {
-  uint64_t key;   // (row-in-window << 32) | neighbour
-  int32_t w;
-};
-
-template <bool W>
-__device__ __forceinline__ bool item_greater(uint64_t ka, int32_t wa, uint64_t kb, int32_t wb) {
-  if (ka != kb) return ka > kb;
-  return W ? (wa > wb) : false;
-}
-
-// Bitonic sort of P (power of two) items in LDS by the whole workgroup.
-template <bool W>
-__device__ void lds_bitonic(uint64_t *key, int32_t *wt, int P) {
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < P; i += blockDim.x) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const bool up = (i & k) == 0;
-          const uint64_t ka = key[i], kb = key[ixj];
-          const int32_t wa = W ? wt[i] : 0, wb = W ? wt[ixj] : 0;
-          if (item_greater<W>(ka, wa, kb, wb) == up) {
-            key[i] = kb;
-            key[ixj] = ka;
-            if (W) {
-              wt[i] = wb;
-              wt[ixj] = wa;
-            }
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// One workgroup per window of up to R rows taken from 'rows' (total elements
-// of the window <= kSortCap by construction of the degree class).
-template <bool W>
+template <bool W, int E>
 __global__ void __launch_bounds__(kBlock)
-relabel_window_sort(const int32_t *rows, int64_t nrows, int R, const int32_t *inv,
-                    const int64_t *off, const int32_t *nbr, const int32_t *wgt,
-                    const int32_t *new_ids, const int64_t *new_off, int32_t *nbr_out,
-                    int32_t *wgt_out) {
-  __shared__ uint64_t key[kSortCap];
-  __shared__ int32_t wt[W ? kSortCap : 1];
-  __shared__ int64_t pre[129];   // prefix of row sizes inside the window
-  __shared__ int32_t row_id[128];
-  __shared__ int64_t src_off[128];
-  const int64_t first = static_cast<int64_t>(blockIdx.x) * R;
-  const int cnt = static_cast<int>(min(static_cast<int64_t>(R), nrows - first));
-  if (threadIdx.x < cnt) {
-    const int32_t r = rows[first + threadIdx.x];
-    row_id[threadIdx.x] = r;
-    const int32_t u = inv[r];
-    src_off[threadIdx.x] = off[u];
-    pre[threadIdx.x + 1] = off[u + 1] - off[u];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    pre[0] = 0;
-    for (int i = 0; i < cnt; i++) pre[i + 1] += pre[i];
-  }
-  __syncthreads();
-  const int total = static_cast<int>(pre[cnt]);
-  int P = 1;
-  while (P < total) P <<= 1;
-  for (int i = threadIdx.x; i < P; i += blockDim.x) {
-    if (i < total) {
-      int lo = 0, hi = cnt;  // last row with pre[row] <= i
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (pre[mid] <= i) lo = mid;
-        else hi = mid;
-      }
-      const int64_t e = src_off[lo] + (i - pre[lo]);
-      const uint32_t v = static_cast<uint32_t>(new_ids[nbr[e]]);
-      key[i] = (static_cast<uint64_t>(lo) << 32) | v;
-      if (W) wt[i] = wgt[e];
-    } else {
-      key[i] = ~0ull;
-      if (W) wt[i] = INT32_MAX;
-    }
-  }
-  __syncthreads();
-  lds_bitonic<W>(key, wt, P);
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    const int lr = static_cast<int>(key[i] >> 32);
-    const int64_t dst = new_off[row_id[lr]] + (i - pre[lr]);
-    nbr_out[dst] = static_cast<int32_t>(key[i] & 0xffffffffu);
-    if (W) wgt_out[dst] = wt[i];
-  }
+relabel_wave_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const int64_t *off,
+                  const int32_t *nbr, const int32_t *wgt, const int32_t *new_ids,
+                  const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out) {
+  using K = typename RowKey<W>::T;
+  const int64_t ri = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (ri >= nrows) return;  // wave-uniform
+  const int32_t r = rows[ri];
+  const int32_t u = inv[r];
+  const int64_t src = off[u];
+  const int64_t len = off[u + 1] - src;
+  const int64_t dst = new_off[r];
+  wave_sort<W, E>(
+      len, [&](int64_t i) { return RowKey<W>::make(new_ids[nbr[src + i]], W ? wgt[src + i] : 0); },
+      [&](int64_t i, K key) {
+        if constexpr (W) {
+          nbr_out[dst + i] = static_cast<int32_t>(key >> 32);
+          wgt_out[dst + i] = static_cast<int32_t>(static_cast<uint32_t>(key) ^ 0x80000000u);
+        } else {
+          nbr_out[dst + i] = static_cast<int32_t>(key);
+        }
+      });
 }
 
 // Big rows: composite key so a plain integer compare gives (v, w) order.
@@ -328,35 +240,26 @@ struct BigKey {
 };
 
 // Chunk sort: chunk c covers elements [cstart, cstart+len) of big row 'row'
-// (staging coordinates: row i of the big list starts at stage_off[i]).
+// (staging coordinates: row i of the big list starts at stage_off[i]); one
+// wave sorts a chunk of up to kSortCap keys in registers.
 template <bool W>
 __global__ void __launch_bounds__(kBlock)
 relabel_big_chunks(const int32_t *big_rows, const int64_t *stage_off, const int32_t *chunk_row,
-                   const int64_t *chunk_start, const int32_t *inv, const int64_t *off,
+                   const int64_t *chunk_start, int64_t nchunks, const int32_t *inv, const int64_t *off,
                    const int32_t *nbr, const int32_t *wgt, const int32_t *new_ids,
                    typename BigKey<W>::T *stage) {
   using K = typename BigKey<W>::T;
-  __shared__ uint64_t key[kSortCap];
-  __shared__ int32_t wt[1];
-  const int bi = chunk_row[blockIdx.x];
-  const int64_t cs = chunk_start[blockIdx.x];
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (c >= nchunks) return;  // wave-uniform
+  const int bi = chunk_row[c];
+  const int64_t cs = chunk_start[c];
   const int32_t u = inv[big_rows[bi]];
-  const int64_t deg = off[u + 1] - off[u];
-  const int len = static_cast<int>(min(static_cast<int64_t>(kSortCap), deg - cs));
-  int P = 1;
-  while (P < len) P <<= 1;
-  for (int i = threadIdx.x; i < P; i += blockDim.x) {
-    if (i < len) {
-      const int64_t e = off[u] + cs + i;
-      key[i] = static_cast<uint64_t>(BigKey<W>::make(new_ids[nbr[e]], W ? wgt[e] : 1));
-    } else {
-      key[i] = ~0ull;
-    }
-  }
-  __syncthreads();
-  lds_bitonic<false>(key, wt, P);
-  for (int i = threadIdx.x; i < len; i += blockDim.x)
-    stage[stage_off[bi] + cs + i] = static_cast<K>(key[i]);
+  const int64_t first = off[u] + cs;
+  const int64_t len = min(static_cast<int64_t>(kSortCap), off[u + 1] - first);
+  K *out = stage + stage_off[bi] + cs;
+  wave_sort<W, kSortCap / kWave>(
+      len, [&](int64_t i) { return BigKey<W>::make(new_ids[nbr[first + i]], W ? wgt[first + i] : 1); },
+      [&](int64_t i, K key) { out[i] = key; });
 }
 
 // One merge pass of run width 'width': element t of the staging layout is
@@ -456,8 +359,8 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
     if (!bins.l[c]) return set_error(ctx, GBK_ENOMEM, "relabel bins");
   }
   {
-    // classes 6..8 share one slot, n + 1 entries each
-    auto *rest = static_cast<int32_t *>(scratch(ctx, kS14, sizeof(int32_t) * 3 * (n + 1)));
+    // classes 6..9 share one slot, n + 1 entries each
+    auto *rest = static_cast<int32_t *>(scratch(ctx, kS14, sizeof(int32_t) * 4 * (n + 1)));
     if (!rest) return set_error(ctx, GBK_ENOMEM, "relabel bins");
     for (int c = 6; c < kClasses; c++) bins.l[c] = rest + (c - 6) * (n + 1);
   }
@@ -486,24 +389,19 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
     }
     GBK_LAUNCH_CHECK(ctx, "relabel_group_sort");
   }
-  // rows of <= 512 entries: one wave each, in registers
-  for (int c = 4; c <= 6; c++) {
+  // rows of <= 2048 entries: one wave each, in registers
+  for (int c = 4; c < kClassBig; c++) {
     const int64_t cnt = static_cast<int64_t>(counts[c]);
     if (cnt == 0) continue;
     const unsigned blocks = static_cast<unsigned>((cnt + kWavesPerBlock - 1) / kWavesPerBlock);
-    if (c == 4)
-      relabel_wave_sort<W, 2><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out);
-    else if (c == 5)
-      relabel_wave_sort<W, 4><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out);
-    else
-      relabel_wave_sort<W, 8><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out);
+    switch (c) {
+      case 4: relabel_wave_sort<W, 2><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+      case 5: relabel_wave_sort<W, 4><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+      case 6: relabel_wave_sort<W, 8><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+      case 7: relabel_wave_sort<W, 16><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+      default: relabel_wave_sort<W, 32><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+    }
     GBK_LAUNCH_CHECK(ctx, "relabel_wave_sort");
-  }
-  if (counts[kClassWindow] > 0) {
-    const int64_t cnt = static_cast<int64_t>(counts[kClassWindow]);
-    relabel_window_sort<W><<<cnt, kBlock, 0, s>>>(lists[kClassWindow], cnt, 1, inv, off, nbr, wgt, new_ids_d,
-                                                   off_out, nbr_out, wgt_out);
-    GBK_LAUNCH_CHECK(ctx, "relabel_window_sort");
   }
   const int nbig = static_cast<int>(counts[kClassBig]);
   int32_t *big_list = lists[kClassBig];
@@ -568,8 +466,9 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
   GBK_HIP(ctx, hipMemcpyAsync(soff_d, soff.data(), sizeof(int64_t) * (nbig + 1), hipMemcpyHostToDevice, s));
   GBK_HIP(ctx, hipMemcpyAsync(crow_d, crow.data(), sizeof(int32_t) * crow.size(), hipMemcpyHostToDevice, s));
   GBK_HIP(ctx, hipMemcpyAsync(cstart_d, cstart.data(), sizeof(int64_t) * cstart.size(), hipMemcpyHostToDevice, s));
-  relabel_big_chunks<W><<<static_cast<unsigned>(crow.size()), kBlock, 0, s>>>(
-      big_d, soff_d, crow_d, cstart_d, inv, off, nbr, wgt, new_ids_d, st0);
+  const int64_t nchunks = static_cast<int64_t>(crow.size());
+  relabel_big_chunks<W><<<static_cast<unsigned>((nchunks + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, s>>>(
+      big_d, soff_d, crow_d, cstart_d, nchunks, inv, off, nbr, wgt, new_ids_d, st0);
   GBK_LAUNCH_CHECK(ctx, "relabel_big_chunks");
   K *src = st0, *dst = st1;
   int active = nbig;  // rows longer than the current width: a prefix
