@@ -259,23 +259,39 @@ __global__ void msbfs_finish(const int64_t *off, int64_t n, int Kp, unsigned lon
                              unsigned long long *front, unsigned long long *next, int32_t *depth_vm,
                              int32_t level, unsigned long long *p_verts, unsigned long long *p_edges) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const int lane = lane_id();
+  const int G = kWave / Kp;  // depth rows one store instruction covers
   unsigned long long verts = 0, edges = 0;
-  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
-    const unsigned long long nx = next[v];
-    const unsigned long long sv = seen[v];
-    const unsigned long long add = nx & ~sv;
-    if (front[v] != add) front[v] = add;
-    if (nx) next[v] = 0;
-    if (add) {
-      seen[v] = sv | add;
-      verts++;
-      edges += static_cast<unsigned long long>(off[v + 1] - off[v]);
-      unsigned long long b = add;
-      while (b) {
-        const int k = __ffsll(static_cast<long long>(b)) - 1;
-        depth_vm[v * Kp + k] = level;
-        b &= b - 1;
+  // the loop bound is wave-uniform (every wave's 64 vertices start at a
+  // multiple of 64), so the shuffles below run on whole waves
+  const int64_t nr = (n + kWave - 1) / kWave * kWave;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < nr; v += stride) {
+    unsigned long long add = 0;
+    if (v < n) {
+      const unsigned long long nx = next[v];
+      const unsigned long long sv = seen[v];
+      add = nx & ~sv;
+      if (front[v] != add) front[v] = add;
+      if (nx) next[v] = 0;
+      if (add) {
+        seen[v] = sv | add;
+        verts++;
+        edges += static_cast<unsigned long long>(off[v + 1] - off[v]);
       }
+    }
+    // depth rows of the wave's new vertices: lane group g of Kp lanes
+    // writes row v0 + j0 + g (lane k: source k), one coalesced masked
+    // store per G rows instead of one scattered store per new source
+    const uint64_t any = __ballot(add != 0);
+    if (any == 0) continue;
+    const int64_t v0 = v - lane;
+    const int k = lane % Kp;
+    for (int j0 = 0; j0 < kWave; j0 += G) {
+      const uint64_t grp = (G == kWave ? ~0ull : ((1ull << G) - 1)) << j0;
+      if ((any & grp) == 0) continue;  // wave-uniform
+      const int j = j0 + lane / Kp;
+      const unsigned long long aj = __shfl(add, j);
+      if ((aj >> k) & 1ull) depth_vm[(v0 + j) * Kp + k] = level;
     }
   }
   block_partial(verts, p_verts);
