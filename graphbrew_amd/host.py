@@ -50,6 +50,9 @@ def lib():
             "gbh_distance_fingerprint": (None, [i64, VP, VP, VP]),
             "gbh_validate_permutation": (i32, [i64, VP]),
             "gbh_mapping_read": (i32, [C.c_char_p, C.POINTER(i64), VP]),
+            "gbh_sg_info": (i32, [C.c_char_p, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64), C.POINTER(i32)]),
+            "gbh_sg_read": (i32, [C.c_char_p, i32, VP, VP, VP, VP, VP, VP, VP]),
+            "gbh_sg_write": (i32, [C.c_char_p, i32, i64, i64, VP, VP, VP, VP, VP, VP, VP]),
             "gbh_mapping_write": (i32, [C.c_char_p, i64, VP]),
             "gbh_source_picker": (i32, [i64, VP, VP, i32, i32, VP, VP]),
             "gbh_afforest_samples": (i32, [i64, i32, VP]),
@@ -190,6 +193,48 @@ def mapping_read(path: str) -> np.ndarray:
 def mapping_write(path: str, ids) -> None:
     ids = np.ascontiguousarray(ids, np.int32)
     _chk(lib().gbh_mapping_write(path.encode(), len(ids), _p(ids)), "mapping_write")
+
+
+def sg_info(path: str) -> dict:
+    """Header of a .sg / .wsg serialized graph (gapbs/reader.h:445-485)."""
+    d, w = C.c_int32(0), C.c_int32(0)
+    n, m = C.c_int64(0), C.c_int64(0)
+    _chk(lib().gbh_sg_info(path.encode(), C.byref(d), C.byref(n), C.byref(m), C.byref(w)), "sg_info")
+    return {"directed": bool(d.value), "n": n.value, "m": m.value, "weighted": bool(w.value)}
+
+
+def sg_read(path: str, weights: bool = False, scheme: str = "unit") -> dict:
+    """Reads a serialized graph: off, nbr, org_ids (+ in_off, in_nbr for a
+    directed graph; + wgt, in_wgt when weights, generated by 'scheme' for a
+    .sg file as ReadSerializedGraph does, gapbs/reader.h:48-59)."""
+    h = sg_info(path)
+    n, m, d = h["n"], h["m"], h["directed"]
+    out = {"directed": d, "off": np.zeros(n + 1, np.int64), "nbr": np.zeros(m, np.int32),
+           "org_ids": np.zeros(n, np.int32)}
+    if d:
+        out["in_off"] = np.zeros(n + 1, np.int64)
+        out["in_nbr"] = np.zeros(m, np.int32)
+    if weights:
+        out["wgt"] = np.zeros(m, np.int32)
+        if d:
+            out["in_wgt"] = np.zeros(m, np.int32)
+    _chk(lib().gbh_sg_read(path.encode(), {"unit": 0, "hash": 1}[scheme], _p(out["off"]), _p(out["nbr"]),
+                           _p(out.get("wgt")), _p(out.get("in_off")), _p(out.get("in_nbr")),
+                           _p(out.get("in_wgt")), _p(out["org_ids"])), "sg_read")
+    return out
+
+
+def sg_write(path: str, off, nbr, wgt=None, in_off=None, in_nbr=None, in_wgt=None, org_ids=None) -> None:
+    """Writes a .sg (wgt None) or .wsg serialized graph (gapbs/writer.h:336-377);
+    a directed graph passes its in-CSR."""
+    off = np.ascontiguousarray(off, np.int64)
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    n, m = len(off) - 1, len(nbr)
+    org = np.arange(n, dtype=np.int32) if org_ids is None else np.ascontiguousarray(org_ids, np.int32)
+    arr = lambda x, t: None if x is None else np.ascontiguousarray(x, t)
+    wgt, in_off, in_nbr, in_wgt = arr(wgt, np.int32), arr(in_off, np.int64), arr(in_nbr, np.int32), arr(in_wgt, np.int32)
+    _chk(lib().gbh_sg_write(path.encode(), int(in_off is not None), n, m, _p(off), _p(nbr), _p(wgt), _p(in_off),
+                            _p(in_nbr), _p(in_wgt), _p(org)), "sg_write")
 
 
 def source_picker(off, count: int, org_ids=None, repeats: int = 1):

@@ -2,7 +2,7 @@
 // bfs, cc, cc_sv, sssp, bc).  Mirrors the reference binaries' interface:
 //   CLI      getopt flags of CLBase/CLApp/CLPageRank/CLDelta/CLIterApp
 //            (gapbs/command_line.h:41, 333, 504-640); -o "id[:opt...]"
-//   build    -g/-u synthetic (always symmetrized) or -f .el/.wel, then the
+//   build    -g/-u synthetic (always symmetrized) or -f .el/.wel/.sg/.wsg, then the
 //            orderings of -o in sequence (MakeGraph, gapbs/builder.h:715-1002)
 //   harness  BenchmarkKernel (gapbs/benchmark.h:435-545): per-trial
 //            "Trial Time", optional analysis / verification, "Average Time",
@@ -297,12 +297,24 @@ struct Session {
     rep.Start();
     std::vector<int32_t> u, v, w;
     Timer t;
+    const std::string &f = cli.filename;
+    const auto has_suffix = [&](const char *suf) {
+      const size_t k = std::strlen(suf);
+      return f.size() > k && f.compare(f.size() - k, k, suf) == 0;
+    };
+    if (has_suffix(".sg") || has_suffix(".wsg")) {
+      LoadSerialized(need_weights);
+      rep.Stop();
+      PrintTime("Representation Build Time", rep.Seconds());
+      PrintLabel("Topology Analysis", "disabled");
+      ApplyOrders(total);
+      return;
+    }
     if (!cli.filename.empty()) {
       t.Start();
-      const std::string &f = cli.filename;
-      const bool wel = f.size() > 4 && f.compare(f.size() - 4, 4, ".wel") == 0;
-      const bool el = f.size() > 3 && f.compare(f.size() - 3, 3, ".el") == 0;
-      if (!wel && !el) Die("Unrecognized suffix: only .el and .wel are read by this driver", -3);
+      const bool wel = has_suffix(".wel");
+      const bool el = has_suffix(".el");
+      if (!wel && !el) Die("Unrecognized suffix: only .el, .wel, .sg and .wsg are read by this driver", -3);
       int64_t m = 0;
       if (gbh_read_el(f.c_str(), wel, &m, nullptr, nullptr, nullptr) != 0) Die("Couldn't open file " + f, -2);
       u.resize(m);
@@ -339,6 +351,52 @@ struct Session {
     rep.Stop();
     PrintTime("Representation Build Time", rep.Seconds());
     PrintLabel("Topology Analysis", "disabled");
+    ApplyOrders(total);
+  }
+
+  // .sg / .wsg (ReadSerializedGraph, gapbs/reader.h:445-578; symmetrize as
+  // MakeGraph does for serialized input, gapbs/builder.h:741-790): the CSR
+  // is uploaded as stored; -s on a directed unweighted file with identity
+  // original ids rebuilds it symmetrized from its edges.
+  void LoadSerialized(bool need_weights) {
+    Timer t;
+    t.Start();
+    const std::string &f = cli.filename;
+    int directed = 0, file_weighted = 0;
+    int64_t n = 0, m = 0;
+    if (gbh_sg_info(f.c_str(), &directed, &n, &m, &file_weighted) != 0) Die("Couldn't open file " + f, -6);
+    std::vector<int64_t> off(n + 1), in_off(directed ? n + 1 : 0);
+    std::vector<int32_t> nbr(m), in_nbr(directed ? m : 0), org(n);
+    std::vector<int32_t> w(need_weights ? m : 0), in_w(need_weights && directed ? m : 0);
+    const int scheme = cli.weight_scheme == "hash" ? 1 : 0;
+    if (gbh_sg_read(f.c_str(), scheme, off.data(), nbr.data(), need_weights ? w.data() : nullptr,
+                    directed ? in_off.data() : nullptr, directed ? in_nbr.data() : nullptr,
+                    need_weights && directed ? in_w.data() : nullptr, org.data()) != 0)
+      Die("Serialized graph " + f + " is truncated or inconsistent", -6);
+    t.Stop();
+    PrintTime("Read Time", t.Seconds());
+    t.Start();
+    if (cli.symmetrize && directed) {
+      if (need_weights) Die("Symmetrizing weighted serialized graphs is unsupported", 1);
+      for (int64_t i = 0; i < n; i++)
+        if (org[i] != i) Die("Cannot symmetrize a serialized graph whose original IDs are not the identity", 1);
+      std::vector<int32_t> u(m);
+      for (int64_t r = 0; r < n; r++)
+        for (int64_t e = off[r]; e < off[r + 1]; e++) u[e] = static_cast<int32_t>(r);
+      double dev_ms = 0;
+      Check(gbk_graph_build(ctx, n, m, u.data(), nbr.data(), nullptr, 1, cli.keep_self, &g, &dev_ms), "build");
+    } else {
+      Check(gbk_graph_upload(ctx, n, directed, off.data(), nbr.data(), need_weights ? w.data() : nullptr,
+                             directed ? in_off.data() : nullptr, directed ? in_nbr.data() : nullptr,
+                             need_weights && directed ? in_w.data() : nullptr, org.data(), &g),
+            "upload");
+    }
+    t.Stop();
+    PrintTime("Build Time", t.Seconds());
+  }
+
+  // The orderings of -o in sequence, then the total preprocessing time.
+  void ApplyOrders(Timer &total) {
     for (const auto &o : cli.orders) {
       std::vector<int32_t> ids = Mapping(o.first, o.second);
       Timer val;

@@ -178,6 +178,154 @@ int gbh_read_el(const char *path, int weighted, int64_t *m, int32_t *u, int32_t 
   return 0;
 }
 
+// ------------------------------------------------------ serialized graphs
+// Layout of .sg / .wsg files (gapbs/writer.h:336-377, gapbs/reader.h:445-578):
+// a one-byte directed flag, int64 edge count m, int64 vertex count n, the
+// out-CSR (int64 offsets[n+1], then m neighbours: int32 ids in .sg, (v, w)
+// int32 pairs in .wsg), for a directed graph the in-CSR the same way, and
+// last the n int32 original ids.
+namespace {
+
+struct SgHeader {
+  bool directed = false;
+  int64_t m = 0, n = 0;
+  bool weighted = false;
+};
+
+bool sg_suffix(const std::string &p, bool *weighted) {
+  if (ends_with(p, ".wsg")) {
+    *weighted = true;
+    return true;
+  }
+  if (ends_with(p, ".sg")) {
+    *weighted = false;
+    return true;
+  }
+  return false;
+}
+
+// Edge weights generated for a .sg file read as weighted (reader.h:48-59):
+// scheme 0 "unit" gives 1; scheme 1 "hash" mixes the original ids (ordered
+// for an undirected graph) with SplitMix64's finalizer.
+int32_t sg_generated_weight(int32_t src, int32_t dst, bool directed, int scheme) {
+  if (scheme == 0) return 1;
+  uint64_t a = static_cast<uint32_t>(src), b = static_cast<uint32_t>(dst);
+  if (!directed && b < a) std::swap(a, b);
+  uint64_t z = ((a << 32) ^ b ^ 0x27491095ULL) + 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  z ^= z >> 31;
+  return static_cast<int32_t>(z % 255ULL + 1ULL);
+}
+
+}  // namespace
+
+int gbh_sg_info(const char *path, int *directed, int64_t *n, int64_t *m, int *weighted) {
+  if (!path || !directed || !n || !m || !weighted) return -1;
+  SgHeader h;
+  if (!sg_suffix(path, &h.weighted)) return -1;
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return -2;
+  char d = 0;
+  f.read(&d, 1);
+  f.read(reinterpret_cast<char *>(&h.m), 8);
+  f.read(reinterpret_cast<char *>(&h.n), 8);
+  if (!f || h.m < 0 || h.n < 0 || h.n > INT32_MAX) return -3;
+  *directed = d != 0;
+  *n = h.n;
+  *m = h.m;
+  *weighted = h.weighted;
+  return 0;
+}
+
+int gbh_sg_read(const char *path, int weight_scheme, int64_t *off, int32_t *nbr, int32_t *wgt, int64_t *in_off,
+                int32_t *in_nbr, int32_t *in_wgt, int32_t *org_ids) {
+  int directed = 0, file_weighted = 0;
+  int64_t n = 0, m = 0;
+  if (int rc = gbh_sg_info(path, &directed, &n, &m, &file_weighted)) return rc;
+  if (!off || !nbr || !org_ids || (weight_scheme != 0 && weight_scheme != 1)) return -1;
+  if (directed && (!in_off || !in_nbr || (wgt && !in_wgt))) return -1;
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return -2;
+  const std::streamoff header = 1 + 8 + 8;
+  const std::streamoff entry = file_weighted ? 8 : 4;
+  const std::streamoff csr_bytes = (n + 1) * 8 + m * entry;
+  f.seekg(0, std::ios::end);
+  const std::streamoff size = f.tellg();
+  if (size != header + csr_bytes * (directed ? 2 : 1) + n * 4) return -3;  // truncated or padded
+  f.seekg(header + csr_bytes * (directed ? 2 : 1));
+  f.read(reinterpret_cast<char *>(org_ids), n * 4);
+  f.seekg(header);
+  std::vector<int32_t> pairs;
+  auto read_csr = [&](int64_t *o, int32_t *nb, int32_t *w, bool in_csr) -> int {
+    f.read(reinterpret_cast<char *>(o), (n + 1) * 8);
+    if (!f || o[0] != 0 || o[n] != m) return -3;
+    for (int64_t v = 0; v < n; v++)
+      if (o[v + 1] < o[v]) return -3;
+    if (file_weighted) {
+      pairs.resize(2 * static_cast<size_t>(m));
+      f.read(reinterpret_cast<char *>(pairs.data()), m * 8);
+      for (int64_t e = 0; e < m; e++) {
+        nb[e] = pairs[2 * e];
+        if (w) w[e] = pairs[2 * e + 1];
+      }
+    } else {
+      f.read(reinterpret_cast<char *>(nb), m * 4);
+      if (w) {
+        // out-CSR rows are sources, in-CSR rows destinations (reader.h:505-564)
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t v = 0; v < n; v++)
+          for (int64_t e = o[v]; e < o[v + 1]; e++) {
+            const int32_t a = org_ids[v], b = nb[e] >= 0 && nb[e] < n ? org_ids[nb[e]] : 0;
+            w[e] = in_csr ? sg_generated_weight(b, a, true, weight_scheme)
+                          : sg_generated_weight(a, b, directed != 0, weight_scheme);
+          }
+      }
+    }
+    if (!f) return -3;
+    for (int64_t e = 0; e < m; e++)
+      if (nb[e] < 0 || nb[e] >= n) return -3;
+    return 0;
+  };
+  if (int rc = read_csr(off, nbr, wgt, false)) return rc;
+  if (directed)
+    if (int rc = read_csr(in_off, in_nbr, wgt ? in_wgt : nullptr, true)) return rc;
+  return 0;
+}
+
+int gbh_sg_write(const char *path, int directed, int64_t n, int64_t m, const int64_t *off, const int32_t *nbr,
+                 const int32_t *wgt, const int64_t *in_off, const int32_t *in_nbr, const int32_t *in_wgt,
+                 const int32_t *org_ids) {
+  bool weighted = false;
+  if (!path || !sg_suffix(path, &weighted) || n < 0 || m < 0 || n > INT32_MAX || !off || !nbr || !org_ids)
+    return -1;
+  if (weighted != (wgt != nullptr) || (directed && (!in_off || !in_nbr || (weighted && !in_wgt)))) return -1;
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return -2;
+  const char d = directed ? 1 : 0;
+  f.write(&d, 1);
+  f.write(reinterpret_cast<const char *>(&m), 8);
+  f.write(reinterpret_cast<const char *>(&n), 8);
+  std::vector<int32_t> pairs;
+  auto write_csr = [&](const int64_t *o, const int32_t *nb, const int32_t *w) {
+    f.write(reinterpret_cast<const char *>(o), (n + 1) * 8);
+    if (!w) {
+      f.write(reinterpret_cast<const char *>(nb), m * 4);
+      return;
+    }
+    pairs.resize(2 * static_cast<size_t>(m));
+    for (int64_t e = 0; e < m; e++) {
+      pairs[2 * e] = nb[e];
+      pairs[2 * e + 1] = w[e];
+    }
+    f.write(reinterpret_cast<const char *>(pairs.data()), m * 8);
+  };
+  write_csr(off, nbr, wgt);
+  if (directed) write_csr(in_off, in_nbr, in_wgt);
+  f.write(reinterpret_cast<const char *>(org_ids), n * 4);
+  return f ? 0 : -3;
+}
+
 int gbh_mapping_identity(int64_t n, int32_t *new_ids) {
   if (n < 0 || (n > 0 && !new_ids)) return -1;
 #pragma omp parallel for

@@ -12,6 +12,7 @@
  *   gbh_mapping_fingerprint               BuilderBase::MappingFingerprint        gapbs/builder.h:1398-1411
  *   gbh_validate_permutation              ValidatePermutationOrThrow             gapbs/builder.h:2042-2081
  *   gbh_mapping_read / gbh_mapping_write  MAP (-o 13) files, WriteLabels         gapbs/builder.h:2125-2235, gapbs/writer.h:380-412
+ *   gbh_sg_info / gbh_sg_read / gbh_sg_write  ReadSerializedGraph, WriteSerializedGraph gapbs/reader.h:445-578, gapbs/writer.h:336-377
  *   gbh_source_picker                     SourcePicker::GenerateConsistentSources gapbs/benchmark.h:115-156
  *   gbh_tier0_features / gbh_frozen_rule  ComputeTier0SampledGraphFeatures,
  *                                         ShouldUseAllKernelLowReuseRule         reorder/reorder_types.h:6066-6195, reorder/reorder_adaptive.h:302-333
@@ -32,6 +33,23 @@ int gbh_insert_weights(int64_t m, int32_t *w);
 /* Reads a .el (weighted=0) or .wel (weighted=1) file.  First call with
  * u == NULL returns the edge count in *m; second call fills the arrays. */
 int gbh_read_el(const char *path, int weighted, int64_t *m, int32_t *u, int32_t *v, int32_t *w);
+
+/* Serialized graphs: ".sg" (int32 neighbours) and ".wsg" ((v, w) int32
+ * pairs).  A one-byte directed flag, int64 m, int64 n, the out-CSR (int64
+ * offsets[n+1], m neighbours), for a directed graph the in-CSR, then the n
+ * int32 original ids.  gbh_sg_info reads the header (weighted = the suffix
+ * is .wsg).  gbh_sg_read fills caller arrays sized from it (in_* only for a
+ * directed graph): with wgt == NULL a .wsg file's weights are dropped; with
+ * wgt given for a .sg file they are generated from the original ids,
+ * weight_scheme 0 "unit" (1) or 1 "hash" (reader.h:48-59).  Returns -3 for
+ * a truncated or inconsistent file.  gbh_sg_write writes .wsg when wgt is
+ * given, else .sg (the path's suffix must agree). */
+int gbh_sg_info(const char *path, int *directed, int64_t *n, int64_t *m, int *weighted);
+int gbh_sg_read(const char *path, int weight_scheme, int64_t *off, int32_t *nbr, int32_t *wgt, int64_t *in_off,
+                int32_t *in_nbr, int32_t *in_wgt, int32_t *org_ids);
+int gbh_sg_write(const char *path, int directed, int64_t n, int64_t m, const int64_t *off, const int32_t *nbr,
+                 const int32_t *wgt, const int64_t *in_off, const int32_t *in_nbr, const int32_t *in_wgt,
+                 const int32_t *org_ids);
 
 int gbh_mapping_identity(int64_t n, int32_t *new_ids);
 int gbh_mapping_random(int64_t n, int32_t *new_ids);

@@ -134,3 +134,40 @@ def test_file_input_and_components():
 def test_unavailable_ordering_fails_cleanly():
     r = run("bfs", "-g", 10, "-n", 1, "-o", "12:leiden")
     assert r.returncode != 0 and "-o 13:<file>" in (r.stderr + r.stdout)
+
+
+# ------------------------------------------------- serialized graph input
+@pytest.mark.gpu
+def test_drivers_read_serialized_graphs(tmp_path):
+    """-f .sg / .wsg (ReadSerializedGraph, gapbs/reader.h:445-578): the CSR
+    is taken as stored.  A .wsg of the generated scale-10 graph gives the
+    same SSSP distance fingerprints as -g 10; BFS and CC verify on the .sg."""
+    g = O.kron(10, weighted=True)
+    sg, wsg = str(tmp_path / "k10.sg"), str(tmp_path / "k10.wsg")
+    H.sg_write(sg, g.off, g.nbr)
+    H.sg_write(wsg, g.off, g.nbr, wgt=g.wgt)
+    for name in ("bfs", "cc", "pr"):
+        r = run(name, "-f", sg, "-n", 2, "-v")
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert labels(r.stdout)["Verification"] == ["PASS", "PASS"]
+    a = run("sssp", "-f", wsg, "-n", 3, "-v", "-d", 2)
+    b = run("sssp", "-g", 10, "-n", 3, "-v", "-d", 2)
+    assert a.returncode == 0 and b.returncode == 0, a.stdout + b.stdout
+    La, Lb = labels(a.stdout), labels(b.stdout)
+    assert La["Verification"] == ["PASS"] * 3
+    assert La["Distance Fingerprint"] == Lb["Distance Fingerprint"]
+
+
+@pytest.mark.gpu
+def test_driver_symmetrizes_directed_sg(tmp_path):
+    """-s on a directed unweighted .sg with identity original ids rebuilds
+    it symmetrized (gapbs/builder.h:741-790); CC then verifies."""
+    rng = np.random.default_rng(8)
+    u = rng.integers(0, 2000, 12000).astype(np.int32)
+    v = rng.integers(0, 2000, 12000).astype(np.int32)
+    d = O.make_graph(u, v, symmetrize=False, n=2000)
+    p = str(tmp_path / "d.sg")
+    H.sg_write(p, d.off, d.nbr, in_off=d.in_off, in_nbr=d.in_nbr)
+    r = run("cc", "-f", p, "-s", "-n", 1, "-v")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert labels(r.stdout)["Verification"] == ["PASS"]

@@ -237,3 +237,95 @@ def test_selector_generators_are_seeded():
         (u2, v2), n2 = S.make_edges(spec)
         assert n1 == n2 and np.array_equal(u1, u2) and np.array_equal(v1, v2)
         assert u1.min() >= 0 and max(u1.max(), v1.max()) < n1 and not np.any(u1 == v1)
+
+
+# ------------------------------------------------------- serialized graphs
+# Layout (gapbs/writer.h:336-377): bool directed, int64 m, int64 n, out
+# offsets[n+1] (int64), m neighbours (int32, or (v, w) int32 pairs in .wsg),
+# for a directed graph the in-CSR, then int32 org_ids[n].
+import struct
+
+from tests import bruteforce as B
+
+
+def _sg_bytes(directed, off, nbr, wgt=None, in_off=None, in_nbr=None, in_wgt=None, org=None):
+    n, m = len(off) - 1, len(nbr)
+
+    def csr(o, nb, w):
+        nb = np.asarray(nb, np.int32)
+        body = nb.tobytes() if w is None else np.stack([nb, np.asarray(w, np.int32)], 1).tobytes()
+        return np.asarray(o, np.int64).tobytes() + body
+
+    out = struct.pack("<?qq", directed, m, n) + csr(off, nbr, wgt)
+    if directed:
+        out += csr(in_off, in_nbr, in_wgt)
+    return out + np.asarray(org if org is not None else np.arange(n), np.int32).tobytes()
+
+
+def test_sg_write_matches_layout_and_reads_back(tmp_path):
+    g = O.kron(8, weighted=True)
+    org = np.random.default_rng(3).permutation(g.n).astype(np.int32)
+    p = str(tmp_path / "g.sg")
+    H.sg_write(p, g.off, g.nbr, org_ids=org)
+    assert open(p, "rb").read() == _sg_bytes(False, g.off, g.nbr, org=org)
+    r = H.sg_read(p)
+    assert not r["directed"] and np.array_equal(r["off"], g.off) and np.array_equal(r["nbr"], g.nbr)
+    assert np.array_equal(r["org_ids"], org)
+    pw = str(tmp_path / "g.wsg")
+    H.sg_write(pw, g.off, g.nbr, wgt=g.wgt, org_ids=org)
+    assert open(pw, "rb").read() == _sg_bytes(False, g.off, g.nbr, g.wgt, org=org)
+    rw = H.sg_read(pw, weights=True)
+    assert np.array_equal(rw["wgt"], g.wgt) and np.array_equal(rw["nbr"], g.nbr)
+    # a .wsg read unweighted drops the weights (reader.h:524-534)
+    assert np.array_equal(H.sg_read(pw)["nbr"], g.nbr)
+    assert H.sg_info(pw) == {"directed": False, "n": g.n, "m": len(g.nbr), "weighted": True}
+
+
+def test_sg_directed_round_trip(tmp_path):
+    rng = np.random.default_rng(4)
+    u = rng.integers(0, 300, 4000).astype(np.int32)
+    v = rng.integers(0, 300, 4000).astype(np.int32)
+    w = rng.integers(1, 256, 4000).astype(np.int32)
+    g = O.make_graph(u, v, w, symmetrize=False, n=300)
+    p = str(tmp_path / "d.wsg")
+    H.sg_write(p, g.off, g.nbr, g.wgt, g.in_off, g.in_nbr, g.in_wgt)
+    assert open(p, "rb").read() == _sg_bytes(True, g.off, g.nbr, g.wgt, g.in_off, g.in_nbr, g.in_wgt)
+    r = H.sg_read(p, weights=True)
+    for k in ("off", "nbr", "wgt", "in_off", "in_nbr", "in_wgt"):
+        assert np.array_equal(r[k], getattr(g, k)), k
+
+
+def test_sg_generated_weights(tmp_path):
+    """A .sg read into a weighted graph: 'unit' gives 1; 'hash' gives
+    Mix64((src << 32) ^ dst ^ 0x27491095) % 255 + 1 over original ids, the
+    pair ordered for an undirected graph (reader.h:48-59).  Mix64 is
+    SplitMix64's output for that state, pinned by its published vector."""
+    g = O.kron(7)
+    org = np.random.default_rng(5).permutation(g.n).astype(np.int32)
+    p = str(tmp_path / "h.sg")
+    H.sg_write(p, g.off, g.nbr, org_ids=org)
+    assert np.all(H.sg_read(p, weights=True, scheme="unit")["wgt"] == 1)
+    got = H.sg_read(p, weights=True, scheme="hash")["wgt"]
+    rows = np.repeat(np.arange(g.n), np.diff(g.off))
+    for e in range(0, len(g.nbr), 7):
+        a, b = int(org[rows[e]]), int(org[g.nbr[e]])
+        a, b = min(a, b), max(a, b)
+        key = ((a << 32) ^ b ^ 0x27491095) & (2**64 - 1)
+        assert got[e] == B.splitmix64_stream(key, 1)[0] % 255 + 1
+    # undirected: both directions of an edge carry the same weight
+    d = {(int(rows[e]), int(g.nbr[e])): int(got[e]) for e in range(len(g.nbr))}
+    assert all(d[(y, x)] == wt for (x, y), wt in d.items())
+
+
+def test_sg_rejects_bad_files(tmp_path):
+    g = O.kron(6)
+    p = str(tmp_path / "t.sg")
+    H.sg_write(p, g.off, g.nbr)
+    data = open(p, "rb").read()
+    open(p, "wb").write(data[:-3])                 # truncated org_ids
+    with pytest.raises(Exception):
+        H.sg_read(p)
+    with pytest.raises(Exception):                 # weights into a .sg name
+        H.sg_write(str(tmp_path / "x.sg"), g.off, g.nbr, wgt=np.ones(len(g.nbr), np.int32))
+    with pytest.raises(Exception):
+        H.sg_info(str(tmp_path / "x.el"))
