@@ -509,18 +509,117 @@ int sort_rows(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_t *nbr, c
 
 using namespace gbk;
 
-extern "C" int gbk_relabel(gbk_ctx *ctx, const gbk_graph *in, const int32_t *new_ids,
-                           gbk_graph **out, gbk_relabel_stats *stats) {
-  if (!ctx || !in || !new_ids || !out) return set_error(ctx, GBK_EINVAL, "relabel: null argument");
-  *out = nullptr;
+// ------------------------------------------------------ mapping producers
+// The orderings that are a stable sort of the vertices by a key of their
+// out-degree, made on the device: every vertex gets the 64-bit key
+// (class << 32 | id) -- ties broken by id -- and the keys are sorted (one
+// wave per 2048 keys in registers, then merge-path passes); the i-th key's
+// vertex gets new id i.
+//   -o 0 original: the identity (reorder_basic.h:65-79)
+//   -o 2 sort: degree descending, then id (reorder_basic.h:182-258)
+//   -o 5 DBG: 8 groups at m/n * {1/2, 1, 2, 4, 8, 16, 32, inf}, the highest
+//        first, id order inside (reorder_hub.h:267-345)
+//   -o 6 HubSortDBG: hubs (degree > integer m/n) by degree then id, both
+//        descending, then the rest in id order (reorder_hub.h:380-502)
+//   -o 7 HubClusterDBG: hubs then the rest, each in id order
+//        (reorder_hub.h:505-582)
+// Random (-o 1) is a sequential Fisher-Yates and the hub placements (-o 3, 4)
+// move displaced vertices one after another: they stay on the host.
+__global__ void mapping_keys(const int64_t *off, int64_t n, int algo, uint64_t avg, uint64_t *keys) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint64_t d = static_cast<uint64_t>(off[v + 1] - off[v]);
+  const uint64_t dc = d < 0xffffffffull ? d : 0xffffffffull;
+  const uint64_t id = static_cast<uint64_t>(v);
+  uint64_t key = id;
+  if (algo == 2) {
+    key = ((0xffffffffull - dc) << 32) | id;
+  } else if (algo == 5) {
+    uint64_t lim = avg / 2;
+    int b = 0;
+    while (b < 7 && d > lim) {
+      lim = b == 0 ? avg : lim * 2;
+      b++;
+    }
+    key = (static_cast<uint64_t>(7 - b) << 32) | id;
+  } else if (algo == 6) {
+    key = d > avg ? ((0xffffffffull - dc) << 31) | (0x7fffffffull - id) : (1ull << 63) | id;
+  } else if (algo == 7) {
+    key = d > avg ? id : (1ull << 63) | id;
+  }
+  keys[v] = key;
+}
+
+__global__ void __launch_bounds__(kBlock)
+mapping_sort_chunks(const uint64_t *keys, int64_t n, uint64_t *out) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  const int64_t first = c * kSortCap;
+  if (first >= n) return;  // wave-uniform
+  const int64_t len = min(static_cast<int64_t>(kSortCap), n - first);
+  wave_sort<true, kSortCap / kWave>(
+      len, [&](int64_t i) { return keys[first + i]; }, [&](int64_t i, uint64_t k) { out[first + i] = k; });
+}
+
+__global__ void mapping_scatter(const uint64_t *sorted, int64_t n, int algo, int32_t *new_ids) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = sorted[i];
+  const int64_t v = algo == 6 && !(k >> 63) ? static_cast<int64_t>(0x7fffffffull - (k & 0x7fffffffull))
+                                            : static_cast<int64_t>(k & 0xffffffffull);
+  new_ids[v] = static_cast<int32_t>(i);
+}
+
+__global__ void mapping_identity(int64_t n, int32_t *new_ids) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < n) new_ids[v] = static_cast<int32_t>(v);
+}
+
+// Mapping 'algo' of g into ids_d (device, n entries).
+static int device_mapping(gbk_ctx *ctx, const gbk_graph *g, int algo, int32_t *ids_d) {
+  const int64_t n = g->n;
+  hipStream_t s = ctx->stream;
+  if (n == 0) return GBK_OK;
+  if (algo == 0) {
+    mapping_identity<<<grid_for(n), kBlock, 0, s>>>(n, ids_d);
+    GBK_LAUNCH_CHECK(ctx, "mapping_identity");
+    return GBK_OK;
+  }
+  auto *k0 = static_cast<uint64_t *>(scratch(ctx, kS10, sizeof(uint64_t) * n));
+  auto *k1 = static_cast<uint64_t *>(scratch(ctx, kS11, sizeof(uint64_t) * n));
+  auto *so = static_cast<int64_t *>(scratch(ctx, kS9, 2 * sizeof(int64_t)));
+  if (!k0 || !k1 || !so) return set_error(ctx, GBK_ENOMEM, "mapping scratch");
+  const int64_t so_h[2] = {0, n};
+  GBK_HIP(ctx, hipMemcpyAsync(so, so_h, sizeof(so_h), hipMemcpyHostToDevice, s));
+  mapping_keys<<<grid_for(n), kBlock, 0, s>>>(g->off, n, algo, static_cast<uint64_t>(g->m / n), k1);
+  const int64_t chunks = (n + kSortCap - 1) / kSortCap;
+  mapping_sort_chunks<<<static_cast<unsigned>((chunks + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, s>>>(
+      k1, n, k0);
+  uint64_t *src = k0, *dst = k1;
+  for (int64_t width = kSortCap; width < n; width *= 2) {
+    const int64_t threads = (n + kMergeItems - 1) / kMergeItems;
+    relabel_merge_pass<uint64_t><<<grid_for(threads), kBlock, 0, s>>>(src, dst, so, 1, n, width);
+    std::swap(src, dst);
+  }
+  mapping_scatter<<<grid_for(n), kBlock, 0, s>>>(src, n, algo, ids_d);
+  GBK_LAUNCH_CHECK(ctx, "mapping");
+  // the host copies of so_h must outlive the copy above
+  GBK_HIP(ctx, hipStreamSynchronize(s));
+  return GBK_OK;
+}
+
+static bool device_mapping_supported(int algo) {
+  return algo == 0 || algo == 2 || algo == 5 || algo == 6 || algo == 7;
+}
+
+// Relabels 'in' by the mapping already in ids_d (device).
+static int relabel_device_ids(gbk_ctx *ctx, const gbk_graph *in, int32_t *ids_d, gbk_graph **out,
+                              gbk_relabel_stats *stats) {
   const int64_t n = in->n;
   hipStream_t s = ctx->stream;
-  auto *ids_d = static_cast<int32_t *>(scratch(ctx, kS0, sizeof(int32_t) * (n + 1)));
   auto *inv = static_cast<int32_t *>(scratch(ctx, kS1, sizeof(int32_t) * (n + 1)));
   auto *deg_new = static_cast<int64_t *>(scratch(ctx, kS2, sizeof(int64_t) * (n + 1)));
   auto *err = static_cast<int32_t *>(scratch(ctx, kS3, 64));
-  if (!ids_d || !inv || !deg_new || !err) return set_error(ctx, GBK_ENOMEM, "relabel scratch");
-  GBK_HIP(ctx, hipMemcpyAsync(ids_d, new_ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  if (!inv || !deg_new || !err) return set_error(ctx, GBK_ENOMEM, "relabel scratch");
   int64_t m_in = in->m;
   if (in->directed)
     GBK_HIP(ctx, hipMemcpyAsync(&m_in, in->in_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -569,4 +668,44 @@ extern "C" int gbk_relabel(gbk_ctx *ctx, const gbk_graph *in, const int32_t *new
   }
   *out = g;
   return GBK_OK;
+}
+
+extern "C" int gbk_relabel(gbk_ctx *ctx, const gbk_graph *in, const int32_t *new_ids,
+                           gbk_graph **out, gbk_relabel_stats *stats) {
+  if (!ctx || !in || !new_ids || !out) return set_error(ctx, GBK_EINVAL, "relabel: null argument");
+  *out = nullptr;
+  auto *ids_d = static_cast<int32_t *>(scratch(ctx, kS0, sizeof(int32_t) * (in->n + 1)));
+  if (!ids_d) return set_error(ctx, GBK_ENOMEM, "relabel scratch");
+  GBK_HIP(ctx, hipMemcpyAsync(ids_d, new_ids, sizeof(int32_t) * in->n, hipMemcpyHostToDevice, ctx->stream));
+  return relabel_device_ids(ctx, in, ids_d, out, stats);
+}
+
+extern "C" int gbk_mapping(gbk_ctx *ctx, const gbk_graph *g, int algo, int32_t *new_ids, double *device_ms) {
+  if (!ctx || !g || !new_ids) return set_error(ctx, GBK_EINVAL, "mapping: null argument");
+  if (!device_mapping_supported(algo)) return set_error(ctx, GBK_EINVAL, "mapping: ordering not made on the device");
+  auto *ids_d = static_cast<int32_t *>(scratch(ctx, kS0, sizeof(int32_t) * (g->n + 1)));
+  if (!ids_d) return set_error(ctx, GBK_ENOMEM, "mapping scratch");
+  EventTimer timer(ctx);
+  if (int rc = device_mapping(ctx, g, algo, ids_d)) return rc;
+  const double ms = timer.stop_ms();
+  if (device_ms) *device_ms = ms;
+  GBK_HIP(ctx, hipMemcpyAsync(new_ids, ids_d, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, ctx->stream));
+  GBK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GBK_OK;
+}
+
+extern "C" int gbk_relabel_by(gbk_ctx *ctx, const gbk_graph *in, int algo, gbk_graph **out,
+                              gbk_relabel_stats *stats, int32_t *new_ids, double *mapping_ms) {
+  if (!ctx || !in || !out) return set_error(ctx, GBK_EINVAL, "relabel: null argument");
+  *out = nullptr;
+  if (!device_mapping_supported(algo)) return set_error(ctx, GBK_EINVAL, "relabel: ordering not made on the device");
+  auto *ids_d = static_cast<int32_t *>(scratch(ctx, kS0, sizeof(int32_t) * (in->n + 1)));
+  if (!ids_d) return set_error(ctx, GBK_ENOMEM, "relabel scratch");
+  EventTimer timer(ctx);
+  if (int rc = device_mapping(ctx, in, algo, ids_d)) return rc;
+  const double ms = timer.stop_ms();
+  if (mapping_ms) *mapping_ms = ms;
+  if (new_ids)
+    GBK_HIP(ctx, hipMemcpyAsync(new_ids, ids_d, sizeof(int32_t) * in->n, hipMemcpyDeviceToHost, ctx->stream));
+  return relabel_device_ids(ctx, in, ids_d, out, stats);
 }

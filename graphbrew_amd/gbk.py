@@ -97,6 +97,8 @@ def lib():
         "gbk_graph_download": (i32, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
         "gbk_graph_free": (i32, [VP, VP]),
         "gbk_relabel": (i32, [VP, VP, VP, pp, C.POINTER(RelabelStats)]),
+        "gbk_mapping": (i32, [VP, VP, i32, VP, C.POINTER(f64)]),
+        "gbk_relabel_by": (i32, [VP, VP, i32, pp, C.POINTER(RelabelStats), VP, C.POINTER(f64)]),
         "gbk_pr": (i32, [VP, VP, i32, i32, f64, i32, VP, C.POINTER(PrStats)]),
         "gbk_bfs": (i32, [VP, VP, C.c_int32, i32, i32, VP, C.POINTER(BfsStats)]),
         "gbk_cc": (i32, [VP, VP, i32, VP, C.POINTER(CcStats)]),
@@ -281,6 +283,24 @@ class Context:
         st = RelabelStats()
         self._check(lib().gbk_relabel(self.h, g.handle, _ptr(ids), C.byref(h), C.byref(st)))
         return self._wrap(h), st
+
+    def mapping(self, g: DeviceGraph, algo: int):
+        """Ordering -o algo (0, 2, 5, 6 or 7) made on the device: (new_ids, ms)."""
+        ids = np.zeros(g.n, np.int32)
+        ms = C.c_double(0)
+        self._check(lib().gbk_mapping(self.h, g.handle, int(algo), _ptr(ids), C.byref(ms)))
+        return ids, ms.value
+
+    def relabel_by(self, g: DeviceGraph, algo: int):
+        """Ordering -o algo made and applied on the device: (graph, stats,
+        new_ids, mapping ms)."""
+        ids = np.zeros(g.n, np.int32)
+        h = VP()
+        st = RelabelStats()
+        ms = C.c_double(0)
+        self._check(lib().gbk_relabel_by(self.h, g.handle, int(algo), C.byref(h), C.byref(st), _ptr(ids),
+                                         C.byref(ms)))
+        return self._wrap(h), st, ids, ms.value
 
     def pr(self, g: DeviceGraph, variant=PR_GS, max_iters=20, tol=1e-4, fixed=False, download=True):
         out = np.empty(g.n, np.float32) if download else None

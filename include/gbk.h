@@ -161,6 +161,20 @@ int gbk_graph_free(gbk_ctx *ctx, gbk_graph *g);
  * GBK_EPERM_MAP and creates nothing. */
 int gbk_relabel(gbk_ctx *ctx, const gbk_graph *in, const int32_t *new_ids,
                 gbk_graph **out, gbk_relabel_stats *stats);
+/* Mapping producers on the device (the cheap orderings whose result is a
+ * stable sort by an out-degree key): algo 0 original (reorder_basic.h:65-79),
+ * 2 sort (reorder_basic.h:182-258), 5 DBG (reorder_hub.h:267-345), 6
+ * HubSortDBG (reorder_hub.h:380-502), 7 HubClusterDBG (reorder_hub.h:505-582);
+ * other orderings return GBK_EINVAL (random and the hub placements are
+ * sequential and stay on the host).  new_ids (host, n) receives the mapping,
+ * equal to gbh_mapping_*; device_ms (may be NULL) its device time. */
+int gbk_mapping(gbk_ctx *ctx, const gbk_graph *g, int algo, int32_t *new_ids, double *device_ms);
+/* gbk_mapping then gbk_relabel without the host round trip of the mapping
+ * (MakeGraph's mapping + RelabelByMapping, gapbs/builder.h:869).  new_ids
+ * (host, may be NULL) receives a copy of the mapping; mapping_ms (may be
+ * NULL) the mapping's device time; stats the relabel's. */
+int gbk_relabel_by(gbk_ctx *ctx, const gbk_graph *in, int algo, gbk_graph **out,
+                   gbk_relabel_stats *stats, int32_t *new_ids, double *mapping_ms);
 
 /* ---- kernels ----------------------------------------------------------- */
 /* scores: host float[n] (may be NULL to leave the result on the device). */

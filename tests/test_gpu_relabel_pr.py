@@ -5,6 +5,7 @@ import pytest
 
 import oracle as O
 from graphbrew_amd import gbk
+from graphbrew_amd import host as H
 
 pytestmark = pytest.mark.gpu
 
@@ -318,3 +319,46 @@ def test_pr_heavy_rows_finished_in_launch(ctx):
     # fixed point instead
     s2, _ = ctx.pr(dg, gbk.PR_GS, max_iters=200, tol=1e-9)
     np.testing.assert_allclose(s2, O.pr_fp64(g), rtol=1e-4, atol=1e-10)
+
+
+# ------------------------------------------------- device mapping producers
+# -o 0, 2, 5, 6, 7 made on the GPU (a stable sort by an out-degree key) are
+# the host producers' mappings exactly (those are pinned to the oracle in
+# tests/test_host.py); n spans one to many 2048-key chunks and merge passes.
+def _odd_graph():
+    rng = np.random.default_rng(9)
+    n = 70001                                   # not a power of two: ragged last chunk
+    u = rng.integers(0, n, 300000).astype(np.int32)
+    v = (rng.zipf(1.6, 300000) % n).astype(np.int32)
+    return O.make_graph(u, v, n=n)
+
+
+MAP_GRAPHS = {
+    "kron10": lambda: O.kron(10),
+    "kron16": lambda: O.kron(16),
+    "uniform14": lambda: O.kron(14, uniform=True),   # many degree ties
+    "star5000": lambda: _star(5000),
+    "odd70001": _odd_graph,
+    "directed": lambda: _heavy_in_rows(),
+}
+
+
+@pytest.mark.parametrize("name", list(MAP_GRAPHS))
+def test_device_mappings_equal_host(ctx, name):
+    g = MAP_GRAPHS[name]()
+    dg = ctx.upload_graph(g)
+    for algo in (0, 2, 5, 6, 7):
+        ids, _ = ctx.mapping(dg, algo)
+        assert np.array_equal(ids, H.mapping(algo, g.off)), (name, algo)
+
+
+def test_relabel_by_equals_relabel(ctx):
+    g = O.kron(13, weighted=True)
+    dg = ctx.upload_graph(g)
+    for algo in (2, 5, 6):
+        r1, _, ids, _ = ctx.relabel_by(dg, algo)
+        assert np.array_equal(ids, H.mapping(algo, g.off))
+        _assert_same_graph(r1, O.relabel(g, ids))
+        r1.free()
+    with pytest.raises(gbk.GbkError):
+        ctx.mapping(dg, 1)                      # random stays on the host
