@@ -198,7 +198,7 @@ class Workload:
 def kt_table(gbk, single):
     t = {
         "sssp": (gbk.KT_SSSP_RELAX, "sssp_relax+heavy_prefix+sssp_relax_heavy (one relaxation round)"),
-        "pr": (gbk.KT_PR_ITER, "pr_pull_iter<GS> (one pull iteration)"),
+        "pr": (gbk.KT_PR_ITER, "pr_pull_block + pr_block_close per row block (one Gauss-Seidel sweep)"),
         "bfs": (gbk.KT_BFS_STEP, "bfs_td_step+heavy_prefix+bfs_td_heavy | bfs_bu_step (one BFS step)"),
         "bc": (gbk.KT_BC_LEVEL, "bc_forward_level+heavy_prefix+bc_forward_heavy | bc_backward_level"),
     }
