@@ -39,6 +39,7 @@ struct Tuning {
   int graphs = 1;               // iteration loops replayed as captured hipGraphs
   int batch_grid = 0;           // workgroups of a source batch's launches (0: as many as stay resident)
   int pr_blocks = 0;            // row blocks of a Gauss-Seidel PageRank sweep (0: by size)
+  int pr_inplace = 0;           // Gauss-Seidel blocks update contributions in place (1) or publish them
   int pr_unroll = 0;            // 64-edge chunks a PageRank wave has in flight (1, 2, 4; 0: 4 for
                                 // Gauss-Seidel blocks, 1 for Jacobi's one launch)
 };

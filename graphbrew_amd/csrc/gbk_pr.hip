@@ -7,7 +7,7 @@
 // sweep keep only the contraction bound (the L1 change shrinks ~0.82x a
 // sweep at Kronecker scale 20, so 20 sweeps do not reach 1e-4 where the
 // reference needs 7).  Here a sweep is a block Gauss-Seidel in a fixed
-// order instead: rows are cut into tile-aligned blocks (pr_blocks, or by
+// order instead: the rows' tiles are cut into equal runs (pr_blocks, or by
 // default one per ~4096 tiles, at most 8), one launch each,
 // and a row of block b reads the new contributions of the
 // rows before block b and the previous sweep's for the rest -- no races,
@@ -85,18 +85,16 @@ __device__ __forceinline__ int64_t lower_index(const int32_t *a, int64_t len, in
   return lo;
 }
 
-// Block b starts at the first tile at or after row b*n/B; its heavy rows are
-// the heavy-list entries in its row range (the list is ascending).  Also
-// clears the heavy rows' accumulators.
-__global__ void pr_blocks_init(RowTiles rt, int64_t n, int B, PrBlock *blk, float *hacc) {
+// Block b holds tiles [b*T/B, (b+1)*T/B) of the T tiles (edge-balanced, so
+// the blocks are too, whatever the order of the rows -- hubs packed at the
+// front by a degree ordering included); its heavy rows are the heavy-list
+// entries in its row range (the list is ascending).  Also clears the heavy
+// rows' accumulators.
+__global__ void pr_blocks_init(RowTiles rt, int B, PrBlock *blk, float *hacc) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t < rt.nheavy) hacc[t] = 0.0f;
   if (t < B) {
-    auto tile_at = [&](int64_t k) -> int64_t {
-      if (k == 0) return 0;
-      if (k == B) return rt.ntiles;
-      return lower_index(rt.tiles, rt.ntiles, n * k / B);
-    };
+    auto tile_at = [&](int64_t k) -> int64_t { return rt.ntiles * k / B; };
     PrBlock pb;
     pb.tile_lo = tile_at(t);
     pb.tile_hi = tile_at(t + 1);
@@ -283,12 +281,12 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
          : rows_grid(rt);
   // the close launch copies about n/B contributions (Gauss-Seidel) or
   // finishes the heavy rows
-  const bool publish = gs && B > 1;
+  const bool publish = gs && B > 1 && !ctx->tune.pr_inplace;
   const unsigned cgrid = publish ? static_cast<unsigned>(std::min<int64_t>(grid_for(n / B + 1), 1024)) : 16;
   const bool close = publish || rt.nheavy;
   // block launches are short and latency-bound: more loads in flight pay;
   // Jacobi's one launch is bound by the random-line rate and loses 7%
-  const int unroll = ctx->tune.pr_unroll ? ctx->tune.pr_unroll : (publish ? 4 : 1);
+  const int unroll = ctx->tune.pr_unroll ? ctx->tune.pr_unroll : (gs && B > 1 ? 4 : 1);
   const size_t nparts = static_cast<size_t>(B) * (grid + (close ? cgrid : 0));
   auto *scores = static_cast<float *>(scratch(ctx, kS0, sizeof(float) * n));
   auto *c0 = static_cast<float *>(scratch(ctx, kS1, sizeof(float) * n));
@@ -311,7 +309,7 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
       // Jacobi: ping-pong buffers; Gauss-Seidel in blocks: c0 read and
       // published into, c1 the side buffer; in one launch: c0 in place
       float *cur = gs ? c0 : ((it & 1) ? c1 : c0);
-      float *nxt = gs ? (B > 1 ? c1 : c0) : ((it & 1) ? c0 : c1);
+      float *nxt = gs ? (publish ? c1 : c0) : ((it & 1) ? c0 : c1);
       for (int b = 0; b < B; b++) {
         switch (unroll) {
           case 1:
@@ -338,7 +336,7 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   LoopGraph loop(ctx);
   if (int rc = loop.capture(iterations)) return rc;
   EventTimer timer(ctx);
-  pr_blocks_init<<<grid_for(std::max<int64_t>(rt.nheavy, B)), kBlock, 0, s>>>(rt, n, B, blocks, hacc);
+  pr_blocks_init<<<grid_for(std::max<int64_t>(rt.nheavy, B)), kBlock, 0, s>>>(rt, B, blocks, hacc);
   pr_init<<<grid_for(n), kBlock, 0, s>>>(g->off, n, init, scores, c0, st);
   if (int rc = loop.run(iterations)) return rc;
   GBK_LAUNCH_CHECK(ctx, "pr_pull_block");

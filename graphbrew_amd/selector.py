@@ -369,7 +369,7 @@ def main(argv=None):
     import argparse
     import sys
     p = argparse.ArgumentParser(description="collect timings / train / evaluate the ordering selector")
-    p.add_argument("cmd", choices=["collect", "train", "evaluate"])
+    p.add_argument("cmd", choices=["collect", "remeasure", "train", "evaluate"])
     p.add_argument("--graphs", type=int, default=96)
     p.add_argument("--min-scale", type=int, default=12)
     p.add_argument("--max-scale", type=int, default=15)
@@ -378,11 +378,22 @@ def main(argv=None):
                    help="output of collect; for train / evaluate a comma-separated list to merge")
     p.add_argument("--model", default=MODEL_PATH)
     p.add_argument("--epochs", type=int, default=40)
+    p.add_argument("--start", type=int, default=0, help="remeasure: first graph of the corpus")
+    p.add_argument("--count", type=int, default=-1, help="remeasure: graphs (-1: to the end)")
+    p.add_argument("--out", default="", help="remeasure: output file")
     a = p.parse_args(argv)
     ctx = gbk.Context(0)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
     if a.cmd == "collect":
         collect(ctx, corpus(a.graphs, seed0=a.seed0, min_scale=a.min_scale, max_scale=a.max_scale), a.corpus, log)
+        return 0
+    if a.cmd == "remeasure":
+        # the same graphs (their specs are stored with the timings) timed again
+        # with the current kernels
+        with open(a.corpus) as f:
+            specs = [row["spec"] for row in json.load(f)]
+        end = len(specs) if a.count < 0 else min(len(specs), a.start + a.count)
+        collect(ctx, specs[a.start:end], a.out or a.corpus, log)
         return 0
     data = []
     for path in a.corpus.split(","):

@@ -183,8 +183,7 @@ def test_pr_heavy_rows_and_directed(ctx):
 # place, as the reference does; held to the verifier and the fixed point
 # above).  Tiles start every 64 rows and
 # wherever the prefix of light in-degrees (rows of at most 2048 in-edges)
-# crosses a multiple of 2048; block b starts at the first tile start at or
-# after row b*n/B.
+# crosses a multiple of 2048; of T tiles, block b starts at tile b*T//B.
 def _heavy_in_rows():
     """Directed graph whose in-rows sit at and around the heavy threshold and
     its chunk size, some far above it."""
@@ -210,12 +209,9 @@ def _block_starts(g, blocks, heavy=2048):
     v = np.arange(g.n)
     flag = (v % 64 == 0)
     flag[1:] |= (lp[1:] // 2048) != (lp[:-1] // 2048)
-    tiles = np.flatnonzero(flag)
-    starts = [0]
-    for b in range(1, blocks):
-        k = np.searchsorted(tiles, g.n * b // blocks)
-        starts.append(int(tiles[k]) if k < len(tiles) else g.n)
-    return np.array(starts + [g.n])
+    tiles = np.append(np.flatnonzero(flag), g.n)
+    T = len(tiles) - 1
+    return np.array([int(tiles[T * b // blocks]) for b in range(blocks + 1)])
 
 
 def _block_gs_fp64(g, blocks, iters):
