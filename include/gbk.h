@@ -109,7 +109,8 @@ int gbk_ctx_sync(gbk_ctx *ctx);
  * as many as the GPU keeps resident, else 8..16384 in multiples of 8);
  * "pr_blocks" (row blocks of a Gauss-Seidel PageRank sweep, run in order: 0
  * picks one per ~4096 row tiles, at most 8; 1 is one launch updating in
- * place); "pr_unroll" (64-edge chunks a PageRank wave keeps in flight: 1, 2,
+ * place); "pr_inplace" (1: the blocks update contributions in place rather than
+ * publishing them block by block); "pr_unroll" (64-edge chunks a PageRank wave keeps in flight: 1, 2,
  * 4, or 0 for 4 in block sweeps and 1 otherwise); and
  * "ktime" (a gbk_ktime_kernel) selecting the kernel whose
  * launches are timed one by one with HIP events (loops then run eagerly).  Returns
