@@ -358,3 +358,21 @@ def test_relabel_by_equals_relabel(ctx):
         r1.free()
     with pytest.raises(gbk.GbkError):
         ctx.mapping(dg, 1)                      # random stays on the host
+
+
+@pytest.mark.parametrize("blocks", [4, 16])
+def test_pr_gs_inplace_blocks_reach_fixed_point(ctx, pr_blocks, blocks):
+    """pr_inplace: ordered blocks that update contributions in place (rows may
+    see same-sweep values of their own block, as in the reference's relaxed
+    sweep): held to the verifier and the FP64 fixed point."""
+    g = O.kron(14)
+    dg = ctx.upload_graph(g)
+    pr_blocks(blocks)
+    ctx.set_param("pr_inplace", 1)
+    try:
+        s, st = ctx.pr(dg, gbk.PR_GS, max_iters=30, tol=1e-4)
+        assert st.final_error < 1e-4 and O.pr_residual(g, s) < 1e-4
+        s2, _ = ctx.pr(dg, gbk.PR_GS, max_iters=200, tol=1e-9)
+        np.testing.assert_allclose(s2, O.pr_fp64(g), rtol=1e-4, atol=1e-10)
+    finally:
+        ctx.set_param("pr_inplace", 0)
