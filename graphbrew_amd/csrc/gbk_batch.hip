@@ -125,6 +125,39 @@ batch_collect(const int32_t *vm, int64_t n, int Kp, int K, int32_t unreached, co
   }
 }
 
+// The counts of batch_collect without the transposition: one streaming pass
+// over the vertex-major array.  The grid stride and the workgroup are
+// multiples of Kp, so a thread always reads the same source's column; its
+// counts meet the workgroup's in LDS once, at the end.
+__global__ void __launch_bounds__(kBlock)
+batch_count(const int32_t *vm, int64_t n, int log_kp, int32_t unreached, const int64_t *off,
+            unsigned long long *partial) {
+  __shared__ unsigned long long cnt[2][kBatchMax];
+  if (threadIdx.x < kBatchMax) {
+    cnt[0][threadIdx.x] = 0;
+    cnt[1][threadIdx.x] = 0;
+  }
+  __syncthreads();
+  unsigned long long reached = 0, edges = 0;
+  const int64_t total = n << log_kp;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+    if (vm[i] != unreached) {
+      const int64_t v = i >> log_kp;
+      reached++;
+      edges += static_cast<unsigned long long>(off[v + 1] - off[v]);
+    }
+  }
+  const int k = static_cast<int>(threadIdx.x & ((1 << log_kp) - 1));
+  atomicAdd(&cnt[0][k], reached);
+  atomicAdd(&cnt[1][k], edges);
+  __syncthreads();
+  if (threadIdx.x < kBatchMax) {
+    partial[static_cast<int64_t>(blockIdx.x) * kBatchMax + threadIdx.x] = cnt[0][threadIdx.x];
+    partial[static_cast<int64_t>(gridDim.x + blockIdx.x) * kBatchMax + threadIdx.x] = cnt[1][threadIdx.x];
+  }
+}
+
 // One workgroup of 1024 lanes: sums the collect partials per source.
 __global__ void batch_collect_sum(const unsigned long long *partial, int grid, unsigned long long *sums) {
   __shared__ unsigned long long red[2][16][kBatchMax];
@@ -1088,7 +1121,13 @@ int collect(gbk_ctx *ctx, const gbk_graph *g, const int32_t *vm, int Kp, int K, 
   auto *part = static_cast<unsigned long long *>(scratch(ctx, kS14, sizeof(unsigned long long) * 2 * grid * kBatchMax + 1024));
   auto *sums = static_cast<unsigned long long *>(pinned(ctx, sizeof(unsigned long long) * 2 * kBatchMax));
   if (!part || !sums) return set_error(ctx, GBK_ENOMEM, "batch collect scratch");
-  batch_collect<<<grid, kBlock, 0, s>>>(vm, g->n, Kp, K, unreached, g->off, out_dev, part);
+  if (out_dev) {
+    batch_collect<<<grid, kBlock, 0, s>>>(vm, g->n, Kp, K, unreached, g->off, out_dev, part);
+  } else {
+    int log_kp = 0;
+    while ((1 << log_kp) < Kp) log_kp++;
+    batch_count<<<grid, kBlock, 0, s>>>(vm, g->n, log_kp, unreached, g->off, part);
+  }
   batch_collect_sum<<<1, 1024, 0, s>>>(part, static_cast<int>(grid), part + 2 * grid * kBatchMax);
   GBK_LAUNCH_CHECK(ctx, "batch_collect");
   GBK_HIP(ctx, hipMemcpyAsync(sums, part + 2 * grid * kBatchMax, sizeof(unsigned long long) * 2 * kBatchMax,

@@ -204,3 +204,27 @@ def test_batch_rejects_bad_input(ctx):
         ctx.sssp_batch(gu, [0], 4)
     gu.free()
     dg.free()
+
+
+@pytest.mark.parametrize("k", [3, 8, 20, 64])
+def test_batch_counts_without_download_equal(ctx, k):
+    """Without the per-source arrays the reached counts and reached edges come
+    from a streaming count over the vertex-major array; they equal the
+    transposing path's and the oracle's."""
+    g = O.kron(12, weighted=True)
+    dg = ctx.upload_graph(g)
+    src = O.source_picker(g, k)[1].astype(np.int32)
+    d1, r1, e1, s1 = ctx.bfs_batch(dg, src)
+    d2, r2, e2, s2 = ctx.bfs_batch(dg, src, download=False)
+    assert d2 is None and np.array_equal(r1, r2) and np.array_equal(e1, e2)
+    assert s1.edges_reached == s2.edges_reached == int(e1.sum())
+    deg = np.diff(g.off)
+    for j in range(k):
+        reached = d1[j] >= 0
+        assert r1[j] == reached.sum() and e1[j] == deg[reached].sum()
+    _, r3, e3, s3 = ctx.sssp_batch(dg, src, 64)
+    _, r4, e4, s4 = ctx.sssp_batch(dg, src, 64, download=False)
+    assert np.array_equal(r3, r4) and np.array_equal(e3, e4) and s3.edges_reached == s4.edges_reached
+    _, b1 = ctx.bc_batch(dg, src)
+    _, b2 = ctx.bc_batch(dg, src, download=False)
+    assert b1.edges_reached == b2.edges_reached
