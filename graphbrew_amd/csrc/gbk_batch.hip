@@ -575,14 +575,7 @@ sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *di
     }
     const bool heavy = len > kBatchHeavy;
     if (heavy && f.leader) {
-      const unsigned long long at = atomicAdd(hq.count, 1ull);
-      if (static_cast<int64_t>(at) < hq.cap) {
-        hq.row[at] = static_cast<int32_t>(row);
-        hq.start[at] = s;
-        hq.len[at] = len;
-      } else {
-        *hq.overflow = 1;
-      }
+      heavy_push(hq, static_cast<int32_t>(row), s, len);
     }
     int64_t du;
     bool act;
@@ -602,14 +595,14 @@ sb_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *p
                HeavyQueue hq) {
   const volatile BatchState *vs = st;
   if (vs->done || vs->phase != 0) return;
-  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
   if (rows == 0) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   constexpr int G = kWave / Kp;
   const int grp = lane_id() / Kp;
   GroupRelax<Kp> f;
   f.setup(seg, nbr, wgt, dist, pend, vtag, near0, near1, cap, delta, st);
-  const int64_t total = hq.prefix[rows];
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
   unsigned long long walked = 0;
@@ -862,14 +855,7 @@ __global__ void bcb_sources(const int32_t *src, int K, int Kp, int32_t *depth, d
 }
 
 __device__ __forceinline__ void bcb_queue_heavy(const HeavyQueue &hq, int32_t row, int64_t s, int64_t len) {
-  const unsigned long long at = atomicAdd(hq.count, 1ull);
-  if (static_cast<int64_t>(at) < hq.cap) {
-    hq.row[at] = row;
-    hq.start[at] = s;
-    hq.len[at] = len;
-  } else {
-    *hq.overflow = 1;
-  }
+  heavy_push(hq, row, s, len);
 }
 
 // Heavy chunk c of the queue: its row and edge range (binary search in the
@@ -935,7 +921,8 @@ template <int Kp>
 __global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
 bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *ltag, int32_t *queue,
                   int64_t cap, int32_t d, BcbCounters *c, HeavyQueue hq) {
-  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
   if (rows == 0) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   constexpr int G = kWave / Kp;
@@ -949,7 +936,6 @@ bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *l
   f.tag = static_cast<uint32_t>(d) + 2;
   f.k = lane_id() % Kp;
   f.leader = f.k == 0;
-  const int64_t total = hq.prefix[rows];
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
   unsigned long long walked = 0;
@@ -1023,7 +1009,8 @@ template <int Kp>
 __global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
 bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const float *coef, int32_t d, BcbCounters *c,
                    HeavyQueue hq, double *hpart) {
-  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
   if (rows == 0) return;
   constexpr int G = kWave / Kp;
   const int grp = lane_id() / Kp;
@@ -1032,7 +1019,6 @@ bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const float *coef, 
   f.coef = coef;
   f.d = d;
   f.k = lane_id() % Kp;
-  const int64_t total = hq.prefix[rows];
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
   unsigned long long walked = 0;
@@ -1054,33 +1040,39 @@ bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const float *coef, 
   if (lane_id() == 0 && walked) atomicAdd(&c->edges_scanned, walked);
 }
 
-// One group per heavy row: the chunks' sums added in chunk order.
+// One wave per heavy row: lane (slot, k) adds source k's partials of chunks
+// slot, slot + G, ... (G = 64 / Kp slots), and the slots' sums meet over
+// shuffles -- a fixed order, so the result does not depend on the schedule.
+// (One lane group walking a 162K-edge row's 635 chunks one by one took
+// 0.4 ms a level.)
 template <int Kp>
 __global__ void __launch_bounds__(kBlock)
 bcb_backward_finish(const int32_t *depth, const double *sigma, float *coef, float *scores, int32_t d,
                     HeavyQueue hq, const double *hpart) {
-  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
   constexpr int G = kWave / Kp;
-  const int k = lane_id() % Kp;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
-  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  for (int64_t base = gw * G; base < rows; base += stride) {
-    const int64_t h = base + lane_id() / Kp;
-    const bool valid = h < rows;
+  const int lane = lane_id();
+  const int k = lane % Kp;
+  const int slot = lane / Kp;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t h = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave; h < rows; h += waves) {
+    const int32_t row = hq.row[h];
+    const bool here = depth[static_cast<int64_t>(row) * Kp + k] == d;
+    const int64_t end = h + 1 < rows ? hq.prefix[h + 1] : total;
+    double tot = 0.0;
+    if (here)
+      for (int64_t cc = hq.prefix[h] + slot; cc < end; cc += G) tot += hpart[cc * Kp + k];
+#pragma unroll
+    for (int o = Kp; o < kWave; o <<= 1) tot += __shfl_xor(tot, o);
     double delta = 0.0;
-    int32_t row = 0;
-    if (valid) {
-      row = hq.row[h];
-      if (depth[static_cast<int64_t>(row) * Kp + k] == d) {
-        double tot = 0.0;
-        for (int64_t cc = hq.prefix[h]; cc < hq.prefix[h + 1]; cc++) tot += hpart[cc * Kp + k];
-        const double su = sigma[static_cast<int64_t>(row) * Kp + k];
-        delta = su * tot;
-        coef[static_cast<int64_t>(row) * Kp + k] = static_cast<float>((1.0 + delta) / su);
-      }
+    if (slot == 0 && here) {
+      const double su = sigma[static_cast<int64_t>(row) * Kp + k];
+      delta = su * tot;
+      coef[static_cast<int64_t>(row) * Kp + k] = static_cast<float>((1.0 + delta) / su);
     }
-    const double tot = group_sum_d<Kp>(delta);
-    if (k == 0 && valid && tot != 0.0) scores[row] += static_cast<float>(tot);
+    const double sum = group_sum_d<Kp>(delta);
+    if (lane == 0 && sum != 0.0) scores[row] += static_cast<float>(sum);
   }
 }
 
@@ -1101,7 +1093,6 @@ static void launch_sb_relax(gbk_ctx *ctx, const gbk_graph *g, int32_t *dist, int
   hipStream_t s = ctx->stream;
   sb_relax<Kp><<<grid, kBlock, 0, s>>>(g->off, g->nbr, g->wgt, dist, pend, vtag, near0, near1, cap, delta,
                                        st, hq);
-  heavy_prefix<<<1, 1024, 0, s>>>(hq);
   sb_relax_heavy<Kp><<<grid, kBlock, 0, s>>>(g->nbr, g->wgt, dist, pend, vtag, near0, near1, cap, delta,
                                              st, hq);
 }
@@ -1197,7 +1188,6 @@ int msbfs_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, int
       msbfs_front_queue<<<n_grid, kBlock, 0, s>>>(front, n, queue, cap, c);
       msbfs_td<<<kGridCap, kBlock, 0, s>>>(g->off, g->nbr, queue, c, front, seen, next,
                                            ctx->tune.heavy_degree, hq);
-      heavy_prefix<<<1, 1024, 0, s>>>(hq);
       msbfs_td_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, front, seen, next, hq);
     }
     kt_end(ctx, kt);
@@ -1385,7 +1375,6 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
     const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
     bcb_forward<Kp><<<grid_f, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, ltag, queue + lb, le - lb, queue, cap,
                                               d, c, hq);
-    heavy_prefix<<<1, 1024, 0, s>>>(hq);
     bcb_forward_heavy<Kp><<<grid_f, kBlock, 0, s>>>(g->nbr, depth, sigma, ltag, queue, cap, d, c, hq);
     kt_end(ctx, kt);
     GBK_LAUNCH_CHECK(ctx, "bc batch forward");
@@ -1402,9 +1391,8 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
     fill<unsigned long long>(ctx, hq.count, 0ull, 1);
     const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
     bcb_backward<Kp><<<grid_b, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, coef, scores, queue + lb, cnt, d, c, hq);
-    heavy_prefix<<<1, 1024, 0, s>>>(hq);
     bcb_backward_heavy<Kp><<<grid_b, kBlock, 0, s>>>(g->nbr, depth, coef, d, c, hq, hpart);
-    bcb_backward_finish<Kp><<<64, kBlock, 0, s>>>(depth, sigma, coef, scores, d, hq, hpart);
+    bcb_backward_finish<Kp><<<1024, kBlock, 0, s>>>(depth, sigma, coef, scores, d, hq, hpart);
     kt_end(ctx, kt);
   }
   GBK_LAUNCH_CHECK(ctx, "bc batch backward");

@@ -153,14 +153,7 @@ bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, i
       start = off[u];
       len = off[u + 1] - start;
       if (len > heavy_deg) {
-        const unsigned long long at = atomicAdd(hq.count, 1ull);
-        if (static_cast<int64_t>(at) < hq.cap) {
-          hq.row[at] = u;
-          hq.start[at] = start;
-          hq.len[at] = len;
-        } else {
-          *hq.overflow = 1;
-        }
+        heavy_push(hq, u, start, len);
         len = 0;
       } else {
         light = true;
@@ -210,10 +203,10 @@ bc_backward_level(const int64_t *off, const int32_t *nbr, const int32_t *list, i
 __global__ void __launch_bounds__(kBlock)
 bc_backward_heavy(const int64_t *off, const int32_t *nbr, const double *sigma, const uint32_t *succ,
                   const float *coef, HeavyQueue hq, double *hpart) {
-  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
   if (rows == 0) return;
   const int64_t edge_base = off[0];
-  const int64_t total = hq.prefix[rows];
   const int lane = lane_id();
   const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave; c < total;
@@ -251,11 +244,13 @@ bc_backward_heavy(const int64_t *off, const int32_t *nbr, const double *sigma, c
 // One lane per heavy row: its chunks' partials added in order.
 __global__ void bc_backward_heavy_finish(HeavyQueue hq, const double *sigma, const double *hpart,
                                          float *coef, float *scores) {
-  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t h = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; h < rows; h += stride) {
     double tot = 0.0;
-    for (int64_t c = hq.prefix[h]; c < hq.prefix[h + 1]; c++) tot += hpart[c];
+    const int64_t end = h + 1 < rows ? hq.prefix[h + 1] : total;
+    for (int64_t c = hq.prefix[h]; c < end; c++) tot += hpart[c];
     const int32_t u = hq.row[h];
     coef[u] = static_cast<float>((1.0 + tot) / sigma[u]);
     scores[u] += static_cast<float>(tot);
@@ -317,7 +312,6 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
       const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
       bc_forward_level<<<expand_grid(le - lb), kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ,
                                                               queue, lb, le, cap, d, H, c, hq);
-      heavy_prefix<<<1, 1024, 0, s>>>(hq);
       bc_forward_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ, queue, cap,
                                                      d, c, hq);
       kt_end(ctx, kt);
@@ -336,7 +330,6 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
       const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
       bc_backward_level<<<expand_grid(cnt), kBlock, 0, s>>>(g->off, g->nbr, queue + lb, cnt, sigma,
                                                            succ, coef, scores, H, hq);
-      heavy_prefix<<<1, 1024, 0, s>>>(hq);
       bc_backward_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->off, g->nbr, sigma, succ, coef, hq, hpart);
       bc_backward_heavy_finish<<<4, kBlock, 0, s>>>(hq, sigma, hpart, coef, scores);
       kt_end(ctx, kt);

@@ -282,7 +282,6 @@ int gbk_bfs_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha, 
       const int kt = kt_begin(ctx, GBK_KT_BFS_STEP);
       bfs_td_step<<<grid, kBlock, 0, s>>>(g->off, g->nbr, parent, queue, qb, qe, cap,
                                           ctx->tune.heavy_degree, c, partial, hq);
-      heavy_prefix<<<1, 1024, 0, s>>>(hq);
       bfs_td_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, parent, queue, cap, c, partial + kGridCap, hq);
       kt_end(ctx, kt);
       bfs_store_count<<<1, 1024, 0, s>>>(partial, static_cast<int>(grid), &c->scout,

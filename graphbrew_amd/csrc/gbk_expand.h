@@ -158,8 +158,8 @@ struct HeavyQueue {
   int32_t *row;
   int64_t *start;
   int64_t *len;
-  int64_t *prefix;            // chunk prefix over the queued rows (cap + 1)
-  unsigned long long *count;  // reset to 0 before the light kernel
+  int64_t *prefix;            // first chunk of each queued row (cap)
+  unsigned long long *count;  // rows << 32 | chunks queued; reset to 0 before the light kernel
   int64_t cap;
   int *overflow;
   int64_t chunk;              // edges of a row per chunk
@@ -174,6 +174,8 @@ constexpr unsigned kHeavyGrid = 512;
 inline bool make_heavy_queue(gbk_ctx *ctx, const gbk_graph *g, int slot, HeavyQueue *hq) {
   const int64_t H = ctx->tune.heavy_degree;
   const int64_t cap = std::min<int64_t>(g->n, g->m / (H > 0 ? H : 1) + 1) + 1;
+  // a level's chunk total is counted in 32 bits (chunks are >= 64 edges)
+  if (g->m / 64 + cap >= (int64_t{1} << 32)) return false;
   const size_t bytes = 128 + cap * (4 + 8 + 8 + 8) + 64;
   char *p = static_cast<char *>(scratch(ctx, slot, bytes));
   if (!p) return false;
@@ -190,6 +192,34 @@ inline bool make_heavy_queue(gbk_ctx *ctx, const gbk_graph *g, int slot, HeavyQu
   return true;
 }
 
+// Queue a heavy row.  One atomic on 'count' reserves the row's slot (high 32
+// bits) together with its chunk range (low 32 bits), so prefix[] ascends in
+// queue order and the heavy kernel needs no scan pass first.  Past 'cap' the
+// row is dropped and 'overflow' is raised for the host to report.
+__device__ __forceinline__ void heavy_push(const HeavyQueue &hq, int32_t row, int64_t start, int64_t len) {
+  const auto chunks = static_cast<unsigned long long>((len + hq.chunk - 1) / hq.chunk);
+  const unsigned long long old = atomicAdd(hq.count, (1ull << 32) | chunks);
+  const auto at = static_cast<int64_t>(old >> 32);
+  if (at < hq.cap) {
+    hq.row[at] = row;
+    hq.start[at] = start;
+    hq.len[at] = len;
+    hq.prefix[at] = static_cast<int64_t>(old & 0xffffffffull);
+  } else {
+    *hq.overflow = 1;
+  }
+}
+
+// Rows stored and chunks they cover (the chunks reserved by dropped rows,
+// which follow every stored row's, are left out).
+__device__ __forceinline__ void heavy_extent(const HeavyQueue &hq, int64_t &rows, int64_t &total) {
+  const unsigned long long w = *hq.count;
+  const auto pushed = static_cast<int64_t>(w >> 32);
+  rows = min(pushed, hq.cap);
+  total = static_cast<int64_t>(w & 0xffffffffull);
+  if (pushed > hq.cap) total = hq.prefix[rows - 1] + (hq.len[rows - 1] + hq.chunk - 1) / hq.chunk;
+}
+
 // Light part of a frontier expansion: rows above heavy_deg go to 'hq'.
 template <class RowLen, class F>
 __device__ void expand_list_light(const int32_t *list, int64_t count, int64_t heavy_deg,
@@ -201,14 +231,7 @@ __device__ void expand_list_light(const int32_t *list, int64_t count, int64_t he
       row = list[i];
       rowlen(row, start, len);
       if (len > heavy_deg) {
-        const unsigned long long at = atomicAdd(hq.count, 1ull);
-        if (static_cast<int64_t>(at) < hq.cap) {
-          hq.row[at] = static_cast<int32_t>(row);
-          hq.start[at] = start;
-          hq.len[at] = len;
-        } else {
-          *hq.overflow = 1;
-        }
+        heavy_push(hq, static_cast<int32_t>(row), start, len);
         len = 0;
       }
     }
@@ -227,50 +250,11 @@ __device__ void expand_list_light_batch(const int32_t *list, int64_t count, int6
       row = list[i];
       rowlen(row, start, len);
       if (len > heavy_deg) {
-        const unsigned long long at = atomicAdd(hq.count, 1ull);
-        if (static_cast<int64_t>(at) < hq.cap) {
-          hq.row[at] = static_cast<int32_t>(row);
-          hq.start[at] = start;
-          hq.len[at] = len;
-        } else {
-          *hq.overflow = 1;
-        }
+        heavy_push(hq, static_cast<int32_t>(row), start, len);
         len = 0;
       }
     }
     wave_walk_batch<U>(row, start, len, f);
-  }
-}
-
-// One workgroup (1024 lanes): prefix[h] = chunks of rows [0, h); prefix[rows]
-// = total.  Run between the light and the heavy kernel.
-static __global__ void heavy_prefix(HeavyQueue hq) {
-  __shared__ int64_t wsum[1024 / kWave];
-  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
-  if (rows == 0) return;  // expand_heavy reads nothing else then
-  const int64_t chunk = hq.chunk;
-  const int t = threadIdx.x;
-  const int nw = static_cast<int>(blockDim.x) / kWave;
-  const int64_t per = (rows + blockDim.x - 1) / blockDim.x;
-  const int64_t b = t * per, e = min(rows, b + per);
-  int64_t s = 0;
-  for (int64_t h = b; h < e; h++) s += (hq.len[h] + chunk - 1) / chunk;
-  // block-wide exclusive scan of the per-thread chunk counts
-  const int64_t incl = wave_incl_scan(s);
-  const int w = t / kWave;
-  if (lane_id() == kWave - 1) wsum[w] = incl;
-  __syncthreads();
-  if (w == 0) {
-    int64_t x = lane_id() < nw ? wsum[lane_id()] : 0;
-    x = wave_incl_scan(x);
-    if (lane_id() < nw) wsum[lane_id()] = x;
-  }
-  __syncthreads();
-  int64_t run = incl - s + (w > 0 ? wsum[w - 1] : 0);
-  if (t == static_cast<int>(blockDim.x) - 1) hq.prefix[rows] = run + s;
-  for (int64_t h = b; h < e; h++) {
-    hq.prefix[h] = run;
-    run += (hq.len[h] + chunk - 1) / chunk;
   }
 }
 
@@ -279,9 +263,9 @@ static __global__ void heavy_prefix(HeavyQueue hq) {
 // the chunk prefix), so the work spreads evenly however many rows there are.
 template <class F>
 __device__ void expand_heavy(const HeavyQueue &hq, F &f) {
-  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
   if (rows == 0) return;
-  const int64_t total = hq.prefix[rows];
   for (int64_t gc = blockIdx.x; gc < total; gc += gridDim.x) {
     int64_t lo = 0, hi = rows;  // last row with prefix[row] <= gc
     while (hi - lo > 1) {
@@ -304,9 +288,9 @@ __device__ void expand_heavy(const HeavyQueue &hq, F &f) {
 // same row; edges past the chunk end are inactive and read edge 0).
 template <int U, class F>
 __device__ void expand_heavy_batch(const HeavyQueue &hq, F &f) {
-  const int64_t rows = min(static_cast<int64_t>(*hq.count), hq.cap);
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
   if (rows == 0) return;
-  const int64_t total = hq.prefix[rows];
   for (int64_t gc = blockIdx.x; gc < total; gc += gridDim.x) {
     int64_t lo = 0, hi = rows;  // last row with prefix[row] <= gc
     while (hi - lo > 1) {

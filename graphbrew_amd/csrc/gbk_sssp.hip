@@ -337,7 +337,6 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
       const int kt = kt_begin(ctx, GBK_KT_SSSP_RELAX);
       sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->nbr, g->wgt, word, near0, near1, cap, delta, H,
                                                st, hq);
-      heavy_prefix<<<1, 1024, 0, s>>>(hq);
       sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, g->wgt, word, near0, near1, cap, delta, st,
                                                      hq);
       kt_end(ctx, kt);
