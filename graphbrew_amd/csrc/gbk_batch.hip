@@ -815,28 +815,37 @@ struct BcbForward {
   }
 };
 
+// The backward pass reads a successor's depth and coefficient as one 64-bit
+// word dc[v * Kp + k] = depth << 32 | coefficient bits: one gathered line per
+// edge instead of one from each of two arrays.  A level stores only the low
+// halves of its own rows' words, which no reader of that level uses.
+__device__ __forceinline__ float *dc_coef(uint64_t *dc, int64_t i) { return reinterpret_cast<float *>(dc + i); }
+
 template <int Kp>
 struct BcbBackward {
-  const int32_t *depth;
-  const float *coef;
+  const uint64_t *dc;
   int32_t d;
   bool act;
   int k;
   double acc;
   static constexpr int U = kBcbUnroll<Kp>;
   __device__ void operator()(const int32_t (&x)[U], const bool (&on)[U]) {
-    int32_t dv[U];
-    float cf[U];
+    uint64_t w[U];
 #pragma unroll
-    for (int j = 0; j < U; j++) {
-      dv[j] = depth[static_cast<int64_t>(x[j]) * Kp + k];
-      cf[j] = coef[static_cast<int64_t>(x[j]) * Kp + k];
-    }
+    for (int j = 0; j < U; j++) w[j] = dc[static_cast<int64_t>(x[j]) * Kp + k];
 #pragma unroll
     for (int j = 0; j < U; j++)
-      if (on[j] && act && dv[j] == d + 1) acc += static_cast<double>(cf[j]);
+      if (on[j] && act && static_cast<int32_t>(w[j] >> 32) == d + 1)
+        acc += static_cast<double>(__uint_as_float(static_cast<uint32_t>(w[j])));
   }
 };
+
+// dc = depth << 32 (coefficients 0) after the forward pass.
+__global__ void bcb_pack(const int32_t *depth, int64_t total, uint64_t *dc) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride)
+    dc[i] = static_cast<uint64_t>(static_cast<uint32_t>(depth[i])) << 32;
+}
 
 __global__ void bcb_init_counters(BcbCounters *c, long long first) {
   if (threadIdx.x == 0) {
@@ -962,13 +971,12 @@ bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *l
 // row's score gains the deltas of all its lanes at this depth.
 template <int Kp>
 __global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
-bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const double *sigma, float *coef,
+bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const double *sigma, uint64_t *dc,
              float *scores, const int32_t *list, int64_t count, int32_t d, BcbCounters *c, HeavyQueue hq) {
   constexpr int G = kWave / Kp;
   const int grp = lane_id() / Kp;
   BcbBackward<Kp> f;
-  f.depth = depth;
-  f.coef = coef;
+  f.dc = dc;
   f.d = d;
   f.k = lane_id() % Kp;
   const bool leader = f.k == 0;
@@ -993,7 +1001,7 @@ bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const
     if (f.act && !heavy) {
       const double su = sigma[row * Kp + f.k];
       delta = su * f.acc;
-      coef[row * Kp + f.k] = static_cast<float>((1.0 + delta) / su);
+      *dc_coef(dc, row * Kp + f.k) = static_cast<float>((1.0 + delta) / su);
     }
     const double tot = group_sum_d<Kp>(delta);
     if (leader && valid && !heavy && tot != 0.0) scores[row] += static_cast<float>(tot);
@@ -1007,7 +1015,7 @@ bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const
 // sum over the chunk's successors.
 template <int Kp>
 __global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
-bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const float *coef, int32_t d, BcbCounters *c,
+bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const uint64_t *dc, int32_t d, BcbCounters *c,
                    HeavyQueue hq, double *hpart) {
   int64_t rows, total;
   heavy_extent(hq, rows, total);
@@ -1015,8 +1023,7 @@ bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const float *coef, 
   constexpr int G = kWave / Kp;
   const int grp = lane_id() / Kp;
   BcbBackward<Kp> f;
-  f.depth = depth;
-  f.coef = coef;
+  f.dc = dc;
   f.d = d;
   f.k = lane_id() % Kp;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
@@ -1047,7 +1054,7 @@ bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const float *coef, 
 // 0.4 ms a level.)
 template <int Kp>
 __global__ void __launch_bounds__(kBlock)
-bcb_backward_finish(const int32_t *depth, const double *sigma, float *coef, float *scores, int32_t d,
+bcb_backward_finish(const int32_t *depth, const double *sigma, uint64_t *dc, float *scores, int32_t d,
                     HeavyQueue hq, const double *hpart) {
   int64_t rows, total;
   heavy_extent(hq, rows, total);
@@ -1069,7 +1076,7 @@ bcb_backward_finish(const int32_t *depth, const double *sigma, float *coef, floa
     if (slot == 0 && here) {
       const double su = sigma[static_cast<int64_t>(row) * Kp + k];
       delta = su * tot;
-      coef[static_cast<int64_t>(row) * Kp + k] = static_cast<float>((1.0 + delta) / su);
+      *dc_coef(dc, static_cast<int64_t>(row) * Kp + k) = static_cast<float>((1.0 + delta) / su);
     }
     const double sum = group_sum_d<Kp>(delta);
     if (lane == 0 && sum != 0.0) scores[row] += static_cast<float>(sum);
@@ -1334,12 +1341,12 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
   const int64_t cap = n * Kp + 64;
   auto *depth = static_cast<int32_t *>(scratch(ctx, kS0, sizeof(int32_t) * n * Kp + 64));
   auto *sigma = static_cast<double *>(scratch(ctx, kS1, sizeof(double) * n * Kp + 64));
-  auto *coef = static_cast<float *>(scratch(ctx, kS2, sizeof(float) * n * Kp + 64));
+  auto *dc = static_cast<uint64_t *>(scratch(ctx, kS2, sizeof(uint64_t) * n * Kp + 64));
   auto *queue = static_cast<int32_t *>(scratch(ctx, kS3, sizeof(int32_t) * cap));
   auto *c = static_cast<BcbCounters *>(scratch(ctx, kS5, sizeof(BcbCounters) + 64));
   auto *ltag = static_cast<uint32_t *>(scratch(ctx, kS8, sizeof(uint32_t) * (n + 1)));
   auto *hc = static_cast<BcbCounters *>(pinned(ctx, sizeof(BcbCounters)));
-  if (!depth || !sigma || !coef || !queue || !c || !ltag || !hc) return set_error(ctx, GBK_ENOMEM, "bc batch scratch");
+  if (!depth || !sigma || !dc || !queue || !c || !ltag || !hc) return set_error(ctx, GBK_ENOMEM, "bc batch scratch");
   HeavyQueue hq;
   {
     const int64_t H = ctx->tune.heavy_degree;
@@ -1386,13 +1393,14 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
     level.push_back(tail);
   }
   const int levels = static_cast<int>(level.size()) - 1;
+  bcb_pack<<<grid_for(n * Kp), kBlock, 0, s>>>(depth, n * Kp, dc);
   for (int32_t d = levels - 1; d >= 0; d--) {
     const int64_t lb = level[d], cnt = level[d + 1] - lb;
     fill<unsigned long long>(ctx, hq.count, 0ull, 1);
     const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
-    bcb_backward<Kp><<<grid_b, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, coef, scores, queue + lb, cnt, d, c, hq);
-    bcb_backward_heavy<Kp><<<grid_b, kBlock, 0, s>>>(g->nbr, depth, coef, d, c, hq, hpart);
-    bcb_backward_finish<Kp><<<1024, kBlock, 0, s>>>(depth, sigma, coef, scores, d, hq, hpart);
+    bcb_backward<Kp><<<grid_b, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, dc, scores, queue + lb, cnt, d, c, hq);
+    bcb_backward_heavy<Kp><<<grid_b, kBlock, 0, s>>>(g->nbr, depth, dc, d, c, hq, hpart);
+    bcb_backward_finish<Kp><<<1024, kBlock, 0, s>>>(depth, sigma, dc, scores, d, hq, hpart);
     kt_end(ctx, kt);
   }
   GBK_LAUNCH_CHECK(ctx, "bc batch backward");
