@@ -217,7 +217,7 @@ msbfs_front_queue(const unsigned long long *front, int64_t n, int32_t *queue, in
     const int64_t v = base + threadIdx.x;
     q.push(v < n && front[v] != 0, static_cast<int32_t>(v));
   }
-  q.flush();
+  q.flush_block();
 }
 
 struct MsTdEdge {
@@ -583,9 +583,8 @@ sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *di
     f.walk(s, heavy ? 0 : len, du, act);
     if (f.leader && !heavy) walked += static_cast<unsigned long long>(len);
   }
-  f.nearq.flush();
-  walked = wave_sum(walked);
-  if (lane_id() == 0 && walked) atomicAdd(&st->edges_scanned, walked);
+  f.nearq.flush_block();
+  block_add(walked, &st->edges_scanned);
 }
 
 template <int Kp>
@@ -628,9 +627,8 @@ sb_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *p
     f.walk(start + first, valid ? last - first : 0, du, act);
     if (f.leader && valid) walked += static_cast<unsigned long long>(last - first);
   }
-  f.nearq.flush();
-  walked = wave_sum(walked);
-  if (lane_id() == 0 && walked) atomicAdd(&st->edges_scanned, walked);
+  f.nearq.flush_block();
+  block_add(walked, &st->edges_scanned);
 }
 
 __global__ void sb_relax_done(BatchState *st, unsigned long long *heavy_count) {
@@ -703,7 +701,7 @@ sb_split(int32_t *pend, int64_t n, int32_t *near0, int32_t *near1, int64_t cap, 
     }
     q.push(take, static_cast<int32_t>(v));
   }
-  q.flush();
+  q.flush_block();
 }
 
 __global__ void sb_split_done(BatchState *st) {
@@ -921,9 +919,8 @@ bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigm
     bcb_edges<Kp>(nbr, s, heavy ? 0 : len, f);
     if (f.leader && !heavy) walked += static_cast<unsigned long long>(len);
   }
-  f.q.flush();
-  walked = wave_sum(walked);
-  if (lane_id() == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+  f.q.flush_block();
+  block_add(walked, &c->edges_scanned);
 }
 
 template <int Kp>
@@ -961,9 +958,8 @@ bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *l
     bcb_edges<Kp>(nbr, valid ? hq.start[h] + first : 0, len, f);
     if (f.leader) walked += static_cast<unsigned long long>(len);
   }
-  f.q.flush();
-  walked = wave_sum(walked);
-  if (lane_id() == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+  f.q.flush_block();
+  block_add(walked, &c->edges_scanned);
 }
 
 // Backward level d: delta of a lane = sigma[u] * sum of coef[v] over its
@@ -1007,8 +1003,7 @@ bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const
     if (leader && valid && !heavy && tot != 0.0) scores[row] += static_cast<float>(tot);
     if (leader && !heavy) walked += static_cast<unsigned long long>(len);
   }
-  walked = wave_sum(walked);
-  if (lane_id() == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+  block_add(walked, &c->edges_scanned);
 }
 
 // Heavy rows of a backward level: hpart[chunk][k] = lane k's coefficient
@@ -1043,8 +1038,7 @@ bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const uint64_t *dc,
     if (valid) hpart[cc * Kp + f.k] = f.acc;
     if (f.k == 0) walked += static_cast<unsigned long long>(len);
   }
-  walked = wave_sum(walked);
-  if (lane_id() == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+  block_add(walked, &c->edges_scanned);
 }
 
 // One wave per heavy row: lane (slot, k) adds source k's partials of chunks

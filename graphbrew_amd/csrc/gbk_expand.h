@@ -490,7 +490,40 @@ struct WaveQueue {
     if (pred) lds[cnt + lanemask_lt_count(mask)] = v;
     cnt += k;
   }
+  // Final flush of a whole workgroup, called by every thread: one tail atomic
+  // per workgroup instead of one per wave (same-address atomics serialize,
+  // and a resident grid ends with thousands of waves flushing at once).
+  __device__ void flush_block() {
+    __shared__ int wcnt[kWavesPerBlock];
+    __shared__ unsigned long long bbase;
+    const int w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane_id() == 0) wcnt[w] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int v = 0; v < kWavesPerBlock; v++) tot += wcnt[v];
+      bbase = tot ? atomicAdd(tail, static_cast<unsigned long long>(tot)) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long base = bbase;
+    for (int v = 0; v < w; v++) base += static_cast<unsigned long long>(wcnt[v]);
+    for (int i = lane_id(); i < cnt; i += kWave) {
+      const int64_t at = static_cast<int64_t>(base) + i;
+      if (at < cap) out[at] = lds[i];
+      else *overflow = 1;
+    }
+    cnt = 0;
+  }
 };
+
+// Adds every lane's v into *total with one atomic per workgroup (every
+// thread calls it).
+__device__ __forceinline__ void block_add(unsigned long long v, unsigned long long *total) {
+  __shared__ unsigned long long red[kWavesPerBlock];
+  v = block_sum(v, red);
+  if (threadIdx.x == 0 && v) atomicAdd(total, v);
+}
 
 // Block-wide partial sum written to partial[blockIdx.x]; a one-workgroup
 // finalize kernel adds the partials (no same-address atomics).
