@@ -197,17 +197,17 @@ class Workload:
 # on its stream around each launch) for the roofline of the dominant family.
 def kt_table(gbk, single):
     t = {
-        "sssp": (gbk.KT_SSSP_RELAX, "sssp_relax+heavy_prefix+sssp_relax_heavy (one relaxation round)"),
+        "sssp": (gbk.KT_SSSP_RELAX, "sssp_relax+sssp_relax_heavy (one relaxation round)"),
         "pr": (gbk.KT_PR_ITER, "pr_pull_block + pr_block_close per row block (one Gauss-Seidel sweep)"),
-        "bfs": (gbk.KT_BFS_STEP, "bfs_td_step+heavy_prefix+bfs_td_heavy | bfs_bu_step (one BFS step)"),
-        "bc": (gbk.KT_BC_LEVEL, "bc_forward_level+heavy_prefix+bc_forward_heavy | bc_backward_level"),
+        "bfs": (gbk.KT_BFS_STEP, "bfs_td_step+bfs_td_heavy | bfs_bu_step (one BFS step)"),
+        "bc": (gbk.KT_BC_LEVEL, "bc_forward_level+bc_forward_heavy | bc_backward_level"),
     }
     if not single:
-        t["sssp"] = (gbk.KT_SSSP_RELAX, "sb_relax<64>+heavy_prefix+sb_relax_heavy<64> "
+        t["sssp"] = (gbk.KT_SSSP_RELAX, "sb_relax<64>+sb_relax_heavy<64> "
                                         "(one relaxation round of a 64-source batch)")
-        t["bfs"] = (gbk.KT_BFS_STEP, "msbfs_front_queue+msbfs_td+heavy_prefix+msbfs_td_heavy | msbfs_bu<8> "
+        t["bfs"] = (gbk.KT_BFS_STEP, "msbfs_front_queue+msbfs_td+msbfs_td_heavy | msbfs_bu<8> "
                                      "(one level of a 64-source batch)")
-        t["bc"] = (gbk.KT_BC_LEVEL, "bcb_forward+heavy_prefix+bcb_forward_heavy | bcb_backward+heavy_prefix+"
+        t["bc"] = (gbk.KT_BC_LEVEL, "bcb_forward+bcb_forward_heavy | bcb_backward+"
                                     "bcb_backward_heavy+bcb_backward_finish (one level of a source batch)")
     return t
 

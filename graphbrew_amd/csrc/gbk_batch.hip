@@ -126,9 +126,10 @@ batch_collect(const int32_t *vm, int64_t n, int Kp, int K, int32_t unreached, co
 }
 
 // The counts of batch_collect without the transposition: one streaming pass
-// over the vertex-major array.  The grid stride and the workgroup are
-// multiples of Kp, so a thread always reads the same source's column; its
-// counts meet the workgroup's in LDS once, at the end.
+// over the vertex-major array, 16 bytes (four sources of one vertex; Kp is a
+// multiple of 4) per lane and load, four loads in flight.  The grid stride
+// is a multiple of Kp, so a thread always reads the same four sources'
+// columns; its counts meet the workgroup's in LDS once, at the end.
 __global__ void __launch_bounds__(kBlock)
 batch_count(const int32_t *vm, int64_t n, int log_kp, int32_t unreached, const int64_t *off,
             unsigned long long *partial) {
@@ -138,19 +139,40 @@ batch_count(const int32_t *vm, int64_t n, int log_kp, int32_t unreached, const i
     cnt[1][threadIdx.x] = 0;
   }
   __syncthreads();
-  unsigned long long reached = 0, edges = 0;
-  const int64_t total = n << log_kp;
+  constexpr int U = 4;
+  unsigned long long reached[4] = {}, edges[4] = {};
+  const int4 *v4 = reinterpret_cast<const int4 *>(vm);
+  const int64_t total = (n << log_kp) >> 2;  // 16-byte items
+  const int log_q = log_kp - 2;              // items per vertex = Kp / 4
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
-    if (vm[i] != unreached) {
-      const int64_t v = i >> log_kp;
-      reached++;
-      edges += static_cast<unsigned long long>(off[v + 1] - off[v]);
+  for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i0 < total; i0 += U * stride) {
+    int4 x[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int64_t i = i0 + j * stride;
+      x[j] = i < total ? v4[i] : make_int4(unreached, unreached, unreached, unreached);
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const bool r[4] = {x[j].x != unreached, x[j].y != unreached, x[j].z != unreached, x[j].w != unreached};
+      if (!(r[0] || r[1] || r[2] || r[3])) continue;
+      const int64_t v = (i0 + j * stride) >> log_q;
+      const auto deg = static_cast<unsigned long long>(off[v + 1] - off[v]);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        reached[q] += r[q];
+        edges[q] += r[q] ? deg : 0ull;
+      }
     }
   }
-  const int k = static_cast<int>(threadIdx.x & ((1 << log_kp) - 1));
-  atomicAdd(&cnt[0][k], reached);
-  atomicAdd(&cnt[1][k], edges);
+  const int k0 = static_cast<int>((threadIdx.x << 2) & ((1u << log_kp) - 1));
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    if (reached[q]) {
+      atomicAdd(&cnt[0][k0 + q], reached[q]);
+      atomicAdd(&cnt[1][k0 + q], edges[q]);
+    }
+  }
   __syncthreads();
   if (threadIdx.x < kBatchMax) {
     partial[static_cast<int64_t>(blockIdx.x) * kBatchMax + threadIdx.x] = cnt[0][threadIdx.x];
@@ -1116,6 +1138,9 @@ int collect(gbk_ctx *ctx, const gbk_graph *g, const int32_t *vm, int Kp, int K, 
   if (out_dev) {
     batch_collect<<<grid, kBlock, 0, s>>>(vm, g->n, Kp, K, unreached, g->off, out_dev, part);
   } else {
+    // 16-byte loads of four sources: Kp >= 4 and an aligned array
+    if (Kp < 4 || (reinterpret_cast<uintptr_t>(vm) & 15))
+      return set_error(ctx, GBK_EINVAL, "batch count: misaligned vertex-major array");
     int log_kp = 0;
     while ((1 << log_kp) < Kp) log_kp++;
     batch_count<<<grid, kBlock, 0, s>>>(vm, g->n, log_kp, unreached, g->off, part);
