@@ -284,9 +284,10 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   const bool publish = gs && B > 1 && !ctx->tune.pr_inplace;
   const unsigned cgrid = publish ? static_cast<unsigned>(std::min<int64_t>(grid_for(n / B + 1), 1024)) : 16;
   const bool close = publish || rt.nheavy;
-  // block launches are short and latency-bound: more loads in flight pay;
-  // Jacobi's one launch is bound by the random-line rate and loses 7%
-  const int unroll = ctx->tune.pr_unroll ? ctx->tune.pr_unroll : (gs && B > 1 ? 4 : 1);
+  // block launches are short and latency-bound: more loads in flight pay
+  // (8 chunks 1% over 4); Jacobi's one launch is bound by the random-line
+  // rate and loses 7%
+  const int unroll = ctx->tune.pr_unroll ? ctx->tune.pr_unroll : (gs && B > 1 ? 8 : 1);
   const size_t nparts = static_cast<size_t>(B) * (grid + (close ? cgrid : 0));
   auto *scores = static_cast<float *>(scratch(ctx, kS0, sizeof(float) * n));
   auto *c0 = static_cast<float *>(scratch(ctx, kS1, sizeof(float) * n));
@@ -318,6 +319,10 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
             break;
           case 2:
             pr_pull_block<2><<<grid, kBlock, 0, s>>>(rt, blocks, b, g->in_nbr, g->off, scores, cur, nxt, st, partial,
+                                                     hacc, base, kDamp);
+            break;
+          case 8:
+            pr_pull_block<8><<<grid, kBlock, 0, s>>>(rt, blocks, b, g->in_nbr, g->off, scores, cur, nxt, st, partial,
                                                      hacc, base, kDamp);
             break;
           default:

@@ -408,7 +408,7 @@ int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value) {
   else if (k == "batch_grid" && value >= 0 && value <= 16384 && value % 8 == 0) t.batch_grid = static_cast<int>(value);
   else if (k == "pr_blocks" && value >= 0 && value <= 1024) t.pr_blocks = static_cast<int>(value);
   else if (k == "pr_inplace" && (value == 0 || value == 1)) t.pr_inplace = static_cast<int>(value);
-  else if (k == "pr_unroll" && (value == 0 || value == 1 || value == 2 || value == 4)) t.pr_unroll = static_cast<int>(value);
+  else if (k == "pr_unroll" && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8)) t.pr_unroll = static_cast<int>(value);
   else if (k == "ktime" && value >= GBK_KT_NONE && value <= GBK_KT_LAST) {
     if (value != GBK_KT_NONE && !ctx->kt_dev) {
       const size_t bytes = sizeof(unsigned long long) * (2 + 2 * kKtPartials);

@@ -111,7 +111,7 @@ int gbk_ctx_sync(gbk_ctx *ctx);
  * picks one per ~4096 row tiles, at most 8; 1 is one launch updating in
  * place); "pr_inplace" (1: the blocks update contributions in place rather than
  * publishing them block by block); "pr_unroll" (64-edge chunks a PageRank wave keeps in flight: 1, 2,
- * 4, or 0 for 4 in block sweeps and 1 otherwise); and
+ * 4, 8, or 0 for 8 in block sweeps and 1 otherwise); and
  * "ktime" (a gbk_ktime_kernel) selecting the kernel whose
  * launches are timed one by one with HIP events (loops then run eagerly).  Returns
  * GBK_EINVAL for an unknown name or bad value.  No reference counterpart:

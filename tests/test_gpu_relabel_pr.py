@@ -273,6 +273,22 @@ def test_pr_gs_block_sweep_matches_fp64(ctx, pr_blocks, name, blocks):
         np.testing.assert_allclose(s, _block_gs_fp64(g, blocks, iters), rtol=1e-5, atol=1e-12)
 
 
+@pytest.mark.parametrize("unroll", [1, 2, 4, 8])
+def test_pr_gs_block_sweep_unrolls_match_fp64(ctx, pr_blocks, unroll):
+    """Every chunks-in-flight setting of the block sweep gives the float64
+    block sweep (the setting only changes the order of a row's float adds)."""
+    g = BLOCK_GRAPHS["kron13"]()
+    dg = ctx.upload_graph(g)
+    pr_blocks(3)
+    ctx.set_param("pr_unroll", unroll)
+    try:
+        s, st = ctx.pr(dg, gbk.PR_GS, max_iters=3, tol=0.0, fixed=True)
+    finally:
+        ctx.set_param("pr_unroll", 0)
+    assert st.iterations == 3
+    np.testing.assert_allclose(s, _block_gs_fp64(g, 3, 3), rtol=1e-5, atol=1e-12)
+
+
 def test_block_gs_reference_model_pins():
     """The float64 model itself: one block is the oracle's Jacobi, and more
     blocks are not (so the tests above tell the read rules apart)."""
