@@ -746,6 +746,8 @@ __global__ void sb_split_done(BatchState *st) {
 struct BcbCounters {
   unsigned long long tail;
   unsigned long long edges_scanned;
+  unsigned long long listed_edges;  // out-edges of the level lists after the first
+  unsigned long long open_edges;    // in-edges of vertices some source has not reached
   int overflow;
   int pad;
 };
@@ -871,6 +873,7 @@ __global__ void bcb_init_counters(BcbCounters *c, long long first) {
   if (threadIdx.x == 0) {
     c->tail = static_cast<unsigned long long>(first);
     c->edges_scanned = 0;
+    c->listed_edges = 0;
     c->overflow = 0;
   }
 }
@@ -982,6 +985,196 @@ bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *l
   }
   f.q.flush_block();
   block_add(walked, &c->edges_scanned);
+}
+
+// Bottom-up forward level d, for levels whose frontier has more out-edges
+// than the vertices not yet reached: every vertex that some lane has not
+// reached sums, lane by lane, sigma over its in-neighbours at depth d and
+// takes depth d + 1 where the sum is positive (sigma >= 1 on every reached
+// vertex).  One group owns a vertex, so there are no atomics; a vertex set
+// here is never at depth d, so concurrent readers decide the same either
+// way.  The sum of integer-valued doubles is exact in any order, as the
+// top-down atomics' is.
+template <int Kp>
+struct BcbPull {
+  const int32_t *depth;
+  const double *sigma;
+  int32_t d;
+  bool act;
+  int k;
+  double acc;
+  static constexpr int U = kBcbUnroll<Kp>;
+  __device__ void operator()(const int32_t (&x)[U], const bool (&on)[U]) {
+    int32_t du[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) du[j] = depth[static_cast<int64_t>(x[j]) * Kp + k];
+#pragma unroll
+    for (int j = 0; j < U; j++)
+      if (on[j] && act && du[j] == d) acc += sigma[static_cast<int64_t>(x[j]) * Kp + k];
+  }
+};
+
+// Light rows of a bottom-up level; rows above kBatchHeavy go to 'hq'.
+template <int Kp>
+__global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
+bcb_forward_pull(const int64_t *ioff, const int32_t *inbr, int64_t n, int K, int32_t *depth, double *sigma,
+                 int32_t *queue, int64_t cap, int32_t d, BcbCounters *c, HeavyQueue hq) {
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  constexpr int G = kWave / Kp;
+  const int grp = lane_id() / Kp;
+  WaveQueue q;
+  q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
+  BcbPull<Kp> f;
+  f.depth = depth;
+  f.sigma = sigma;
+  f.d = d;
+  f.k = lane_id() % Kp;
+  const bool leader = f.k == 0;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  unsigned long long walked = 0;
+  for (int64_t base = gw * G; base < n; base += waves * G) {
+    const int64_t v = base + grp;
+    const bool valid = v < n;
+    f.act = valid && f.k < K && depth[v * Kp + f.k] == -1;
+    const bool any = group_any<Kp>(f.act);
+    int64_t s = 0, len = 0;
+    if (any) {
+      s = ioff[v];
+      len = ioff[v + 1] - s;
+    }
+    const bool heavy = len > kBatchHeavy;
+    if (heavy && leader) heavy_push(hq, static_cast<int32_t>(v), s, len);
+    f.acc = 0.0;
+    bcb_edges<Kp>(inbr, s, heavy ? 0 : len, f);
+    const bool found = f.act && !heavy && f.acc > 0.0;
+    if (found) {
+      depth[v * Kp + f.k] = d + 1;
+      sigma[v * Kp + f.k] = f.acc;
+    }
+    const bool anyf = group_any<Kp>(found);
+    q.push(leader && anyf, static_cast<int32_t>(v));
+    if (leader && !heavy) walked += static_cast<unsigned long long>(len);
+  }
+  q.flush_block();
+  block_add(walked, &c->edges_scanned);
+}
+
+// Heavy rows of a bottom-up level: hpart[chunk][k] = lane k's sigma sum over
+// the chunk's in-neighbours at depth d.
+template <int Kp>
+__global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
+bcb_pull_heavy(const int32_t *inbr, int K, const int32_t *depth, const double *sigma, int32_t d, BcbCounters *c,
+               HeavyQueue hq, double *hpart) {
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
+  if (rows == 0) return;
+  constexpr int G = kWave / Kp;
+  const int grp = lane_id() / Kp;
+  BcbPull<Kp> f;
+  f.depth = depth;
+  f.sigma = sigma;
+  f.d = d;
+  f.k = lane_id() % Kp;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
+  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  unsigned long long walked = 0;
+  for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
+    const int64_t cc = c0 + grp;
+    const bool valid = cc < total;
+    int64_t h = 0, first = 0, len = 0, row = 0;
+    if (valid) {
+      bcb_chunk(hq, rows, cc, h, first, len);
+      row = hq.row[h];
+    }
+    f.act = valid && f.k < K && depth[row * Kp + f.k] == -1;
+    f.acc = 0.0;
+    bcb_edges<Kp>(inbr, valid ? hq.start[h] + first : 0, len, f);
+    if (valid) hpart[cc * Kp + f.k] = f.acc;
+    if (f.k == 0) walked += static_cast<unsigned long long>(len);
+  }
+  block_add(walked, &c->edges_scanned);
+}
+
+// One wave per heavy row of a bottom-up level: its chunks' sums meet as in
+// bcb_backward_finish; lanes with a positive sum take depth d + 1 and the
+// row joins the next level's list.
+template <int Kp>
+__global__ void __launch_bounds__(kBlock)
+bcb_pull_finish(int K, int32_t *depth, double *sigma, int32_t d, HeavyQueue hq, const double *hpart,
+                int32_t *queue, int64_t cap, BcbCounters *c) {
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  WaveQueue q;
+  q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
+  constexpr int G = kWave / Kp;
+  const int lane = lane_id();
+  const int k = lane % Kp;
+  const int slot = lane / Kp;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t h = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave; h < rows; h += waves) {
+    const int32_t row = hq.row[h];
+    const bool need = k < K && depth[static_cast<int64_t>(row) * Kp + k] == -1;
+    const int64_t end = h + 1 < rows ? hq.prefix[h + 1] : total;
+    double tot = 0.0;
+    if (need)
+      for (int64_t cc = hq.prefix[h] + slot; cc < end; cc += G) tot += hpart[cc * Kp + k];
+#pragma unroll
+    for (int o = Kp; o < kWave; o <<= 1) tot += __shfl_xor(tot, o);
+    const bool found = slot == 0 && need && tot > 0.0;
+    if (found) {
+      depth[static_cast<int64_t>(row) * Kp + k] = d + 1;
+      sigma[static_cast<int64_t>(row) * Kp + k] = tot;
+    }
+    const bool anyf = __ballot(found) != 0;
+    q.push(lane == 0 && anyf, row);
+  }
+  q.flush_block();
+}
+
+// In-edges of the vertices that some source (lane k < K) has not reached,
+// added into c->open_edges: what a bottom-up level would walk.  16-byte
+// loads of four lanes; a vertex's Kp / 4 items sit in one aligned lane
+// group of the wave and are OR-ed over shuffles.
+__global__ void __launch_bounds__(kBlock)
+bcb_open_edges(const int32_t *depth, int64_t n, int log_kp, int K, const int64_t *ioff, BcbCounters *c) {
+  const int4 *d4 = reinterpret_cast<const int4 *>(depth);
+  const int log_q = log_kp - 2;
+  const int q = 1 << log_q;
+  const int64_t total = (n << log_kp) >> 2;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  unsigned long long e = 0;
+  for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x; i0 < total; i0 += stride) {
+    const int64_t i = i0 + threadIdx.x;
+    int open = 0;
+    if (i < total) {
+      const int4 x = d4[i];
+      const int k0 = static_cast<int>((i & (q - 1)) << 2);
+      open = (k0 < K && x.x == -1) || (k0 + 1 < K && x.y == -1) || (k0 + 2 < K && x.z == -1) ||
+             (k0 + 3 < K && x.w == -1);
+    }
+    for (int o = 1; o < q; o <<= 1) open |= __shfl_xor(open, o);
+    if (i < total && (i & (q - 1)) == 0 && open) {
+      const int64_t v = i >> log_q;
+      e += static_cast<unsigned long long>(ioff[v + 1] - ioff[v]);
+    }
+  }
+  block_add(e, &c->open_edges);
+}
+
+// Out-edges of the level list [le, tail) added into c->listed_edges: the
+// host compares the next frontier's edges with those not yet listed.
+__global__ void __launch_bounds__(kBlock)
+bcb_list_edges(const int64_t *off, const int32_t *queue, int64_t le, int64_t cap, BcbCounters *c) {
+  const int64_t tail = min(static_cast<int64_t>(c->tail), cap);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  unsigned long long e = 0;
+  for (int64_t i = le + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < tail; i += stride) {
+    const int32_t v = queue[i];
+    e += static_cast<unsigned long long>(off[v + 1] - off[v]);
+  }
+  block_add(e, &c->listed_edges);
 }
 
 // Backward level d: delta of a lane = sigma[u] * sum of coef[v] over its
@@ -1388,6 +1581,8 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
                                    resident_grid(ctx, reinterpret_cast<const void *>(&bcb_forward_heavy<Kp>)));
   const unsigned grid_b = std::min(resident_grid(ctx, reinterpret_cast<const void *>(&bcb_backward<Kp>)),
                                    resident_grid(ctx, reinterpret_cast<const void *>(&bcb_backward_heavy<Kp>)));
+  const unsigned grid_p = std::min(resident_grid(ctx, reinterpret_cast<const void *>(&bcb_forward_pull<Kp>)),
+                                   resident_grid(ctx, reinterpret_cast<const void *>(&bcb_pull_heavy<Kp>)));
   GBK_HIP(ctx, hipMemcpyAsync(queue, uniq.data(), sizeof(int32_t) * uniq.size(), hipMemcpyHostToDevice, s));
   fill<int32_t>(ctx, depth, -1, n * Kp);
   fill<double>(ctx, sigma, 0.0, n * Kp);
@@ -1395,14 +1590,43 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
   bcb_init_counters<<<1, 1, 0, s>>>(c, static_cast<long long>(uniq.size()));
   bcb_sources<<<1, kBatchMax, 0, s>>>(dsrc, K, Kp, depth, sigma);
   std::vector<int64_t> level = {0, static_cast<int64_t>(uniq.size())};
+  // direction per level (bfs.cc:231-262 weighs frontier edges against the
+  // unexplored ones the same way): bottom-up while the frontier's out-edges
+  // exceed the edges of the vertices not yet listed at any level
+  int64_t listed = 0, frontier_edges = 0;
+  int log_kp = 0;
+  while ((1 << log_kp) < Kp) log_kp++;
   for (int32_t d = 0;; d++) {
     const int64_t lb = level[level.size() - 2], le = level.back();
     fill<unsigned long long>(ctx, hq.count, 0ull, 1);
+    // m - listed bounds the bottom-up work from below; only past it is the
+    // exact figure counted (one streaming pass over the depths), and only
+    // for narrow rows: at 32 and 64 lanes the pass reads n * Kp * 4 bytes,
+    // and a top-down frontier edge costs more than the bound's error (64
+    // sources: 50.7 ms on the bound, 52.9 with the count at any ratio)
+    bool pull = ctx->tune.bc_pull == 2;
+    const int64_t weighed = frontier_edges * ctx->tune.bc_pull_ratio;
+    if (ctx->tune.bc_pull == 1 && weighed > g->m - listed && Kp >= 32) {
+      pull = true;
+    } else if (ctx->tune.bc_pull == 1 && weighed > g->m - listed) {
+      fill<unsigned long long>(ctx, &c->open_edges, 0ull, 1);
+      bcb_open_edges<<<1024, kBlock, 0, s>>>(depth, n, log_kp, K, g->in_off, c);
+      GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcbCounters), hipMemcpyDeviceToHost, s));
+      GBK_HIP(ctx, hipStreamSynchronize(s));
+      pull = weighed > static_cast<int64_t>(hc->open_edges);
+    }
     const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
-    bcb_forward<Kp><<<grid_f, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, ltag, queue + lb, le - lb, queue, cap,
-                                              d, c, hq);
-    bcb_forward_heavy<Kp><<<grid_f, kBlock, 0, s>>>(g->nbr, depth, sigma, ltag, queue, cap, d, c, hq);
+    if (pull) {
+      bcb_forward_pull<Kp><<<grid_p, kBlock, 0, s>>>(g->in_off, g->in_nbr, n, K, depth, sigma, queue, cap, d, c, hq);
+      bcb_pull_heavy<Kp><<<grid_p, kBlock, 0, s>>>(g->in_nbr, K, depth, sigma, d, c, hq, hpart);
+      bcb_pull_finish<Kp><<<1024, kBlock, 0, s>>>(K, depth, sigma, d, hq, hpart, queue, cap, c);
+    } else {
+      bcb_forward<Kp><<<grid_f, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, ltag, queue + lb, le - lb, queue,
+                                                cap, d, c, hq);
+      bcb_forward_heavy<Kp><<<grid_f, kBlock, 0, s>>>(g->nbr, depth, sigma, ltag, queue, cap, d, c, hq);
+    }
     kt_end(ctx, kt);
+    if (ctx->tune.bc_pull == 1) bcb_list_edges<<<256, kBlock, 0, s>>>(g->off, queue, le, cap, c);
     GBK_LAUNCH_CHECK(ctx, "bc batch forward");
     GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcbCounters), hipMemcpyDeviceToHost, s));
     GBK_HIP(ctx, hipStreamSynchronize(s));
@@ -1410,6 +1634,8 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
     const int64_t tail = static_cast<int64_t>(hc->tail);
     if (tail == le) break;
     level.push_back(tail);
+    frontier_edges = static_cast<int64_t>(hc->listed_edges) - listed;
+    listed = static_cast<int64_t>(hc->listed_edges);
   }
   const int levels = static_cast<int>(level.size()) - 1;
   bcb_pack<<<grid_for(n * Kp), kBlock, 0, s>>>(depth, n * Kp, dc);

@@ -40,8 +40,11 @@ struct Tuning {
   int batch_grid = 0;           // workgroups of a source batch's launches (0: as many as stay resident)
   int pr_blocks = 0;            // row blocks of a Gauss-Seidel PageRank sweep (0: by size)
   int pr_inplace = 0;           // Gauss-Seidel blocks update contributions in place (1) or publish them
-  int pr_unroll = 0;            // 64-edge chunks a PageRank wave has in flight (1, 2, 4; 0: 4 for
+  int pr_unroll = 0;            // 64-edge chunks a PageRank wave has in flight (1, 2, 4, 8; 0: 8 for
                                 // Gauss-Seidel blocks, 1 for Jacobi's one launch)
+  int bc_pull = 1;              // BC batch forward levels: 0 top-down, 1 bottom-up when the
+                                // frontier has more out-edges than the unlisted vertices, 2 bottom-up
+  int bc_pull_ratio = 1;        // a frontier edge weighs this many open edges in that choice
 };
 constexpr int kGridCapRuntime = 2048;  // upper bound of relax_grid
 constexpr int64_t kHeavyChunkEdges = 1024;  // heavy-row chunk of the all-rows kernels

@@ -111,7 +111,10 @@ int gbk_ctx_sync(gbk_ctx *ctx);
  * picks one per ~4096 row tiles, at most 8; 1 is one launch updating in
  * place); "pr_inplace" (1: the blocks update contributions in place rather than
  * publishing them block by block); "pr_unroll" (64-edge chunks a PageRank wave keeps in flight: 1, 2,
- * 4, 8, or 0 for 8 in block sweeps and 1 otherwise); and
+ * 4, 8, or 0 for 8 in block sweeps and 1 otherwise); "bc_pull" (BC batch forward levels: 0
+ * top-down only, 1 (default) bottom-up from in-neighbours when the frontier's out-edges outweigh
+ * those of the vertices some source has not reached, 2 bottom-up at every level); "bc_pull_ratio"
+ * (1..1024, default 1: how many such edges one frontier edge weighs in that choice); and
  * "ktime" (a gbk_ktime_kernel) selecting the kernel whose
  * launches are timed one by one with HIP events (loops then run eagerly).  Returns
  * GBK_EINVAL for an unknown name or bad value.  No reference counterpart:

@@ -178,6 +178,28 @@ def test_bc_batch_matches_brandes(ctx, name, k):
     dg.free()
 
 
+@pytest.mark.parametrize("name", ["kron13", "star3000", "disconnected", "directed"])
+@pytest.mark.parametrize("k", [4, 64])
+@pytest.mark.parametrize("pull", [0, 2])
+def test_bc_batch_directions_match_brandes(ctx, name, k, pull):
+    """Forward levels forced top-down (0) or bottom-up from in-neighbours at
+    every level (2) give the serial Brandes scores and the same reached
+    out-edges; the default switches by frontier edges per level."""
+    g = BC_GRAPHS[name]()
+    dg = ctx.upload_graph(g)
+    srcs = _sources(g, k, seed=300 + k)
+    ctx.set_param("bc_pull", pull)
+    try:
+        scores, st = ctx.bc_batch(dg, srcs)
+    finally:
+        ctx.set_param("bc_pull", 1)
+    ref = O.bc_serial(g, srcs)
+    np.testing.assert_allclose(scores, ref, rtol=1e-4, atol=1e-6)
+    deg = np.diff(g.off)
+    assert st.edges_reached == sum(int(deg[O.bfs_depths(g, int(s)) >= 0].sum()) for s in srcs)
+    dg.free()
+
+
 def test_bc_batch_equals_one_source_runs(ctx):
     g = O.kron(14)
     dg = ctx.upload_graph(g)
