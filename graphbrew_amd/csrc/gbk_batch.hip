@@ -579,7 +579,8 @@ sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *di
   constexpr int G = kWave / Kp;
   const int grp = lane_id() / Kp;
   const int cur = vs->cur;
-  const int64_t count = static_cast<int64_t>(vs->near_cnt[cur]);
+  // the tail counts past cap only when a push overflowed (reported by the host)
+  const int64_t count = min(static_cast<int64_t>(vs->near_cnt[cur]), cap);
   const int32_t *list = cur ? near1 : near0;
   GroupRelax<Kp> f;
   f.setup(seg, nbr, wgt, dist, pend, vtag, near0, near1, cap, delta, st);

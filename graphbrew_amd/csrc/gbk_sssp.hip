@@ -138,7 +138,8 @@ sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, unsigned 
   if (vs->done || vs->phase != 0) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   const int cur = vs->cur;
-  const int64_t count = static_cast<int64_t>(vs->near_cnt[cur]);
+  // the tail counts past cap only when a push overflowed (reported by the host)
+  const int64_t count = min(static_cast<int64_t>(vs->near_cnt[cur]), cap);
   RelaxEdge f;
   relax_setup(f, seg, nbr, wgt, word, near0, near1, cap, delta, st);
   const int64_t lower = delta * f.bucket;
