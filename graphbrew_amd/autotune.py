@@ -9,6 +9,11 @@ results: bit-exact; PR: 1e-4 relative).  Swept, per kernel:
          then relax_grid in {256, 512, 1024, 2048} workgroups
   sssp batch  bucket width in {64, 256, 1024, 4096, 2^20}
   bfs    bu_group in {1, 2, 4, 8, 16} x (alpha, beta) grid (bfs.cc:165)
+  pr     Gauss-Seidel row blocks in {1, 2, 4, 8, 16} x in place / published
+         (converged vectors within 2e-3 in L1 of the default's: each is within
+         about 5.7 x tol of the fixed point at damping 0.85)
+  bc batch  forward direction rule: top-down, bottom-up, or by frontier
+         edges with ratio {1, 2, 4}
   all    heavy_degree in {1024, 2048, 4096, 8192}          (rows per workgroup)
 
 Results are cached per (graph shape, kernel) in a JSON file so that a
@@ -32,6 +37,8 @@ ALPHA_BETA = ((15, 18), (8, 18), (30, 18), (15, 9), (15, 36))
 HEAVY = (1024, 2048, 4096, 8192)
 GRIDS = (256, 512, 1024, 2048)
 BATCH_DELTAS = (64, 256, 1024, 4096, 1 << 20)
+PR_BLOCKS = (1, 2, 4, 8, 16)
+BC_PULL = ((0, 1), (2, 1), (1, 1), (1, 2), (1, 4))   # (bc_pull, bc_pull_ratio)
 
 
 def _median(xs: Sequence[float]) -> float:
@@ -123,6 +130,65 @@ class Autotuner:
         self._save()
         return best
 
+    def tune_pr(self, g: gbk.DeviceGraph, tol: float = 1e-4) -> dict:
+        """Row blocks and in-place updates of the Gauss-Seidel PageRank."""
+        k = self.key(g, "pr")
+        if k in self.cache:
+            return self.cache[k]
+        saved = (self.ctx.get_param("pr_blocks"), self.ctx.get_param("pr_inplace"))
+        ref = self.ctx.pr(g, gbk.PR_GS, tol=tol)[0]
+        best = None
+        try:
+            for blocks, inplace in itertools.product(PR_BLOCKS, (0, 1)):
+                self.ctx.set_param("pr_blocks", blocks)
+                self.ctx.set_param("pr_inplace", inplace)
+                out, st = self.ctx.pr(g, gbk.PR_GS, tol=tol)
+                if st.final_error >= tol:
+                    continue  # not converged within the iteration cap
+                if float(np.abs(out.astype(np.float64) - ref).sum()) > 20 * tol:
+                    raise AssertionError(f"pr blocks={blocks} inplace={inplace} left the fixed point")
+                ms = _median([self.ctx.pr(g, gbk.PR_GS, tol=tol, download=False)[1].device_ms
+                              for _ in range(self.repeats)])
+                self.log.append({"kernel": "pr", "pr_blocks": blocks, "pr_inplace": inplace, "ms": ms})
+                if best is None or ms < best["ms"]:
+                    best = {"pr_blocks": blocks, "pr_inplace": inplace, "ms": ms}
+        finally:
+            self.ctx.set_param("pr_blocks", saved[0])
+            self.ctx.set_param("pr_inplace", saved[1])
+        self.cache[k] = best
+        self._save()
+        return best
+
+    def tune_bc_batch(self, g: gbk.DeviceGraph, sources: Iterable[int]) -> dict:
+        """Direction rule of the BC batch's forward levels (scores 1e-4)."""
+        k = self.key(g, "bc_batch")
+        if k in self.cache:
+            return self.cache[k]
+        srcs = np.asarray([int(s) for s in sources], np.int32)
+        saved = (self.ctx.get_param("bc_pull"), self.ctx.get_param("bc_pull_ratio"))
+        best = None
+        ref = None
+        try:
+            for pull, ratio in BC_PULL:
+                self.ctx.set_param("bc_pull", pull)
+                self.ctx.set_param("bc_pull_ratio", ratio)
+                out = self.ctx.bc_batch(g, srcs)[0]
+                if ref is None:
+                    ref = out
+                else:
+                    np.testing.assert_allclose(out, ref, rtol=1e-4, atol=1e-6)
+                ms = _median([self.ctx.bc_batch(g, srcs, download=False)[1].device_ms
+                              for _ in range(self.repeats)])
+                self.log.append({"kernel": "bc_batch", "bc_pull": pull, "bc_pull_ratio": ratio, "ms": ms})
+                if best is None or ms < best["ms"]:
+                    best = {"bc_pull": pull, "bc_pull_ratio": ratio, "ms": ms}
+        finally:
+            self.ctx.set_param("bc_pull", saved[0])
+            self.ctx.set_param("bc_pull_ratio", saved[1])
+        self.cache[k] = best
+        self._save()
+        return best
+
     def tune_bfs(self, g: gbk.DeviceGraph, sources: Iterable[int]) -> dict:
         k = self.key(g, "bfs")
         if k in self.cache:
@@ -172,7 +238,8 @@ class Autotuner:
         return best
 
     def apply(self, tuned: dict) -> None:
-        for name in ("bu_group", "heavy_degree", "relax_grid"):
+        for name in ("bu_group", "heavy_degree", "relax_grid", "pr_blocks", "pr_inplace", "bc_pull",
+                     "bc_pull_ratio"):
             if name in tuned:
                 self.ctx.set_param(name, tuned[name])
         if "alpha" in tuned:

@@ -163,3 +163,44 @@ def test_autotuner_sssp_batch(ctx):
     for j, s in enumerate(srcs):
         assert np.array_equal(dist[j], O.dijkstra(g, int(s)))
     dg.free()
+
+
+def test_autotuner_pr_blocks(ctx):
+    """Every block layout that converges is timed and its vector checked
+    against the default's; the pick leaves the parameters restored."""
+    from graphbrew_amd.autotune import PR_BLOCKS
+    g = O.kron(13)
+    dg = ctx.upload_graph(g)
+    t = Autotuner(ctx, repeats=1)
+    best = t.tune_pr(dg)
+    assert best["pr_blocks"] in PR_BLOCKS and best["pr_inplace"] in (0, 1)
+    assert 1 <= len([r for r in t.log if r["kernel"] == "pr"]) <= 2 * len(PR_BLOCKS)
+    assert ctx.get_param("pr_blocks") == 0 and ctx.get_param("pr_inplace") == 0
+    t.apply(best)
+    try:
+        s, st = ctx.pr(dg, gbk.PR_GS)
+        assert st.final_error < 1e-4 and O.pr_residual(g, s) < 1e-4
+    finally:
+        ctx.set_param("pr_blocks", 0)
+        ctx.set_param("pr_inplace", 0)
+    dg.free()
+
+
+def test_autotuner_bc_batch_direction(ctx):
+    from graphbrew_amd.autotune import BC_PULL
+    g = O.kron(12)
+    dg = ctx.upload_graph(g)
+    t = Autotuner(ctx, repeats=1)
+    srcs = O.source_picker(g, 8)[1]
+    best = t.tune_bc_batch(dg, srcs)
+    assert (best["bc_pull"], best["bc_pull_ratio"]) in BC_PULL
+    assert len([r for r in t.log if r["kernel"] == "bc_batch"]) == len(BC_PULL)
+    assert ctx.get_param("bc_pull") == 1 and ctx.get_param("bc_pull_ratio") == 1
+    t.apply(best)
+    try:
+        scores = ctx.bc_batch(dg, srcs)[0]
+    finally:
+        ctx.set_param("bc_pull", 1)
+        ctx.set_param("bc_pull_ratio", 1)
+    np.testing.assert_allclose(scores, O.bc_serial(g, srcs), rtol=1e-4, atol=1e-6)
+    dg.free()
