@@ -1665,8 +1665,9 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
 int bc_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, float *scores,
                    gbk_batch_stats *acc) {
   // up to 4 sources take 16 B depth rows: half the footprint of width 8, and
-  // 16 edges in flight per wave
-  switch (K <= 4 ? 4 : batch_width(K)) {
+  // 16 edges in flight per wave; 17..32 sources take 64-lane rows, whose
+  // wave-uniform walk beats two 32-lane groups per wave
+  switch (K <= 4 ? 4 : K > 16 ? 64 : batch_width(K)) {
     case 4: return bc_batch_run<4>(ctx, g, src, K, scores, acc);
     case 8: return bc_batch_run<8>(ctx, g, src, K, scores, acc);
     case 16: return bc_batch_run<16>(ctx, g, src, K, scores, acc);
