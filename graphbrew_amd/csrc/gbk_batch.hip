@@ -1440,7 +1440,8 @@ int sssp_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K
                      gbk_batch_stats *acc) {
   const int64_t n = g->n;
   hipStream_t s = ctx->stream;
-  const int Kp = batch_width(K);
+  // 17..32 sources take 64-lane rows too when the wave-uniform walk wins
+  const int Kp = K > 16 && ctx->tune.sssp_wide ? kWave : batch_width(K);
   const int64_t cap = n + 64;
   auto *dist = static_cast<int32_t *>(scratch(ctx, kS0, sizeof(int32_t) * n * Kp + 64));
   auto *pend = static_cast<int32_t *>(scratch(ctx, kS1, sizeof(int32_t) * (n + 1)));

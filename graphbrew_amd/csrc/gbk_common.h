@@ -45,6 +45,7 @@ struct Tuning {
   int bc_pull = 1;              // BC batch forward levels: 0 top-down, 1 bottom-up when the
                                 // frontier has more out-edges than the unlisted vertices, 2 bottom-up
   int bc_pull_ratio = 1;        // a frontier edge weighs this many open edges in that choice
+  int sssp_wide = 1;            // SSSP batches of 17..32 sources on 64-lane rows
 };
 constexpr int kGridCapRuntime = 2048;  // upper bound of relax_grid
 constexpr int64_t kHeavyChunkEdges = 1024;  // heavy-row chunk of the all-rows kernels
