@@ -887,10 +887,6 @@ __global__ void bcb_sources(const int32_t *src, int K, int Kp, int32_t *depth, d
   }
 }
 
-__device__ __forceinline__ void bcb_queue_heavy(const HeavyQueue &hq, int32_t row, int64_t s, int64_t len) {
-  heavy_push(hq, row, s, len);
-}
-
 // Heavy chunk c of the queue: its row and edge range (binary search in the
 // chunk prefix).
 __device__ __forceinline__ void bcb_chunk(const HeavyQueue &hq, int64_t rows, int64_t c, int64_t &h,
@@ -941,7 +937,7 @@ bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigm
     f.act = valid && depth[row * Kp + f.k] == d;
     f.su = f.act ? sigma[row * Kp + f.k] : 0.0;
     const bool heavy = len > kBatchHeavy;
-    if (heavy && f.leader) bcb_queue_heavy(hq, static_cast<int32_t>(row), s, len);
+    if (heavy && f.leader) heavy_push(hq, static_cast<int32_t>(row), s, len);
     bcb_edges<Kp>(nbr, s, heavy ? 0 : len, f);
     if (f.leader && !heavy) walked += static_cast<unsigned long long>(len);
   }
@@ -1206,7 +1202,7 @@ bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const
     }
     f.act = valid && depth[row * Kp + f.k] == d;
     const bool heavy = len > kBatchHeavy;
-    if (heavy && leader) bcb_queue_heavy(hq, static_cast<int32_t>(row), s, len);
+    if (heavy && leader) heavy_push(hq, static_cast<int32_t>(row), s, len);
     f.acc = 0.0;
     bcb_edges<Kp>(nbr, s, heavy ? 0 : len, f);
     double delta = 0.0;

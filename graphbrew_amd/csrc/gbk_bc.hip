@@ -109,7 +109,7 @@ bc_forward_level(const int64_t *off, const int32_t *nbr, int32_t *depth, double 
     l = off[u + 1] - s;
   };
   expand_list_light(queue + lb, le - lb, heavy_deg, rowlen, f, hq);
-  f.q.flush();
+  f.q.flush_block();
 }
 
 // Heavy rows of the level, spread over all workgroups.
@@ -127,7 +127,7 @@ bc_forward_heavy(const int64_t *off, const int32_t *nbr, int32_t *depth, double 
   f.edge_base = off[0];
   f.d = d;
   expand_heavy(hq, f);
-  f.q.flush();
+  f.q.flush_block();
 }
 
 // One level of the dependency accumulation; coef[v] = (1 + delta[v]) /

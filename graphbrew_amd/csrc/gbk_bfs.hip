@@ -77,7 +77,7 @@ bfs_td_step(const int64_t *out_off, const int32_t *out_nbr, int32_t *parent, int
     l = out_off[u + 1] - s;
   };
   expand_list_light(queue + qb, qe - qb, heavy_deg, rowlen, f, hq);
-  f.q.flush();
+  f.q.flush_block();
   block_partial(f.scout, partial);
 }
 
@@ -92,7 +92,7 @@ bfs_td_heavy(const int32_t *out_nbr, int32_t *parent, int32_t *queue, int64_t ca
   f.scout = 0;
   f.q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
   expand_heavy(hq, f);
-  f.q.flush();
+  f.q.flush_block();
   block_partial(f.scout, partial);
 }
 
@@ -153,7 +153,7 @@ bfs_bitmap_to_queue(const uint32_t *bm, int64_t n, int32_t *queue, int64_t cap, 
     const bool set = v < n && bit_get(bm, v);
     q.push(set, static_cast<int32_t>(v));
   }
-  q.flush();
+  q.flush_block();
 }
 
 // Writes the summed partials (one or two ranges) into a counter (one workgroup).

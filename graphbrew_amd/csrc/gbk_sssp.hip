@@ -148,7 +148,7 @@ sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, unsigned 
     l = word_dist(word[u]) >= lower ? off[u + 1] - s : 0;
   };
   expand_list_light_batch<kRelaxBatch>(cur ? near1 : near0, count, heavy_deg, rowlen, f, hq);
-  f.nearq.flush();
+  f.nearq.flush_block();
 }
 
 // Heavy rows of the near queue, spread over all workgroups.
@@ -161,7 +161,7 @@ sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, unsigned long long *wor
   RelaxEdge f;
   relax_setup(f, seg, nbr, wgt, word, near0, near1, cap, delta, st);
   expand_heavy_batch<kRelaxBatch>(hq, f);
-  f.nearq.flush();
+  f.nearq.flush_block();
 }
 
 // Work of the coming relaxation round, for the kernel timer only: rows of
@@ -273,7 +273,7 @@ sssp_split(const unsigned long long *word, int64_t n, int32_t *near0, int32_t *n
     }
     q.push(take, static_cast<int32_t>(v));
   }
-  q.flush();
+  q.flush_block();
 }
 
 __global__ void sssp_split_done(SsspState *st) {
