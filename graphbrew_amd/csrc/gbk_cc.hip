@@ -95,6 +95,26 @@ struct HookEdge {
     const int32_t low = cu + (cv - high);
     if (high == ld_agent(&comp[high]) && atomicCAS(&comp[high], high, low) == high) changed = 1;
   }
+  // U edges per lane: their neighbour ids and labels are all in flight
+  // before the first comparison (the CAS still decides every hook)
+  template <int U>
+  __device__ __forceinline__ void batch(const int64_t (&u)[U], const int64_t (&k)[U], const bool (&act)[U]) {
+    int32_t v[U], cu[U], cv[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) v[j] = act[j] ? nbr[k[j]] : 0;
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      cu[j] = ld_agent(&comp[u[j]]);
+      cv[j] = ld_agent(&comp[v[j]]);
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      if (!act[j] || cu[j] == cv[j]) continue;
+      const int32_t high = cu[j] > cv[j] ? cu[j] : cv[j];
+      const int32_t low = cu[j] + (cv[j] - high);
+      if (high == ld_agent(&comp[high]) && atomicCAS(&comp[high], high, low) == high) changed = 1;
+    }
+  }
 };
 
 // SV hook over every edge (cc_sv.cc:86-106).
@@ -105,7 +125,7 @@ cc_sv_hook(RowTiles rt, const int32_t *nbr, int32_t *comp, int *change) {
     s = rt.off[u];
     l = rt.off[u + 1] - s;
   };
-  expand_rows(rt, rowlen, f);
+  expand_rows_batch<4>(rt, rowlen, f);
   if (f.changed) *change = 1;
 }
 

@@ -395,6 +395,38 @@ __device__ void expand_rows(const RowTiles &rt, RowLen rowlen, F &f) {
   }
 }
 
+// expand_rows with the batched walk on the light tiles (the functor's
+// batch<U>() gets U edges per lane per trip; heavy chunks call it edge by
+// edge as expand_rows does).
+template <int U, class RowLen, class F>
+__device__ void expand_rows_batch(const RowTiles &rt, RowLen rowlen, F &f) {
+  const int64_t chunks = rt.nheavy ? rt.hprefix[rt.nheavy] : 0;
+  for (int64_t gc = blockIdx.x; gc < chunks; gc += gridDim.x) {
+    int64_t h, first, last;
+    heavy_chunk_of(rt, gc, h, first, last);
+    const int64_t u = rt.heavy[h];
+    int64_t s, l;
+    rowlen(u, s, l);
+    const int64_t hi = min(last, l);
+    for (int64_t base = first; base < hi; base += blockDim.x) {
+      const int64_t e = base + threadIdx.x;
+      const bool active = e < hi;
+      f(u, active ? s + e : 0, active);
+    }
+  }
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave; t < rt.ntiles;
+       t += waves) {
+    const int64_t r = rt.tiles[t] + lane_id();
+    int64_t start = 0, len = 0;
+    if (r < rt.tiles[t + 1]) {
+      const int64_t deg = rt.off[r + 1] - rt.off[r];
+      if (deg <= rt.heavy_deg) rowlen(r, start, len);
+    }
+    wave_walk_batch<U>(r < rt.tiles[t + 1] ? r : rt.tiles[t], start, len, f);
+  }
+}
+
 // Same over all rows [0, n) (row = list index).
 template <class RowLen, class F>
 __device__ void expand_all(int64_t n, int64_t heavy_deg, RowLen rowlen, F &f) {
