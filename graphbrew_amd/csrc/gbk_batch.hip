@@ -389,6 +389,8 @@ static void launch_msbfs_bu(gbk_ctx *ctx, int G, const gbk_graph *g, const unsig
     case 2: msbfs_bu<2><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
     case 4: msbfs_bu<4><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
     case 16: msbfs_bu<16><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
+    case 32: msbfs_bu<32><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
+    case 64: msbfs_bu<64><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
     default: msbfs_bu<8><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
   }
 }
@@ -1398,7 +1400,7 @@ int msbfs_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, int
   for (int32_t level = 1; level <= n + 1; level++) {
     const int kt = kt_begin(ctx, GBK_KT_BFS_STEP);
     if (fedges > g->m / alpha) {
-      launch_msbfs_bu(ctx, 8, g, seen, front, next, full, bu_grid);
+      launch_msbfs_bu(ctx, ctx->tune.msbfs_group, g, seen, front, next, full, bu_grid);
     } else {
       fill<unsigned long long>(ctx, hq.count, 0ull, 1);
       msbfs_front_queue<<<n_grid, kBlock, 0, s>>>(front, n, queue, cap, c);

@@ -115,7 +115,8 @@ int gbk_ctx_sync(gbk_ctx *ctx);
  * top-down only, 1 (default) bottom-up from in-neighbours when the frontier's out-edges outweigh
  * those of the vertices some source has not reached, 2 bottom-up at every level); "bc_pull_ratio"
  * (1..1024, default 1: how many such edges one frontier edge weighs in that choice); "sssp_wide"
- * (1, default: SSSP batches of 17..32 sources run on 64-lane rows; 0: 32-lane rows); and
+ * (1, default: SSSP batches of 17..32 sources run on 64-lane rows; 0: 32-lane rows); "msbfs_group"
+ * (lanes per vertex in the BFS batch's bottom-up step: 1, 2, 4, ..., 64; default 32); and
  * "ktime" (a gbk_ktime_kernel) selecting the kernel whose
  * launches are timed one by one with HIP events (loops then run eagerly).  Returns
  * GBK_EINVAL for an unknown name or bad value.  No reference counterpart:

@@ -250,3 +250,21 @@ def test_batch_counts_without_download_equal(ctx, k):
     _, b1 = ctx.bc_batch(dg, src)
     _, b2 = ctx.bc_batch(dg, src, download=False)
     assert b1.edges_reached == b2.edges_reached
+
+
+@pytest.mark.parametrize("group", [1, 8, 32, 64])
+def test_bfs_batch_bottom_up_groups(ctx, group):
+    """Every lanes-per-vertex width of the batch's bottom-up step gives the
+    oracle's depths for every source (the step only changes who ORs which
+    in-neighbour words)."""
+    g = O.kron(13)
+    dg = ctx.upload_graph(g)
+    srcs = _sources(g, 64, seed=400)
+    ctx.set_param("msbfs_group", group)
+    try:
+        depth = ctx.bfs_batch(dg, srcs)[0]
+    finally:
+        ctx.set_param("msbfs_group", 32)
+    for j, s in enumerate(srcs):
+        assert np.array_equal(depth[j], O.bfs_depths(g, int(s))), (group, j)
+    dg.free()
