@@ -45,7 +45,7 @@ constexpr int64_t kBatchChunk = 256;
 // BC batch: edges in flight per trip, and the waves per SIMD its kernels'
 // registers are capped for (64 lanes: fewer registers, more waves measured
 // faster; narrower groups keep their registers).
-template <int Kp> constexpr int kBcbUnroll = Kp == 64 ? 4 : 8;
+template <int Kp> constexpr int kBcbUnroll = Kp == 64 ? 4 : Kp <= 8 ? 16 : 8;
 template <int Kp> constexpr int kBcbWaves = Kp == 64 ? 8 : 1;
 constexpr int kRelaxUnroll = 4;   // edges in flight per trip (SSSP batch: fewer registers, more waves)
 
