@@ -195,7 +195,7 @@ class Workload:
 
 # Kernels followed one launch at a time (HIP events recorded by the library
 # on its stream around each launch) for the roofline of the dominant family.
-def kt_table(gbk, single):
+def kt_table(gbk, single, msbfs_group=32):
     t = {
         "sssp": (gbk.KT_SSSP_RELAX, "sssp_relax+sssp_relax_heavy (one relaxation round)"),
         "pr": (gbk.KT_PR_ITER, "pr_pull_block + pr_block_close per row block (one Gauss-Seidel sweep)"),
@@ -205,8 +205,8 @@ def kt_table(gbk, single):
     if not single:
         t["sssp"] = (gbk.KT_SSSP_RELAX, "sb_relax<64>+sb_relax_heavy<64> "
                                         "(one relaxation round of a 64-source batch)")
-        t["bfs"] = (gbk.KT_BFS_STEP, "msbfs_front_queue+msbfs_td+msbfs_td_heavy | msbfs_bu<8> "
-                                     "(one level of a 64-source batch)")
+        t["bfs"] = (gbk.KT_BFS_STEP, "msbfs_front_queue+msbfs_td+msbfs_td_heavy | "
+                                     f"msbfs_bu<{msbfs_group}> (one level of a 64-source batch)")
         t["bc"] = (gbk.KT_BC_LEVEL, "bcb_forward+bcb_forward_heavy | bcb_backward+"
                                     "bcb_backward_heavy+bcb_backward_finish (one level of a source batch)")
     return t
@@ -427,7 +427,7 @@ def main():
         log(f"autotune: {tuned}")
         for row in tuner.log:
             log(f"  {row}")
-    kts = kt_table(gbk, args.single_source)
+    kts = kt_table(gbk, args.single_source, ctx.get_param("msbfs_group"))
     if wrec:
         family = max((f for f in wrec if f in kts), key=lambda f: wrec[f][0], default=None)
     else:

@@ -33,6 +33,7 @@ from . import gbk
 
 DELTAS = (1, 8, 32, 64, 128, 256, 512)
 BU_GROUPS = (1, 2, 4, 8, 16)
+MSBFS_GROUPS = (4, 8, 16, 32, 64)
 ALPHA_BETA = ((15, 18), (8, 18), (30, 18), (15, 9), (15, 36))
 HEAVY = (1024, 2048, 4096, 8192)
 GRIDS = (256, 512, 1024, 2048)
@@ -189,6 +190,34 @@ class Autotuner:
         self._save()
         return best
 
+    def tune_bfs_batch(self, g: gbk.DeviceGraph, sources: Iterable[int],
+                       groups: Sequence[int] = MSBFS_GROUPS) -> dict:
+        """Lanes per vertex of the BFS batch's bottom-up step (msbfs_group);
+        every candidate's depths must equal those of the current setting."""
+        k = self.key(g, "bfs_batch")
+        if k in self.cache:
+            return self.cache[k]
+        srcs = np.asarray([int(s) for s in sources], np.int32)
+        saved = self.ctx.get_param("msbfs_group")
+        ref = self.ctx.bfs_batch(g, srcs)[0]
+        best = None
+        try:
+            for grp in groups:
+                self.ctx.set_param("msbfs_group", grp)
+                out = self.ctx.bfs_batch(g, srcs)[0]
+                if not np.array_equal(out, ref):
+                    raise AssertionError(f"bfs batch msbfs_group={grp} changed depths")
+                ms = _median([self.ctx.bfs_batch(g, srcs, download=False)[3].device_ms
+                              for _ in range(self.repeats)])
+                self.log.append({"kernel": "bfs_batch", "msbfs_group": grp, "ms": ms})
+                if best is None or ms < best["ms"]:
+                    best = {"msbfs_group": grp, "ms": ms}
+        finally:
+            self.ctx.set_param("msbfs_group", saved)
+        self.cache[k] = best
+        self._save()
+        return best
+
     def tune_bfs(self, g: gbk.DeviceGraph, sources: Iterable[int]) -> dict:
         k = self.key(g, "bfs")
         if k in self.cache:
@@ -239,7 +268,7 @@ class Autotuner:
 
     def apply(self, tuned: dict) -> None:
         for name in ("bu_group", "heavy_degree", "relax_grid", "pr_blocks", "pr_inplace", "bc_pull",
-                     "bc_pull_ratio"):
+                     "bc_pull_ratio", "msbfs_group"):
             if name in tuned:
                 self.ctx.set_param(name, tuned[name])
         if "alpha" in tuned:

@@ -1388,7 +1388,8 @@ int msbfs_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, int
   if (int rc = upload_sources(ctx, src, K, &dsrc)) return rc;
   const unsigned long long full = K == 64 ? ~0ull : ((1ull << K) - 1);
   const unsigned n_grid = expand_grid(n);
-  const unsigned bu_grid = expand_grid((n + kBlock / 8 - 1) / (kBlock / 8) * kBlock);
+  // the bottom-up step gives each vertex msbfs_group lanes (grid-stride beyond the cap)
+  const unsigned bu_grid = expand_grid(n * static_cast<int64_t>(ctx->tune.msbfs_group));
   const int64_t alpha = ctx->tune.bfs_alpha;
   EventTimer timer(ctx);
   fill<unsigned long long>(ctx, seen, 0ull, n);

@@ -219,6 +219,9 @@ int set_error(gbk_ctx *ctx, int code, const std::string &msg);
 // exactly, with no second round of workgroups.
 unsigned resident_grid(gbk_ctx *ctx, const void *kernel);
 void *scratch(gbk_ctx *ctx, int slot, size_t bytes);   // nullptr on failure
+// Pinned host staging for counters read back by entry points: one fixed
+// area of kPinnedBytes per context, so pointers stay valid for its lifetime.
+constexpr size_t kPinnedBytes = 64 * 1024;
 void *pinned(gbk_ctx *ctx, size_t bytes);
 int check_hip(gbk_ctx *ctx, hipError_t e, const char *what);
 

@@ -56,13 +56,19 @@ unsigned resident_grid(gbk_ctx *ctx, const void *kernel) {
   return grid;
 }
 
+// One fixed staging area per context, allocated on first use and never
+// reallocated: a pointer handed out earlier in an entry point stays valid
+// after later, larger requests (callers hold counters across helpers that
+// also stage through it).  Requests above kPinnedBytes fail.
 void *pinned(gbk_ctx *ctx, size_t bytes) {
-  if (ctx->pinned_bytes >= bytes) return ctx->pinned;
-  if (ctx->pinned) hipHostFree(ctx->pinned);
-  ctx->pinned = nullptr;
-  ctx->pinned_bytes = 0;
-  if (hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
-  ctx->pinned_bytes = bytes;
+  if (bytes > kPinnedBytes) return nullptr;
+  if (!ctx->pinned) {
+    if (hipHostMalloc(&ctx->pinned, kPinnedBytes, hipHostMallocDefault) != hipSuccess) {
+      ctx->pinned = nullptr;
+      return nullptr;
+    }
+    ctx->pinned_bytes = kPinnedBytes;
+  }
   return ctx->pinned;
 }
 
@@ -560,6 +566,20 @@ __global__ void iota_i32(int32_t *a, int64_t n) {
   if (i < n) a[i] = static_cast<int32_t>(i);
 }
 
+// A caller's CSR is checked on the host before anything reaches the device:
+// offsets start at 0 and never decrease, and every neighbour id lies in
+// [0, n).  Every kernel indexes per-vertex arrays by neighbour id, so one
+// bad entry would otherwise be an out-of-bounds device access.
+static const char *check_csr(int64_t n, const int64_t *off, const int32_t *nbr) {
+  if (off[0] != 0) return "offsets must start at 0";
+  for (int64_t i = 0; i < n; i++)
+    if (off[i + 1] < off[i]) return "offsets decrease";
+  const int64_t m = off[n];
+  for (int64_t k = 0; k < m; k++)
+    if (nbr[k] < 0 || nbr[k] >= n) return "neighbour id out of range";
+  return nullptr;
+}
+
 extern "C" int gbk_graph_upload(gbk_ctx *ctx, int64_t n, int directed, const int64_t *off,
                                 const int32_t *nbr, const int32_t *wgt, const int64_t *in_off,
                                 const int32_t *in_nbr, const int32_t *in_wgt,
@@ -569,10 +589,13 @@ extern "C" int gbk_graph_upload(gbk_ctx *ctx, int64_t n, int directed, const int
   if (off[0] != 0) return set_error(ctx, GBK_EINVAL, "upload: off[0] must be 0");
   const int64_t m = off[n];
   if (m < 0 || (m > 0 && !nbr)) return set_error(ctx, GBK_EINVAL, "upload: bad edge count");
+  if (const char *bad = check_csr(n, off, nbr)) return set_error(ctx, GBK_EINVAL, std::string("upload: ") + bad);
   int64_t m_in = m;
   if (directed) {
     if (!in_off || (in_off[n] > 0 && !in_nbr) || in_off[0] != 0)
       return set_error(ctx, GBK_EINVAL, "upload: directed graph needs its in-CSR");
+    if (const char *bad = check_csr(n, in_off, in_nbr))
+      return set_error(ctx, GBK_EINVAL, std::string("upload: in-CSR ") + bad);
     m_in = in_off[n];
     if (wgt && !in_wgt) return set_error(ctx, GBK_EINVAL, "upload: in-CSR weights missing");
   }

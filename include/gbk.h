@@ -136,7 +136,9 @@ int gbk_ktime_read(gbk_ctx *ctx, double *total_ms, int64_t *launches,
  * off[0] == 0; rows sorted.  wgt may be NULL (unweighted).  For directed
  * graphs the in-CSR (in_off/in_nbr[/in_wgt]) is required; for undirected
  * graphs pass NULL and the out arrays serve as in arrays.  org_ids (length
- * n, new id -> original id) may be NULL for the identity. */
+ * n, new id -> original id) may be NULL for the identity.  Offsets that do
+ * not start at 0 or that decrease, and neighbour ids outside [0, n), return
+ * GBK_EINVAL before anything is copied (checked on the host). */
 int gbk_graph_upload(gbk_ctx *ctx, int64_t n, int directed,
                      const int64_t *off, const int32_t *nbr, const int32_t *wgt,
                      const int64_t *in_off, const int32_t *in_nbr,

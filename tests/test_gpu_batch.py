@@ -268,3 +268,20 @@ def test_bfs_batch_bottom_up_groups(ctx, group):
     for j, s in enumerate(srcs):
         assert np.array_equal(depth[j], O.bfs_depths(g, int(s))), (group, j)
     dg.free()
+
+
+def test_bc_batch_first_call_on_new_context():
+    """A BC batch as the first call on a fresh context: the counter staging it
+    reads after the per-source count pass stays valid (the staging area is
+    never reallocated), and the scores equal serial Brandes."""
+    g = O.kron(12)
+    c = gbk.Context(0)
+    try:
+        dg = c.upload_graph(g)
+        srcs = _sources(g, 4, seed=17)
+        scores, st = c.bc_batch(dg, srcs)
+        np.testing.assert_allclose(scores, O.bc_serial(g, srcs), rtol=1e-4, atol=1e-6)
+        assert st.rounds > 0
+        dg.free()
+    finally:
+        c.close()

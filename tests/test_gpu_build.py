@@ -52,3 +52,28 @@ def test_build_duplicates_keep_smallest_weight(ctx):
     for sym in (True, False):
         dg, _ = ctx.build(u, v, w, symmetrize=sym)
         _same(dg, O.make_graph(u, v, w, symmetrize=sym))
+
+
+@pytest.mark.parametrize("case", ["bad_id", "negative_id", "decreasing", "bad_in_id"])
+def test_upload_rejects_malformed_csr(ctx, case):
+    """gbk_graph_upload checks a caller's CSR on the host before any kernel
+    sees it: out-of-range neighbour ids and decreasing offsets are EINVAL."""
+    from graphbrew_amd import gbk
+    off = np.array([0, 2, 3, 4], np.int64)
+    nbr = np.array([1, 2, 0, 0], np.int32)
+    in_off, in_nbr, directed = None, None, False
+    if case == "bad_id":
+        nbr[2] = 3
+    elif case == "negative_id":
+        nbr[0] = -1
+    elif case == "decreasing":
+        off = np.array([0, 3, 2, 4], np.int64)
+    else:
+        directed = True
+        in_off = np.array([0, 2, 3, 4], np.int64)
+        in_nbr = np.array([1, 2, 0, 7], np.int32)
+    with pytest.raises(gbk.GbkError) as e:
+        ctx.upload(3, off, nbr, in_off=in_off, in_nbr=in_nbr, directed=directed)
+    assert e.value.code == gbk.GBK_EINVAL
+    ok = ctx.upload(3, np.array([0, 2, 3, 4], np.int64), np.array([1, 2, 0, 0], np.int32))
+    ok.free()

@@ -204,3 +204,25 @@ def test_autotuner_bc_batch_direction(ctx):
         ctx.set_param("bc_pull_ratio", 1)
     np.testing.assert_allclose(scores, O.bc_serial(g, srcs), rtol=1e-4, atol=1e-6)
     dg.free()
+
+
+def test_autotuner_bfs_batch_group(ctx):
+    """Every bottom-up width of the BFS batch gives the oracle's depths; the
+    sweep restores the parameter and the pick applies."""
+    from graphbrew_amd.autotune import MSBFS_GROUPS
+    g = O.kron(13)
+    dg = ctx.upload_graph(g)
+    t = Autotuner(ctx, repeats=1)
+    srcs = O.source_picker(g, 64)[1]
+    best = t.tune_bfs_batch(dg, srcs)
+    assert best["msbfs_group"] in MSBFS_GROUPS
+    assert len([r for r in t.log if r["kernel"] == "bfs_batch"]) == len(MSBFS_GROUPS)
+    assert ctx.get_param("msbfs_group") == 32
+    t.apply(best)
+    try:
+        depth = ctx.bfs_batch(dg, srcs)[0]
+    finally:
+        ctx.set_param("msbfs_group", 32)
+    for j in (0, 17, 63):
+        assert np.array_equal(depth[j], O.bfs_depths(g, int(srcs[j])))
+    dg.free()
