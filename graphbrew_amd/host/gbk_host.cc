@@ -101,8 +101,11 @@ int gbh_generate_el(int scale, int degree, int uniform, int32_t *u, int32_t *v) 
       std::mt19937 rng(kSeed + static_cast<uint32_t>(sidx));
       const int64_t e1 = std::min(m, (sidx + 1) * kEdgesPerStream);
       for (int64_t e = sidx * kEdgesPerStream; e < e1; e++) {
-        u[e] = static_cast<int32_t>(draw(rng));
+        // Edge(udist(), udist()) (generator.h:112): the reference's GCC build
+        // evaluates the constructor arguments right to left, so the
+        // destination takes the first draw of each edge.
         v[e] = static_cast<int32_t>(draw(rng));
+        u[e] = static_cast<int32_t>(draw(rng));
       }
     }
     return 0;

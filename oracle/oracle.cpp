@@ -135,11 +135,10 @@ int orc_generate_el(int scale, int degree, int uniform, int32_t *u,
         rng.seed(static_cast<uint32_t>(kRandSeed + block / kGenBlock));
         const int64_t end = std::min(block + kGenBlock, m);
         for (int64_t e = block; e < end; e++) {
-          // The two draws of one edge; which one is the source does not
-          // matter because generated graphs are always symmetrized
-          // (command_line.h:114-115).
-          u[e] = static_cast<int32_t>(udist());
+          // Edge(udist(), udist()) (generator.h:112): GCC evaluates the
+          // arguments right to left, so the destination is drawn first.
           v[e] = static_cast<int32_t>(udist());
+          u[e] = static_cast<int32_t>(udist());
         }
       }
     }

@@ -43,8 +43,11 @@ def _uniform_el_python(scale, degree):
     for b in range(0, m, block):
         rng = B.MT19937(27491095 + b // block)
         for _ in range(b, min(b + block, m)):
-            us.append(rng() % n)   # n is a power of two: no rejection cutoff
-            vs.append(rng() % n)
+            # Edge(udist(), udist()): the reference's GCC build evaluates the
+            # arguments right to left, so the destination takes the first draw
+            # (the reference compiled here shows it; tests/golden/ref_vectors.json)
+            vs.append(rng() % n)   # n is a power of two: no rejection cutoff
+            us.append(rng() % n)
     return np.array(us, np.int32), np.array(vs, np.int32)
 
 
@@ -125,8 +128,8 @@ def test_generator_block_reseed_beyond_first_block():
     # 2^18 edges per block: scale 15 x 16 = 2 blocks (pins the reseeding).
     u, v = O.generate_el(15, 16, uniform=True)
     rng = B.MT19937(27491095 + 1)
-    first = (rng() % 32768, rng() % 32768)
-    assert (int(u[1 << 18]), int(v[1 << 18])) == first
+    first_v, first_u = rng() % 32768, rng() % 32768  # destination drawn first
+    assert (int(u[1 << 18]), int(v[1 << 18])) == (first_u, first_v)
 
 
 # ------------------------------------------------------------------ CSR build
