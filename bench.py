@@ -293,16 +293,10 @@ def many_graphs(ctx, gbk, count):
     'count' seeded R-MAT graphs of uneven size (scales 10-16, edge factor
     8-32, varied a/b/c), once as one union batch and once one graph per
     call; device times from the library's hipEvents."""
-    from graphbrew_amd import selector as S
-    rng = np.random.default_rng(20240)
+    from graphbrew_amd import workloads as WL
     graphs, edges = [], 0
     t0 = time.time()
-    for i in range(count):
-        scale = int(rng.integers(10, 17))
-        ef = int(rng.choice([8, 16, 32]))
-        a = float(rng.uniform(0.45, 0.6))
-        b = c = float(rng.uniform(0.1, (1 - a) / 2 - 0.02))
-        (u, v), n = S.rmat(scale, ef, a, b, c, seed=5000 + i, permute=True)
+    for u, v, n in WL.many_small_graphs(count):
         g, _ = ctx.build(u, v, None, n, True)
         graphs.append(g)
         edges += g.m
