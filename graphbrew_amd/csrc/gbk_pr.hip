@@ -38,6 +38,9 @@
 // launches return at once -- no host synchronisation inside the iteration
 // loop.
 #include <algorithm>
+#include <mutex>
+#include <optional>
+#include <string>
 
 #include "gbk_expand.h"
 
@@ -255,6 +258,184 @@ pr_block_close(RowTiles rt, const PrBlock *blocks, int b, int64_t row_hi, const 
   block_partial(err, partial + static_cast<int64_t>(b) * gridDim.x);
 }
 
+// ------------------------------------------------------------ host side
+// Launch shape of one PageRank call, the same for both entry points.
+struct PrPlan {
+  int B;             // Gauss-Seidel row blocks (1 for Jacobi)
+  unsigned grid;     // workgroups of a block's launch
+  unsigned cgrid;    // workgroups of a close launch
+  bool publish;      // Gauss-Seidel blocks publish through a side buffer
+  bool close;        // a close launch follows each block
+  int unroll;        // 64-edge chunks in flight per wave
+  size_t nparts;     // per-workgroup L1 partials per iteration
+};
+
+// Buffers of one call (scores may be the caller's).
+struct PrBufs {
+  float *scores, *c0, *c1;
+  PrState *st;
+  double *errs, *partial;
+  float *hacc;
+  PrBlock *blocks;
+};
+
+// Workgroups of kBlock lanes of pr_pull_block that stay resident, read from
+// the context's cache when present, else queried without caching (the
+// no-allocation entry point inserts nothing into the cache).
+unsigned pr_resident(gbk_ctx *ctx, bool cache) {
+  const void *k = reinterpret_cast<const void *>(pr_pull_block<4>);
+  if (cache || ctx->tune.batch_grid > 0) return resident_grid(ctx, k);
+  auto it = ctx->resident.find(k);
+  if (it != ctx->resident.end()) return it->second;
+  int cu = ctx->cu_count, per_cu = 0;
+  if (cu == 0 && (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cu < 1))
+    cu = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) != hipSuccess || per_cu < 1) per_cu = 4;
+  return static_cast<unsigned>(per_cu) * static_cast<unsigned>(cu);
+}
+
+PrPlan pr_plan(gbk_ctx *ctx, const RowTiles &rt, int64_t n, bool gs, bool cache) {
+  PrPlan p;
+  // Gauss-Seidel blocks: one per ~4096 tiles (a block's launch then keeps
+  // the GPU's waves busy), at most 8; a small graph's one block is Jacobi
+  int64_t want = ctx->tune.pr_blocks ? ctx->tune.pr_blocks : std::min<int64_t>(8, rt.ntiles / 4096);
+  p.B = gs ? static_cast<int>(std::max<int64_t>(1, std::min(want, std::max<int64_t>(rt.ntiles, 1)))) : 1;
+  // a block's launch covers about 1/B of the tiles, about one per wave;
+  // Jacobi's one launch all of them
+  const int64_t block_units = (rt.ntiles / p.B) / kWavesPerBlock + 1;
+  p.grid = gs ? static_cast<unsigned>(std::min<int64_t>(pr_resident(ctx, cache), std::max<int64_t>(block_units, 64)))
+              : rows_grid(rt);
+  // the close launch copies about n/B contributions (Gauss-Seidel) or
+  // finishes the heavy rows
+  p.publish = gs && p.B > 1 && !ctx->tune.pr_inplace;
+  p.cgrid = p.publish ? static_cast<unsigned>(std::min<int64_t>(grid_for(n / p.B + 1), 1024)) : 16;
+  p.close = p.publish || rt.nheavy;
+  // block launches are short and latency-bound: more loads in flight pay
+  // (8 chunks 1% over 4); Jacobi's one launch is bound by the random-line
+  // rate and loses 7%
+  p.unroll = ctx->tune.pr_unroll ? ctx->tune.pr_unroll : (gs && p.B > 1 ? 8 : 1);
+  p.nparts = static_cast<size_t>(p.B) * (p.grid + (p.close ? p.cgrid : 0));
+  return p;
+}
+
+// Carves the buffers (scores excluded) from 'base' at 256-byte alignment
+// and returns the bytes they take; base may be null to measure.
+size_t pr_carve(char *base, int64_t n, int max_iters, const PrPlan &p, int64_t nheavy, PrBufs *b) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *q = base ? base + off : nullptr;
+    off += (bytes + 255) / 256 * 256;
+    return q;
+  };
+  b->c0 = reinterpret_cast<float *>(take(sizeof(float) * n));
+  b->c1 = reinterpret_cast<float *>(take(sizeof(float) * n));
+  b->st = reinterpret_cast<PrState *>(take(sizeof(PrState)));
+  b->errs = reinterpret_cast<double *>(take(sizeof(double) * max_iters));
+  b->partial = reinterpret_cast<double *>(take(sizeof(double) * p.nparts));
+  b->hacc = reinterpret_cast<float *>(take(sizeof(float) * (nheavy + 1)));
+  b->blocks = reinterpret_cast<PrBlock *>(take(sizeof(PrBlock) * p.B));
+  return off;
+}
+
+// Queues the whole call on the context stream: block table, initial
+// scores, then max_iters iterations (captured once and replayed when
+// 'loop' is given, else launched one by one); kernels only.
+int pr_enqueue(gbk_ctx *ctx, const gbk_graph *g, const RowTiles &rt, const PrPlan &p, const PrBufs &b, bool gs,
+               int max_iters, double tol, int fixed_work, LoopGraph *loop, std::optional<EventTimer> *timer) {
+  const int64_t n = g->n;
+  hipStream_t s = ctx->stream;
+  const float kDamp = 0.85f;
+  const float init = 1.0f / static_cast<float>(n);
+  const float base = (1.0f - kDamp) / static_cast<float>(n);
+  // all iterations are queued; after convergence the kernels return at once
+  // on the state's done flag
+  auto iterations = [&]() {
+    for (int it = 0; it < max_iters; it++) {
+      const int kt = loop ? kt_begin(ctx, GBK_KT_PR_ITER) : -1;
+      // Jacobi: ping-pong buffers; Gauss-Seidel in blocks: c0 read and
+      // published into, c1 the side buffer; in one launch: c0 in place
+      float *cur = gs ? b.c0 : ((it & 1) ? b.c1 : b.c0);
+      float *nxt = gs ? (p.publish ? b.c1 : b.c0) : ((it & 1) ? b.c0 : b.c1);
+      for (int blk = 0; blk < p.B; blk++) {
+        switch (p.unroll) {
+          case 1:
+            pr_pull_block<1><<<p.grid, kBlock, 0, s>>>(rt, b.blocks, blk, g->in_nbr, g->off, b.scores, cur, nxt,
+                                                       b.st, b.partial, b.hacc, base, kDamp);
+            break;
+          case 2:
+            pr_pull_block<2><<<p.grid, kBlock, 0, s>>>(rt, b.blocks, blk, g->in_nbr, g->off, b.scores, cur, nxt,
+                                                       b.st, b.partial, b.hacc, base, kDamp);
+            break;
+          case 8:
+            pr_pull_block<8><<<p.grid, kBlock, 0, s>>>(rt, b.blocks, blk, g->in_nbr, g->off, b.scores, cur, nxt,
+                                                       b.st, b.partial, b.hacc, base, kDamp);
+            break;
+          default:
+            pr_pull_block<4><<<p.grid, kBlock, 0, s>>>(rt, b.blocks, blk, g->in_nbr, g->off, b.scores, cur, nxt,
+                                                       b.st, b.partial, b.hacc, base, kDamp);
+        }
+        if (p.close)
+          pr_block_close<<<p.cgrid, kBlock, 0, s>>>(rt, b.blocks, blk, blk + 1 == p.B ? n : -1, g->off, b.scores,
+                                                    p.publish ? cur : nxt, p.publish ? nxt : nullptr, b.st,
+                                                    b.partial + static_cast<size_t>(p.B) * p.grid, b.hacc, base,
+                                                    kDamp);
+      }
+      kt_end(ctx, kt);
+      pr_finish<<<1, 1024, 0, s>>>(b.partial, static_cast<int>(p.nparts), b.st, b.errs, it, tol, fixed_work);
+    }
+  };
+  if (loop)
+    if (int rc = loop->capture(iterations)) return rc;
+  if (timer) timer->emplace(ctx);
+  pr_blocks_init<<<grid_for(std::max<int64_t>(rt.nheavy, p.B)), kBlock, 0, s>>>(rt, p.B, b.blocks, b.hacc);
+  pr_init<<<grid_for(n), kBlock, 0, s>>>(g->off, n, init, b.scores, b.c0, b.st);
+  if (loop) {
+    if (int rc = loop->run(iterations)) return rc;
+  } else {
+    iterations();
+  }
+  GBK_LAUNCH_CHECK(ctx, "pr_pull_block");
+  return GBK_OK;
+}
+
+// Iterations executed, last L1 change and first converged iteration (-1
+// if none) as doubles, for callers that keep the results on the device.
+__global__ void pr_report(const PrState *st, double *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    out[0] = st->iters;
+    out[1] = st->last_err;
+    out[2] = st->converged;
+  }
+}
+
+// The no-allocation variant's workspace: one array per GPU, held until the
+// process ends, shared by every call on that GPU.  Host code uses its
+// device address (hipGetSymbolAddress, once per device), never the name.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+static __device__ __align__(256) char noalloc_scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+std::once_flag noalloc_once[kMaxDevices];
+void *noalloc_ptr[kMaxDevices];
+hipError_t noalloc_err[kMaxDevices];
+
+int noalloc_workspace(gbk_ctx *ctx, char **out) {
+  const int dev = ctx->device;
+  if (dev < 0 || dev >= kMaxDevices) return set_error(ctx, GBK_EINVAL, "pr noalloc: device index");
+  std::call_once(noalloc_once[dev], [dev]() {
+    int prev = 0;
+    const bool restore = hipGetDevice(&prev) == hipSuccess && prev != dev;
+    noalloc_err[dev] = hipSetDevice(dev);
+    if (noalloc_err[dev] == hipSuccess)
+      noalloc_err[dev] = hipGetSymbolAddress(&noalloc_ptr[dev], HIP_SYMBOL(noalloc_scratch));
+    if (restore) (void)hipSetDevice(prev);
+  });
+  if (noalloc_err[dev] != hipSuccess || !noalloc_ptr[dev])
+    return check_hip(ctx, noalloc_err[dev] != hipSuccess ? noalloc_err[dev] : hipErrorInvalidSymbol,
+                     "pr noalloc: workspace address");
+  *out = static_cast<char *>(noalloc_ptr[dev]);
+  return GBK_OK;
+}
+
 }  // namespace gbk
 
 using namespace gbk;
@@ -268,93 +449,60 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   hipStream_t s = ctx->stream;
   const RowTiles rt = row_tiles(g, true);
   const bool gs = variant == GBK_PR_GS;
-  // Gauss-Seidel blocks: one per ~4096 tiles (a block's launch then keeps
-  // the GPU's waves busy), at most 8; a small graph's one block is Jacobi
-  int64_t want = ctx->tune.pr_blocks ? ctx->tune.pr_blocks : std::min<int64_t>(8, rt.ntiles / 4096);
-  const int B = gs ? static_cast<int>(std::max<int64_t>(1, std::min(want, std::max<int64_t>(rt.ntiles, 1)))) : 1;
-  // a block's launch covers about 1/B of the tiles, about one per wave;
-  // Jacobi's one launch all of them
-  const int64_t block_units = (rt.ntiles / B) / kWavesPerBlock + 1;
-  const unsigned grid =
-      gs ? static_cast<unsigned>(std::min<int64_t>(resident_grid(ctx, reinterpret_cast<const void *>(pr_pull_block<4>)),
-                                                   std::max<int64_t>(block_units, 64)))
-         : rows_grid(rt);
-  // the close launch copies about n/B contributions (Gauss-Seidel) or
-  // finishes the heavy rows
-  const bool publish = gs && B > 1 && !ctx->tune.pr_inplace;
-  const unsigned cgrid = publish ? static_cast<unsigned>(std::min<int64_t>(grid_for(n / B + 1), 1024)) : 16;
-  const bool close = publish || rt.nheavy;
-  // block launches are short and latency-bound: more loads in flight pay
-  // (8 chunks 1% over 4); Jacobi's one launch is bound by the random-line
-  // rate and loses 7%
-  const int unroll = ctx->tune.pr_unroll ? ctx->tune.pr_unroll : (gs && B > 1 ? 8 : 1);
-  const size_t nparts = static_cast<size_t>(B) * (grid + (close ? cgrid : 0));
-  auto *scores = static_cast<float *>(scratch(ctx, kS0, sizeof(float) * n));
-  auto *c0 = static_cast<float *>(scratch(ctx, kS1, sizeof(float) * n));
-  auto *c1 = static_cast<float *>(scratch(ctx, kS2, sizeof(float) * n));
-  auto *st = static_cast<PrState *>(scratch(ctx, kS3, sizeof(PrState)));
-  auto *errs = static_cast<double *>(scratch(ctx, kS4, sizeof(double) * max_iters));
-  auto *partial = static_cast<double *>(scratch(ctx, kS5, sizeof(double) * nparts));
-  auto *hacc = static_cast<float *>(scratch(ctx, kS6, sizeof(float) * (rt.nheavy + 1)));
-  auto *blocks = static_cast<PrBlock *>(scratch(ctx, kS8, sizeof(PrBlock) * B));
-  if (!scores || !c0 || !c1 || !st || !errs || !partial || !hacc || !blocks)
+  const PrPlan p = pr_plan(ctx, rt, n, gs, true);
+  PrBufs b;
+  b.scores = static_cast<float *>(scratch(ctx, kS0, sizeof(float) * n));
+  b.c0 = static_cast<float *>(scratch(ctx, kS1, sizeof(float) * n));
+  b.c1 = static_cast<float *>(scratch(ctx, kS2, sizeof(float) * n));
+  b.st = static_cast<PrState *>(scratch(ctx, kS3, sizeof(PrState)));
+  b.errs = static_cast<double *>(scratch(ctx, kS4, sizeof(double) * max_iters));
+  b.partial = static_cast<double *>(scratch(ctx, kS5, sizeof(double) * p.nparts));
+  b.hacc = static_cast<float *>(scratch(ctx, kS6, sizeof(float) * (rt.nheavy + 1)));
+  b.blocks = static_cast<PrBlock *>(scratch(ctx, kS8, sizeof(PrBlock) * p.B));
+  if (!b.scores || !b.c0 || !b.c1 || !b.st || !b.errs || !b.partial || !b.hacc || !b.blocks)
     return set_error(ctx, GBK_ENOMEM, "pr scratch");
-  const float kDamp = 0.85f;
-  const float init = 1.0f / static_cast<float>(n);
-  const float base = (1.0f - kDamp) / static_cast<float>(n);
-  // all iterations are queued (one captured graph); after convergence the
-  // kernels return at once on the state's done flag
-  auto iterations = [&]() {
-    for (int it = 0; it < max_iters; it++) {
-      const int kt = kt_begin(ctx, GBK_KT_PR_ITER);
-      // Jacobi: ping-pong buffers; Gauss-Seidel in blocks: c0 read and
-      // published into, c1 the side buffer; in one launch: c0 in place
-      float *cur = gs ? c0 : ((it & 1) ? c1 : c0);
-      float *nxt = gs ? (publish ? c1 : c0) : ((it & 1) ? c0 : c1);
-      for (int b = 0; b < B; b++) {
-        switch (unroll) {
-          case 1:
-            pr_pull_block<1><<<grid, kBlock, 0, s>>>(rt, blocks, b, g->in_nbr, g->off, scores, cur, nxt, st, partial,
-                                                     hacc, base, kDamp);
-            break;
-          case 2:
-            pr_pull_block<2><<<grid, kBlock, 0, s>>>(rt, blocks, b, g->in_nbr, g->off, scores, cur, nxt, st, partial,
-                                                     hacc, base, kDamp);
-            break;
-          case 8:
-            pr_pull_block<8><<<grid, kBlock, 0, s>>>(rt, blocks, b, g->in_nbr, g->off, scores, cur, nxt, st, partial,
-                                                     hacc, base, kDamp);
-            break;
-          default:
-            pr_pull_block<4><<<grid, kBlock, 0, s>>>(rt, blocks, b, g->in_nbr, g->off, scores, cur, nxt, st, partial,
-                                                     hacc, base, kDamp);
-        }
-        if (close)
-          pr_block_close<<<cgrid, kBlock, 0, s>>>(rt, blocks, b, b + 1 == B ? n : -1, g->off, scores,
-                                                  publish ? cur : nxt, publish ? nxt : nullptr, st,
-                                                  partial + static_cast<size_t>(B) * grid, hacc, base, kDamp);
-      }
-      kt_end(ctx, kt);
-      pr_finish<<<1, 1024, 0, s>>>(partial, static_cast<int>(nparts), st, errs, it, tol, fixed_work);
-    }
-  };
   LoopGraph loop(ctx);
-  if (int rc = loop.capture(iterations)) return rc;
-  EventTimer timer(ctx);
-  pr_blocks_init<<<grid_for(std::max<int64_t>(rt.nheavy, B)), kBlock, 0, s>>>(rt, B, blocks, hacc);
-  pr_init<<<grid_for(n), kBlock, 0, s>>>(g->off, n, init, scores, c0, st);
-  if (int rc = loop.run(iterations)) return rc;
-  GBK_LAUNCH_CHECK(ctx, "pr_pull_block");
-  const double ms = timer.stop_ms();
+  std::optional<EventTimer> timer;
+  if (int rc = pr_enqueue(ctx, g, rt, p, b, gs, max_iters, tol, fixed_work, &loop, &timer)) return rc;
+  const double ms = timer->stop_ms();
   PrState h{};
-  GBK_HIP(ctx, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
-  if (scores_out) GBK_HIP(ctx, hipMemcpyAsync(scores_out, scores, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+  GBK_HIP(ctx, hipMemcpyAsync(&h, b.st, sizeof(h), hipMemcpyDeviceToHost, s));
+  if (scores_out) GBK_HIP(ctx, hipMemcpyAsync(scores_out, b.scores, sizeof(float) * n, hipMemcpyDeviceToHost, s));
   GBK_HIP(ctx, hipStreamSynchronize(s));
   if (stats) {
     stats->device_ms = ms;
     stats->iterations = h.iters;
     stats->converged_iter = h.converged;
     stats->final_error = h.last_err;
+  }
+  return GBK_OK;
+}
+
+extern "C" int64_t gbk_pr_noalloc_max_bytes(void) { return static_cast<int64_t>(kNoAllocBytes); }
+
+extern "C" int gbk_pr_noalloc(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_iters, double tol,
+                              int fixed_work, float *scores_dev, double *report_dev) {
+  if (!ctx || !g || !scores_dev || max_iters <= 0 || (variant != GBK_PR_GS && variant != GBK_PR_SPMV))
+    return set_error(ctx, GBK_EINVAL, "pr noalloc: bad arguments");
+  const int64_t n = g->n;
+  if (n == 0) return set_error(ctx, GBK_EINVAL, "pr noalloc: empty graph");
+  const RowTiles rt = row_tiles(g, true);
+  const bool gs = variant == GBK_PR_GS;
+  const PrPlan p = pr_plan(ctx, rt, n, gs, false);
+  PrBufs b;
+  const size_t need = pr_carve(nullptr, n, max_iters, p, rt.nheavy, &b);
+  if (need > kNoAllocBytes)
+    return set_error(ctx, GBK_EINVAL,
+                     "pr noalloc: needs " + std::to_string(need) + " bytes of workspace, more than its " +
+                         std::to_string(kNoAllocBytes) + " (about 8 bytes per vertex: n up to ~8M)");
+  char *ws = nullptr;
+  if (int rc = noalloc_workspace(ctx, &ws)) return rc;
+  pr_carve(ws, n, max_iters, p, rt.nheavy, &b);
+  b.scores = scores_dev;
+  if (int rc = pr_enqueue(ctx, g, rt, p, b, gs, max_iters, tol, fixed_work, nullptr, nullptr)) return rc;
+  if (report_dev) {
+    pr_report<<<1, 1, 0, ctx->stream>>>(b.st, report_dev);
+    GBK_LAUNCH_CHECK(ctx, "pr noalloc report");
   }
   return GBK_OK;
 }

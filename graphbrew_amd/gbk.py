@@ -100,6 +100,8 @@ def lib():
         "gbk_mapping": (i32, [VP, VP, i32, VP, C.POINTER(f64)]),
         "gbk_relabel_by": (i32, [VP, VP, i32, pp, C.POINTER(RelabelStats), VP, C.POINTER(f64)]),
         "gbk_pr": (i32, [VP, VP, i32, i32, f64, i32, VP, C.POINTER(PrStats)]),
+        "gbk_pr_noalloc": (i32, [VP, VP, i32, i32, f64, i32, VP, VP]),
+        "gbk_pr_noalloc_max_bytes": (C.c_int64, []),
         "gbk_bfs": (i32, [VP, VP, C.c_int32, i32, i32, VP, C.POINTER(BfsStats)]),
         "gbk_cc": (i32, [VP, VP, i32, VP, C.POINTER(CcStats)]),
         "gbk_sssp": (i32, [VP, VP, C.c_int32, C.c_int32, VP, C.POINTER(SsspStats)]),

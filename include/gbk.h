@@ -188,6 +188,23 @@ int gbk_relabel_by(gbk_ctx *ctx, const gbk_graph *in, int algo, gbk_graph **out,
 /* scores: host float[n] (may be NULL to leave the result on the device). */
 int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_iters,
            double tol, int fixed_work, float *scores, gbk_pr_stats *stats);
+/* PageRank without any allocation: gbk_pr's kernels with their workspace
+ * in one fixed device array per GPU (gbk_pr_noalloc_max_bytes(), 64 MiB:
+ * about 8 bytes per vertex, so n up to ~8M), held until the process ends.
+ * No reference counterpart: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc/hipFree or allocating library routine, does not
+ * synchronise and only launches kernels on the context stream, so it may be
+ * captured.  scores_dev: device float[n] (caller-owned) receiving the
+ * scores; report_dev: device double[3] or NULL receiving iterations, the
+ * last L1 change and the first converged iteration (-1 if none).  Results
+ * are ready once the context stream has reached the call.  A graph whose
+ * workspace exceeds the array returns GBK_EINVAL.  Every call on one GPU
+ * shares the array: calls that overlap on one GPU (from two contexts or
+ * streams) race; each call initialises what it reads. */
+int gbk_pr_noalloc(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_iters, double tol,
+                   int fixed_work, float *scores_dev, double *report_dev);
+int64_t gbk_pr_noalloc_max_bytes(void);
 /* parent: host int32[n], -1 for unreached; alpha/beta <= 0 use defaults. */
 int gbk_bfs(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int alpha,
             int beta, int32_t *parent, gbk_bfs_stats *stats);
