@@ -102,6 +102,7 @@ def lib():
         "gbk_pr": (i32, [VP, VP, i32, i32, f64, i32, VP, C.POINTER(PrStats)]),
         "gbk_pr_noalloc": (i32, [VP, VP, i32, i32, f64, i32, VP, VP]),
         "gbk_pr_noalloc_max_bytes": (C.c_int64, []),
+        "gbk_tier0_features": (i32, [VP, VP, C.c_int64, C.c_uint64, VP]),
         "gbk_bfs": (i32, [VP, VP, C.c_int32, i32, i32, VP, C.POINTER(BfsStats)]),
         "gbk_cc": (i32, [VP, VP, i32, VP, C.POINTER(CcStats)]),
         "gbk_sssp": (i32, [VP, VP, C.c_int32, C.c_int32, VP, C.POINTER(SsspStats)]),
@@ -405,6 +406,14 @@ class Context:
         if rc < 0:
             self._check(rc)
         return out
+
+    def tier0_features(self, g: DeviceGraph, sample_size: int = 0, seed: int = 0) -> dict:
+        """The adaptive selector's Tier-0 features, computed on the device
+        (the names and order of graphbrew_amd.host.tier0_features)."""
+        from .host import TIER0_NAMES
+        out = np.zeros(7, np.float64)
+        self._check(lib().gbk_tier0_features(self.h, g.handle, int(sample_size), int(seed), _ptr(out)))
+        return dict(zip(TIER0_NAMES, out.tolist()))
 
     def mlp(self, in_dim: int, seed: int = 0) -> "Mlp":
         return Mlp(self, in_dim, seed)

@@ -59,6 +59,8 @@ def lib():
             "gbh_tier0_features": (i32, [i64, VP, VP, i64, C.c_uint64, VP]),
             "gbh_modeled_property_bytes": (i64, [i32, i64, i64]),
             "gbh_frozen_rule": (i32, [i32, f64, i64, f64, f64, f64]),
+            "gbh_budgeted_rule": (i32, [i32, f64, i64, f64, f64, f64]),
+            "gbh_llc_bytes": (i64, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -268,6 +270,32 @@ def modeled_property_bytes(bench: str, n: int, m: int) -> int:
 def frozen_rule(bench: str, reuse: float, n: int, avg_degree: float, degree_cv: float,
                 wsr: float) -> bool:
     return bool(lib().gbh_frozen_rule(BENCH_CODES[bench], reuse, n, avg_degree, degree_cv, wsr))
+
+
+def budgeted_rule(bench: str, reuse: float, n: int, avg_degree: float, degree_cv: float,
+                  hub_concentration: float) -> bool:
+    return bool(lib().gbh_budgeted_rule(BENCH_CODES[bench], reuse, n, avg_degree, degree_cv, hub_concentration))
+
+
+def llc_bytes() -> int:
+    return int(lib().gbh_llc_bytes())
+
+
+def rule_pick(rule: str, bench: str, reuse: float, n: int, m: int, tier0: dict, llc: int = 0):
+    """The reference's deployable rules (reorder_adaptive.h:240-383):
+    "allkernel-lowreuse-rule" asks for the Leiden-Gorder8 composition when
+    its predicate holds, "budgeted-rule" for the budgeted Leiden-Gorder
+    composition; otherwise both fall back to DBG (-o 5), their arm in builds
+    without Rabbit order.  Returns (wants_leiden, working-set ratio)."""
+    llc = llc or llc_bytes()
+    wsr = modeled_property_bytes(bench, n, m) / llc if llc > 0 else 0.0
+    if rule == "allkernel-lowreuse-rule":
+        hit = frozen_rule(bench, reuse, n, tier0["avg_degree"], tier0["degree_cv"], wsr)
+    elif rule == "budgeted-rule":
+        hit = budgeted_rule(bench, reuse, n, tier0["avg_degree"], tier0["degree_cv"], tier0["hub_concentration"])
+    else:
+        raise ValueError(rule)
+    return hit, wsr
 
 
 # ------------------------------------------------------------- verifiers

@@ -6,6 +6,7 @@
 int main(int argc, char *argv[]) {
   drv::Session s;
   s.cli.name = "betweenness-centrality";
+  s.cli.kernel = "bc";  // SetBenchmarkTypeHint (bench/src/bc.cc main)
   s.cli.max_iters = 1;
   if (!s.cli.Parse(argc, argv, "i:")) return -1;
   if (!s.cli.sources.empty() && s.cli.max_iters != 1)

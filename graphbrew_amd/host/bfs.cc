@@ -4,6 +4,7 @@
 int main(int argc, char *argv[]) {
   drv::Session s;
   s.cli.name = "breadth-first search";
+  s.cli.kernel = "bfs";  // SetBenchmarkTypeHint (bench/src/bfs.cc main)
   if (!s.cli.Parse(argc, argv, "")) return -1;
   s.MakeGraph(false);
   int64_t n;

@@ -6,6 +6,7 @@
 int main(int argc, char *argv[]) {
   drv::Session s;
   s.cli.name = "connected-components-afforest";
+  s.cli.kernel = "cc";  // SetBenchmarkTypeHint (bench/src/cc.cc main)
   if (!s.cli.Parse(argc, argv, "")) return -1;
   s.MakeGraph(false);
   int64_t n;

@@ -4,6 +4,7 @@
 int main(int argc, char *argv[]) {
   drv::Session s;
   s.cli.name = "connected-components";
+  s.cli.kernel = "cc_sv";  // SetBenchmarkTypeHint (bench/src/cc_sv.cc main)
   if (!s.cli.Parse(argc, argv, "")) return -1;
   s.MakeGraph(false);
   int64_t n;

@@ -11,7 +11,9 @@
 // The graph lives in HBM (libgbk.so); host copies are made only for the
 // verifiers and the analysis printouts, as the reference keeps everything on
 // the host.  Orderings produced by the reference's own reorderers (Rabbit,
-// GOrder, COrder, RCM, GraphBrew/Leiden, ...) enter through -o 13:<file>.
+// GOrder, COrder, RCM, GraphBrew/Leiden, ...) enter through -o 13:<file>, or
+// for -o 11:bnf and the GraphBrew specs of drv_adaptive.h as mapping files
+// under GBK_MAPPING_DIR; -o 14 is the adaptive selector (drv_adaptive.h).
 #pragma once
 
 #include <execinfo.h>
@@ -59,6 +61,7 @@ struct Timer {
 
 struct Cli {
   std::string name;
+  std::string kernel;  // pr, pr_spmv, bfs, cc, cc_sv, sssp or bc: the benchmark type of -o 14
   std::string filename;
   int scale = -1, degree = 16;
   bool symmetrize = false, uniform = false, keep_self = false, logging = false;
@@ -176,6 +179,12 @@ struct Cli {
   }
 };
 
+}  // namespace drv
+
+#include "drv_adaptive.h"
+
+namespace drv {
+
 // Host view of the device graph (downloaded on demand).
 struct HostGraph {
   int64_t n = 0, m = 0;
@@ -225,7 +234,151 @@ struct Session {
     return host;
   }
 
+  // Mapping files of the reference's CPU reorderers: GBK_MAPPING_DIR/
+  // <graph>_<tag>.so, <graph> = kron<scale>_ef<degree>, uniform<scale>_ef<degree>
+  // or the input file's name without its suffix ("" when absent).
+  std::string GraphKey() const {
+    if (!cli.filename.empty()) {
+      std::string b = cli.filename.substr(cli.filename.rfind('/') + 1);
+      return b.substr(0, b.rfind('.'));
+    }
+    return std::string(cli.uniform ? "uniform" : "kron") + std::to_string(cli.scale) + "_ef" +
+           std::to_string(cli.degree);
+  }
+  std::string MappingFile(const std::string &tag) const {
+    const char *dir = std::getenv("GBK_MAPPING_DIR");
+    if (!dir || tag.empty()) return "";
+    const std::string p = std::string(dir) + "/" + GraphKey() + "_" + tag + ".so";
+    return FileExists(p) ? p : "";
+  }
+
+  // -o 14: the learned pick or a frozen rule's (see drv_adaptive.h); returns
+  // the ordering to apply in its place.
+  std::pair<int, std::vector<std::string>> Adaptive(const std::vector<std::string> &opt) {
+    const AdaptivePolicy pol = ParseAdaptivePolicy(opt);
+    int64_t n = 0, m = 0;
+    int d = 0, w = 0;
+    gbk_graph_info(g, &n, &m, &d, &w);
+    const int bench = BenchCode(cli.kernel);
+    printf("=== Full-Graph Adaptive Mode ===\n");
+    printf("Nodes: %" PRId64 ", Edges: %" PRId64 "\n", n, m);
+    Timer ft;
+    ft.Start();
+    double t0[7];
+    Check(gbk_tier0_features(ctx, g, 0, 0, t0), "tier0 features");
+    const int64_t pb = bench ? gbh_modeled_property_bytes(bench, n, m) : 0;
+    const int64_t llc = gbh_llc_bytes();
+    const double wsr = llc > 0 ? static_cast<double>(pb) / static_cast<double>(llc) : 0.0;
+    ft.Stop();
+    PrintTime("Degree Variance", t0[1]);
+    PrintTime("Hub Concentration", t0[2]);
+    PrintTime("Normalized Edge Span", t0[3]);
+    PrintTime("Window Neighbor Overlap", t0[4]);
+    PrintTime("Packing Factor", t0[5]);
+    PrintTime("Property Working Set Bytes", static_cast<double>(pb));
+    PrintTime("LLC Capacity Bytes", static_cast<double>(llc));
+    PrintTime("Property WSR LLC", wsr);
+    // AdaptiveReuseBucket (reorder_types.h:1276-1284)
+    const double r = pol.reuse;
+    const int bucket = r <= 1 ? 0 : r <= 5 ? 1 : r <= 10 ? 2 : r <= 20 ? 3 : r <= 50 ? 4 : r <= 100 ? 5 : 6;
+    printf("Adaptive Tier0 Features: {\"log10_nodes\":%.17g,\"log10_edges\":%.17g,\"avg_degree\":%.17g,"
+           "\"degree_cv\":%.17g,\"hub_concentration\":%.17g,\"normalized_edge_span\":%.17g,"
+           "\"window_neighbor_overlap\":%.17g,\"property_wsr_llc\":%.17g,\"kernel_class\":%d,"
+           "\"reuse_bucket\":%d}\n",
+           std::log10(static_cast<double>(n) + 1.0), std::log10(static_cast<double>(m) + 1.0), t0[0], t0[1],
+           t0[2], t0[3], t0[4], wsr, bench, bucket);
+    PrintTime("Adaptive Feature Time", ft.Seconds());
+    printf("Selection: model=%s criterion=%s\n", pol.model.c_str(), pol.criterion.c_str());
+    Timer mt;
+    mt.Start();
+    std::string rule = pol.model;
+    int pick = -1;
+    if (pol.model == "perceptron") {
+      if (pol.criterion == "fastest-reorder") {
+        pick = 0;  // the original order needs no mapping and no relabel
+      } else {
+        // the model is trained for reuse 1 and 2; kernel-only criteria use 2
+        const double reuse = pol.criterion == "best-endtoend" ? std::min(2.0, std::max(1.0, pol.reuse)) : 2.0;
+        SelectorModel mdl;
+        std::vector<double> pred;
+        if (mdl.Load(SelectorModelPath()) && LearnedPredict(ctx, g, mdl, cli.kernel, reuse, pred)) {
+          int a = 0;
+          for (int i = 1; i < static_cast<int>(pred.size()); i++)
+            if (pred[i] < pred[a]) a = i;
+          pick = pred[a] < 1.0 - mdl.margin ? a : 0;
+          std::string line;
+          char buf[64];
+          for (size_t i = 0; i < pred.size(); i++) {
+            snprintf(buf, sizeof(buf), "%s%s=%.4f", i ? " " : "", kOrderNames[i], pred[i]);
+            line += buf;
+          }
+          printf("Adaptive Learned Prediction: %s\n", line.c_str());
+        } else {
+          PrintLabel("Adaptive Learned Model", "unavailable");
+          rule = "allkernel-lowreuse-rule";
+        }
+      }
+    }
+    std::pair<int, std::vector<std::string>> out{5, {}};
+    if (pick >= 0) {
+      out = {pick, {}};
+    } else {
+      const bool hit = rule == "budgeted-rule" ? gbh_budgeted_rule(bench, pol.reuse, n, t0[0], t0[1], t0[2])
+                                               : gbh_frozen_rule(bench, pol.reuse, n, t0[0], t0[1], wsr);
+      PrintLabel("Adaptive Rule", std::string(rule) + (hit ? " match" : " fallback"));
+      if (hit) {
+        const std::string arm = rule == "budgeted-rule" ? kBudgetedArm : kLowReuseArm;
+        std::vector<std::string> o;
+        std::stringstream ss(arm);
+        std::string tok;
+        while (std::getline(ss, tok, ':')) o.push_back(tok);
+        PrintLabel("Adaptive Rule Arm", "12:" + arm);
+        if (!MappingFile(ReordererTag(12, o)).empty()) {
+          out = {12, o};
+        } else {
+          // the GraphBrew reorderer is CPU code: without its mapping file the
+          // rule's fallback arm of Rabbit-less builds, DBG, stands in
+          PrintLabel("Adaptive Arm Substitute", "5 (no GraphBrew mapping file)");
+        }
+      }
+    }
+    mt.Stop();
+    PrintTime("Adaptive Model Time", mt.Seconds());
+    std::string spec = std::to_string(out.first);
+    for (const auto &p : out.second) spec += ":" + p;
+    PrintLabel("Adaptive Selected", spec);
+    return out;
+  }
+
+  int order_pos = 0;  // index of the -o being applied (mapping files fit the first only)
+
   std::vector<int32_t> Mapping(int algo, const std::vector<std::string> &opt) {
+    if (algo == 14) {
+      const auto pick = Adaptive(opt);
+      return Mapping(pick.first, pick.second);
+    }
+    if (algo == 11 || algo == 12) {
+      // the reference's RCM and GraphBrew reorderers run on the CPU; their
+      // mappings enter as files
+      const std::string tag = ReordererTag(algo, opt);
+      const std::string path = MappingFile(tag);
+      if (path.empty() || order_pos != 0)
+        Die("Ordering " + std::to_string(algo) + (opt.empty() ? "" : ":" + JoinOpts(opt)) +
+            " is produced by the reference reorderer on the CPU; pass its mapping with -o 13:<file>" +
+            (tag.empty() ? std::string() : " (or as GBK_MAPPING_DIR/" + GraphKey() + "_" + tag + ".so, first -o)"));
+      int64_t n = 0, m = 0, len = 0;
+      int d = 0, w = 0;
+      gbk_graph_info(g, &n, &m, &d, &w);
+      std::vector<int32_t> ids(n);
+      Timer t;
+      t.Start();
+      if (gbh_mapping_read(path.c_str(), &len, nullptr) != 0 || len != n) Die("Mapping file " + path + " does not fit the graph");
+      gbh_mapping_read(path.c_str(), &len, ids.data());
+      t.Stop();
+      PrintLabel("Reorder Mapping File", path.substr(path.rfind('/') + 1));
+      PrintTime("Load Map Time", t.Seconds());
+      return ids;
+    }
     // the orderings that sort by an out-degree key are made on the device
     static const std::unordered_map<int, const char *> device_made = {
         {0, "Original Map Time"}, {2, "Sort Map Time"}, {5, "DBG Map Time"},
@@ -417,6 +570,7 @@ struct Session {
   void ApplyOrders(Timer &total) {
     for (const auto &o : cli.orders) {
       std::vector<int32_t> ids = Mapping(o.first, o.second);
+      order_pos++;
       Timer val;
       val.Start();
       const int bad = gbh_validate_permutation(static_cast<int64_t>(ids.size()), ids.data());

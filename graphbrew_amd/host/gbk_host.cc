@@ -8,6 +8,7 @@
 #include "gbk_host.h"
 
 #include <omp.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cerrno>
@@ -630,6 +631,25 @@ int gbh_frozen_rule(int bench, double reuse, int64_t n, double avg_degree, doubl
   if (!kernel_ok || reuse > 2.0 || n < 1000 || wsr > 3.2) return 0;
   if (degree_cv > 8.0) return 1;
   return degree_cv <= 2.68 && (avg_degree <= 60.0 || wsr <= 0.82);
+}
+
+// ShouldUseBudgetedLeidenGorder (reorder_adaptive.h:240-257): PageRank
+// (either variant) used once on a dense, moderately skewed graph whose hubs
+// hold at least 30% of the sampled degree.
+int gbh_budgeted_rule(int bench, double reuse, int64_t n, double avg_degree, double degree_cv,
+                      double hub_concentration) {
+  const bool pr = bench == 1 || bench == 7;
+  return pr && reuse <= 1.0 && n >= 1000 && avg_degree >= 45.0 && degree_cv < 3.0 && hub_concentration >= 0.3;
+}
+
+// GetLLCSizeBytes (reorder_types.h:6030-6038): the host's L3, else its L2,
+// else 30 MB.  The frozen rules' working-set ratio divides by it.
+int64_t gbh_llc_bytes(void) {
+  long llc = sysconf(_SC_LEVEL3_CACHE_SIZE);
+  if (llc > 0) return llc;
+  llc = sysconf(_SC_LEVEL2_CACHE_SIZE);
+  if (llc > 0) return llc;
+  return int64_t(30) * 1024 * 1024;
 }
 
 }  // extern "C"

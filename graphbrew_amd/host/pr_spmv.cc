@@ -4,6 +4,7 @@
 int main(int argc, char *argv[]) {
   drv::Session s;
   s.cli.name = "pagerank";
+  s.cli.kernel = "pr_spmv";  // SetBenchmarkTypeHint (bench/src/pr_spmv.cc main)
   if (!s.cli.Parse(argc, argv, "i:t:F")) return -1;
   s.MakeGraph(false);
   drv::PrintLabel("PR Mode", s.cli.fixed ? "fixed-work" : "convergence");

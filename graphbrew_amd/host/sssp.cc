@@ -10,6 +10,7 @@ static std::string hex2(const uint64_t h[2]) {
 int main(int argc, char *argv[]) {
   drv::Session s;
   s.cli.name = "single-source shortest-path";
+  s.cli.kernel = "sssp";  // SetBenchmarkTypeHint (bench/src/sssp.cc main)
   if (!s.cli.Parse(argc, argv, "d:")) return -1;
   s.MakeGraph(true);
   int64_t n;

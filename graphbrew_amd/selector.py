@@ -37,8 +37,12 @@ ORDER_NAMES = ("original", "random", "sort", "hubsort", "hubcluster", "dbg", "hu
 KERNELS = ("pr", "pr_spmv", "bfs", "cc", "cc_sv", "sssp", "bc")
 REUSE = (1, 2)
 FALLBACK = 5  # DBG: the frozen rule's deployable pick here
+# orderings the drivers make on the device (gbk_mapping / gbk_relabel_by):
+# their mapping cost is device time; the others are made on the host
+DEVICE_MAPPED = (2, 5, 6, 7)
 IN_DIM = gbk.NUM_FEATURES + len(ORDERINGS) + len(KERNELS) + 1
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_model.json")
+MODEL_TEXT_PATH = os.path.splitext(MODEL_PATH)[0] + ".txt"
 
 
 # ----------------------------------------------------------- graph families
@@ -180,9 +184,7 @@ def measure_graph(ctx: gbk.Context, spec: dict, repeats: int = 3) -> dict:
     src_orig = int(H.source_picker(off, 1)[0][0])
     rows = {}
     for a in ORDERINGS:
-        t0 = time.perf_counter()
-        ids = H.mapping(a, off)
-        map_ms = (time.perf_counter() - t0) * 1e3
+        ids, map_ms = mapping_ms(ctx, g, off, a)
         if a == 0:
             # the graph is already in this order: nothing to map or relabel
             rg, rgw, rel_ms, rel_w_ms, map_ms = g, gw, 0.0, 0.0, 0.0
@@ -196,10 +198,49 @@ def measure_graph(ctx: gbk.Context, spec: dict, repeats: int = 3) -> dict:
         if a != 0:
             rg.free()
             rgw.free()
-    out = {"spec": spec, "n": g.n, "m": g.m, "features": [float(x) for x in feats], "orderings": rows}
+    out = {"spec": spec, "n": g.n, "m": g.m, "features": [float(x) for x in feats], "orderings": rows,
+           "tier0": list(ctx.tier0_features(g).values())}
     g.free()
     gw.free()
     return out
+
+
+def mapping_ms(ctx: gbk.Context, g, off, a: int, repeats: int = 3):
+    """(new_ids, ms) of ordering a as the drivers make it: device time for
+    DEVICE_MAPPED (median of repeats), host wall time otherwise."""
+    if a in DEVICE_MAPPED:
+        runs = [ctx.mapping(g, a) for _ in range(repeats)]
+        return runs[0][0], _median([ms for _, ms in runs])
+    t0 = time.perf_counter()
+    ids = H.mapping(a, off)
+    return ids, (time.perf_counter() - t0) * 1e3
+
+
+def remap(ctx: gbk.Context, data: Sequence[dict], log=None) -> None:
+    """Rebuilds every stored graph on the device and, in place, replaces the
+    mapping times of DEVICE_MAPPED orderings by their device times (the host
+    time is kept as map_host_ms) and stores the device Tier-0 features (the
+    frozen rule's inputs)."""
+    for i, row in enumerate(data):
+        (u, v), n = make_edges(row["spec"])
+        g, _ = ctx.build(u, v, None, n, True)
+        if g.n != row["n"] or g.m != row["m"]:
+            raise RuntimeError(f"graph {i} rebuilt with different size")
+        row["tier0"] = list(ctx.tier0_features(g).values())
+        for a in DEVICE_MAPPED:
+            o = row["orderings"][str(a)]
+            o.setdefault("map_host_ms", o["map_ms"])
+            o["map_ms"] = mapping_ms(ctx, g, None, a)[1]
+        g.free()
+        if log:
+            log(f"remap {i + 1}/{len(data)} n={row['n']} m={row['m']}")
+
+
+def rule_decision(row: dict, kernel: str, reuse: int, llc: int = 0) -> bool:
+    """The frozen all-kernel rule's decision on a stored graph (True: the
+    Leiden-Gorder8 arm, which is not produced here; False: its fallback DBG)."""
+    t0 = dict(zip(H.TIER0_NAMES, row["tier0"]))
+    return H.rule_pick("allkernel-lowreuse-rule", kernel, reuse, row["n"], row["m"], t0, llc)[0]
 
 
 def collect(ctx: gbk.Context, specs: Sequence[dict], path: Optional[str] = None, log=None) -> List[dict]:
@@ -311,6 +352,22 @@ class Selector:
                        "params": [[float(f"{x:.7g}") for x in p] for p in np.atleast_2d(self.params)],
                        "mean": list(map(float, self.mean)),
                        "std": list(map(float, self.std)), "margin": self.margin, "meta": self.meta}, f)
+        self.save_text(os.path.splitext(path)[0] + ".txt")
+
+    def save_text(self, path: str = MODEL_TEXT_PATH):
+        """The model for the C++ drivers (-o 14): a header line
+        "gbk-selector 1 in_dim members margin features orderings kernels", the
+        kernel names, the feature means, the feature deviations, then one line
+        of parameters per ensemble member (gbk_mlp_set_params layout)."""
+        P = np.atleast_2d(self.params)
+        with open(path, "w") as f:
+            f.write(f"gbk-selector 1 {IN_DIM} {len(P)} {self.margin:.9g} {gbk.NUM_FEATURES} "
+                    f"{len(ORDERINGS)} {len(KERNELS)}\n")
+            f.write(" ".join(KERNELS) + "\n")
+            f.write(" ".join(f"{x:.17g}" for x in self.mean) + "\n")
+            f.write(" ".join(f"{x:.17g}" for x in self.std) + "\n")
+            for p in P:
+                f.write(" ".join(f"{float(x):.9g}" for x in p) + "\n")
 
     @classmethod
     def load(cls, path: str = MODEL_PATH) -> "Selector":
@@ -331,6 +388,8 @@ def evaluate(ctx: gbk.Context, sel: "Selector", data: Sequence[dict], margin: Op
     names = ("model", "fallback", "original")
     res = {r: {k: [] for k in names} for r in REUSE}
     picks = []
+    leiden = 0
+    llc = H.llc_bytes()
     for g in data:
         for k in KERNELS:
             for r in REUSE:
@@ -339,6 +398,10 @@ def evaluate(ctx: gbk.Context, sel: "Selector", data: Sequence[dict], margin: Op
                 p = sel.pick(ctx, g["features"], k, r, margin)
                 picks.append(p)
                 res[r]["model"].append(e[p] / best - 1)
+                # the rule's deployable pick: DBG on its fallback branch, and DBG
+                # again where it asks for Leiden-Gorder8, not produced here
+                if "tier0" in g and rule_decision(g, k, r, llc):
+                    leiden += 1
                 res[r]["fallback"].append(e[FALLBACK] / best - 1)
                 res[r]["original"].append(e[0] / best - 1)
 
@@ -351,6 +414,7 @@ def evaluate(ctx: gbk.Context, sel: "Selector", data: Sequence[dict], margin: Op
     out["by_reuse"] = {str(r): {k: summary(res[r][k]) for k in names} for r in REUSE}
     out["picks"] = {ORDER_NAMES[a]: int(c) for a, c in zip(*np.unique(picks, return_counts=True))}
     out["cases"] = len(picks)
+    out["rule_asks_leiden_gorder8"] = leiden
     return out
 
 
@@ -369,7 +433,7 @@ def main(argv=None):
     import argparse
     import sys
     p = argparse.ArgumentParser(description="collect timings / train / evaluate the ordering selector")
-    p.add_argument("cmd", choices=["collect", "remeasure", "train", "evaluate"])
+    p.add_argument("cmd", choices=["collect", "remeasure", "remap", "train", "evaluate"])
     p.add_argument("--graphs", type=int, default=96)
     p.add_argument("--min-scale", type=int, default=12)
     p.add_argument("--max-scale", type=int, default=15)
@@ -394,6 +458,13 @@ def main(argv=None):
             specs = [row["spec"] for row in json.load(f)]
         end = len(specs) if a.count < 0 else min(len(specs), a.start + a.count)
         collect(ctx, specs[a.start:end], a.out or a.corpus, log)
+        return 0
+    if a.cmd == "remap":
+        with open(a.corpus) as f:
+            data = json.load(f)
+        remap(ctx, data, log)
+        with open(a.out or a.corpus, "w") as f:
+            json.dump(data, f)
         return 0
     data = []
     for path in a.corpus.split(","):

@@ -291,6 +291,15 @@ int gbk_pr_many(gbk_ctx *ctx, const gbk_graph *g, const int64_t *vbase, int coun
  * GBK_NUM_FEATURES, or a negative error code. */
 int gbk_graph_features(gbk_ctx *ctx, const gbk_graph *g, float *features, int count);
 
+/* The adaptive selector's Tier-0 features (ComputeTier0SampledGraphFeatures,
+ * reorder/reorder_types.h:6066-6195) computed on the device: feat receives 7
+ * doubles, avg_degree, degree_cv, hub_concentration, normalized_edge_span,
+ * window_neighbor_overlap, packing_factor, internal_density (the order of
+ * gbh_tier0_features).  sample_size 0 takes the reference's default
+ * max(1024, min(sqrt(n), 8192)); at most 8192.  Equal to the host
+ * restatement bit for bit except the edge span (a few ulps). */
+int gbk_tier0_features(gbk_ctx *ctx, const gbk_graph *g, int64_t sample_size, uint64_t seed, double *feat);
+
 typedef struct gbk_mlp gbk_mlp;
 /* in_dim in [1, 48]; Glorot-uniform weights from 'seed', zero biases. */
 int gbk_mlp_create(gbk_ctx *ctx, int in_dim, uint64_t seed, gbk_mlp **out);

@@ -16,6 +16,8 @@
  *   gbh_source_picker                     SourcePicker::GenerateConsistentSources gapbs/benchmark.h:115-156
  *   gbh_tier0_features / gbh_frozen_rule  ComputeTier0SampledGraphFeatures,
  *                                         ShouldUseAllKernelLowReuseRule         reorder/reorder_types.h:6066-6195, reorder/reorder_adaptive.h:302-333
+ *   gbh_budgeted_rule / gbh_llc_bytes     ShouldUseBudgetedLeidenGorder,
+ *                                         GetLLCSizeBytes                        reorder/reorder_adaptive.h:240-257, reorder/reorder_types.h:6030-6038
  * All calls return 0 on success and a negative code on bad input.
  */
 #ifndef GBK_HOST_H_
@@ -91,6 +93,11 @@ int gbh_tier0_features(int64_t n, const int64_t *off, const int32_t *nbr, int64_
 int64_t gbh_modeled_property_bytes(int bench, int64_t n, int64_t m);
 int gbh_frozen_rule(int bench, double reuse, int64_t n, double avg_degree, double degree_cv,
                     double wsr);
+/* ShouldUseBudgetedLeidenGorder (reorder/reorder_adaptive.h:240-257). */
+int gbh_budgeted_rule(int bench, double reuse, int64_t n, double avg_degree, double degree_cv,
+                      double hub_concentration);
+/* GetLLCSizeBytes (reorder/reorder_types.h:6030-6038): host L3, else L2, else 30 MB. */
+int64_t gbh_llc_bytes(void);
 
 /* ---- built-in verifiers (gapbs/benchmark.h:475-485) and fingerprints ---- */
 /* BFSVerifier (graphbrew/bfs_common.h:31-97): 1 = PASS. */

@@ -8,11 +8,13 @@ Run in the build container, where the reference tree exists:
 The vectors are data only: sha256 digests of the reference's outputs (edge
 lists, weights, relabelled CSRs, mappings), the Tier-0 feature values and the
 modeled property bytes, on inputs every test can rebuild (the oracle's
-generator and CSR build, seeded numpy integer draws).  The two mapping files
+generator and CSR build, seeded numpy integer draws).  The mapping files
 are the reference's CPU reorderers on Kronecker scale 16 (edge factor 16):
-RCM in its "bnf" variant (-o 11:bnf) and the GraphBrew Leiden-Gorder8
-composition (-o 12:leiden:compose:sg_none:comm_size_desc:intra_gorder:gw8),
-as binary int32 MAP files (new_ids[old] = new), the format -o 13 reads.
+RCM in its "bnf" variant (-o 11:bnf), the GraphBrew Leiden-Gorder8
+composition (-o 12:leiden:compose:sg_none:comm_size_desc:intra_gorder:gw8)
+and the two adaptive rules' Leiden-Gorder arms, as binary int32 MAP files
+(new_ids[old] = new), the format -o 13 reads; the drivers find them by name
+under GBK_MAPPING_DIR.
 """
 from __future__ import annotations
 
@@ -107,7 +109,16 @@ def main() -> None:
     os.makedirs(MAPS, exist_ok=True)
     g16 = O.kron(16)
     files = {}
-    for tag, ids in (("rcm_bnf", R.mapping(g16, 11)), ("leiden_gorder8", R.graphbrew_mapping(g16))):
+    arms = {
+        # the all-kernel low-reuse rule's arm and the budgeted rule's arm
+        # (reorder_adaptive.h:344-371, 262-283): parallel community
+        # detection, so a run's output is one valid mapping among several
+        "leiden_gorder8_lowreuse": R.GRAPHBREW_LEIDEN_GORDER8 + ":cd_parallel:sgmb4096:gordf5000:norefine:2:2",
+        "leiden_gorder32_budgeted": "leiden:compose:sg_none:comm_identity:intra_gorder:gw32:gordf500:cd_parallel:1:1",
+    }
+    made = [("rcm_bnf", R.mapping(g16, 11)), ("leiden_gorder8", R.graphbrew_mapping(g16))]
+    made += [(tag, R.graphbrew_mapping(g16, spec)) for tag, spec in arms.items()]
+    for tag, ids in made:
         assert O.validate_permutation(ids) == 0
         path = os.path.join(MAPS, f"kron16_ef16_{tag}.so")
         ids.astype("<i4").tofile(path)

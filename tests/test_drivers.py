@@ -171,3 +171,104 @@ def test_driver_symmetrizes_directed_sg(tmp_path):
     r = run("cc", "-f", p, "-s", "-n", 1, "-v")
     assert r.returncode == 0, r.stdout + r.stderr
     assert labels(r.stdout)["Verification"] == ["PASS"]
+
+
+# ------------------------------------------- reference reorderers and -o 14
+MAPS = os.path.join(ROOT, "tests", "golden", "mappings")
+LEIDEN8 = "leiden:compose:sg_none:comm_size_desc:intra_gorder:gw8"
+
+
+def run_env(name, *args, env=None, timeout=180):
+    exe = os.path.join(BIN, name)
+    if not os.path.exists(exe):
+        pytest.skip("drivers not built")
+    e = dict(os.environ)
+    e.pop("GBK_MAPPING_DIR", None)
+    e.update(env or {})
+    print("driver:", name, *args, flush=True)
+    return subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=timeout, env=e)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spec,tag", [("11:bnf", "rcm_bnf"), ("12:" + LEIDEN8, "leiden_gorder8")])
+def test_reference_reorderer_mapping_files(spec, tag):
+    """-o 11:bnf and -o 12:<Leiden-Gorder8> take the reference reorderers'
+    mappings from GBK_MAPPING_DIR; the applied mapping is the file's
+    (fingerprint) and BFS verifies on the relabelled graph."""
+    r = run_env("bfs", "-g", 16, "-n", 2, "-v", "-o", spec, env={"GBK_MAPPING_DIR": MAPS})
+    assert r.returncode == 0, r.stdout + r.stderr
+    L = labels(r.stdout)
+    ids = H.mapping_read(os.path.join(MAPS, f"kron16_ef16_{tag}.so"))
+    assert L["Mapping Fingerprint"] == [H.mapping_fingerprint(ids)]
+    assert L["Reorder Mapping File"] == [f"kron16_ef16_{tag}.so"]
+    assert L["Verification"] == ["PASS", "PASS"]
+    r = run_env("bfs", "-g", 16, "-n", 1, "-o", spec)
+    assert r.returncode != 0 and "-o 13:<file>" in (r.stderr + r.stdout)
+
+
+def _rule_expectation(kernel, reuse, rule="allkernel-lowreuse-rule"):
+    g = O.kron(16)
+    t0 = H.tier0_features(g.off, g.nbr)
+    hit, wsr = H.rule_pick(rule, kernel, reuse, g.n, g.m, t0)
+    return g, hit, wsr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel,reuse", [("bfs", 1), ("pr_spmv", 2), ("cc", 1), ("sssp", 3)])
+def test_adaptive_frozen_rule(kernel, reuse):
+    """-o 14:_:_:_:allkernel-lowreuse-rule:best-endtoend:<reuse> decides by
+    the reference's predicate on the device Tier-0 features; its Leiden-Gorder8
+    arm comes from the mapping file, else DBG stands in; the fallback is DBG."""
+    g, hit, wsr = _rule_expectation(kernel, reuse)
+    spec = f"14:_:_:_:allkernel-lowreuse-rule:best-endtoend:{reuse}"
+    dbg = H.mapping_fingerprint(H.mapping_dbg(g.off))
+    arm = H.mapping_fingerprint(H.mapping_read(os.path.join(MAPS, "kron16_ef16_leiden_gorder8_lowreuse.so")))
+    for env, want in (({"GBK_MAPPING_DIR": MAPS}, arm if hit else dbg), ({}, dbg)):
+        r = run_env(kernel, "-g", 16, "-n", 1, "-o", spec, env=env)
+        assert r.returncode == 0, r.stdout + r.stderr
+        L = labels(r.stdout)
+        assert L["Selection"] == ["model=allkernel-lowreuse-rule criterion=best-endtoend"]
+        assert float(L["Property WSR LLC"][0]) == pytest.approx(wsr, abs=6e-6)  # printed %3.5lf
+        assert L["Adaptive Rule"] == ["allkernel-lowreuse-rule " + ("match" if hit else "fallback")]
+        assert L["Mapping Fingerprint"] == [want], (env, hit)
+
+
+@pytest.mark.gpu
+def test_adaptive_budgeted_rule_and_learned_pick(ctx):
+    """budgeted-rule follows ShouldUseBudgetedLeidenGorder; the default
+    (perceptron) model is the learned selector, whose pick equals the Python
+    selector's for the same graph, kernel and reuse."""
+    g, hit, _ = _rule_expectation("pr", 1, "budgeted-rule")
+    r = run_env("pr", "-g", 16, "-n", 1, "-o", "14:_:_:_:budgeted-rule:best-endtoend:1",
+                env={"GBK_MAPPING_DIR": MAPS})
+    assert r.returncode == 0, r.stdout + r.stderr
+    L = labels(r.stdout)
+    assert L["Adaptive Rule"] == ["budgeted-rule " + ("match" if hit else "fallback")]
+    from graphbrew_amd import selector as S
+    sel = S.Selector.load()
+    dg = ctx.upload_graph(g)
+    feats = ctx.features(dg)
+    for kernel, reuse in (("bfs", 1), ("cc_sv", 2), ("pr", 2)):
+        want = sel.pick(ctx, feats, kernel, reuse)
+        r = run_env(kernel, "-g", 16, "-n", 1, "-o", f"14:_:_:_:perceptron:best-endtoend:{reuse}")
+        assert r.returncode == 0, r.stdout + r.stderr
+        L = labels(r.stdout)
+        assert "Adaptive Learned Prediction" in L
+        assert L["Adaptive Selected"] == [str(want)], (kernel, reuse)
+    dg.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spec,msg", [
+    ("14:x", "AdaptiveOrder reserved fields must be empty"),
+    ("14:_:_:_:allkernel-lowreuse-rule", "requires an explicit reuse count"),
+    ("14:_:_:_:allkernel-lowreuse-rule:fastest-execution:1", "requires best-endtoend criterion"),
+    ("14:_:_:_:dt:e2e:1", "offline-only"),
+    ("14:_:_:_:perceptron:e2e:0", "reuse count must be positive"),
+    ("14:_:_:_:perceptron:e2e:1:extra", "no longer accepts"),
+    ("14:_:_:_:nosuch", "Unknown adaptive selection model"),
+])
+def test_adaptive_policy_errors(spec, msg):
+    """ParseDeployableSelectionPolicy's refusals (reorder_adaptive.h:165-237)."""
+    r = run_env("bfs", "-g", 10, "-n", 1, "-o", spec)
+    assert r.returncode != 0 and msg in (r.stderr + r.stdout), r.stderr + r.stdout
