@@ -242,6 +242,9 @@ int sort_rows(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_t *nbr, c
               int32_t *wgt_out);
 // Stable compaction of ids [0,n) where flag(i) != 0 (flags given as int32).
 int compact_flags(gbk_ctx *ctx, const int32_t *flags, int64_t n, int32_t *out, int64_t *count);
+// The reference's random mapping (SplitMix64 Fisher-Yates, reorder_basic.h:
+// 104-135) of n vertices into new_ids (device), gbk_shuffle.hip.
+int random_mapping(gbk_ctx *ctx, int64_t n, int32_t *new_ids);
 // Heavy-row list for a CSR.
 int build_heavy_list(gbk_ctx *ctx, const int64_t *off, int64_t n, int64_t threshold,
                      int32_t **list, int64_t *count, int64_t *max_degree,

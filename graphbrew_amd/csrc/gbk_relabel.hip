@@ -670,6 +670,7 @@ static int device_mapping(gbk_ctx *ctx, const gbk_graph *g, int algo, int32_t *i
     GBK_LAUNCH_CHECK(ctx, "mapping_identity");
     return GBK_OK;
   }
+  if (algo == 1) return random_mapping(ctx, n, ids_d);
   auto *k0 = static_cast<uint64_t *>(scratch(ctx, kS10, sizeof(uint64_t) * n));
   auto *k1 = static_cast<uint64_t *>(scratch(ctx, kS11, sizeof(uint64_t) * n));
   auto *so = static_cast<int64_t *>(scratch(ctx, kS9, 2 * sizeof(int64_t)));
@@ -694,7 +695,7 @@ static int device_mapping(gbk_ctx *ctx, const gbk_graph *g, int algo, int32_t *i
 }
 
 static bool device_mapping_supported(int algo) {
-  return algo == 0 || algo == 2 || algo == 5 || algo == 6 || algo == 7;
+  return algo == 0 || algo == 1 || algo == 2 || algo == 5 || algo == 6 || algo == 7;
 }
 
 // Relabels 'in' by the mapping already in ids_d (device).

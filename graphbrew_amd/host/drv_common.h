@@ -381,7 +381,7 @@ struct Session {
     }
     // the orderings that sort by an out-degree key are made on the device
     static const std::unordered_map<int, const char *> device_made = {
-        {0, "Original Map Time"}, {2, "Sort Map Time"}, {5, "DBG Map Time"},
+        {0, "Original Map Time"}, {1, "Random Map Time"}, {2, "Sort Map Time"}, {5, "DBG Map Time"},
         {6, "HubSortDBG Map Time"}, {7, "HubClusterDBG Map Time"}};
     const auto dm = device_made.find(algo);
     if (dm != device_made.end() && opt.empty()) {
@@ -394,6 +394,7 @@ struct Session {
       double ms = 0;
       Check(gbk_mapping(ctx, g, algo, ids.data(), &ms), "mapping");
       t.Stop();
+      if (algo == 1) PrintLabel("Random Seed", "0");
       PrintTime(dm->second, t.Seconds());
       return ids;
     }

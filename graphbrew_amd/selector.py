@@ -39,7 +39,7 @@ REUSE = (1, 2)
 FALLBACK = 5  # DBG: the frozen rule's deployable pick here
 # orderings the drivers make on the device (gbk_mapping / gbk_relabel_by):
 # their mapping cost is device time; the others are made on the host
-DEVICE_MAPPED = (2, 5, 6, 7)
+DEVICE_MAPPED = (1, 2, 5, 6, 7)
 IN_DIM = gbk.NUM_FEATURES + len(ORDERINGS) + len(KERNELS) + 1
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_model.json")
 MODEL_TEXT_PATH = os.path.splitext(MODEL_PATH)[0] + ".txt"

@@ -169,12 +169,14 @@ int gbk_graph_free(gbk_ctx *ctx, gbk_graph *g);
  * GBK_EPERM_MAP and creates nothing. */
 int gbk_relabel(gbk_ctx *ctx, const gbk_graph *in, const int32_t *new_ids,
                 gbk_graph **out, gbk_relabel_stats *stats);
-/* Mapping producers on the device (the cheap orderings whose result is a
- * stable sort by an out-degree key): algo 0 original (reorder_basic.h:65-79),
- * 2 sort (reorder_basic.h:182-258), 5 DBG (reorder_hub.h:267-345), 6
- * HubSortDBG (reorder_hub.h:380-502), 7 HubClusterDBG (reorder_hub.h:505-582);
- * other orderings return GBK_EINVAL (random and the hub placements are
- * sequential and stay on the host).  new_ids (host, n) receives the mapping,
+/* Mapping producers on the device: algo 0 original (reorder_basic.h:65-79),
+ * 1 random (reorder_basic.h:104-135: the SplitMix64 Fisher-Yates shuffle,
+ * run as deterministic reservations, the same permutation), and the
+ * orderings whose result is a stable sort by an out-degree key: 2 sort
+ * (reorder_basic.h:182-258), 5 DBG (reorder_hub.h:267-345), 6 HubSortDBG
+ * (reorder_hub.h:380-502), 7 HubClusterDBG (reorder_hub.h:505-582); other
+ * orderings return GBK_EINVAL (the hub placements of 3 and 4 move vertices
+ * one after another and stay on the host).  new_ids (host, n) receives the mapping,
  * equal to gbh_mapping_*; device_ms (may be NULL) its device time. */
 int gbk_mapping(gbk_ctx *ctx, const gbk_graph *g, int algo, int32_t *new_ids, double *device_ms);
 /* gbk_mapping then gbk_relabel without the host round trip of the mapping

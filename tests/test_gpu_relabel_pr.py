@@ -359,7 +359,7 @@ MAP_GRAPHS = {
 def test_device_mappings_equal_host(ctx, name):
     g = MAP_GRAPHS[name]()
     dg = ctx.upload_graph(g)
-    for algo in (0, 2, 5, 6, 7):
+    for algo in (0, 1, 2, 5, 6, 7):
         ids, _ = ctx.mapping(dg, algo)
         assert np.array_equal(ids, H.mapping(algo, g.off)), (name, algo)
 
@@ -367,13 +367,32 @@ def test_device_mappings_equal_host(ctx, name):
 def test_relabel_by_equals_relabel(ctx):
     g = O.kron(13, weighted=True)
     dg = ctx.upload_graph(g)
-    for algo in (2, 5, 6):
+    for algo in (1, 2, 5, 6):
         r1, _, ids, _ = ctx.relabel_by(dg, algo)
         assert np.array_equal(ids, H.mapping(algo, g.off))
         _assert_same_graph(r1, O.relabel(g, ids))
         r1.free()
     with pytest.raises(gbk.GbkError):
-        ctx.mapping(dg, 1)                      # random stays on the host
+        ctx.mapping(dg, 3)                      # HubSort stays on the host
+
+
+def _path(n):
+    u = np.arange(max(n - 1, 1), dtype=np.int32) % max(n, 1)
+    v = (u + 1) % max(n, 1)
+    return O.make_graph(u[:max(n - 1, 0)], v[:max(n - 1, 0)], n=n)
+
+
+@pytest.mark.parametrize("n", [2, 3, 1000, 70001, 1 << 22])
+def test_device_random_mapping_equals_reference(ctx, n):
+    """-o 1 on the device (parallel Fisher-Yates by deterministic
+    reservations) equals the host's SplitMix64 Fisher-Yates, which equals the
+    reference's GenerateRandomMapping (tests/test_oracle_ref.py)."""
+    g = _path(n)
+    dg = ctx.upload_graph(g)
+    ids, ms = ctx.mapping(dg, 1)
+    assert np.array_equal(ids, H.mapping_random(n)), n
+    print(f"random mapping n={n}: {ms:.3f} ms on the device", flush=True)
+    dg.free()
 
 
 @pytest.mark.parametrize("blocks", [4, 16])
