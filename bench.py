@@ -54,6 +54,8 @@ def parse():
                    help="first copy MIB MiB device to device (calibrates HBM byte counters)")
     p.add_argument("--many-graphs", type=int, default=256,
                    help="small seeded R-MAT graphs (scales 10-16, ef 8-32) run as one batch; 0 skips")
+    p.add_argument("--no-legs", action="store_true",
+                   help="skip the other orderings and the one-source calls after the timed region")
     p.add_argument("--no-selector", action="store_true",
                    help="skip the learned-order vs identity end-to-end comparison")
     p.add_argument("--single-source", action="store_true",
@@ -150,7 +152,7 @@ class Workload:
         if not hasattr(self, "srcs"):
             off = g.download()["off"]
             org = g.download()["org_ids"]
-            self.srcs = H.source_picker(off, max(a.bfs_sources, a.sssp_sources, a.bc_sources), org)[1]
+            self.srcs_orig, self.srcs = H.source_picker(off, max(a.bfs_sources, a.sssp_sources, a.bc_sources), org)
             self.deg = np.diff(off)
         if "pr" in self.kernels:
             _, st = ctx.pr(g, gbk.PR_GS, 20, 1e-4, False, download=False)
@@ -212,6 +214,11 @@ def kt_table(gbk, single, msbfs_group=32):
     return t
 
 
+def workload_name(args):
+    return (f"{'uniform' if args.uniform else 'kronecker'}-scale{args.scale}-ef{args.degree}"
+            f" relabel+pr+bfs{args.bfs_sources}+cc+cc_sv+sssp{args.sssp_sources}+bc{args.bc_sources}")
+
+
 def launch_bytes(family, kt, work, rec):
     """Algorithmic bytes of one launch of the followed kernel.
 
@@ -231,16 +238,17 @@ def launch_bytes(family, kt, work, rec):
     return rec[family][2] / max(kt["launches"], 1), "per-source figure / launches"
 
 
-def load_traffic(workload, family, single):
-    """HBM bytes per launch of the followed kernel, measured with rocprofv3
-    PMC counters for this workload and mode (profiles/traffic.json)."""
+def load_traffic(workload, family, single, field="bytes_per_launch"):
+    """HBM bytes per launch of the followed kernel (per call for the BFS
+    batch), measured with rocprofv3 PMC counters for this workload and mode
+    (profiles/traffic.json)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
     e = t.get(f"{workload}|{family}|{'single' if single else 'batched'}")
-    return None if e is None else e.get("bytes_per_launch")
+    return None if e is None else e.get(field)
 
 
 def selector_e2e(work, gbk, H):
@@ -333,45 +341,157 @@ def many_graphs(ctx, gbk, count):
     return out
 
 
-def cpu_baseline(work, budget_s):
-    """CPU oracle (OpenMP restatement of the reference kernels) on a bounded
-    sample of the same relabelled graph: PR 2 fixed iterations, then BFS and
-    SSSP sources one at a time, CC once, BC one source, until the budget."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(work, budget_s, kern, gpu_scores):
+    """The CPU oracle (an OpenMP restatement of the reference kernels) on the
+    same workload as the GPU step: both relabels from the same graphs, PR
+    Gauss-Seidel to the same tolerance, Afforest, SV, and BFS / SSSP (the
+    same delta) / BC from the same SourcePicker sources.  The per-source
+    kernels run on a bounded sample of the sources and are scaled to the
+    step's source counts.  value = the GPU step's traversed edges / the CPU
+    step time.  Also the checker of the GPU run's PageRank: its L1 distance
+    from the FP64 fixed point (the reference prints its own L1 change,
+    pr.cc:175)."""
     import oracle as O
+    a = work.args
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    d = work.rg.download()
-    g = O.Graph(n=work.rg.n, off=d["off"], nbr=d["nbr"], org_ids=d["org_ids"])
-    dw = work.rgw.download()
-    gw = O.Graph(n=work.rgw.n, off=dw["off"], nbr=dw["nbr"], wgt=dw["wgt"], org_ids=dw["org_ids"])
-    srcs = work.srcs
-    deg = np.diff(g.off)
-    edges, secs, parts = 0, 0.0, []
+    gi, gwi = work.g.download(), work.gw.download()
+    g0 = O.Graph(n=work.g.n, off=gi["off"], nbr=gi["nbr"])
+    gw0 = O.Graph(n=work.gw.n, off=gwi["off"], nbr=gwi["nbr"], wgt=gwi["wgt"])
+    del gi, gwi
     t = time.time()
-    O.pr_gs(g, max_iters=2, tol=0.0, fixed=True, threads=threads)
-    dt = time.time() - t
-    edges += 2 * g.m
-    secs += dt
-    parts.append("pr x2 iters")
+    g = O.relabel(g0, work.ids)
+    gw = O.relabel(gw0, work.ids)
+    rel_s = time.time() - t
+    del g0, gw0
+    out = {"relabel": {"s": rel_s, "edges": 2 * g.m}}
+    t = time.time()
+    _, it, _ = O.pr_gs(g, max_iters=20, tol=1e-4, threads=threads)
+    out["pr"] = {"s": time.time() - t, "edges": g.m * it, "iterations": it}
     t = time.time()
     O.cc_afforest(g, threads=threads)
-    secs += time.time() - t
-    edges += g.m
-    parts.append("cc afforest")
-    i = 0
-    while secs < budget_s and i < len(srcs):
-        s = int(srcs[i])
+    out["cc"] = {"s": time.time() - t, "edges": g.m}
+    t = time.time()
+    O.cc_sv(g, threads=threads)
+    out["cc_sv"] = {"s": time.time() - t, "edges": None}
+    deg = np.diff(g.off)
+    t = time.time()
+    O.bc(g, work.srcs[:a.bc_sources], threads=threads)
+    out["bc"] = {"s": time.time() - t, "edges": None}
+    spent = sum(v["s"] for v in out.values())
+    # per-source kernels: sources one at a time until the budget is used
+    bfs_s = sssp_s = 0.0
+    k = 0
+    delta = a.delta if a.single_source else a.batch_delta
+    while k < max(a.bfs_sources, a.sssp_sources) and (k < 2 or spent + bfs_s + sssp_s < budget_s):
+        s = int(work.srcs[k])
         t = time.time()
-        p = O.bfs_do(g, s, threads=threads)
-        secs += time.time() - t
-        edges += int(deg[p >= 0].sum())
+        O.bfs_do(g, s, threads=threads)
+        bfs_s += time.time() - t
         t = time.time()
-        dist = O.sssp_delta(gw, s, work.args.delta, threads=threads)
-        secs += time.time() - t
-        edges += int(deg[dist != O.K_DIST_INF].sum())
-        i += 1
-    parts.append(f"bfs+sssp x{i} sources")
-    return {"value": edges / secs / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": f"scale-{work.args.scale} graph, {', '.join(parts)}; {secs:.1f}s of CPU work"}
+        O.sssp_delta(gw, s, delta, threads=threads)
+        sssp_s += time.time() - t
+        k += 1
+    out["bfs"] = {"s": bfs_s * a.bfs_sources / k, "edges": None, "sampled_sources": k}
+    out["sssp"] = {"s": sssp_s * a.sssp_sources / k, "edges": None, "sampled_sources": k}
+    step_s = sum(v["s"] for v in out.values())
+    step_edges = sum(kern[name]["edges_per_step"] for name in kern)
+    per = {}
+    for name, v in out.items():
+        e = kern.get(name, {}).get("edges_per_step")
+        per[name] = {"s_per_step": round(v["s"], 4),
+                     "gteps": round(e / v["s"] / 1e9, 4) if e and v["s"] > 0 else None,
+                     "gpu_speedup": round(v["s"] * 1e3 / kern[name]["ms_per_step"], 1) if name in kern else None}
+        for extra in ("iterations", "sampled_sources"):
+            if extra in v:
+                per[name][extra] = v[extra]
+    res = {"value": step_edges / step_s / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+           "cpu": cpu_model(),
+           "sample": (f"the scale-{a.scale} step on the same graphs: relabel x2, pr (gs, tol 1e-4), cc, cc_sv, "
+                      f"bc x{a.bc_sources}; bfs and sssp (delta {delta}) on {k} of the sources, scaled to "
+                      f"{a.bfs_sources} / {a.sssp_sources}; {step_s:.1f}s of CPU per step"),
+           "kernels": per}
+    if gpu_scores is not None:
+        fp = O.pr_fp64(g)
+        res["pr_l1_error_vs_fp64"] = float(np.abs(gpu_scores.astype(np.float64) - fp).sum())
+    return res
+
+
+def orderings_leg(work, gbk, orders=(1, 2, 5)):
+    """The step's kernels once under other orderings made on the device
+    (random, degree sort, DBG): mapping and relabel device time and each
+    kernel's device ms and GTEPS, next to the identity step.  After the
+    timed region."""
+    from graphbrew_amd import selector as S
+    ctx, a = work.ctx, work.args
+    src_orig = work.srcs_orig
+    out = {}
+    for algo in orders:
+        rg, st, ids, map_ms = ctx.relabel_by(work.g, algo)
+        rgw, stw, _, _ = ctx.relabel_by(work.gw, algo)
+        srcs = ids[src_orig]
+        k = {}
+        _, p = ctx.pr(rg, gbk.PR_GS, 20, 1e-4, False, download=False)
+        k["pr"] = (p.device_ms, rg.m * p.iterations)
+        _, _, _, b = ctx.bfs_batch(rg, srcs[:a.bfs_sources], download=False)
+        k["bfs"] = (b.device_ms, b.edges_reached)
+        _, c = ctx.cc(rg, gbk.CC_AFFOREST, download=False)
+        k["cc"] = (c.device_ms, rg.m)
+        _, c = ctx.cc(rg, gbk.CC_SV, download=False)
+        k["cc_sv"] = (c.device_ms, rg.m * c.iterations)
+        _, _, _, q = ctx.sssp_batch(rgw, srcs[:a.sssp_sources], a.batch_delta, download=False)
+        k["sssp"] = (q.device_ms, q.edges_reached)
+        _, bc = ctx.bc_batch(rg, srcs[:a.bc_sources], download=False)
+        k["bc"] = (bc.device_ms, 2 * bc.edges_reached)
+        step = st.device_ms + stw.device_ms + sum(ms for ms, _ in k.values())
+        edges = rg.m + rgw.m + sum(e for _, e in k.values())
+        out[S.ORDER_NAMES[algo]] = {
+            "mapping_ms": round(map_ms, 3), "relabel_ms": round(st.device_ms + stw.device_ms, 3),
+            "step_ms": round(step, 3), "gteps": round(edges / step / 1e6, 2),
+            "kernels_ms": {n: round(ms, 3) for n, (ms, _) in k.items()}}
+        rg.free()
+        rgw.free()
+    return out
+
+
+def single_source_leg(work, gbk):
+    """The kernels the drop-in drivers run one source at a time: BFS and
+    SSSP from each of the step's sources (one call per source, delta of
+    --delta) and BC with one traversal per source.  After the timed region."""
+    ctx, a = work.ctx, work.args
+    g, gw, n = work.rg, work.rgw, work.rg.n
+    out = {}
+    ms = edges = nbytes = 0.0
+    for s in work.srcs[:a.bfs_sources]:
+        _, st = ctx.bfs(g, int(s), download=False)
+        ms += st.device_ms
+        edges += st.edges_reached
+        nbytes += bytes_bfs(n, st.edges_reached)
+    out["bfs"] = {"ms": round(ms, 3), "sources": a.bfs_sources, "gteps": round(edges / ms / 1e6, 2),
+                  "hbm_frac": round(nbytes / (ms / 1e3) / HBM_PEAK, 4)}
+    ms = edges = nbytes = 0.0
+    for s in work.srcs[:a.sssp_sources]:
+        _, st = ctx.sssp(gw, int(s), a.delta, download=False)
+        ms += st.device_ms
+        edges += st.edges_reached
+        nbytes += bytes_sssp(n, st.edges_reached)
+    out["sssp"] = {"ms": round(ms, 3), "sources": a.sssp_sources, "delta": a.delta,
+                   "gteps": round(edges / ms / 1e6, 2), "hbm_frac": round(nbytes / (ms / 1e3) / HBM_PEAK, 4)}
+    _, st = ctx.bc(g, work.srcs[:a.bc_sources], download=False)
+    out["bc"] = {"ms": round(st.device_ms, 3), "sources": a.bc_sources,
+                 "gteps": round(2 * st.edges_reached / st.device_ms / 1e6, 2),
+                 "hbm_frac": round(bytes_bc(n, st.edges_reached) / (st.device_ms / 1e3) / HBM_PEAK, 4)}
+    return out
 
 
 def main():
@@ -462,9 +582,20 @@ def main():
         secs = ms / 1e3
         total_edges += edges
         kern[name] = {"ms_per_step": ms / args.steps, "gteps": edges / secs / 1e9 if secs else 0.0,
+                      "edges_per_step": edges / args.steps,
                       "gbps": nbytes / secs / 1e9 if secs else 0.0,
                       "hbm_frac": nbytes / secs / HBM_PEAK if secs else 0.0,
                       "calls_per_step": calls / args.steps}
+        if name == "bfs" and not args.single_source:
+            # the bit-parallel batch reads each edge once per level for all 64
+            # sources: the per-source byte formula x 64 is a one-source-
+            # equivalent rate, not traffic; the counted bytes of a batch call
+            # (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/traffic.json) give
+            # the fraction of the HBM rate actually moved
+            kern[name]["hbm_frac_one_source_equivalent"] = kern[name].pop("hbm_frac")
+            per_call = load_traffic(workload_name(args), "bfs", False, "bytes_per_call")
+            kern[name]["hbm_frac"] = (per_call * calls / secs / HBM_PEAK) if per_call and secs else None
+            kern[name]["hbm_frac_basis"] = "counted bytes per batch call" if per_call else "not counted"
     if scan and "sssp" in kern:
         # work the batch actually did: queued rows relaxed and their edges
         # walked (each edge once for all sources of the batch)
@@ -472,8 +603,7 @@ def main():
                              "edges_scanned_per_call": scan[3] / scan[0],
                              "edges_scanned_per_us": scan[3] / (rec["sssp"][0] * 1e3)})
     value = total_edges / (total_ms / 1e3) / 1e9
-    workload = (f"{'uniform' if args.uniform else 'kronecker'}-scale{args.scale}-ef{args.degree}"
-                f" relabel+pr+bfs{args.bfs_sources}+cc+cc_sv+sssp{args.sssp_sources}+bc{args.bc_sources}")
+    workload = workload_name(args)
     roof = None
     if family is not None and kt and kt["launches"]:
         per_launch, how = launch_bytes(family, kt, work, rec)
@@ -506,12 +636,22 @@ def main():
         "roofline": roof,
         "wall_s": round(wall, 2),
     }
+    gpu_scores = None
+    if "pr" in work.kernels:
+        gpu_scores, pst = ctx.pr(work.rg, gbk.PR_GS, 20, 1e-4, False)
+        out["pr_final_error"] = pst.final_error  # the L1 change of the last sweep (pr.cc:175)
+    if not args.no_legs and len(work.kernels) == 7:
+        out["orderings"] = orderings_leg(work, gbk)
+        if not args.single_source:
+            out["single_source"] = single_source_leg(work, gbk)
     if not args.no_selector:
         out["selector_e2e"] = selector_e2e(work, gbk, H)
     if args.many_graphs > 0:
         out["many_graphs"] = many_graphs(ctx, gbk, args.many_graphs)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(work, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(work, args.cpu_budget, kern, gpu_scores)
+        if "pr_l1_error_vs_fp64" in out["cpu_baseline"]:
+            out["pr_l1_error_vs_fp64"] = out["cpu_baseline"].pop("pr_l1_error_vs_fp64")
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
