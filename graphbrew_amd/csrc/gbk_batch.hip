@@ -39,6 +39,8 @@
 namespace gbk {
 
 constexpr int kBatchMax = 64;
+constexpr int kMsLevelsPerBlock = 4;   // BFS batch levels queued per host check
+constexpr int kBcbLevelsPerBlock = 4;  // BC batch forward levels queued per host check
 constexpr int32_t kBatchInf = INT32_MAX / 2;  // sssp.cc:67
 constexpr int64_t kBatchHeavy = 256;          // rows above this are chunked over groups
 constexpr int64_t kBatchChunk = 256;
@@ -204,11 +206,17 @@ __global__ void batch_collect_sum(const unsigned long long *partial, int grid, u
 }
 
 // ------------------------------------------------------------------- BFS
+// Level state of a BFS batch, on the device: the host queues levels blind
+// and reads it once per block of levels.
 struct MsCounters {
   unsigned long long tail;
   unsigned long long fverts;  // vertices in the new frontier
   unsigned long long fedges;  // their out-degree sum
   int overflow;
+  int done;                   // the last level found nothing: later launches return at once
+  int dir;                    // this level: 0 top-down, 1 bottom-up
+  int level;                  // depth this level assigns
+  int levels;                 // levels executed
   int pad;
 };
 
@@ -226,11 +234,21 @@ __global__ void msbfs_sources(const int32_t *src, int K, int Kp, unsigned long l
     c->fverts = 0;
     c->fedges = 0;
     c->overflow = 0;
+    c->done = 0;
+    c->dir = 0;
+    c->level = 1;
+    c->levels = 0;
   }
+}
+
+__device__ __forceinline__ bool ms_skip(const MsCounters *c, int dir) {
+  const volatile MsCounters *v = c;
+  return v->done || v->dir != dir;
 }
 
 __global__ void __launch_bounds__(kBlock)
 msbfs_front_queue(const unsigned long long *front, int64_t n, int32_t *queue, int64_t cap, MsCounters *c) {
+  if (ms_skip(c, 0)) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   WaveQueue q;
   q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
@@ -259,6 +277,7 @@ __global__ void __launch_bounds__(kBlock)
 msbfs_td(const int64_t *off, const int32_t *nbr, const int32_t *queue, const MsCounters *c,
          const unsigned long long *front, const unsigned long long *seen, unsigned long long *next,
          int64_t heavy_deg, HeavyQueue hq) {
+  if (ms_skip(c, 0)) return;
   MsTdEdge f{nbr, front, seen, next};
   auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
     s = off[u];
@@ -269,7 +288,8 @@ msbfs_td(const int64_t *off, const int32_t *nbr, const int32_t *queue, const MsC
 
 __global__ void __launch_bounds__(kBlock)
 msbfs_td_heavy(const int32_t *nbr, const unsigned long long *front, const unsigned long long *seen,
-               unsigned long long *next, HeavyQueue hq) {
+               unsigned long long *next, HeavyQueue hq, const MsCounters *c) {
+  if (ms_skip(c, 0)) return;
   MsTdEdge f{nbr, front, seen, next};
   expand_heavy(hq, f);
 }
@@ -279,7 +299,9 @@ msbfs_td_heavy(const int32_t *nbr, const unsigned long long *front, const unsign
 template <int G>
 __global__ void __launch_bounds__(kBlock)
 msbfs_bu(const int64_t *in_off, const int32_t *in_nbr, int64_t n, const unsigned long long *seen,
-         const unsigned long long *front, unsigned long long *next, unsigned long long full, int swz) {
+         const unsigned long long *front, unsigned long long *next, unsigned long long full, int swz,
+         const MsCounters *c) {
+  if (ms_skip(c, 1)) return;
   const int glane = lane_id() % G;
   const int64_t groups_per_block = kBlock / G;
   const int64_t tiles = (n + groups_per_block - 1) / groups_per_block;
@@ -312,7 +334,9 @@ msbfs_bu(const int64_t *in_off, const int32_t *in_nbr, int64_t n, const unsigned
 
 __global__ void msbfs_finish(const int64_t *off, int64_t n, int Kp, unsigned long long *seen,
                              unsigned long long *front, unsigned long long *next, int32_t *depth_vm,
-                             int32_t level, unsigned long long *p_verts, unsigned long long *p_edges) {
+                             const MsCounters *c, unsigned long long *p_verts, unsigned long long *p_edges) {
+  if (reinterpret_cast<const volatile MsCounters *>(c)->done) return;
+  const int32_t level = reinterpret_cast<const volatile MsCounters *>(c)->level;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   const int lane = lane_id();
   const int G = kWave / Kp;  // depth rows one store instruction covers
@@ -353,7 +377,11 @@ __global__ void msbfs_finish(const int64_t *off, int64_t n, int Kp, unsigned lon
   block_partial(edges, p_edges);
 }
 
-__global__ void msbfs_store(const unsigned long long *partial, int count, MsCounters *c) {
+// Ends a level: the new frontier's size and out-edges, then the next
+// level's direction (bfs.cc:231-262: bottom-up while the frontier's
+// out-edges exceed m / alpha) or the end of the traversal.
+__global__ void msbfs_store(const unsigned long long *partial, int count, MsCounters *c, long long bu_edges) {
+  if (reinterpret_cast<volatile MsCounters *>(c)->done) return;
   __shared__ unsigned long long red[2][1024 / kWave];
   unsigned long long a = 0, b = 0;
   for (int i = threadIdx.x; i < count; i += blockDim.x) {
@@ -376,22 +404,29 @@ __global__ void msbfs_store(const unsigned long long *partial, int count, MsCoun
     c->fverts = x;
     c->fedges = y;
     c->tail = 0;
+    c->levels += 1;
+    if (x == 0) {
+      c->done = 1;
+    } else {
+      c->level += 1;
+      c->dir = static_cast<long long>(y) > bu_edges ? 1 : 0;
+    }
   }
 }
 
 static void launch_msbfs_bu(gbk_ctx *ctx, int G, const gbk_graph *g, const unsigned long long *seen,
                             const unsigned long long *front, unsigned long long *next,
-                            unsigned long long full, unsigned grid) {
+                            unsigned long long full, unsigned grid, const MsCounters *c) {
   const int swz = ctx->tune.xcd_swizzle;
   hipStream_t s = ctx->stream;
   switch (G) {
-    case 1: msbfs_bu<1><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
-    case 2: msbfs_bu<2><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
-    case 4: msbfs_bu<4><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
-    case 16: msbfs_bu<16><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
-    case 32: msbfs_bu<32><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
-    case 64: msbfs_bu<64><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
-    default: msbfs_bu<8><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz); break;
+    case 1: msbfs_bu<1><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz, c); break;
+    case 2: msbfs_bu<2><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz, c); break;
+    case 4: msbfs_bu<4><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz, c); break;
+    case 16: msbfs_bu<16><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz, c); break;
+    case 32: msbfs_bu<32><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz, c); break;
+    case 64: msbfs_bu<64><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz, c); break;
+    default: msbfs_bu<8><<<grid, kBlock, 0, s>>>(g->in_off, g->in_nbr, g->n, seen, front, next, full, swz, c); break;
   }
 }
 
@@ -746,14 +781,29 @@ __global__ void sb_split_done(BatchState *st) {
 // only where the vertex is at depth d for their source.  A lane's successor
 // test is depth[v] == depth[u] + 1, the relation the reference's successor
 // bitmap records (bc.cc:104-112), so no per-source bitmap is kept.
+// Forward state of a BC batch, on the device: the host queues levels blind
+// and reads it once per block of levels.  Level d's frontier is
+// queue[lb, le); lvl[d] holds the level starts (lvl[0] = 0).
 struct BcbCounters {
   unsigned long long tail;
   unsigned long long edges_scanned;
   unsigned long long listed_edges;  // out-edges of the level lists after the first
   unsigned long long open_edges;    // in-edges of vertices some source has not reached
+  long long lb, le;                 // the current level's list
+  long long listed;                 // listed_edges when the current level began
+  long long weighed;                // its frontier edges x bc_pull_ratio
   int overflow;
-  int pad;
+  int d;                            // current level
+  int done;                         // the last level listed nothing: later launches return at once
+  int pull;                         // this level runs bottom-up
+  int need_count;                   // the direction waits for the open-edge count
+  int levels;                       // levels executed
 };
+
+__device__ __forceinline__ bool bcb_skip(const BcbCounters *c, int pull) {
+  const volatile BcbCounters *v = c;
+  return v->done || v->need_count || v->pull != pull;
+}
 
 template <int Kp>
 __device__ __forceinline__ double group_sum_d(double v) {
@@ -872,13 +922,72 @@ __global__ void bcb_pack(const int32_t *depth, int64_t total, uint64_t *dc) {
     dc[i] = static_cast<uint64_t>(static_cast<uint32_t>(depth[i])) << 32;
 }
 
-__global__ void bcb_init_counters(BcbCounters *c, long long first) {
+__global__ void bcb_init_counters(BcbCounters *c, long long first, int pull, long long *lvl) {
   if (threadIdx.x == 0) {
     c->tail = static_cast<unsigned long long>(first);
     c->edges_scanned = 0;
     c->listed_edges = 0;
+    c->open_edges = 0;
+    c->lb = 0;
+    c->le = first;
+    c->listed = 0;
+    c->weighed = 0;
     c->overflow = 0;
+    c->d = 0;
+    c->done = 0;
+    c->pull = pull;
+    c->need_count = 0;
+    c->levels = 0;
+    lvl[0] = 0;
+    lvl[1] = first;
   }
+}
+
+// Ends forward level d: the next level's list is queue[le, tail); none ends
+// the traversal.  Otherwise the next level's direction (bfs.cc:231-262
+// weighs frontier edges against unexplored ones the same way): mode 2 always
+// bottom-up; mode 1 bottom-up while the frontier's out-edges (x ratio)
+// exceed the edges of the vertices some source has not reached -- bounded
+// below by m - listed, counted exactly (bcb_open_edges) past that bound
+// unless the rows are wide (count_wide 0), where a top-down frontier edge
+// costs more than the bound's error.
+__global__ void bcb_level_end(BcbCounters *c, long long *lvl, long long cap, long long m, int mode, long long ratio,
+                              int count_wide) {
+  if (threadIdx.x != 0 || c->done) return;
+  const long long tail = static_cast<long long>(c->tail);
+  if (c->overflow || tail > cap) {
+    c->overflow = 1;
+    c->done = 1;
+    return;
+  }
+  c->levels = c->d + 1;
+  if (tail == c->le) {
+    c->done = 1;
+    return;
+  }
+  lvl[c->d + 2] = tail;
+  const long long frontier = static_cast<long long>(c->listed_edges) - c->listed;
+  c->listed = static_cast<long long>(c->listed_edges);
+  c->lb = c->le;
+  c->le = tail;
+  c->d += 1;
+  c->pull = mode == 2;
+  c->weighed = frontier * ratio;
+  if (mode == 1 && c->weighed > m - c->listed) {
+    if (count_wide) {
+      c->open_edges = 0;
+      c->need_count = 1;
+    } else {
+      c->pull = 1;
+    }
+  }
+}
+
+// The direction of a level that waited for the open-edge count.
+__global__ void bcb_pull_decide(BcbCounters *c) {
+  if (threadIdx.x != 0 || c->done || !c->need_count) return;
+  c->pull = c->weighed > static_cast<long long>(c->open_edges);
+  c->need_count = 0;
 }
 
 __global__ void bcb_sources(const int32_t *src, int K, int Kp, int32_t *depth, double *sigma) {
@@ -910,8 +1019,11 @@ __device__ __forceinline__ void bcb_chunk(const HeavyQueue &hq, int64_t rows, in
 template <int Kp>
 __global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
 bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *ltag,
-            const int32_t *list, int64_t count, int32_t *queue, int64_t cap, int32_t d, BcbCounters *c,
-            HeavyQueue hq) {
+            int32_t *queue, int64_t cap, BcbCounters *c, HeavyQueue hq) {
+  if (bcb_skip(c, 0)) return;
+  const int32_t d = c->d;
+  const int32_t *list = queue + c->lb;
+  const int64_t count = c->le - c->lb;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   constexpr int G = kWave / Kp;
   const int grp = lane_id() / Kp;
@@ -950,7 +1062,9 @@ bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigm
 template <int Kp>
 __global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
 bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *ltag, int32_t *queue,
-                  int64_t cap, int32_t d, BcbCounters *c, HeavyQueue hq) {
+                  int64_t cap, BcbCounters *c, HeavyQueue hq) {
+  if (bcb_skip(c, 0)) return;
+  const int32_t d = c->d;
   int64_t rows, total;
   heavy_extent(hq, rows, total);
   if (rows == 0) return;
@@ -1017,7 +1131,9 @@ struct BcbPull {
 template <int Kp>
 __global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
 bcb_forward_pull(const int64_t *ioff, const int32_t *inbr, int64_t n, int K, int32_t *depth, double *sigma,
-                 int32_t *queue, int64_t cap, int32_t d, BcbCounters *c, HeavyQueue hq) {
+                 int32_t *queue, int64_t cap, BcbCounters *c, HeavyQueue hq) {
+  if (bcb_skip(c, 1)) return;
+  const int32_t d = c->d;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   constexpr int G = kWave / Kp;
   const int grp = lane_id() / Kp;
@@ -1063,8 +1179,10 @@ bcb_forward_pull(const int64_t *ioff, const int32_t *inbr, int64_t n, int K, int
 // the chunk's in-neighbours at depth d.
 template <int Kp>
 __global__ void __launch_bounds__(kBlock, kBcbWaves<Kp>)
-bcb_pull_heavy(const int32_t *inbr, int K, const int32_t *depth, const double *sigma, int32_t d, BcbCounters *c,
+bcb_pull_heavy(const int32_t *inbr, int K, const int32_t *depth, const double *sigma, BcbCounters *c,
                HeavyQueue hq, double *hpart) {
+  if (bcb_skip(c, 1)) return;
+  const int32_t d = c->d;
   int64_t rows, total;
   heavy_extent(hq, rows, total);
   if (rows == 0) return;
@@ -1100,8 +1218,10 @@ bcb_pull_heavy(const int32_t *inbr, int K, const int32_t *depth, const double *s
 // row joins the next level's list.
 template <int Kp>
 __global__ void __launch_bounds__(kBlock)
-bcb_pull_finish(int K, int32_t *depth, double *sigma, int32_t d, HeavyQueue hq, const double *hpart,
+bcb_pull_finish(int K, int32_t *depth, double *sigma, HeavyQueue hq, const double *hpart,
                 int32_t *queue, int64_t cap, BcbCounters *c) {
+  if (bcb_skip(c, 1)) return;
+  const int32_t d = c->d;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   WaveQueue q;
   q.init(seg + (threadIdx.x / kWave) * kSeg, &c->tail, queue, cap, &c->overflow);
@@ -1138,6 +1258,10 @@ bcb_pull_finish(int K, int32_t *depth, double *sigma, int32_t d, HeavyQueue hq, 
 // group of the wave and are OR-ed over shuffles.
 __global__ void __launch_bounds__(kBlock)
 bcb_open_edges(const int32_t *depth, int64_t n, int log_kp, int K, const int64_t *ioff, BcbCounters *c) {
+  {
+    const volatile BcbCounters *v = c;
+    if (v->done || !v->need_count) return;
+  }
   const int4 *d4 = reinterpret_cast<const int4 *>(depth);
   const int log_q = log_kp - 2;
   const int q = 1 << log_q;
@@ -1163,9 +1287,12 @@ bcb_open_edges(const int32_t *depth, int64_t n, int log_kp, int K, const int64_t
 }
 
 // Out-edges of the level list [le, tail) added into c->listed_edges: the
-// host compares the next frontier's edges with those not yet listed.
+// next level's direction compares the frontier's edges with those not yet
+// listed.
 __global__ void __launch_bounds__(kBlock)
-bcb_list_edges(const int64_t *off, const int32_t *queue, int64_t le, int64_t cap, BcbCounters *c) {
+bcb_list_edges(const int64_t *off, const int32_t *queue, int64_t cap, BcbCounters *c) {
+  if (reinterpret_cast<const volatile BcbCounters *>(c)->done) return;
+  const int64_t le = c->le;
   const int64_t tail = min(static_cast<int64_t>(c->tail), cap);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   unsigned long long e = 0;
@@ -1390,36 +1517,44 @@ int msbfs_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, int
   const unsigned n_grid = expand_grid(n);
   // the bottom-up step gives each vertex msbfs_group lanes (grid-stride beyond the cap)
   const unsigned bu_grid = expand_grid(n * static_cast<int64_t>(ctx->tune.msbfs_group));
-  const int64_t alpha = ctx->tune.bfs_alpha;
+  const long long bu_edges = g->m / ctx->tune.bfs_alpha;
+  const int G = ctx->tune.msbfs_group;
+  // one block of levels: every level queues both directions; the one the
+  // device state did not choose, and every level after the last, returns at
+  // once, so blocks are queued blind (and replayed from a captured graph)
+  // with one host check of the state per block
+  auto levels_block = [&]() {
+    for (int i = 0; i < kMsLevelsPerBlock; i++) {
+      const int kt = kt_begin(ctx, GBK_KT_BFS_STEP);
+      launch_msbfs_bu(ctx, G, g, seen, front, next, full, bu_grid, c);
+      fill<unsigned long long>(ctx, hq.count, 0ull, 1);
+      msbfs_front_queue<<<n_grid, kBlock, 0, s>>>(front, n, queue, cap, c);
+      msbfs_td<<<kGridCap, kBlock, 0, s>>>(g->off, g->nbr, queue, c, front, seen, next, ctx->tune.heavy_degree, hq);
+      msbfs_td_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, front, seen, next, hq, c);
+      kt_end(ctx, kt);
+      msbfs_finish<<<n_grid, kBlock, 0, s>>>(g->off, n, Kp, seen, front, next, dvm, c, partial, partial + kGridCap);
+      msbfs_store<<<1, 1024, 0, s>>>(partial, static_cast<int>(n_grid), c, bu_edges);
+    }
+  };
+  LoopGraph loop(ctx);
+  if (int rc = loop.capture(levels_block)) return rc;
   EventTimer timer(ctx);
   fill<unsigned long long>(ctx, seen, 0ull, n);
   fill<unsigned long long>(ctx, front, 0ull, n);
   fill<unsigned long long>(ctx, next, 0ull, n);
   fill<int32_t>(ctx, dvm, -1, n * Kp);
   msbfs_sources<<<1, kBatchMax, 0, s>>>(dsrc, K, Kp, seen, front, dvm, c);
-  int64_t fedges = 0, levels = 0;
-  for (int32_t level = 1; level <= n + 1; level++) {
-    const int kt = kt_begin(ctx, GBK_KT_BFS_STEP);
-    if (fedges > g->m / alpha) {
-      launch_msbfs_bu(ctx, ctx->tune.msbfs_group, g, seen, front, next, full, bu_grid);
-    } else {
-      fill<unsigned long long>(ctx, hq.count, 0ull, 1);
-      msbfs_front_queue<<<n_grid, kBlock, 0, s>>>(front, n, queue, cap, c);
-      msbfs_td<<<kGridCap, kBlock, 0, s>>>(g->off, g->nbr, queue, c, front, seen, next,
-                                           ctx->tune.heavy_degree, hq);
-      msbfs_td_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, front, seen, next, hq);
-    }
-    kt_end(ctx, kt);
-    msbfs_finish<<<n_grid, kBlock, 0, s>>>(g->off, n, Kp, seen, front, next, dvm, level, partial,
-                                           partial + kGridCap);
-    msbfs_store<<<1, 1024, 0, s>>>(partial, static_cast<int>(n_grid), c);
+  GBK_LAUNCH_CHECK(ctx, "bfs batch sources");
+  int64_t levels = 0;
+  for (int64_t queued = 0;; queued += kMsLevelsPerBlock) {
+    if (queued > n + 1) return set_error(ctx, GBK_EHIP, "bfs batch: more levels than vertices");
+    if (int rc = loop.run(levels_block)) return rc;
     GBK_LAUNCH_CHECK(ctx, "bfs batch level");
     GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(MsCounters), hipMemcpyDeviceToHost, s));
     GBK_HIP(ctx, hipStreamSynchronize(s));
     if (hc->overflow) return set_error(ctx, GBK_EHIP, "bfs batch: queue overflow");
-    levels++;
-    if (hc->fverts == 0) break;
-    fedges = static_cast<int64_t>(hc->fedges);
+    levels = hc->levels;
+    if (hc->done) break;
   }
   int64_t etot = 0;
   if (int rc = collect(ctx, g, dvm, Kp, K, -1, out, reached, edges, first, &etot)) return rc;
@@ -1584,60 +1719,55 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
                                    resident_grid(ctx, reinterpret_cast<const void *>(&bcb_backward_heavy<Kp>)));
   const unsigned grid_p = std::min(resident_grid(ctx, reinterpret_cast<const void *>(&bcb_forward_pull<Kp>)),
                                    resident_grid(ctx, reinterpret_cast<const void *>(&bcb_pull_heavy<Kp>)));
+  auto *lvl = static_cast<long long *>(scratch(ctx, kS6, sizeof(long long) * (n + 3)));
+  if (!lvl) return set_error(ctx, GBK_ENOMEM, "bc batch levels");
+  int log_kp = 0;
+  while ((1 << log_kp) < Kp) log_kp++;
+  const int mode = ctx->tune.bc_pull;
+  // one block of forward levels, queued blind: the direction of each level,
+  // its list and the end of the traversal are device state (bcb_level_end),
+  // so a block is captured once per call and replayed with one host check of
+  // the state per block
+  auto levels_block = [&]() {
+    for (int i = 0; i < kBcbLevelsPerBlock; i++) {
+      if (mode == 1 && Kp < 32) {
+        bcb_open_edges<<<1024, kBlock, 0, s>>>(depth, n, log_kp, K, g->in_off, c);
+        bcb_pull_decide<<<1, 1, 0, s>>>(c);
+      }
+      fill<unsigned long long>(ctx, hq.count, 0ull, 1);
+      const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
+      bcb_forward_pull<Kp><<<grid_p, kBlock, 0, s>>>(g->in_off, g->in_nbr, n, K, depth, sigma, queue, cap, c, hq);
+      bcb_pull_heavy<Kp><<<grid_p, kBlock, 0, s>>>(g->in_nbr, K, depth, sigma, c, hq, hpart);
+      bcb_pull_finish<Kp><<<1024, kBlock, 0, s>>>(K, depth, sigma, hq, hpart, queue, cap, c);
+      bcb_forward<Kp><<<grid_f, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, ltag, queue, cap, c, hq);
+      bcb_forward_heavy<Kp><<<grid_f, kBlock, 0, s>>>(g->nbr, depth, sigma, ltag, queue, cap, c, hq);
+      kt_end(ctx, kt);
+      if (mode == 1) bcb_list_edges<<<256, kBlock, 0, s>>>(g->off, queue, cap, c);
+      bcb_level_end<<<1, 1, 0, s>>>(c, lvl, cap, g->m, mode, ctx->tune.bc_pull_ratio, Kp < 32);
+    }
+  };
+  LoopGraph loop(ctx);
+  if (int rc = loop.capture(levels_block)) return rc;
+  EventTimer timer(ctx);
   GBK_HIP(ctx, hipMemcpyAsync(queue, uniq.data(), sizeof(int32_t) * uniq.size(), hipMemcpyHostToDevice, s));
   fill<int32_t>(ctx, depth, -1, n * Kp);
   fill<double>(ctx, sigma, 0.0, n * Kp);
   fill<uint32_t>(ctx, ltag, 0u, n);
-  bcb_init_counters<<<1, 1, 0, s>>>(c, static_cast<long long>(uniq.size()));
+  bcb_init_counters<<<1, 1, 0, s>>>(c, static_cast<long long>(uniq.size()), mode == 2, lvl);
   bcb_sources<<<1, kBatchMax, 0, s>>>(dsrc, K, Kp, depth, sigma);
-  std::vector<int64_t> level = {0, static_cast<int64_t>(uniq.size())};
-  // direction per level (bfs.cc:231-262 weighs frontier edges against the
-  // unexplored ones the same way): bottom-up while the frontier's out-edges
-  // exceed the edges of the vertices not yet listed at any level
-  int64_t listed = 0, frontier_edges = 0;
-  int log_kp = 0;
-  while ((1 << log_kp) < Kp) log_kp++;
-  for (int32_t d = 0;; d++) {
-    const int64_t lb = level[level.size() - 2], le = level.back();
-    fill<unsigned long long>(ctx, hq.count, 0ull, 1);
-    // m - listed bounds the bottom-up work from below; only past it is the
-    // exact figure counted (one streaming pass over the depths), and only
-    // for narrow rows: at 32 and 64 lanes the pass reads n * Kp * 4 bytes,
-    // and a top-down frontier edge costs more than the bound's error (64
-    // sources: 50.7 ms on the bound, 52.9 with the count at any ratio)
-    bool pull = ctx->tune.bc_pull == 2;
-    const int64_t weighed = frontier_edges * ctx->tune.bc_pull_ratio;
-    if (ctx->tune.bc_pull == 1 && weighed > g->m - listed && Kp >= 32) {
-      pull = true;
-    } else if (ctx->tune.bc_pull == 1 && weighed > g->m - listed) {
-      fill<unsigned long long>(ctx, &c->open_edges, 0ull, 1);
-      bcb_open_edges<<<1024, kBlock, 0, s>>>(depth, n, log_kp, K, g->in_off, c);
-      GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcbCounters), hipMemcpyDeviceToHost, s));
-      GBK_HIP(ctx, hipStreamSynchronize(s));
-      pull = weighed > static_cast<int64_t>(hc->open_edges);
-    }
-    const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
-    if (pull) {
-      bcb_forward_pull<Kp><<<grid_p, kBlock, 0, s>>>(g->in_off, g->in_nbr, n, K, depth, sigma, queue, cap, d, c, hq);
-      bcb_pull_heavy<Kp><<<grid_p, kBlock, 0, s>>>(g->in_nbr, K, depth, sigma, d, c, hq, hpart);
-      bcb_pull_finish<Kp><<<1024, kBlock, 0, s>>>(K, depth, sigma, d, hq, hpart, queue, cap, c);
-    } else {
-      bcb_forward<Kp><<<grid_f, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, ltag, queue + lb, le - lb, queue,
-                                                cap, d, c, hq);
-      bcb_forward_heavy<Kp><<<grid_f, kBlock, 0, s>>>(g->nbr, depth, sigma, ltag, queue, cap, d, c, hq);
-    }
-    kt_end(ctx, kt);
-    if (ctx->tune.bc_pull == 1) bcb_list_edges<<<256, kBlock, 0, s>>>(g->off, queue, le, cap, c);
+  GBK_LAUNCH_CHECK(ctx, "bc batch sources");
+  for (int64_t queued = 0;; queued += kBcbLevelsPerBlock) {
+    if (queued > n + 1) return set_error(ctx, GBK_EHIP, "bc batch: more levels than vertices");
+    if (int rc = loop.run(levels_block)) return rc;
     GBK_LAUNCH_CHECK(ctx, "bc batch forward");
     GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcbCounters), hipMemcpyDeviceToHost, s));
     GBK_HIP(ctx, hipStreamSynchronize(s));
-    if (hc->overflow || static_cast<int64_t>(hc->tail) > cap) return set_error(ctx, GBK_EHIP, "bc batch: queue overflow");
-    const int64_t tail = static_cast<int64_t>(hc->tail);
-    if (tail == le) break;
-    level.push_back(tail);
-    frontier_edges = static_cast<int64_t>(hc->listed_edges) - listed;
-    listed = static_cast<int64_t>(hc->listed_edges);
+    if (hc->overflow) return set_error(ctx, GBK_EHIP, "bc batch: queue overflow");
+    if (hc->done) break;
   }
+  std::vector<int64_t> level(static_cast<size_t>(hc->levels) + 1);
+  GBK_HIP(ctx, hipMemcpyAsync(level.data(), lvl, sizeof(long long) * level.size(), hipMemcpyDeviceToHost, s));
+  GBK_HIP(ctx, hipStreamSynchronize(s));
   const int levels = static_cast<int>(level.size()) - 1;
   bcb_pack<<<grid_for(n * Kp), kBlock, 0, s>>>(depth, n * Kp, dc);
   for (int32_t d = levels - 1; d >= 0; d--) {
@@ -1652,6 +1782,7 @@ int bc_batch_run(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K, fl
   GBK_LAUNCH_CHECK(ctx, "bc batch backward");
   int64_t etot = 0;
   if (int rc = collect(ctx, g, depth, Kp, K, -1, nullptr, nullptr, nullptr, 0, &etot)) return rc;
+  acc->device_ms += timer.stop_ms();
   GBK_HIP(ctx, hipMemcpyAsync(hc, c, sizeof(BcbCounters), hipMemcpyDeviceToHost, s));
   GBK_HIP(ctx, hipStreamSynchronize(s));
   if (hc->overflow) return set_error(ctx, GBK_EHIP, "bc batch: heavy queue overflow");
@@ -1716,17 +1847,23 @@ extern "C" int gbk_bc_batch(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sou
   auto *pmax = static_cast<unsigned int *>(scratch(ctx, kS11, sizeof(unsigned int) * kGridCap));
   if (!scores || !pmax) return set_error(ctx, GBK_ENOMEM, "bc batch scores");
   gbk_batch_stats acc{};
-  EventTimer timer(ctx);
-  fill<float>(ctx, scores, 0.0f, n);
+  // device time: each chunk times its own region (after capturing its level
+  // loop), plus the clearing and the normalisation
+  {
+    EventTimer t0(ctx);
+    fill<float>(ctx, scores, 0.0f, n);
+    acc.device_ms += t0.stop_ms();
+  }
   for (int first = 0; first < num_sources; first += kBatchMax) {
     const int K = std::min(kBatchMax, num_sources - first);
     if (int rc = bc_batch_chunk(ctx, g, sources + first, K, scores, &acc)) return rc;
   }
+  EventTimer timer(ctx);
   const unsigned ng = expand_grid(n);
   bc_max<<<ng, kBlock, 0, s>>>(scores, n, pmax);
   bc_normalize<<<ng, kBlock, 0, s>>>(scores, n, pmax, static_cast<int>(ng));
   GBK_LAUNCH_CHECK(ctx, "bc batch normalize");
-  acc.device_ms = timer.stop_ms();
+  acc.device_ms += timer.stop_ms();
   if (scores_out) {
     GBK_HIP(ctx, hipMemcpyAsync(scores_out, scores, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     GBK_HIP(ctx, hipStreamSynchronize(s));

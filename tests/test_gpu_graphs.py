@@ -97,3 +97,50 @@ def test_pr_many_replays_equal_eager(ctx, graphs_on):
     for d in dgs:
         d.free()
     ug.free()
+
+
+@pytest.mark.parametrize("name", ["kron13", "path"])
+def test_bfs_batch_replays_equal_eager(ctx, graphs_on, name):
+    """BFS batch levels run as device-driven blocks (direction and end of the
+    traversal decided on the device), captured once per call: replays equal
+    eager runs and the oracle's depths, also on a long path (many blocks of
+    levels, the last ones returning at once)."""
+    g = O.kron(13) if name == "kron13" else O.make_graph(np.arange(999, dtype=np.int32),
+                                                          np.arange(1, 1000, dtype=np.int32))
+    dg = ctx.upload_graph(g)
+    srcs = O.source_picker(g, 64)[1]
+    refs = [O.bfs_depths(g, int(s)) for s in srcs]
+    for rep in range(3):
+        _mode(ctx, True)
+        depth, reached, edges, st = ctx.bfs_batch(dg, srcs)
+        _mode(ctx, False)
+        eager, r0, e0, st0 = ctx.bfs_batch(dg, srcs)
+        ctx.cc(dg, gbk.CC_AFFOREST)  # eager work between the replays
+        assert st.rounds == st0.rounds and np.array_equal(reached, r0) and np.array_equal(edges, e0)
+        for j in range(0, len(srcs), 7):
+            assert np.array_equal(depth[j], refs[j]) and np.array_equal(eager[j], refs[j]), (rep, j)
+    dg.free()
+
+
+@pytest.mark.parametrize("pull", [0, 1, 2])
+def test_bc_batch_replays_equal_eager(ctx, graphs_on, pull):
+    """BC batch forward levels as device-driven blocks: replays equal eager
+    runs and serial Brandes, for every direction rule (1 counts the open
+    edges on the device before choosing)."""
+    g = O.kron(12)
+    dg = ctx.upload_graph(g)
+    srcs = O.source_picker(g, 8)[1]
+    ref = O.bc_serial(g, srcs)
+    ctx.set_param("bc_pull", pull)
+    try:
+        for rep in range(3):
+            _mode(ctx, True)
+            s1, st1 = ctx.bc_batch(dg, srcs)
+            _mode(ctx, False)
+            s0, st0 = ctx.bc_batch(dg, srcs)
+            assert st1.rounds == st0.rounds and st1.edges_reached == st0.edges_reached
+            np.testing.assert_allclose(s1, ref, rtol=1e-4, atol=1e-6)
+            np.testing.assert_allclose(s0, ref, rtol=1e-4, atol=1e-6)
+    finally:
+        ctx.set_param("bc_pull", 1)
+    dg.free()
