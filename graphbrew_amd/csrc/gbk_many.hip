@@ -28,8 +28,8 @@ namespace gbk {
 
 __global__ void union_rows(const int64_t *off, int64_t n, int64_t vbase, int64_t ebase, int64_t *off_out,
                            const int32_t *org, int32_t *org_out) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v < n) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
     off_out[vbase + v] = off[v] - off[0] + ebase;
     org_out[vbase + v] = static_cast<int32_t>(vbase + org[v]);
   }
@@ -37,8 +37,8 @@ __global__ void union_rows(const int64_t *off, int64_t n, int64_t vbase, int64_t
 
 __global__ void union_edges(const int32_t *nbr, const int32_t *wgt, int64_t m, int64_t vbase, int64_t ebase,
                             int32_t *nbr_out, int32_t *wgt_out) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e < m) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m; e += stride) {
     nbr_out[ebase + e] = static_cast<int32_t>(nbr[e] + vbase);
     if (wgt) wgt_out[ebase + e] = wgt[e];
   }

@@ -356,8 +356,11 @@ template <typename K>
 __global__ void __launch_bounds__(kBlock)
 relabel_merge_pass(const K *src, K *dst, const int64_t *stage_off, int nbig, int64_t total,
                    int64_t width) {
-  const int64_t t0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kMergeItems;
-  if (t0 >= total) return;
+  // grid-stride over groups of kMergeItems outputs (the staged rows can hold
+  // more items than one launch has threads times kMergeItems)
+  const int64_t step = static_cast<int64_t>(gridDim.x) * blockDim.x * kMergeItems;
+  for (int64_t t0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kMergeItems; t0 < total;
+       t0 += step) {
   // row containing t0
   int lo = 0, hi = nbig;
   while (hi - lo > 1) {
@@ -393,6 +396,7 @@ relabel_merge_pass(const K *src, K *dst, const int64_t *stage_off, int nbig, int
       dst[t++] = x;
     }
   }
+  }
 }
 
 template <bool W>
@@ -400,8 +404,8 @@ __global__ void relabel_big_scatter(const typename BigKey<W>::T *stage0, const t
                                     const int64_t *stage_off, const int32_t *big_rows, int nbig,
                                     int64_t total, const int64_t *new_off, int32_t *nbr_out,
                                     int32_t *wgt_out) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= total) return;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
   int lo = 0, hi = nbig;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -418,6 +422,7 @@ __global__ void relabel_big_scatter(const typename BigKey<W>::T *stage0, const t
   const int64_t dst = new_off[big_rows[lo]] + (t - stage_off[lo]);
   nbr_out[dst] = BigKey<W>::v(key);
   if (W) wgt_out[dst] = BigKey<W>::w(key);
+  }
 }
 
 __global__ void gather_i64(const int64_t *src, const int32_t *idx, int64_t n, int64_t *out) {

@@ -60,6 +60,7 @@ def lib():
             "gbh_modeled_property_bytes": (i64, [i32, i64, i64]),
             "gbh_frozen_rule": (i32, [i32, f64, i64, f64, f64, f64]),
             "gbh_budgeted_rule": (i32, [i32, f64, i64, f64, f64, f64]),
+            "gbh_generate_lattice": (i64, [i64, i64, f64, C.c_uint64, i32, VP, VP, VP]),
             "gbh_llc_bytes": (i64, []),
         }
         for name, (res, args) in sig.items():
@@ -92,6 +93,20 @@ def generate_rmat(scale: int, edges: int, a: float, b: float, c: float, seed: in
     v = np.empty(edges, np.int32)
     _chk(lib().gbh_generate_rmat(scale, edges, a, b, c, seed, int(permute), _p(u), _p(v)), "generate_rmat")
     return u, v
+
+
+def generate_lattice(rows: int, cols: int, rewire: float = 0.001, seed: int = 1, max_weight: int = 1000,
+                     weighted: bool = True):
+    """Road-like grid (4-neighbour, rewired heads, weights U[1, max_weight]):
+    (u, v, w or None), n = rows * cols."""
+    m = int(lib().gbh_generate_lattice(rows, cols, rewire, seed, max_weight, None, None, None))
+    if m < 0:
+        raise ValueError("bad lattice arguments")
+    u = np.empty(m, np.int32)
+    v = np.empty(m, np.int32)
+    w = np.empty(m, np.int32) if weighted else None
+    _chk(lib().gbh_generate_lattice(rows, cols, rewire, seed, max_weight, _p(u), _p(v), _p(w)) - m, "lattice")
+    return u, v, w
 
 
 def insert_weights(m: int) -> np.ndarray:

@@ -795,3 +795,40 @@ extern "C" int gbh_generate_rmat(int scale, int64_t edges, double a, double b, d
   }
   return 0;
 }
+
+// A road-like grid: rows x cols vertices (id = r * cols + c), each joined to
+// its right and lower neighbour; with probability 'rewire' an edge's head is
+// replaced by a uniform random vertex (self loops and duplicates are left to
+// the CSR build's squish).  Weights, when w is given, are uniform in
+// [1, max_weight].  Every edge draws from a counter-based stream (SplitMix64
+// of the seed and the edge index), so the output does not depend on the
+// thread count.  Edge count: rows (cols - 1) + (rows - 1) cols.  No
+// reference counterpart: the road-network graphs of the reference's
+// experiments are downloaded files; this is their synthetic stand-in.
+extern "C" int64_t gbh_generate_lattice(int64_t rows, int64_t cols, double rewire, uint64_t seed, int max_weight,
+                                        int32_t *u, int32_t *v, int32_t *w) {
+  if (rows < 1 || cols < 1 || rows * cols > INT32_MAX || rewire < 0 || rewire > 1 || max_weight < 1) return -1;
+  const int64_t horiz = rows * (cols - 1), total = horiz + (rows - 1) * cols;
+  if (!u || !v) return total;
+  const uint64_t n = static_cast<uint64_t>(rows * cols);
+  const uint64_t cut = static_cast<uint64_t>(rewire * 18446744073709551615.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < total; e++) {
+    int64_t a, b;
+    if (e < horiz) {
+      const int64_t r = e / (cols - 1), c = e % (cols - 1);
+      a = r * cols + c;
+      b = a + 1;
+    } else {
+      const int64_t k = e - horiz;
+      a = k;
+      b = k + cols;
+    }
+    uint64_t state = seed * 0x9E3779B97F4A7C15ULL ^ (static_cast<uint64_t>(e) * 0xD1B54A32D192ED03ULL);
+    if (rewire > 0 && splitmix_next(state) <= cut) b = static_cast<int64_t>(splitmix_next(state) % n);
+    u[e] = static_cast<int32_t>(a);
+    v[e] = static_cast<int32_t>(b);
+    if (w) w[e] = static_cast<int32_t>(1 + splitmix_next(state) % static_cast<uint64_t>(max_weight));
+  }
+  return total;
+}

@@ -73,6 +73,15 @@ int gbh_mapping_hubclusterdbg(int64_t n, const int64_t *off, int32_t *new_ids);
  * probabilities and seed, gapbs/generator.h:89-186). */
 int gbh_generate_rmat(int scale, int64_t edges, double a, double b, double c, uint64_t seed,
                       int permute, int32_t *u, int32_t *v);
+/* Road-like grid of rows x cols vertices (4-neighbour: right and lower
+ * edges), each edge's head rewired to a uniform random vertex with
+ * probability 'rewire', weights U[1, max_weight] into w (may be NULL);
+ * counter-based seeded stream.  With u == NULL returns the edge count
+ * rows (cols - 1) + (rows - 1) cols; else fills the arrays and returns it;
+ * -1 on bad arguments.  No reference counterpart (the reference's road
+ * graphs are downloaded files). */
+int64_t gbh_generate_lattice(int64_t rows, int64_t cols, double rewire, uint64_t seed, int max_weight,
+                             int32_t *u, int32_t *v, int32_t *w);
 uint64_t gbh_mapping_fingerprint(int64_t n, const int32_t *new_ids);
 /* 0 ok, -1 out-of-range id, -2 duplicate id. */
 int gbh_validate_permutation(int64_t n, const int32_t *ids);

@@ -76,6 +76,7 @@ def lib():
             "orc_distance_fingerprint": (None, [i64, I32P, VP, U64P]),
             "orc_bc": (i32, [i64, I64P, I32P, I32P, i32, i32, F32P]),
             "orc_bc_serial": (i32, [i64, I64P, I32P, I32P, i32, F32P]),
+            "orc_bc_fp64": (i32, [i64, I64P, I32P, I32P, i32, i32, F64P]),
             "orc_bc_within_tolerance": (i32, [C.c_float, C.c_float]),
             "orc_tier0_features": (i32, [i64, i64, I64P, I32P, i64, C.c_uint64, F64P]),
             "orc_modeled_property_bytes": (i64, [i32, i64, i64]),
@@ -420,6 +421,14 @@ def bc(g: Graph, sources, threads=1) -> np.ndarray:
     s = np.ascontiguousarray(sources, np.int32)
     out = np.empty(g.n, np.float32)
     _check(lib().orc_bc(g.n, g.off, g.nbr, s, len(s), threads, out), "bc")
+    return out
+
+
+def bc_fp64(g: Graph, sources, threads=1) -> np.ndarray:
+    """Brandes with double deltas and scores: the exact yardstick."""
+    s = np.ascontiguousarray(sources, np.int32)
+    out = np.empty(g.n, np.float64)
+    _check(lib().orc_bc_fp64(g.n, g.off, g.nbr, s, len(s), threads, out), "bc_fp64")
     return out
 
 

@@ -986,15 +986,19 @@ void orc_distance_fingerprint(int64_t n, const int32_t *dist,
 // Brandes + PBFS (bc.cc:71-237): level-synchronous forward pass with double
 // path counts and an edge-indexed successor bitmap, then a deepest-first
 // backward pass; scores normalised by their maximum.
-int orc_bc(int64_t n, const int64_t *off, const int32_t *nbr,
-           const int32_t *sources, int num_sources, int threads,
-           float *scores) {
+// Score is float for the reference's arithmetic; double gives the exact
+// yardstick that float accumulation over a hub's successors drifts from.
+extern "C++" {
+template <class Score>
+int bc_brandes(int64_t n, const int64_t *off, const int32_t *nbr,
+                      const int32_t *sources, int num_sources, int threads,
+                      Score *scores) {
   ThreadScope ts(threads);
   const int64_t m = off[n] - off[0];
-  std::fill(scores, scores + n, 0.0f);
+  std::fill(scores, scores + n, Score(0));
   std::vector<double> sigma(n);
   std::vector<int32_t> depths(n);
-  std::vector<float> deltas(n);
+  std::vector<Score> deltas(n);
   std::vector<int32_t> queue(std::max<int64_t>(n, 1));
   Bits succ(m);
   for (int it = 0; it < num_sources; it++) {
@@ -1036,12 +1040,12 @@ int orc_bc(int64_t n, const int64_t *off, const int32_t *nbr,
       q_end = tail;
     }
     depth_index.push_back(q_end);
-    std::fill(deltas.begin(), deltas.end(), 0.0f);
+    std::fill(deltas.begin(), deltas.end(), Score(0));
     for (int d = static_cast<int>(depth_index.size()) - 2; d >= 0; d--) {
 #pragma omp parallel for schedule(dynamic, 64)
       for (int64_t i = depth_index[d]; i < depth_index[d + 1]; i++) {
         const int32_t u = queue[i];
-        float delta_u = 0;
+        Score delta_u = 0;
         for (int64_t k = off[u]; k < off[u + 1]; k++) {
           if (succ.get(k - off[0])) {
             const int32_t v = nbr[k];
@@ -1053,10 +1057,24 @@ int orc_bc(int64_t n, const int64_t *off, const int32_t *nbr,
       }
     }
   }
-  float biggest = 0;
+  Score biggest = 0;
   for (int64_t v = 0; v < n; v++) biggest = std::max(biggest, scores[v]);
   for (int64_t v = 0; v < n; v++) scores[v] = scores[v] / biggest;
   return 0;
+}
+}  // extern "C++"
+
+int orc_bc(int64_t n, const int64_t *off, const int32_t *nbr,
+           const int32_t *sources, int num_sources, int threads,
+           float *scores) {
+  return bc_brandes<float>(n, off, nbr, sources, num_sources, threads, scores);
+}
+
+// The same traversal with double deltas and scores.
+int orc_bc_fp64(int64_t n, const int64_t *off, const int32_t *nbr,
+                const int32_t *sources, int num_sources, int threads,
+                double *scores) {
+  return bc_brandes<double>(n, off, nbr, sources, num_sources, threads, scores);
 }
 
 // BCVerifier's serial recomputation (bc.cc:256-300).

@@ -390,7 +390,7 @@ def test_bc_on_fixtures(name):
     g, _ = fixture(name)
     srcs = [x for x in range(g.n) if g.off[x + 1] > g.off[x]][:4]
     exp = B.brandes(g.n, g.off, g.nbr, srcs)
-    for got in (O.bc(g, srcs), O.bc_serial(g, srcs)):
+    for got in (O.bc(g, srcs), O.bc_serial(g, srcs), O.bc_fp64(g, srcs, threads=2)):
         assert np.allclose(got, exp, rtol=1e-5, atol=1e-6)
 
 
@@ -398,6 +398,7 @@ def test_bc_path_closed_form():
     g, _ = fixture("path.el")
     # from source 0 on a 6-path, delta(k) = number of vertices beyond k.
     assert np.allclose(O.bc_serial(g, [0]), np.array([5, 4, 3, 2, 1, 0]) / 5.0)
+    assert np.array_equal(O.bc_fp64(g, [0]), np.array([5, 4, 3, 2, 1, 0]) / 5.0)
 
 
 def test_bc_kron_parallel_matches_serial():
@@ -408,6 +409,7 @@ def test_bc_kron_parallel_matches_serial():
     assert all(O.bc_within_tolerance(float(x), float(y)) for x, y in zip(b, a))
     exp = B.brandes(g.n, g.off, g.nbr, srcs.tolist())
     assert np.allclose(b, exp, rtol=1e-4, atol=1e-6)
+    assert np.allclose(O.bc_fp64(g, srcs, threads=4), exp, rtol=1e-12, atol=1e-14)
 
 
 def test_bc_tolerance_reference_asserts():
