@@ -200,7 +200,8 @@ class Workload:
 def kt_table(gbk, single, msbfs_group=32):
     t = {
         "sssp": (gbk.KT_SSSP_RELAX, "sssp_relax+sssp_relax_heavy (one relaxation round)"),
-        "pr": (gbk.KT_PR_ITER, "pr_pull_block + pr_block_close per row block (one Gauss-Seidel sweep)"),
+        "pr": (gbk.KT_PR_ITER, "pr_seg_block + pr_block_close per row block (one Gauss-Seidel sweep; "
+                               "pr_pull_block on graphs of at most 2^19 vertices)"),
         "bfs": (gbk.KT_BFS_STEP, "bfs_td_step+bfs_td_heavy | bfs_bu_step (one BFS step)"),
         "bc": (gbk.KT_BC_LEVEL, "bc_forward_level+bc_forward_heavy | bc_backward_level"),
     }
@@ -594,7 +595,9 @@ def main():
             # the fraction of the HBM rate actually moved
             kern[name]["hbm_frac_one_source_equivalent"] = kern[name].pop("hbm_frac")
             per_call = load_traffic(workload_name(args), "bfs", False, "bytes_per_call")
-            kern[name]["hbm_frac"] = (per_call * calls / secs / HBM_PEAK) if per_call and secs else None
+            batch_calls = calls / args.bfs_sources  # one batch call carries every source
+            kern[name]["hbm_frac"] = (per_call * batch_calls / secs / HBM_PEAK) if per_call and secs else None
+            kern[name]["counted_bytes_per_batch_call"] = per_call
             kern[name]["hbm_frac_basis"] = "counted bytes per batch call" if per_call else "not counted"
     if scan and "sssp" in kern:
         # work the batch actually did: queued rows relaxed and their edges

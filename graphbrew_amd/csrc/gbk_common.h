@@ -47,6 +47,8 @@ struct Tuning {
   int bc_pull_ratio = 1;        // a frontier edge weighs this many open edges in that choice
   int sssp_wide = 1;            // SSSP batches of 17..32 sources on 64-lane rows
   int msbfs_group = 32;         // lanes per vertex in the BFS batch's bottom-up step (1..64)
+  int pr_segment = 0;           // PageRank segmented sweep: 0 by size, -1 never, k > 0 smallest
+                                // segment width 2^k tried (6..26)
 };
 constexpr int kGridCapRuntime = 2048;  // upper bound of relax_grid
 constexpr int64_t kHeavyChunkEdges = 1024;  // heavy-row chunk of the all-rows kernels

@@ -8,7 +8,7 @@
 // sweep at Kronecker scale 20, so 20 sweeps do not reach 1e-4 where the
 // reference needs 7).  Here a sweep is a block Gauss-Seidel in a fixed
 // order instead: the rows' tiles are cut into equal runs (pr_blocks, or by
-// default one per ~4096 tiles, at most 8), one launch each,
+// default one per ~4096 tiles, at most 2), one launch each,
 // and a row of block b reads the new contributions of the
 // rows before block b and the previous sweep's for the rest -- no races,
 // the same answer every run.  The gathers read one buffer: a block's new
@@ -258,6 +258,358 @@ pr_block_close(RowTiles rt, const PrBlock *blocks, int b, int64_t row_hi, const 
   block_partial(err, partial + static_cast<int64_t>(b) * gridDim.x);
 }
 
+// ------------------------------------------------------- segmented sweep
+// Once the contributions outgrow an XCD's 4 MB L2 (more than 2^19 vertices),
+// each 4-byte gather of the row-tile sweep above pulls a whole line from the
+// Infinity Cache.  The segmented sweep cuts the source range into S <= 32
+// segments of 2^lw vertices (2 MiB of contributions at the smallest width)
+// and has every wave walk its rows' in-edges segment by segment, so the
+// waves of an XCD gather from one L2-sized slice at a time (CSR segmenting:
+// Zhang, Kiriansky, Mendis, Amarasinghe, Zaharia and Shun, "Making caches
+// work for graph analytics", IEEE BigData 2017).
+//
+// Layout, rebuilt by each call from the in-CSR and its tiles:
+//  * records: tile t's light in-edges ordered by (segment, row, position in
+//    the row), stored from in_off[tiles[t]] on -- inside the tile's own
+//    edge range; a record is (row - tiles[t]) << 26 | (source - segment
+//    base);
+//  * tseg: per tile, the S + 1 offsets of its segments' records;
+//  * heavy rows: each row's ascending in-edges cut at the segment bounds
+//    into pieces, pieces into 256-edge chunks; hpc[s] is segment s's chunk
+//    prefix over the heavy list, hps[s][h] the first edge of row h's piece.
+// A sweep launch gives each wave G consecutive tiles of the block, their
+// rows' sums in LDS; for s = 0 .. S-1 the wave adds the tiles' segment-s
+// records, then takes its share of the block's segment-s heavy chunks,
+// whose sums go to the heavy rows' accumulators (closed as above).  The
+// reads and writes of contributions are the row-tile sweep's, so both
+// variants keep their semantics; only the order of the float sums differs.
+constexpr int kSegMax = 32;          // segments of the source range
+constexpr int kSegMinLog2 = 19;      // 2^19 sources = 2 MiB of contributions
+constexpr int kSegMaxLog2 = 26;      // records keep 26 bits of local source
+constexpr int kSegMaxG = 16;         // tiles of one wave (64 rows each)
+constexpr int64_t kSegChunk = 256;   // heavy-row edges per chunk
+constexpr int kSegLaneShift = 26;
+constexpr int kSegBuildU = 4;        // chunks in flight per wave in the layout build
+
+struct SegLayout {
+  int S = 0;
+  int lw = 0;
+  int32_t *rec = nullptr;
+  int32_t *tseg = nullptr;
+  int64_t *hps = nullptr;
+  int64_t *hpc = nullptr;
+  int64_t *hcnt = nullptr;  // build scratch: chunk counts per (segment, heavy row)
+};
+
+// A record's bits: the row's lane in its tile (6) over the local source (26).
+__device__ __forceinline__ int32_t seg_record(int lane, int32_t local) {
+  return static_cast<int32_t>((static_cast<uint32_t>(lane) << kSegLaneShift) | static_cast<uint32_t>(local));
+}
+
+__device__ __forceinline__ int64_t lower_edge(const int32_t *nbr, int64_t lo, int64_t hi, int64_t key) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (nbr[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// 6-step owner search over 32-bit exclusive prefixes (a tile's edges).
+__device__ __forceinline__ int tile_owner32(int32_t excl, int32_t e) {
+  int lo = 0;
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1) {
+    const int probe = lo + step;
+    const int32_t x = __shfl(excl, probe & (kWave - 1));
+    if (probe < kWave && x <= e) lo = probe;
+  }
+  return lo;
+}
+
+// One wave per tile.  Each lane counts its row's light in-edges per segment
+// by binary search over the row's ascending sources, the segments are laid
+// out, then one coalesced pass writes the records, each edge also compared
+// with its predecessor in the row.  A tile with a row whose sources are not
+// ascending (an uploaded CSR need not be sorted) is then redone lane by
+// lane: counted and scattered by each lane walking its own row in order.
+// Writes stay inside the tile's record range in every case.
+__global__ void __launch_bounds__(kBlock)
+seg_records(RowTiles rt, const int32_t *in_nbr, int S, int lw, int32_t *rec, int32_t *tseg) {
+  __shared__ int32_t cnt_all[kWavesPerBlock][kSegMax * kWave];
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  int32_t *cnt = cnt_all[wv];
+  const int32_t mask = (1 << lw) - 1;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wv; t < rt.ntiles;
+       t += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
+    const int64_t row0 = rt.tiles[t];
+    const int64_t u = row0 + lane;
+    const int64_t tbase = rt.off[row0];
+    int64_t start = 0, len = 0;
+    if (u < rt.tiles[t + 1]) {
+      start = rt.off[u];
+      len = rt.off[u + 1] - start;
+    }
+    if (len > rt.heavy_deg) len = 0;
+    const int32_t *row = in_nbr + start;
+    // per-segment counts of this lane's row, if its sources ascend
+    int64_t prev = 0;
+    for (int s = 0; s < S; s++) {
+      const int64_t nb = s + 1 < S ? lower_edge(row, prev, len, static_cast<int64_t>(s + 1) << lw) : len;
+      cnt[s * kWave + lane] = static_cast<int32_t>(nb - prev);
+      prev = nb;
+    }
+    const int32_t len32 = static_cast<int32_t>(len);
+    const int32_t rel = static_cast<int32_t>(start - tbase);
+    const int32_t incl = wave_incl_scan(len32);
+    const int32_t excl = incl - len32;
+    const int32_t total = __shfl(incl, kWave - 1);
+    bool sorted = true;
+    for (int pass = 0; pass < 2; pass++) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      // segment s's records start at 'at'; within it the rows in order.  A
+      // sorted row's segment-s edges are contiguous from position 'rowpre'
+      // of the row, so its slot holds first - rowpre (add the position);
+      // otherwise the slot holds the next free record of (row, segment).
+      int32_t at = 0, rowpre = 0;
+      for (int s = 0; s < S; s++) {
+        const int32_t cs = cnt[s * kWave + lane];
+        const int32_t ci = wave_incl_scan(cs);
+        const int32_t first = at + ci - cs;
+        cnt[s * kWave + lane] = sorted ? first - rowpre : first;
+        if (lane == 0) tseg[t * (S + 1) + s] = at;
+        rowpre += cs;
+        at += __shfl(ci, kWave - 1);
+      }
+      if (lane == 0) tseg[t * (S + 1) + S] = at;
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      int32_t *out = rec + tbase;
+      if (sorted) {
+        bool bad = false;
+        for (int32_t c = 0; c < total; c += kSegBuildU * kWave) {
+          int32_t v[kSegBuildU], pv[kSegBuildU], k[kSegBuildU];
+          int own[kSegBuildU];
+#pragma unroll
+          for (int j = 0; j < kSegBuildU; j++) {
+            const int32_t e = c + j * kWave + lane;
+            const int lo = tile_owner32(excl, e);
+            // shuffles in uniform control flow: the owner lane may be past the end
+            k[j] = e - __shfl(excl, lo);
+            const int32_t orel = __shfl(rel, lo);
+            own[j] = e < total ? lo : kWave;
+            v[j] = e < total ? in_nbr[tbase + orel + k[j]] : 0;
+            pv[j] = e < total && k[j] > 0 ? in_nbr[tbase + orel + k[j] - 1] : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < kSegBuildU; j++) {
+            if (own[j] < kWave) {
+              bad |= v[j] < pv[j];
+              const int32_t dst = cnt[(v[j] >> lw) * kWave + own[j]] + k[j];
+              if (dst >= 0 && dst < total) out[dst] = seg_record(own[j], v[j] & mask);
+            }
+          }
+        }
+        if (__ballot(bad) == 0) break;
+        // some row is not ascending: count again lane by lane and redo
+        sorted = false;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        for (int s = 0; s < S; s++) cnt[s * kWave + lane] = 0;
+        for (int64_t q = 0; q < len; q++) cnt[(row[q] >> lw) * kWave + lane]++;
+      } else {
+        for (int64_t q = 0; q < len; q++) {
+          const int32_t v = row[q];
+          const int32_t pos = cnt[(v >> lw) * kWave + lane]++;
+          out[pos] = seg_record(lane, v & mask);
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+
+// One workgroup per heavy row: its pieces' first edges and chunk counts per
+// segment.  A row whose sources are not ascending is one piece in segment
+// 0 (every edge is still summed once; it only loses the locality).
+__global__ void __launch_bounds__(kBlock)
+seg_heavy_pieces(RowTiles rt, const int32_t *in_nbr, int S, int lw, int64_t *hps, int64_t *hcnt) {
+  __shared__ int bad;
+  for (int64_t h = blockIdx.x; h < rt.nheavy; h += gridDim.x) {
+    const int64_t a = rt.off[rt.heavy[h]], b = rt.off[rt.heavy[h] + 1];
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    bool mine = false;
+    for (int64_t k = a + 1 + threadIdx.x; k < b; k += blockDim.x) mine |= in_nbr[k] < in_nbr[k - 1];
+    if (mine) atomicOr(&bad, 1);
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < S) {
+      const int s = threadIdx.x;
+      int64_t lo, hi;
+      if (bad == 0) {
+        lo = s == 0 ? a : lower_edge(in_nbr, a, b, static_cast<int64_t>(s) << lw);
+        hi = s + 1 == S ? b : lower_edge(in_nbr, a, b, static_cast<int64_t>(s + 1) << lw);
+      } else {
+        lo = s == 0 ? a : b;
+        hi = b;
+      }
+      hps[s * rt.nheavy + h] = lo;
+      hcnt[s * rt.nheavy + h] = (hi - lo + kSegChunk - 1) / kSegChunk;
+    }
+    __syncthreads();
+  }
+}
+
+// One workgroup per segment: hpc[s][h] = chunks of segment s before heavy
+// row h (nheavy + 1 entries).
+__global__ void __launch_bounds__(kBlock)
+seg_heavy_prefix(const int64_t *hcnt, int64_t nheavy, int64_t *hpc) {
+  __shared__ int64_t red[kWavesPerBlock];
+  const int64_t s = blockIdx.x;
+  const int64_t *c = hcnt + s * nheavy;
+  int64_t *o = hpc + s * (nheavy + 1);
+  const int wv = threadIdx.x / kWave;
+  int64_t carry = 0;
+  for (int64_t base = 0; base < nheavy; base += blockDim.x) {
+    const int64_t i = base + threadIdx.x;
+    const int64_t x = i < nheavy ? c[i] : 0;
+    const int64_t wi = wave_incl_scan(x);
+    if (lane_id() == kWave - 1) red[wv] = wi;
+    __syncthreads();
+    int64_t before = 0, tot = 0;
+    for (int w = 0; w < kWavesPerBlock; w++) {
+      if (w < wv) before += red[w];
+      tot += red[w];
+    }
+    if (i < nheavy) o[i] = carry + before + wi - x;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) o[nheavy] = carry;
+}
+
+// One row block of a sweep, segment by segment (see above).  Each wave takes
+// parts of G tiles, one per round (the rounds are the same for every wave);
+// the heavy chunks are shared out in round 0.
+template <int U>
+__global__ void __launch_bounds__(kBlock)
+pr_seg_block(SegLayout L, RowTiles rt, const PrBlock *blocks, int b, int G, const int32_t *in_nbr,
+             const int64_t *out_off, float *scores, const float *contrib_cur, float *contrib_next,
+             const PrState *st, double *partial, float *hacc, float base, float damp) {
+  if (*reinterpret_cast<const volatile int *>(&st->done)) return;
+  __shared__ float acc_all[kWavesPerBlock][kWave * kSegMaxG];
+  const PrBlock blk = blocks[b];
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  float *acc = acc_all[wv];
+  const int S = L.S;
+  const int32_t mask = (1 << L.lw) - 1;
+  const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wv;
+  const int64_t nparts = (blk.tile_hi - blk.tile_lo + G - 1) / G;
+  double err = 0;
+  for (int64_t round = 0; round == 0 || round * nw < nparts; round++) {
+    const int64_t p = round * nw + gw;
+    const bool has = p < nparts;
+    const int64_t t0 = blk.tile_lo + p * G;
+    const int64_t t1 = has ? min(t0 + G, blk.tile_hi) : t0;
+    const int64_t row0 = has ? rt.tiles[t0] : 0;
+    const int rows = has ? static_cast<int>(rt.tiles[t1] - row0) : 0;
+    for (int k = lane; k < rows; k += kWave) acc[k] = 0.0f;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    // lane j < t1 - t0 holds tile t0 + j: its records' base and first row
+    const bool tl = lane < t1 - t0;
+    const int64_t tj = t0 + lane;
+    const int64_t tbase = tl ? rt.off[rt.tiles[tj]] : 0;
+    const int trow = tl ? static_cast<int>(rt.tiles[tj] - row0) : 0;
+    const int32_t *ts = L.tseg + (tl ? tj : 0) * (S + 1);
+    int32_t lo = tl ? ts[0] : 0;
+    int32_t hi = tl ? ts[1] : 0;
+    for (int s = 0; s < S; s++) {
+      const int32_t nhi = tl && s + 2 <= S ? ts[s + 2] : 0;
+      if (has) {
+        const int32_t len = hi - lo;
+        const int64_t pstart = tbase + lo;
+        const int32_t incl = wave_incl_scan(len);
+        const int32_t excl = incl - len;
+        const int32_t total = __shfl(incl, kWave - 1);
+        const int64_t sbase = static_cast<int64_t>(s) << L.lw;
+        for (int32_t c = 0; c < total; c += U * kWave) {
+          int owner[U];
+          int32_t r[U];
+#pragma unroll
+          for (int j = 0; j < U; j++) {
+            const int32_t e = c + j * kWave + lane;
+            const int pc = tile_owner32(excl, e);
+            const bool act = e < total;
+            const int64_t ostart = __shfl(pstart, pc);
+            const int32_t oexcl = __shfl(excl, pc);
+            const int orow = __shfl(trow, pc);
+            // records stream through once: non-temporal, so they do not
+            // push the contributions' slice out of the L2
+            r[j] = act ? __builtin_nontemporal_load(L.rec + ostart + (e - oexcl)) : 0;
+            owner[j] = act ? orow + static_cast<int>(static_cast<uint32_t>(r[j]) >> kSegLaneShift) : kWave * kSegMaxG;
+          }
+          float val[U];
+#pragma unroll
+          for (int j = 0; j < U; j++)
+            val[j] = owner[j] < kWave * kSegMaxG ? contrib_cur[sbase + (r[j] & mask)] : 0.0f;
+#pragma unroll
+          for (int j = 0; j < U; j++) {
+            bool tail;
+            const float run = seg_run_sum(val[j], owner[j], tail);
+            if (owner[j] < kWave * kSegMaxG && tail) acc[owner[j]] += run;
+          }
+        }
+      }
+      if (round == 0 && rt.nheavy) {
+        const int64_t *pc = L.hpc + static_cast<int64_t>(s) * (rt.nheavy + 1);
+        const int64_t ce = pc[blk.heavy_hi];
+        for (int64_t ci = pc[blk.heavy_lo] + gw; ci < ce; ci += nw) {
+          int64_t l = blk.heavy_lo, h = blk.heavy_hi;
+          while (l < h) {
+            const int64_t mid = (l + h) >> 1;
+            if (pc[mid] <= ci) l = mid + 1;
+            else h = mid;
+          }
+          const int64_t hh = l - 1;
+          const int64_t first = L.hps[s * rt.nheavy + hh] + (ci - pc[hh]) * kSegChunk;
+          const int64_t pend = s + 1 < S ? L.hps[(s + 1) * rt.nheavy + hh] : rt.off[rt.heavy[hh] + 1];
+          const int64_t last = min(first + kSegChunk, pend);
+          int32_t v[kSegChunk / kWave];
+#pragma unroll
+          for (int j = 0; j < kSegChunk / kWave; j++) {
+            const int64_t k = first + j * kWave + lane;
+            v[j] = k < last ? in_nbr[k] : -1;
+          }
+          float part = 0.0f;
+#pragma unroll
+          for (int j = 0; j < kSegChunk / kWave; j++) part += v[j] >= 0 ? contrib_cur[v[j]] : 0.0f;
+          part = wave_sum(part);
+          if (lane == 0) atomicAdd(&hacc[hh], part);
+        }
+      }
+      lo = hi;
+      hi = nhi;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    for (int k = lane; k < rows; k += kWave) {
+      const int64_t u = row0 + k;
+      if (rt.off[u + 1] - rt.off[u] <= rt.heavy_deg)
+        err += pr_store(u, acc[k], out_off, scores, contrib_next, base, damp);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  }
+  block_partial(err, partial + static_cast<int64_t>(b) * gridDim.x);
+}
+
 // ------------------------------------------------------------ host side
 // Launch shape of one PageRank call, the same for both entry points.
 struct PrPlan {
@@ -268,6 +620,8 @@ struct PrPlan {
   bool close;        // a close launch follows each block
   int unroll;        // 64-edge chunks in flight per wave
   size_t nparts;     // per-workgroup L1 partials per iteration
+  bool seg = false;  // segmented sweep (pr_seg_block)
+  int G = 1;         // tiles per wave part of the segmented sweep
 };
 
 // Buffers of one call (scores may be the caller's).
@@ -294,11 +648,34 @@ unsigned pr_resident(gbk_ctx *ctx, bool cache) {
   return static_cast<unsigned>(per_cu) * static_cast<unsigned>(cu);
 }
 
-PrPlan pr_plan(gbk_ctx *ctx, const RowTiles &rt, int64_t n, bool gs, bool cache) {
+// log2 of the segment width of the segmented sweep for n vertices, 0 when
+// it does not apply (one segment, or the "pr_segment" parameter is -1); a
+// positive parameter is the smallest width tried.
+int seg_log2(const gbk_ctx *ctx, int64_t n) {
+  const int force = ctx->tune.pr_segment;
+  if (force < 0) return 0;
+  int lw = force > 0 ? force : kSegMinLog2;
+  auto segs = [&](int w) { return (n + (int64_t(1) << w) - 1) >> w; };
+  while (lw < kSegMaxLog2 && segs(lw) > kSegMax) lw++;
+  return segs(lw) >= 2 && segs(lw) <= kSegMax ? lw : 0;
+}
+
+const void *seg_kernel(int unroll) {
+  switch (unroll) {
+    case 1: return reinterpret_cast<const void *>(pr_seg_block<1>);
+    case 2: return reinterpret_cast<const void *>(pr_seg_block<2>);
+    case 8: return reinterpret_cast<const void *>(pr_seg_block<8>);
+    default: return reinterpret_cast<const void *>(pr_seg_block<4>);
+  }
+}
+
+PrPlan pr_plan(gbk_ctx *ctx, const RowTiles &rt, int64_t n, bool gs, bool cache, bool seg = false) {
   PrPlan p;
   // Gauss-Seidel blocks: one per ~4096 tiles (a block's launch then keeps
-  // the GPU's waves busy), at most 8; a small graph's one block is Jacobi
-  int64_t want = ctx->tune.pr_blocks ? ctx->tune.pr_blocks : std::min<int64_t>(8, rt.ntiles / 4096);
+  // the GPU's waves busy), at most 2 -- at Kronecker scale 22 two blocks
+  // converge in the sweeps eight do (6) and save six launch tails a sweep
+  // (8.9 against 10.0 ms); a small graph's one block is Jacobi
+  int64_t want = ctx->tune.pr_blocks ? ctx->tune.pr_blocks : std::min<int64_t>(2, rt.ntiles / 4096);
   p.B = gs ? static_cast<int>(std::max<int64_t>(1, std::min(want, std::max<int64_t>(rt.ntiles, 1)))) : 1;
   // a block's launch covers about 1/B of the tiles, about one per wave;
   // Jacobi's one launch all of them
@@ -314,6 +691,16 @@ PrPlan pr_plan(gbk_ctx *ctx, const RowTiles &rt, int64_t n, bool gs, bool cache)
   // (8 chunks 1% over 4); Jacobi's one launch is bound by the random-line
   // rate and loses 7%
   p.unroll = ctx->tune.pr_unroll ? ctx->tune.pr_unroll : (gs && p.B > 1 ? 8 : 1);
+  if (seg) {
+    // resident waves; each takes parts of G tiles so that a block's parts
+    // are about one per wave
+    p.seg = true;
+    p.unroll = ctx->tune.pr_unroll ? ctx->tune.pr_unroll : 4;
+    p.grid = resident_grid(ctx, seg_kernel(p.unroll));
+    const int64_t waves = static_cast<int64_t>(p.grid) * kWavesPerBlock;
+    const int64_t per_block = (rt.ntiles + p.B - 1) / p.B;
+    p.G = static_cast<int>(std::min<int64_t>(kSegMaxG, std::max<int64_t>(1, (per_block + waves - 1) / waves)));
+  }
   p.nparts = static_cast<size_t>(p.B) * (p.grid + (p.close ? p.cgrid : 0));
   return p;
 }
@@ -341,7 +728,8 @@ size_t pr_carve(char *base, int64_t n, int max_iters, const PrPlan &p, int64_t n
 // scores, then max_iters iterations (captured once and replayed when
 // 'loop' is given, else launched one by one); kernels only.
 int pr_enqueue(gbk_ctx *ctx, const gbk_graph *g, const RowTiles &rt, const PrPlan &p, const PrBufs &b, bool gs,
-               int max_iters, double tol, int fixed_work, LoopGraph *loop, std::optional<EventTimer> *timer) {
+               int max_iters, double tol, int fixed_work, LoopGraph *loop, std::optional<EventTimer> *timer,
+               const SegLayout *L = nullptr) {
   const int64_t n = g->n;
   hipStream_t s = ctx->stream;
   const float kDamp = 0.85f;
@@ -357,7 +745,25 @@ int pr_enqueue(gbk_ctx *ctx, const gbk_graph *g, const RowTiles &rt, const PrPla
       float *cur = gs ? b.c0 : ((it & 1) ? b.c1 : b.c0);
       float *nxt = gs ? (p.publish ? b.c1 : b.c0) : ((it & 1) ? b.c0 : b.c1);
       for (int blk = 0; blk < p.B; blk++) {
-        switch (p.unroll) {
+        if (p.seg) {
+          switch (p.unroll) {
+            case 1:
+              pr_seg_block<1><<<p.grid, kBlock, 0, s>>>(*L, rt, b.blocks, blk, p.G, g->in_nbr, g->off, b.scores, cur,
+                                                        nxt, b.st, b.partial, b.hacc, base, kDamp);
+              break;
+            case 2:
+              pr_seg_block<2><<<p.grid, kBlock, 0, s>>>(*L, rt, b.blocks, blk, p.G, g->in_nbr, g->off, b.scores, cur,
+                                                        nxt, b.st, b.partial, b.hacc, base, kDamp);
+              break;
+            case 8:
+              pr_seg_block<8><<<p.grid, kBlock, 0, s>>>(*L, rt, b.blocks, blk, p.G, g->in_nbr, g->off, b.scores, cur,
+                                                        nxt, b.st, b.partial, b.hacc, base, kDamp);
+              break;
+            default:
+              pr_seg_block<4><<<p.grid, kBlock, 0, s>>>(*L, rt, b.blocks, blk, p.G, g->in_nbr, g->off, b.scores, cur,
+                                                        nxt, b.st, b.partial, b.hacc, base, kDamp);
+          }
+        } else switch (p.unroll) {
           case 1:
             pr_pull_block<1><<<p.grid, kBlock, 0, s>>>(rt, b.blocks, blk, g->in_nbr, g->off, b.scores, cur, nxt,
                                                        b.st, b.partial, b.hacc, base, kDamp);
@@ -387,6 +793,15 @@ int pr_enqueue(gbk_ctx *ctx, const gbk_graph *g, const RowTiles &rt, const PrPla
   if (loop)
     if (int rc = loop->capture(iterations)) return rc;
   if (timer) timer->emplace(ctx);
+  if (p.seg) {
+    // the segmented layout of this graph, built inside the timed region
+    seg_records<<<grid_for(rt.ntiles, kWavesPerBlock), kBlock, 0, s>>>(rt, g->in_nbr, L->S, L->lw, L->rec, L->tseg);
+    if (rt.nheavy)
+      seg_heavy_pieces<<<static_cast<unsigned>(std::min<int64_t>(rt.nheavy, 4096)), kBlock, 0, s>>>(
+          rt, g->in_nbr, L->S, L->lw, L->hps, L->hcnt);
+    seg_heavy_prefix<<<L->S, kBlock, 0, s>>>(L->hcnt, rt.nheavy, L->hpc);
+    GBK_LAUNCH_CHECK(ctx, "pr segment layout");
+  }
   pr_blocks_init<<<grid_for(std::max<int64_t>(rt.nheavy, p.B)), kBlock, 0, s>>>(rt, p.B, b.blocks, b.hacc);
   pr_init<<<grid_for(n), kBlock, 0, s>>>(g->off, n, init, b.scores, b.c0, b.st);
   if (loop) {
@@ -449,7 +864,22 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   hipStream_t s = ctx->stream;
   const RowTiles rt = row_tiles(g, true);
   const bool gs = variant == GBK_PR_GS;
-  const PrPlan p = pr_plan(ctx, rt, n, gs, true);
+  SegLayout L;
+  // 32-bit record counts and in-tile offsets: light rows up to 2^24 edges,
+  // rows below 2^30
+  L.lw = rt.ntiles > 0 && rt.heavy_deg <= (int64_t(1) << 24) && g->max_in_degree < (int64_t(1) << 30)
+             ? seg_log2(ctx, n) : 0;
+  if (L.lw) {
+    L.S = static_cast<int>((n + (int64_t(1) << L.lw) - 1) >> L.lw);
+    const int64_t S = L.S, nh = std::max<int64_t>(rt.nheavy, 1);
+    L.rec = static_cast<int32_t *>(scratch(ctx, kS9, sizeof(int32_t) * std::max<int64_t>(g->m, 1)));
+    L.tseg = static_cast<int32_t *>(scratch(ctx, kS10, sizeof(int32_t) * rt.ntiles * (S + 1)));
+    L.hps = static_cast<int64_t *>(scratch(ctx, kS11, sizeof(int64_t) * S * nh));
+    L.hpc = static_cast<int64_t *>(scratch(ctx, kS12, sizeof(int64_t) * S * (rt.nheavy + 1)));
+    L.hcnt = static_cast<int64_t *>(scratch(ctx, kS13, sizeof(int64_t) * S * nh));
+    if (!L.rec || !L.tseg || !L.hps || !L.hpc || !L.hcnt) return set_error(ctx, GBK_ENOMEM, "pr segment scratch");
+  }
+  const PrPlan p = pr_plan(ctx, rt, n, gs, true, L.lw != 0);
   PrBufs b;
   b.scores = static_cast<float *>(scratch(ctx, kS0, sizeof(float) * n));
   b.c0 = static_cast<float *>(scratch(ctx, kS1, sizeof(float) * n));
@@ -463,7 +893,7 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
     return set_error(ctx, GBK_ENOMEM, "pr scratch");
   LoopGraph loop(ctx);
   std::optional<EventTimer> timer;
-  if (int rc = pr_enqueue(ctx, g, rt, p, b, gs, max_iters, tol, fixed_work, &loop, &timer)) return rc;
+  if (int rc = pr_enqueue(ctx, g, rt, p, b, gs, max_iters, tol, fixed_work, &loop, &timer, &L)) return rc;
   const double ms = timer->stop_ms();
   PrState h{};
   GBK_HIP(ctx, hipMemcpyAsync(&h, b.st, sizeof(h), hipMemcpyDeviceToHost, s));

@@ -420,6 +420,8 @@ int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value) {
   else if (k == "sssp_wide" && (value == 0 || value == 1)) t.sssp_wide = static_cast<int>(value);
   else if (k == "msbfs_group" && value >= 1 && value <= 64 && (value & (value - 1)) == 0)
     t.msbfs_group = static_cast<int>(value);
+  else if (k == "pr_segment" && (value == -1 || value == 0 || (value >= 6 && value <= 26)))
+    t.pr_segment = static_cast<int>(value);
   else if (k == "ktime" && value >= GBK_KT_NONE && value <= GBK_KT_LAST) {
     if (value != GBK_KT_NONE && !ctx->kt_dev) {
       const size_t bytes = sizeof(unsigned long long) * (2 + 2 * kKtPartials);
@@ -454,6 +456,7 @@ int gbk_ctx_get_param(gbk_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "bc_pull_ratio") *value = t.bc_pull_ratio;
   else if (k == "sssp_wide") *value = t.sssp_wide;
   else if (k == "msbfs_group") *value = t.msbfs_group;
+  else if (k == "pr_segment") *value = t.pr_segment;
   else if (k == "ktime") *value = ctx->ktime;
   else return set_error(ctx, GBK_EINVAL, "unknown parameter: " + k);
   return GBK_OK;

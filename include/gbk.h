@@ -108,10 +108,13 @@ int gbk_ctx_sync(gbk_ctx *ctx);
  * one); "batch_grid" (workgroups of the source-batch relax launches: 0 launches
  * as many as the GPU keeps resident, else 8..16384 in multiples of 8);
  * "pr_blocks" (row blocks of a Gauss-Seidel PageRank sweep, run in order: 0
- * picks one per ~4096 row tiles, at most 8; 1 is one launch updating in
+ * picks one per ~4096 row tiles, at most 2; 1 is one launch updating in
  * place); "pr_inplace" (1: the blocks update contributions in place rather than
  * publishing them block by block); "pr_unroll" (64-edge chunks a PageRank wave keeps in flight: 1, 2,
- * 4, 8, or 0 for 8 in block sweeps and 1 otherwise); "bc_pull" (BC batch forward levels: 0
+ * 4, 8, or 0 for 8 in block sweeps, 4 in segmented sweeps and 1 otherwise); "pr_segment"
+ * (PageRank's segmented sweep, which walks in-edges by source segment so gathers stay in an XCD's
+ * L2: 0 (default) on graphs above 2^19 vertices, -1 never, 6..26 the smallest segment width 2^k
+ * tried, widened until at most 32 segments remain); "bc_pull" (BC batch forward levels: 0
  * top-down only, 1 (default) bottom-up from in-neighbours when the frontier's out-edges outweigh
  * those of the vertices some source has not reached, 2 bottom-up at every level); "bc_pull_ratio"
  * (1..1024, default 1: how many such edges one frontier edge weighs in that choice); "sssp_wide"
