@@ -174,6 +174,14 @@ struct gbk_graph {
   int32_t *tiles_out = nullptr, *tiles_in = nullptr;
   int64_t ntiles_out = 0, ntiles_in = 0;
   int64_t *hchunk_out = nullptr, *hchunk_in = nullptr;
+  // Smallest and largest out-edge weight, found by the first call that needs
+  // them (-1 until then); the graph is immutable, so they stay valid.
+  int64_t wmin = -1, wmax = -1;
+  // One-source SSSP's light/heavy layout of the out-CSR for delta
+  // 'part_delta' (-1: none yet): each row's edges lighter than delta first,
+  // then the rest, each class in CSR order; plc[u] = light edges of row u.
+  int64_t part_delta = -1;
+  int32_t *pnbr = nullptr, *pwgt = nullptr, *plc = nullptr;
   std::vector<void *> owned;   // device allocations freed with the graph
 };
 

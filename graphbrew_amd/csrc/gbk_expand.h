@@ -29,6 +29,20 @@ __device__ __forceinline__ void st_agent(int32_t *p, int32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Lane whose range [excl, excl + len) holds edge e of the union (6-step
+// shuffle search over the non-decreasing exclusive prefixes); call in
+// uniform control flow.
+__device__ __forceinline__ int walk_owner(int64_t excl, int64_t e) {
+  int lo = 0;
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1) {
+    const int probe = lo + step;
+    const int64_t x = __shfl(excl, probe & (kWave - 1));
+    if (probe < kWave && x <= e) lo = probe;
+  }
+  return lo;
+}
+
 // Walks the union of the lanes' edge ranges.  f(row, edge_index, active).
 template <class F>
 __device__ __forceinline__ void wave_walk(int64_t row, int64_t start, int64_t len, F &f) {

@@ -23,15 +23,28 @@
 //             holding an unsettled vertex and a second pass queues that
 //             bucket.  Two streaming passes per bucket cost far less than a
 //             pending list would in random atomics (one per improvement).
+// Light and heavy edges (Meyer and Sanders, "Delta-stepping: a
+// parallelizable shortest path algorithm", J. Algorithms 2003): the rounds
+// of a bucket relax only edges lighter than delta -- the only ones whose
+// candidates can land in the bucket itself -- and once the bucket is
+// settled one pass relaxes the heavy edges of each of its vertices, from
+// final distances.  A vertex re-queued by an improvement therefore re-reads
+// the distances behind its light edges only (on Kronecker scale 22 with
+// weights U[1,255] and delta 64 the reference's loop scans each edge about
+// four times per source).  The settled vertices are listed as the rounds
+// process them, once per bucket (a per-vertex stamp of the last bucket).
+// The distances are the reference's: both are exact shortest paths.
 // Each lane relaxes four edges per trip (independent loads and atomics in
 // flight together): the path is bound by the latency of dependent random
 // accesses, not by bandwidth.
-// Kernels of one step: relax, then min-bucket and split (each acting only in
-// its phase), each followed by a one-workgroup kernel that advances the
-// device state block.  The host queues steps without waiting and checks the
+// Kernels of one step: relax (a light round in phase 0, the heavy pass in
+// phase 3), then min-bucket and split (each acting only in its phase), each
+// followed by a one-workgroup kernel that advances the device state block.  The host queues steps without waiting and checks the
 // 'done' flag after 2, 4, ... 64 steps.  Distances are integers, so the
 // result is exact whatever the order of relaxations.
+#include <algorithm>
 #include <climits>
+#include <vector>
 
 #include "gbk_expand.h"
 
@@ -47,6 +60,7 @@ __device__ __forceinline__ unsigned long long make_word(int64_t d, uint32_t tag)
 
 struct SsspState {
   unsigned long long near_cnt[2];
+  unsigned long long settled_cnt;  // vertices of the bucket listed for its heavy pass
   unsigned long long next_bucket;
   long long bucket;
   long long relax_rounds;
@@ -54,14 +68,18 @@ struct SsspState {
   int cur, phase, done, overflow;
 };
 
-__global__ void sssp_init(int64_t n, int32_t source, unsigned long long *word, int32_t *near0,
+__global__ void sssp_init(int64_t n, int32_t source, unsigned long long *word, int32_t *stamp, int32_t *near0,
                           SsspState *st) {
   const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v < n) word[v] = make_word(v == source ? 0 : kDistInf, 0);
+  if (v < n) {
+    word[v] = make_word(v == source ? 0 : kDistInf, 0);
+    stamp[v] = -1;
+  }
   if (v == 0) {
     near0[0] = source;
     st->near_cnt[0] = 1;
     st->near_cnt[1] = 0;
+    st->settled_cnt = 0;
     st->next_bucket = ULLONG_MAX;
     st->bucket = 0;
     st->relax_rounds = 0;
@@ -81,12 +99,14 @@ struct RelaxEdge {
   int64_t delta;
   int64_t bucket;
   uint32_t tag;  // this round's tag, never 0
+  bool heavy;    // the heavy pass: edges of weight >= delta, nothing queued
 
   template <int U>
   __device__ __forceinline__ void batch(const int64_t (&u)[U], const int64_t (&k)[U], const bool (&act)[U]) {
     int32_t v[U];
     int64_t cand[U];
     unsigned long long wv[U];
+    bool take[U];
 #pragma unroll
     for (int j = 0; j < U; j++) {
       v[j] = nbr[k[j]];
@@ -94,8 +114,10 @@ struct RelaxEdge {
     }
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      cand[j] += word_dist(word[u[j]]);
-      wv[j] = word[v[j]];
+      // an edge of the other class reads no distance
+      take[j] = act[j] && (cand[j] >= delta) == heavy;
+      cand[j] += take[j] ? word_dist(word[u[j]]) : 0;
+      wv[j] = take[j] ? word[v[j]] : 0ull;
     }
     bool push[U];
 #pragma unroll
@@ -103,8 +125,8 @@ struct RelaxEdge {
       push[j] = false;
       // test first: most relaxations do not improve, and a plain load can
       // only overstate dist[v] (values only decrease), never hide a win
-      if (act[j] && cand[j] < word_dist(wv[j])) {
-        const bool here = cand[j] / delta == bucket;
+      if (take[j] && cand[j] < word_dist(wv[j])) {
+        const bool here = !heavy && cand[j] / delta == bucket;
         const unsigned long long old = atomicMin(&word[v[j]], make_word(cand[j], here ? tag : 0u));
         push[j] = here && cand[j] < word_dist(old) && static_cast<uint32_t>(old) != tag;
       }
@@ -120,6 +142,7 @@ __device__ __forceinline__ void relax_setup(RelaxEdge &f, int32_t *seg, const in
                                             int64_t delta, SsspState *st) {
   const volatile SsspState *vs = st;
   const int cur = vs->cur;
+  f.heavy = vs->phase == 3;
   f.nbr = nbr;
   f.wgt = wgt;
   f.word = word;
@@ -130,33 +153,65 @@ __device__ __forceinline__ void relax_setup(RelaxEdge &f, int32_t *seg, const in
   f.tag = static_cast<uint32_t>(vs->relax_rounds + 1);
 }
 
+// A light round (phase 0) over the near queue -- each row not already
+// listed in this bucket goes on the settled list -- or the heavy pass
+// (phase 3) over the settled list.
+// Rows are read in the light/heavy layout (nbr, wgt, lc): a light round
+// walks a row's light edges, the heavy pass its heavy ones.  Without heavy
+// edges in the graph no vertex is listed.
 __global__ void __launch_bounds__(kBlock)
-sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, unsigned long long *word,
-           int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, int64_t heavy_deg,
-           SsspState *st, HeavyQueue hq) {
+sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, const int32_t *lc,
+           unsigned long long *word, int32_t *near0, int32_t *near1, int32_t *settled, int32_t *stamp,
+           int64_t cap, int64_t delta, int64_t heavy_deg, int has_heavy, SsspState *st, HeavyQueue hq) {
   const volatile SsspState *vs = st;
-  if (vs->done || vs->phase != 0) return;
+  if (vs->done || (vs->phase != 0 && vs->phase != 3)) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  __shared__ int32_t sseg[kWavesPerBlock * kSeg];
   const int cur = vs->cur;
-  // the tail counts past cap only when a push overflowed (reported by the host)
-  const int64_t count = min(static_cast<int64_t>(vs->near_cnt[cur]), cap);
+  const bool heavy = vs->phase == 3;
+  // the tails count past cap only when a push overflowed (reported by the host)
+  const int64_t count = min(static_cast<int64_t>(heavy ? vs->settled_cnt : vs->near_cnt[cur]), cap);
+  const int32_t *list = heavy ? settled : (cur ? near1 : near0);
   RelaxEdge f;
   relax_setup(f, seg, nbr, wgt, word, near0, near1, cap, delta, st);
+  WaveQueue sq;
+  sq.init(sseg + (threadIdx.x / kWave) * kSeg, &st->settled_cnt, settled, cap, &st->overflow);
   const int64_t lower = delta * f.bucket;
-  auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
-    s = off[u];
-    l = word_dist(word[u]) >= lower ? off[u + 1] - s : 0;
-  };
-  expand_list_light_batch<kRelaxBatch>(cur ? near1 : near0, count, heavy_deg, rowlen, f, hq);
+  const int32_t bucket = static_cast<int32_t>(f.bucket);
+  for (int64_t tile = blockIdx.x; tile * kBlock < count; tile += gridDim.x) {
+    const int64_t i = tile * kBlock + threadIdx.x;
+    int64_t row = 0, start = 0, len = 0;
+    bool first = false;
+    if (i < count) {
+      row = list[i];
+      const int64_t deg = off[row + 1] - off[row];
+      const int64_t nl = lc[row];
+      if (heavy) {
+        start = off[row] + nl;
+        len = deg - nl;
+      } else if (word_dist(word[row]) >= lower) {
+        start = off[row];
+        len = nl;
+        if (has_heavy && deg > nl) first = atomicExch(&stamp[row], bucket) != bucket;
+      }
+      if (len > heavy_deg) {
+        heavy_push(hq, static_cast<int32_t>(row), start, len);
+        len = 0;
+      }
+    }
+    sq.push(first, static_cast<int32_t>(row));
+    wave_walk_batch<kRelaxBatch>(row, start, len, f);
+  }
   f.nearq.flush_block();
+  sq.flush_block();
 }
 
-// Heavy rows of the near queue, spread over all workgroups.
+// Heavy rows of the round's list, spread over all workgroups.
 __global__ void __launch_bounds__(kBlock)
 sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, unsigned long long *word, int32_t *near0,
                  int32_t *near1, int64_t cap, int64_t delta, SsspState *st, HeavyQueue hq) {
   const volatile SsspState *vs = st;
-  if (vs->done || vs->phase != 0) return;
+  if (vs->done || (vs->phase != 0 && vs->phase != 3)) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   RelaxEdge f;
   relax_setup(f, seg, nbr, wgt, word, near0, near1, cap, delta, st);
@@ -167,22 +222,24 @@ sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, unsigned long long *wor
 // Work of the coming relaxation round, for the kernel timer only: rows of
 // the near queue that are not stale and their edges, as per-workgroup
 // partials (rows in partial[0, grid), edges in partial[grid, 2 grid)).
-__global__ void sssp_count_round(const int64_t *off, const unsigned long long *word,
-                                 const int32_t *near0, const int32_t *near1, int64_t delta,
-                                 const SsspState *st, unsigned long long *partial) {
+__global__ void sssp_count_round(const int64_t *off, const int32_t *lc, const unsigned long long *word,
+                                 const int32_t *near0, const int32_t *near1, const int32_t *settled,
+                                 int64_t delta, const SsspState *st, unsigned long long *partial) {
   const volatile SsspState *vs = st;
   unsigned long long rows = 0, edges = 0;
-  if (!vs->done && vs->phase == 0) {
+  if (!vs->done && (vs->phase == 0 || vs->phase == 3)) {
     const int cur = vs->cur;
-    const int32_t *q = cur ? near1 : near0;
-    const int64_t count = static_cast<int64_t>(vs->near_cnt[cur]);
-    const int64_t lower = delta * vs->bucket;
+    const bool heavy = vs->phase == 3;
+    const int32_t *q = heavy ? settled : (cur ? near1 : near0);
+    const int64_t count = static_cast<int64_t>(heavy ? vs->settled_cnt : vs->near_cnt[cur]);
+    const int64_t lower = heavy ? 0 : delta * vs->bucket;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
       const int32_t u = q[i];
       if (word_dist(word[u]) >= lower) {
         rows++;
-        edges += static_cast<unsigned long long>(off[u + 1] - off[u]);
+        const int64_t deg = off[u + 1] - off[u];
+        edges += static_cast<unsigned long long>(heavy ? deg - lc[u] : lc[u]);
       }
     }
   }
@@ -192,14 +249,219 @@ __global__ void sssp_count_round(const int64_t *off, const unsigned long long *w
 
 // One workgroup: closes a relaxation round (swap the near queues; when the
 // next one is empty the bucket is done and the search for the next opens).
-__global__ void sssp_relax_done(SsspState *st, unsigned long long *heavy_count) {
-  if (threadIdx.x != 0 || st->done || st->phase != 0) return;
+// After the heavy pass the search for the next bucket opens.
+__global__ void sssp_relax_done(SsspState *st, unsigned long long *heavy_count, int has_heavy) {
+  if (threadIdx.x != 0 || st->done) return;
+  if (st->phase == 3) {
+    *heavy_count = 0;
+    st->settled_cnt = 0;
+    st->phase = 1;
+    return;
+  }
+  if (st->phase != 0) return;
   *heavy_count = 0;
   const int cur = st->cur;
   st->near_cnt[cur] = 0;
   st->cur = 1 - cur;
   st->relax_rounds += 1;
-  if (st->near_cnt[1 - cur] == 0) st->phase = 1;
+  if (st->near_cnt[1 - cur] == 0) st->phase = has_heavy ? 3 : 1;
+}
+
+// ------------------------------------------------ light/heavy row layout
+// Rows of the out-CSR's tiles are walked by one wave per tile in 64-edge
+// chunks; a lane's rank within its row's run of the chunk comes from
+// ballots of the light flag, the run's earlier chunks from per-row carries
+// in LDS.  Rows above the heavy degree take one workgroup each.  Positions
+// are fixed by the CSR order, so the layout is the same every run.
+
+__device__ __forceinline__ uint64_t run_before(int run_lo) {
+  // lanes run_lo .. lane-1
+  const uint64_t lt = (uint64_t(1) << lane_id()) - 1;
+  return lt & ~((uint64_t(1) << run_lo) - 1);
+}
+
+// Pass 1: light edges per row.
+__global__ void __launch_bounds__(kBlock)
+part_count(RowTiles rt, const int32_t *wgt, int64_t delta, int32_t *lc) {
+  __shared__ int32_t cnt_all[kWavesPerBlock][kWave];
+  __shared__ int32_t red[kWavesPerBlock];
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  int32_t *cnt = cnt_all[wv];
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wv; t < rt.ntiles;
+       t += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
+    const int64_t u = rt.tiles[t] + lane;
+    const bool valid = u < rt.tiles[t + 1];
+    int64_t start = 0, len = 0;
+    if (valid) {
+      start = rt.off[u];
+      len = rt.off[u + 1] - start;
+    }
+    const bool tiled = valid && len <= rt.heavy_deg;
+    if (!tiled) len = 0;
+    cnt[lane] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const int64_t incl = wave_incl_scan(len);
+    const int64_t excl = incl - len;
+    const int64_t total = __shfl(incl, kWave - 1);
+    for (int64_t c = 0; c < total; c += kWave) {
+      const int64_t e = c + lane;
+      const int lo = walk_owner(excl, e);
+      const bool act = e < total;
+      const int64_t oexcl = __shfl(excl, lo);
+      const int64_t ostart = __shfl(start, lo);
+      const bool light = act && wgt[ostart + (e - oexcl)] < delta;
+      const uint64_t lm = __ballot(light);
+      const int nxt = __shfl_down(lo, 1);
+      const bool tail = act && (lane == kWave - 1 || e + 1 >= total || nxt != lo);
+      const int run_lo = static_cast<int>(oexcl > c ? oexcl - c : 0);
+      if (tail) cnt[lo] += __popcll(lm & (run_before(run_lo) | (uint64_t(1) << lane)));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    if (tiled) lc[u] = cnt[lane];
+  }
+  for (int64_t h = blockIdx.x; h < rt.nheavy; h += gridDim.x) {
+    const int64_t row = rt.heavy[h];
+    int32_t c = 0;
+    for (int64_t k = rt.off[row] + threadIdx.x; k < rt.off[row + 1]; k += blockDim.x) c += wgt[k] < delta;
+    c = block_sum(c, red);
+    if (threadIdx.x == 0) lc[row] = c;
+  }
+}
+
+// Pass 2: each edge to its place.
+__global__ void __launch_bounds__(kBlock)
+part_place(RowTiles rt, const int32_t *nbr, const int32_t *wgt, int64_t delta, const int32_t *lc,
+           int32_t *pnbr, int32_t *pwgt) {
+  __shared__ int32_t carry_all[kWavesPerBlock][2 * kWave];
+  __shared__ int32_t wl[kWavesPerBlock], wh[kWavesPerBlock];
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  int32_t *carry = carry_all[wv];  // [0, 64): light placed so far, [64, 128): heavy
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wv; t < rt.ntiles;
+       t += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
+    const int64_t u = rt.tiles[t] + lane;
+    const bool valid = u < rt.tiles[t + 1];
+    int64_t start = 0, len = 0;
+    if (valid) {
+      start = rt.off[u];
+      len = rt.off[u + 1] - start;
+    }
+    const bool tiled = valid && len <= rt.heavy_deg;
+    if (!tiled) len = 0;
+    const int32_t my_light = tiled ? lc[u] : 0;
+    carry[lane] = 0;
+    carry[kWave + lane] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const int64_t incl = wave_incl_scan(len);
+    const int64_t excl = incl - len;
+    const int64_t total = __shfl(incl, kWave - 1);
+    for (int64_t c = 0; c < total; c += kWave) {
+      const int64_t e = c + lane;
+      const int lo = walk_owner(excl, e);
+      const bool act = e < total;
+      const int64_t oexcl = __shfl(excl, lo);
+      const int64_t ostart = __shfl(start, lo);
+      const int32_t olight = __shfl(my_light, lo);
+      const int nxt = __shfl_down(lo, 1);
+      int32_t v = 0, w = 0;
+      if (act) {
+        v = nbr[ostart + (e - oexcl)];
+        w = wgt[ostart + (e - oexcl)];
+      }
+      const bool light = act && w < delta;
+      const uint64_t lm = __ballot(light);
+      const uint64_t hm = __ballot(act && !light);
+      const int run_lo = static_cast<int>(oexcl > c ? oexcl - c : 0);
+      const uint64_t before = run_before(run_lo);
+      const int32_t cl = carry[lo], ch = carry[kWave + lo];
+      if (act) {
+        const int64_t pos = light ? ostart + cl + __popcll(lm & before)
+                                  : ostart + olight + ch + __popcll(hm & before);
+        pnbr[pos] = v;
+        pwgt[pos] = w;
+      }
+      const bool tail = act && (lane == kWave - 1 || e + 1 >= total || nxt != lo);
+      if (tail) {
+        const uint64_t run = before | (uint64_t(1) << lane);
+        carry[lo] = cl + __popcll(lm & run);
+        carry[kWave + lo] = ch + __popcll(hm & run);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  for (int64_t h = blockIdx.x; h < rt.nheavy; h += gridDim.x) {
+    const int64_t row = rt.heavy[h];
+    const int64_t a = rt.off[row], b = rt.off[row + 1];
+    const int32_t L = lc[row];
+    int64_t cl = 0, ch = 0;
+    for (int64_t base = a; base < b; base += blockDim.x) {
+      const int64_t k = base + threadIdx.x;
+      const bool act = k < b;
+      int32_t v = 0, w = 0;
+      if (act) {
+        v = nbr[k];
+        w = wgt[k];
+      }
+      const bool light = act && w < delta;
+      const uint64_t lm = __ballot(light), hm = __ballot(act && !light);
+      __syncthreads();
+      if (lane == 0) {
+        wl[wv] = __popcll(lm);
+        wh[wv] = __popcll(hm);
+      }
+      __syncthreads();
+      int64_t bl = 0, bh = 0, tl = 0, th = 0;
+      for (int i = 0; i < kWavesPerBlock; i++) {
+        if (i < wv) {
+          bl += wl[i];
+          bh += wh[i];
+        }
+        tl += wl[i];
+        th += wh[i];
+      }
+      const uint64_t lt = (uint64_t(1) << lane) - 1;
+      if (act) {
+        const int64_t pos = light ? a + cl + bl + __popcll(lm & lt) : a + L + ch + bh + __popcll(hm & lt);
+        pnbr[pos] = v;
+        pwgt[pos] = w;
+      }
+      cl += tl;
+      ch += th;
+    }
+  }
+}
+
+// Smallest and largest weight (per-workgroup partials, then one workgroup).
+__global__ void weight_range(const int32_t *w, int64_t m, long long *partial) {
+  long long lo = LLONG_MAX, hi = LLONG_MIN;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride) {
+    lo = min(lo, static_cast<long long>(w[i]));
+    hi = max(hi, static_cast<long long>(w[i]));
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o));
+    hi = max(hi, __shfl_xor(hi, o));
+  }
+  __shared__ long long red[2][kWavesPerBlock];
+  if (lane_id() == 0) {
+    red[0][threadIdx.x / kWave] = lo;
+    red[1][threadIdx.x / kWave] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kWavesPerBlock; i++) {
+      lo = min(lo, red[0][i]);
+      hi = max(hi, red[1][i]);
+    }
+    partial[2 * blockIdx.x] = lo;
+    partial[2 * blockIdx.x + 1] = hi;
+  }
 }
 
 // Smallest bucket above the current one that holds a reached vertex.
@@ -310,7 +572,7 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   hipStream_t s = ctx->stream;
   const int64_t cap = n + 64;
   auto *word = static_cast<unsigned long long *>(scratch(ctx, kS5, sizeof(unsigned long long) * cap));
-  auto *q = static_cast<int32_t *>(scratch(ctx, kS7, sizeof(int32_t) * cap * 2));
+  auto *q = static_cast<int32_t *>(scratch(ctx, kS7, sizeof(int32_t) * cap * 4));
   auto *st = static_cast<SsspState *>(scratch(ctx, kS8, sizeof(SsspState) + 64));
   auto *fin = static_cast<unsigned long long *>(scratch(ctx, kS9, 64));
   auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS10, sizeof(unsigned long long) * 2 * kGridCap));
@@ -319,8 +581,66 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   if (!word || !q || !st || !fin || !partial || !hs || !make_heavy_queue(ctx, g, kS11, &hq))
     return set_error(ctx, GBK_ENOMEM, "sssp scratch");
   hq.overflow = &st->overflow;
-  int32_t *near0 = q, *near1 = q + cap;
+  int32_t *near0 = q, *near1 = q + cap, *settled = q + 2 * cap, *stamp = q + 3 * cap;
+  if (g->wmin < 0) {
+    // the graph's weight range, once: which edge classes a delta leaves
+    constexpr unsigned kRangeGrid = 1024;
+    static_assert(2 * kRangeGrid <= 2 * kGridCap, "weight range partials fit the partial buffer");
+    auto *rp = reinterpret_cast<long long *>(partial);
+    std::vector<long long> h(2 * kRangeGrid);
+    int64_t lo = 0, hi = 0;
+    if (g->m > 0) {
+      weight_range<<<kRangeGrid, kBlock, 0, s>>>(g->wgt, g->m, rp);
+      GBK_LAUNCH_CHECK(ctx, "sssp weight range");
+      GBK_HIP(ctx, hipMemcpyAsync(h.data(), rp, sizeof(long long) * h.size(), hipMemcpyDeviceToHost, s));
+      GBK_HIP(ctx, hipStreamSynchronize(s));
+      lo = LLONG_MAX;
+      hi = LLONG_MIN;
+      for (unsigned i = 0; i < kRangeGrid; i++) {
+        lo = std::min<int64_t>(lo, h[2 * i]);
+        hi = std::max<int64_t>(hi, h[2 * i + 1]);
+      }
+    }
+    auto *mg = const_cast<gbk_graph *>(g);
+    mg->wmin = std::max<int64_t>(lo, 0);
+    mg->wmax = std::max<int64_t>(hi, 0);
+  }
+  // heavy edges (weight >= delta) exist
+  const int has_heavy = g->m > 0 && g->wmax >= delta;
   const int64_t H = ctx->tune.heavy_degree;
+  // The light/heavy layout for this delta, kept with the graph; its first
+  // call with a delta pays for it (timed on its own and added to the call's
+  // device time).  It exists before the step loop is captured: a captured
+  // launch keeps the pointers it was recorded with.
+  double layout_ms = 0;
+  if (g->part_delta != delta) {
+    EventTimer layout_timer(ctx);
+    auto *mg = const_cast<gbk_graph *>(g);
+    if (!mg->pnbr) {
+      void *a = nullptr, *b = nullptr, *c = nullptr;
+      if (hipMalloc(&a, sizeof(int32_t) * std::max<int64_t>(g->m, 1)) != hipSuccess ||
+          hipMalloc(&b, sizeof(int32_t) * std::max<int64_t>(g->m, 1)) != hipSuccess ||
+          hipMalloc(&c, sizeof(int32_t) * std::max<int64_t>(n, 1)) != hipSuccess) {
+        hipFree(a);
+        hipFree(b);
+        hipFree(c);
+        return set_error(ctx, GBK_ENOMEM, "sssp light/heavy layout");
+      }
+      mg->owned.push_back(a);
+      mg->owned.push_back(b);
+      mg->owned.push_back(c);
+      mg->pnbr = static_cast<int32_t *>(a);
+      mg->pwgt = static_cast<int32_t *>(b);
+      mg->plc = static_cast<int32_t *>(c);
+    }
+    const RowTiles rt = row_tiles(g, false);
+    const unsigned pgrid = grid_for(std::max<int64_t>(rt.ntiles, 1), kWavesPerBlock);
+    part_count<<<pgrid, kBlock, 0, s>>>(rt, g->wgt, delta, mg->plc);
+    part_place<<<pgrid, kBlock, 0, s>>>(rt, g->nbr, g->wgt, delta, mg->plc, mg->pnbr, mg->pwgt);
+    GBK_LAUNCH_CHECK(ctx, "sssp light/heavy layout");
+    layout_ms = layout_timer.stop_ms();
+    mg->part_delta = delta;
+  }
   const unsigned relax_grid = static_cast<unsigned>(ctx->tune.relax_grid);
   const bool count = ctx->ktime == GBK_KT_SSSP_RELAX && ctx->kt_dev;
   const unsigned count_grid = static_cast<unsigned>(kKtPartials < 256 ? kKtPartials : 256);
@@ -330,18 +650,19 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   auto step = [&]() {
       if (count) {
         unsigned long long *part = ctx->kt_dev + 2;
-        sssp_count_round<<<count_grid, kBlock, 0, s>>>(g->off, word, near0, near1, delta, st, part);
+        sssp_count_round<<<count_grid, kBlock, 0, s>>>(g->off, g->plc, word, near0, near1, settled, delta, st,
+                                                       part);
         sum_partials<unsigned long long><<<1, 1024, 0, s>>>(part, static_cast<int>(count_grid), ctx->kt_dev, 1);
         sum_partials<unsigned long long><<<1, 1024, 0, s>>>(part + count_grid, static_cast<int>(count_grid),
                                                           ctx->kt_dev + 1, 1);
       }
       const int kt = kt_begin(ctx, GBK_KT_SSSP_RELAX);
-      sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->nbr, g->wgt, word, near0, near1, cap, delta, H,
-                                               st, hq);
-      sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->nbr, g->wgt, word, near0, near1, cap, delta, st,
+      sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->pnbr, g->pwgt, g->plc, word, near0, near1, settled,
+                                               stamp, cap, delta, H, has_heavy, st, hq);
+      sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->pnbr, g->pwgt, word, near0, near1, cap, delta, st,
                                                      hq);
       kt_end(ctx, kt);
-      sssp_relax_done<<<1, kWave, 0, s>>>(st, hq.count);
+      sssp_relax_done<<<1, kWave, 0, s>>>(st, hq.count, has_heavy);
       sssp_min_bucket<<<scan_grid, kBlock, 0, s>>>(word, n, delta, st, partial);
       sssp_min_done<<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), st);
       sssp_split<<<scan_grid, kBlock, 0, s>>>(word, n, near0, near1, cap, delta, st);
@@ -356,7 +677,7 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   LoopGraph loop(ctx);
   if (int rc = loop.capture(block)) return rc;
   EventTimer timer(ctx);
-  sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, word, near0, st);
+  sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, word, stamp, near0, st);
   int batch = kStepsPerGraph;
   int64_t launched = 0;
   for (;;) {
@@ -378,7 +699,7 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), fin, 0);
   sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial + kGridCap, static_cast<int>(scan_grid), fin + 1, 0);
   GBK_LAUNCH_CHECK(ctx, "sssp_finish");
-  const double ms = timer.stop_ms();
+  const double ms = timer.stop_ms() + layout_ms;
   unsigned long long hf[2] = {0, 0};
   GBK_HIP(ctx, hipMemcpyAsync(hf, fin, sizeof(hf), hipMemcpyDeviceToHost, s));
   GBK_HIP(ctx, hipStreamSynchronize(s));
