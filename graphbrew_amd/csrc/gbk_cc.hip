@@ -9,6 +9,7 @@
 // Link always sees other workgroups' hooks (no stale per-CU cache line).
 #include <random>
 #include <unordered_map>
+#include <utility>
 
 #include "gbk_expand.h"
 
@@ -82,21 +83,32 @@ cc_final_link(RowTiles rt, const int32_t *nbr, int64_t skip, int32_t c, int32_t 
   expand_rows(rt, rowlen, f);
 }
 
+// SV hook (cc_sv.cc:86-106) made deterministic: a sweep reads the labels
+// as they stood before it ('comp', not written during the sweep) and lowers
+// each offered root's entry in 'nxt' (a copy of them) to the smallest label
+// offered, by atomic minimum.  The reference's plain stores let whichever
+// offer lands last win, so its sweep count depends on the schedule (4 to 7
+// sweeps at Kronecker scale 22 on the GPU with first-offer-wins); taking the
+// smallest makes every sweep a function of the previous one and converges
+// in 3-4 sweeps there.  Labels only decrease along a hook (low < high), so
+// no cycle forms, and the converged labels -- each component's smallest
+// vertex id -- are the reference's.
 struct HookEdge {
   const int32_t *nbr;
-  int32_t *comp;
+  const int32_t *comp;
+  int32_t *nxt;
   int changed;
   __device__ void operator()(int64_t u, int64_t k, bool active) {
     if (!active) return;
-    const int32_t cu = ld_agent(&comp[u]);
-    const int32_t cv = ld_agent(&comp[nbr[k]]);
+    const int32_t cu = comp[u];
+    const int32_t cv = comp[nbr[k]];
     if (cu == cv) return;
     const int32_t high = cu > cv ? cu : cv;
     const int32_t low = cu + (cv - high);
-    if (high == ld_agent(&comp[high]) && atomicCAS(&comp[high], high, low) == high) changed = 1;
+    if (high == comp[high] && low < ld_agent(&nxt[high]) && atomicMin(&nxt[high], low) > low) changed = 1;
   }
   // U edges per lane: their neighbour ids and labels are all in flight
-  // before the first comparison (the CAS still decides every hook)
+  // before the first comparison
   template <int U>
   __device__ __forceinline__ void batch(const int64_t (&u)[U], const int64_t (&k)[U], const bool (&act)[U]) {
     int32_t v[U], cu[U], cv[U];
@@ -104,23 +116,24 @@ struct HookEdge {
     for (int j = 0; j < U; j++) v[j] = act[j] ? nbr[k[j]] : 0;
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      cu[j] = ld_agent(&comp[u[j]]);
-      cv[j] = ld_agent(&comp[v[j]]);
+      cu[j] = comp[u[j]];
+      cv[j] = comp[v[j]];
     }
 #pragma unroll
     for (int j = 0; j < U; j++) {
       if (!act[j] || cu[j] == cv[j]) continue;
       const int32_t high = cu[j] > cv[j] ? cu[j] : cv[j];
       const int32_t low = cu[j] + (cv[j] - high);
-      if (high == ld_agent(&comp[high]) && atomicCAS(&comp[high], high, low) == high) changed = 1;
+      // nxt only decreases: a stale read can only send an atomic that was
+      // not needed, never skip one that was
+      if (high == comp[high] && low < ld_agent(&nxt[high]) && atomicMin(&nxt[high], low) > low) changed = 1;
     }
   }
 };
 
-// SV hook over every edge (cc_sv.cc:86-106).
 __global__ void __launch_bounds__(kBlock)
-cc_sv_hook(RowTiles rt, const int32_t *nbr, int32_t *comp, int *change) {
-  HookEdge f{nbr, comp, 0};
+cc_sv_hook(RowTiles rt, const int32_t *nbr, const int32_t *comp, int32_t *nxt, int *change) {
+  HookEdge f{nbr, comp, nxt, 0};
   auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
     s = rt.off[u];
     l = rt.off[u + 1] - s;
@@ -187,13 +200,17 @@ int gbk_cc_device(gbk_ctx *ctx, const gbk_graph *g, int variant, int32_t *comp,
     GBK_LAUNCH_CHECK(ctx, "afforest");
   } else {
     auto *change = static_cast<int *>(scratch(ctx, kS5, 64));
+    auto *other = static_cast<int32_t *>(scratch(ctx, kS6, sizeof(int32_t) * (n + 1)));
     auto *hc = static_cast<int *>(pinned(ctx, sizeof(int)));
-    if (!change || !hc) return set_error(ctx, GBK_ENOMEM, "sv flag");
+    if (!change || !other || !hc) return set_error(ctx, GBK_ENOMEM, "sv flag");
+    int32_t *cur = comp, *nxt = other;
     for (;;) {
       fill<int>(ctx, change, 0, 1);
-      cc_sv_hook<<<rows_grid(out_rows), kBlock, 0, s>>>(out_rows, g->nbr, comp, change);
-      cc_compress<<<grid_for(n), kBlock, 0, s>>>(comp, n);
+      GBK_HIP(ctx, hipMemcpyAsync(nxt, cur, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+      cc_sv_hook<<<rows_grid(out_rows), kBlock, 0, s>>>(out_rows, g->nbr, cur, nxt, change);
+      cc_compress<<<grid_for(n), kBlock, 0, s>>>(nxt, n);
       GBK_LAUNCH_CHECK(ctx, "sv");
+      std::swap(cur, nxt);
       GBK_HIP(ctx, hipMemcpyAsync(hc, change, sizeof(int), hipMemcpyDeviceToHost, s));
       GBK_HIP(ctx, hipStreamSynchronize(s));
       iterations++;
@@ -201,6 +218,7 @@ int gbk_cc_device(gbk_ctx *ctx, const gbk_graph *g, int variant, int32_t *comp,
       // every productive round removes at least one root
       if (iterations > n + 2) return set_error(ctx, GBK_EHIP, "sv: no convergence");
     }
+    if (cur != comp) GBK_HIP(ctx, hipMemcpyAsync(comp, cur, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
   }
   const double ms = timer.stop_ms();
   if (stats) {
