@@ -87,28 +87,54 @@ cc_final_link(RowTiles rt, const int32_t *nbr, int64_t skip, int32_t c, int32_t 
 // as they stood before it ('comp', not written during the sweep) and lowers
 // each offered root's entry in 'nxt' (a copy of them) to the smallest label
 // offered, by atomic minimum.  The reference's plain stores let whichever
-// offer lands last win, so its sweep count depends on the schedule (4 to 7
-// sweeps at Kronecker scale 22 on the GPU with first-offer-wins); taking the
-// smallest makes every sweep a function of the previous one and converges
-// in 3-4 sweeps there.  Labels only decrease along a hook (low < high), so
+// offer lands last win, so its sweep count depends on the schedule (5 to
+// 7.5 sweeps at Kronecker scale 22 on the GPU with first-offer-wins);
+// taking the smallest makes every sweep a function of the previous one and
+// converges in 4 there.  Labels only decrease along a hook (low < high), so
 // no cycle forms, and the converged labels -- each component's smallest
 // vertex id -- are the reference's.
+// The labels are fully compressed between sweeps, so every label is a
+// root and the reference's root test always holds.  An edge offers its
+// higher label's root the lower one; on an undirected graph each edge is
+// seen from both rows, so it is taken from the row whose own label is the
+// higher: all of a row's offers then go to the same root and are reduced to
+// their minimum in registers, one atomic per row run.  A directed graph's
+// edge is seen from one row only, so its offers to the other end's root
+// are sent edge by edge.
+__device__ __forceinline__ int32_t seg_run_min(int32_t x, int32_t key, bool &tail) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int32_t y = __shfl_up(x, o);
+    const int32_t k = __shfl_up(key, o);
+    if (lane >= o && k == key) x = y < x ? y : x;
+  }
+  const int32_t next = __shfl_down(key, 1);
+  tail = (lane == kWave - 1) || (next != key);
+  return x;
+}
+
 struct HookEdge {
   const int32_t *nbr;
   const int32_t *comp;
   int32_t *nxt;
+  int directed;
   int changed;
-  __device__ void operator()(int64_t u, int64_t k, bool active) {
-    if (!active) return;
-    const int32_t cu = comp[u];
-    const int32_t cv = comp[nbr[k]];
-    if (cu == cv) return;
-    const int32_t high = cu > cv ? cu : cv;
-    const int32_t low = cu + (cv - high);
-    if (high == comp[high] && low < ld_agent(&nxt[high]) && atomicMin(&nxt[high], low) > low) changed = 1;
+  __device__ __forceinline__ void offer(int32_t root, int32_t low) {
+    // nxt only decreases: a stale read (this XCD's L2 may hold an older
+    // line) can only send an atomic that was not needed, never skip one
+    // that was
+    if (low < root && low < nxt[root] && atomicMin(&nxt[root], low) > low) changed = 1;
   }
-  // U edges per lane: their neighbour ids and labels are all in flight
-  // before the first comparison
+  // Heavy rows: every lane of the wave is on the same row.
+  __device__ void operator()(int64_t u, int64_t k, bool active) {
+    const int32_t cu = comp[u];
+    const int32_t cv = active ? comp[nbr[k]] : cu;
+    if (directed && cv > cu) offer(cv, cu);
+    const int32_t m = wave_min(cv < cu ? cv : INT32_MAX);
+    if (lane_id() == 0 && m < cu) offer(cu, m);
+  }
+  // Tiled rows: U edges per lane in flight, rows in contiguous lane runs.
   template <int U>
   __device__ __forceinline__ void batch(const int64_t (&u)[U], const int64_t (&k)[U], const bool (&act)[U]) {
     int32_t v[U], cu[U], cv[U];
@@ -117,23 +143,21 @@ struct HookEdge {
 #pragma unroll
     for (int j = 0; j < U; j++) {
       cu[j] = comp[u[j]];
-      cv[j] = comp[v[j]];
+      cv[j] = act[j] ? comp[v[j]] : cu[j];
     }
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      if (!act[j] || cu[j] == cv[j]) continue;
-      const int32_t high = cu[j] > cv[j] ? cu[j] : cv[j];
-      const int32_t low = cu[j] + (cv[j] - high);
-      // nxt only decreases: a stale read can only send an atomic that was
-      // not needed, never skip one that was
-      if (high == comp[high] && low < ld_agent(&nxt[high]) && atomicMin(&nxt[high], low) > low) changed = 1;
+      if (directed && cv[j] > cu[j]) offer(cv[j], cu[j]);
+      bool tail;
+      const int32_t m = seg_run_min(cv[j] < cu[j] ? cv[j] : INT32_MAX, static_cast<int32_t>(u[j]), tail);
+      if (tail && m < cu[j]) offer(cu[j], m);
     }
   }
 };
 
 __global__ void __launch_bounds__(kBlock)
-cc_sv_hook(RowTiles rt, const int32_t *nbr, const int32_t *comp, int32_t *nxt, int *change) {
-  HookEdge f{nbr, comp, nxt, 0};
+cc_sv_hook(RowTiles rt, const int32_t *nbr, const int32_t *comp, int32_t *nxt, int directed, int *change) {
+  HookEdge f{nbr, comp, nxt, directed, 0};
   auto rowlen = [&](int64_t u, int64_t &s, int64_t &l) {
     s = rt.off[u];
     l = rt.off[u + 1] - s;
@@ -207,7 +231,7 @@ int gbk_cc_device(gbk_ctx *ctx, const gbk_graph *g, int variant, int32_t *comp,
     for (;;) {
       fill<int>(ctx, change, 0, 1);
       GBK_HIP(ctx, hipMemcpyAsync(nxt, cur, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
-      cc_sv_hook<<<rows_grid(out_rows), kBlock, 0, s>>>(out_rows, g->nbr, cur, nxt, change);
+      cc_sv_hook<<<rows_grid(out_rows), kBlock, 0, s>>>(out_rows, g->nbr, cur, nxt, g->directed, change);
       cc_compress<<<grid_for(n), kBlock, 0, s>>>(nxt, n);
       GBK_LAUNCH_CHECK(ctx, "sv");
       std::swap(cur, nxt);

@@ -98,6 +98,15 @@ __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    T w = __shfl_xor(v, o);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
 // Inclusive scan across the wave.
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
