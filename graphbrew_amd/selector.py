@@ -108,17 +108,22 @@ def powerlaw(n, gamma, avg_degree, seed, permute=True):
     return _finish(stubs[0::2], stubs[1::2], n, rng, permute), n
 
 
-def corpus(count: int, seed0: int = 1000, min_scale: int = 12, max_scale: int = 16) -> List[dict]:
-    """Seeded graph specifications, cycling through the four families."""
+def corpus(count: int, seed0: int = 1000, min_scale: int = 12, max_scale: int = 16,
+           skewed: bool = False, families: Sequence[str] = ("rmat", "sbm", "lattice", "powerlaw")) -> List[dict]:
+    """Seeded graph specifications, cycling through the four families.
+    skewed: R-MAT's b = c drawn near their upper bound (1 - a) / 2, as in the
+    Kronecker generator's own 0.57 / 0.19 / 0.19, so heavy-tailed degrees are
+    seen under permuted ids too and not only in generator order."""
     rng = np.random.default_rng(seed0)
     specs = []
     for i in range(count):
         seed = seed0 + i
         scale = int(rng.integers(min_scale, max_scale + 1))
-        fam = ("rmat", "sbm", "lattice", "powerlaw")[i % 4]
+        fam = families[i % len(families)]
         if fam == "rmat":
             a = float(rng.uniform(0.45, 0.65))
-            b = c = float(rng.uniform(0.1, (1 - a) / 2 - 0.02))
+            lo = (1 - a) / 2 - 0.03 if skewed else 0.1
+            b = c = float(rng.uniform(lo, (1 - a) / 2 - 0.005 if skewed else (1 - a) / 2 - 0.02))
             specs.append(dict(family=fam, scale=scale, ef=int(rng.choice([8, 16, 32])), a=a, b=b, c=c,
                               seed=seed, permute=bool(rng.random() < 0.7)))
         elif fam == "sbm":
@@ -438,6 +443,8 @@ def main(argv=None):
     p.add_argument("--min-scale", type=int, default=12)
     p.add_argument("--max-scale", type=int, default=15)
     p.add_argument("--seed0", type=int, default=1000)
+    p.add_argument("--skewed", action="store_true", help="collect: R-MAT b = c near (1 - a) / 2")
+    p.add_argument("--families", default="rmat,sbm,lattice,powerlaw", help="collect: families to cycle through")
     p.add_argument("--corpus", default=os.path.join(os.path.dirname(MODEL_PATH), "selector_corpus.json"),
                    help="output of collect; for train / evaluate a comma-separated list to merge")
     p.add_argument("--model", default=MODEL_PATH)
@@ -449,7 +456,8 @@ def main(argv=None):
     ctx = gbk.Context(0)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
     if a.cmd == "collect":
-        collect(ctx, corpus(a.graphs, seed0=a.seed0, min_scale=a.min_scale, max_scale=a.max_scale), a.corpus, log)
+        collect(ctx, corpus(a.graphs, seed0=a.seed0, min_scale=a.min_scale, max_scale=a.max_scale,
+                            skewed=a.skewed, families=a.families.split(",")), a.corpus, log)
         return 0
     if a.cmd == "remeasure":
         # the same graphs (their specs are stored with the timings) timed again
