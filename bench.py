@@ -465,6 +465,37 @@ def orderings_leg(work, gbk, orders=(1, 2, 5)):
     return out
 
 
+def config1_leg(ctx, gbk, H, calls=5):
+    """PageRank (Gauss-Seidel, tol 1e-4) on Kronecker scale 16, edge factor
+    16, under the identity and under the reference's own Leiden-Gorder8
+    composition and RCM-BNF mappings of that graph (files the reference's
+    CPU pipeline wrote, tests/golden/mappings; -o 12 / -o 11 in the
+    drivers).  Relabel and the median of `calls` PR calls per ordering, the
+    mapping file's fingerprint with each.  After the timed region."""
+    gold = os.path.join(ROOT, "tests", "golden", "mappings")
+    u, v = H.generate_el(16, 16)
+    g, _ = ctx.build(u, v, None, -1, True)
+    out = {"workload": f"kronecker-scale16-ef16 pr-gs n={g.n} m={g.m}"}
+    maps = {"identity": H.mapping_identity(g.n)}
+    for tag in ("leiden_gorder8", "rcm_bnf"):
+        path = os.path.join(gold, f"kron16_ef16_{tag}.so")
+        if os.path.exists(path):
+            maps[tag] = H.mapping_read(path)
+    for tag, ids in maps.items():
+        if len(ids) != g.n:
+            out[tag] = {"error": "mapping does not fit the graph"}
+            continue
+        rg, rst = ctx.relabel(g, ids)
+        runs = [ctx.pr(rg, gbk.PR_GS, 20, 1e-4, False, download=False)[1] for _ in range(calls)]
+        ms = sorted(r.device_ms for r in runs)[len(runs) // 2]
+        out[tag] = {"fingerprint": H.mapping_fingerprint(ids), "relabel_ms": round(rst.device_ms, 3),
+                    "pr_ms": round(ms, 3), "pr_iterations": runs[0].iterations,
+                    "pr_gteps": round(g.m * runs[0].iterations / ms / 1e6, 2)}
+        rg.free()
+    g.free()
+    return out
+
+
 def single_source_leg(work, gbk):
     """The kernels the drop-in drivers run one source at a time: BFS and
     SSSP from each of the step's sources (one call per source, delta of
@@ -647,6 +678,7 @@ def main():
         out["orderings"] = orderings_leg(work, gbk)
         if not args.single_source:
             out["single_source"] = single_source_leg(work, gbk)
+        out["config1"] = config1_leg(ctx, gbk, H)
     if not args.no_selector:
         out["selector_e2e"] = selector_e2e(work, gbk, H)
     if args.many_graphs > 0:
