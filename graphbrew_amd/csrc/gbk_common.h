@@ -191,6 +191,11 @@ struct gbk_graph {
   // then the rest, each class in CSR order; plc[u] = light edges of row u.
   int64_t part_delta = -1;
   int32_t *pnbr = nullptr, *pwgt = nullptr, *plc = nullptr;
+  // PageRank's segmented layout of the in-CSR for segment width 2^seg_lw
+  // (0: none yet), one allocation of seg_bytes, built by the first call.
+  int seg_lw = 0;
+  void *seg_mem = nullptr;
+  size_t seg_bytes = 0;
   std::vector<void *> owned;   // device allocations freed with the graph
 };
 

@@ -10,8 +10,9 @@
 // order instead: the rows' tiles are cut into equal runs (pr_blocks, or by
 // default one per ~4096 tiles, at most 2), one launch each,
 // and a row of block b reads the new contributions of the
-// rows before block b and the previous sweep's for the rest -- no races,
-// the same answer every run.  The gathers read one buffer: a block's new
+// rows before block b and the previous sweep's for the rest -- no races:
+// every run reads the same values (heavy rows' chunk sums are added by
+// float atomics, so their last bits can round differently).  The gathers read one buffer: a block's new
 // contributions go to a side buffer and a small close launch copies them
 // in before the next block (selecting between two buffers per edge costs
 // 20%).  A graph too small to fill the GPU from a block's launch runs one
@@ -294,12 +295,36 @@ constexpr int kSegBuildU = 4;        // chunks in flight per wave in the layout 
 struct SegLayout {
   int S = 0;
   int lw = 0;
+  bool build = false;       // the call builds it (else it is the graph's, built before)
   int32_t *rec = nullptr;
   int32_t *tseg = nullptr;
   int64_t *hps = nullptr;
   int64_t *hpc = nullptr;
-  int64_t *hcnt = nullptr;  // build scratch: chunk counts per (segment, heavy row)
+  int64_t *hcnt = nullptr;  // chunk counts per (segment, heavy row)
+  int64_t *cbase = nullptr; // S + 1: first entry of segment s in crow
+  int32_t *crow = nullptr;  // heavy row of each chunk, segment by segment
 };
+
+// Bytes of the layout's arrays, carved from one allocation (256-byte
+// aligned) when 'base' is given.
+size_t seg_carve(char *base, int64_t m, int64_t ntiles, int64_t nheavy, int S, SegLayout *L) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *q = base ? base + off : nullptr;
+    off += (bytes + 255) / 256 * 256;
+    return q;
+  };
+  const int64_t nh = std::max<int64_t>(nheavy, 1);
+  L->rec = reinterpret_cast<int32_t *>(take(sizeof(int32_t) * std::max<int64_t>(m, 1)));
+  L->tseg = reinterpret_cast<int32_t *>(take(sizeof(int32_t) * ntiles * (S + 1)));
+  L->hps = reinterpret_cast<int64_t *>(take(sizeof(int64_t) * S * nh));
+  L->hpc = reinterpret_cast<int64_t *>(take(sizeof(int64_t) * S * (nheavy + 1)));
+  L->hcnt = reinterpret_cast<int64_t *>(take(sizeof(int64_t) * S * nh));
+  L->cbase = reinterpret_cast<int64_t *>(take(sizeof(int64_t) * (S + 1)));
+  // every piece ends at most one chunk early: m / 256 + S (nheavy + 1) bounds the chunks
+  L->crow = reinterpret_cast<int32_t *>(take(sizeof(int32_t) * ((m + kSegChunk - 1) / kSegChunk + S * (nheavy + 1))));
+  return off;
+}
 
 // A record's bits: the row's lane in its tile (6) over the local source (26).
 __device__ __forceinline__ int32_t seg_record(int lane, int32_t local) {
@@ -492,6 +517,26 @@ seg_heavy_prefix(const int64_t *hcnt, int64_t nheavy, int64_t *hpc) {
   if (threadIdx.x == 0) o[nheavy] = carry;
 }
 
+// cbase[s] = chunks of the segments before s (one thread).
+__global__ void seg_chunk_bases(const int64_t *hpc, int64_t nheavy, int S, int64_t *cbase) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int64_t at = 0;
+  for (int s = 0; s < S; s++) {
+    cbase[s] = at;
+    at += hpc[static_cast<int64_t>(s) * (nheavy + 1) + nheavy];
+  }
+  cbase[S] = at;
+}
+
+// crow[cbase[s] + c] = the heavy row of chunk c of segment s.
+__global__ void seg_chunk_rows(const int64_t *hpc, const int64_t *cbase, int64_t nheavy, int S, int32_t *crow) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<int64_t>(S) * nheavy) return;
+  const int64_t s = i / nheavy, h = i % nheavy;
+  const int64_t *pc = hpc + s * (nheavy + 1);
+  for (int64_t c = pc[h]; c < pc[h + 1]; c++) crow[cbase[s] + c] = static_cast<int32_t>(h);
+}
+
 // One row block of a sweep, segment by segment (see above).  Each wave takes
 // parts of G tiles, one per round (the rounds are the same for every wave);
 // the heavy chunks are shared out in round 0.
@@ -570,14 +615,9 @@ pr_seg_block(SegLayout L, RowTiles rt, const PrBlock *blocks, int b, int G, cons
       if (round == 0 && rt.nheavy) {
         const int64_t *pc = L.hpc + static_cast<int64_t>(s) * (rt.nheavy + 1);
         const int64_t ce = pc[blk.heavy_hi];
+        const int32_t *cr = L.crow + L.cbase[s];
         for (int64_t ci = pc[blk.heavy_lo] + gw; ci < ce; ci += nw) {
-          int64_t l = blk.heavy_lo, h = blk.heavy_hi;
-          while (l < h) {
-            const int64_t mid = (l + h) >> 1;
-            if (pc[mid] <= ci) l = mid + 1;
-            else h = mid;
-          }
-          const int64_t hh = l - 1;
+          const int64_t hh = cr[ci];
           const int64_t first = L.hps[s * rt.nheavy + hh] + (ci - pc[hh]) * kSegChunk;
           const int64_t pend = s + 1 < S ? L.hps[(s + 1) * rt.nheavy + hh] : rt.off[rt.heavy[hh] + 1];
           const int64_t last = min(first + kSegChunk, pend);
@@ -793,13 +833,18 @@ int pr_enqueue(gbk_ctx *ctx, const gbk_graph *g, const RowTiles &rt, const PrPla
   if (loop)
     if (int rc = loop->capture(iterations)) return rc;
   if (timer) timer->emplace(ctx);
-  if (p.seg) {
-    // the segmented layout of this graph, built inside the timed region
+  if (p.seg && L->build) {
+    // the graph's segmented layout, built by its first call inside the
+    // timed region and kept with the graph
     seg_records<<<grid_for(rt.ntiles, kWavesPerBlock), kBlock, 0, s>>>(rt, g->in_nbr, L->S, L->lw, L->rec, L->tseg);
     if (rt.nheavy)
       seg_heavy_pieces<<<static_cast<unsigned>(std::min<int64_t>(rt.nheavy, 4096)), kBlock, 0, s>>>(
           rt, g->in_nbr, L->S, L->lw, L->hps, L->hcnt);
     seg_heavy_prefix<<<L->S, kBlock, 0, s>>>(L->hcnt, rt.nheavy, L->hpc);
+    seg_chunk_bases<<<1, 1, 0, s>>>(L->hpc, rt.nheavy, L->S, L->cbase);
+    if (rt.nheavy)
+      seg_chunk_rows<<<grid_for(static_cast<int64_t>(L->S) * rt.nheavy), kBlock, 0, s>>>(L->hpc, L->cbase, rt.nheavy,
+                                                                                     L->S, L->crow);
     GBK_LAUNCH_CHECK(ctx, "pr segment layout");
   }
   pr_blocks_init<<<grid_for(std::max<int64_t>(rt.nheavy, p.B)), kBlock, 0, s>>>(rt, p.B, b.blocks, b.hacc);
@@ -870,14 +915,29 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   L.lw = rt.ntiles > 0 && rt.heavy_deg <= (int64_t(1) << 24) && g->max_in_degree < (int64_t(1) << 30)
              ? seg_log2(ctx, n) : 0;
   if (L.lw) {
+    // the layout lives with the graph (immutable), built once per segment
+    // width; a call with another width rebuilds it in place
     L.S = static_cast<int>((n + (int64_t(1) << L.lw) - 1) >> L.lw);
-    const int64_t S = L.S, nh = std::max<int64_t>(rt.nheavy, 1);
-    L.rec = static_cast<int32_t *>(scratch(ctx, kS9, sizeof(int32_t) * std::max<int64_t>(g->m, 1)));
-    L.tseg = static_cast<int32_t *>(scratch(ctx, kS10, sizeof(int32_t) * rt.ntiles * (S + 1)));
-    L.hps = static_cast<int64_t *>(scratch(ctx, kS11, sizeof(int64_t) * S * nh));
-    L.hpc = static_cast<int64_t *>(scratch(ctx, kS12, sizeof(int64_t) * S * (rt.nheavy + 1)));
-    L.hcnt = static_cast<int64_t *>(scratch(ctx, kS13, sizeof(int64_t) * S * nh));
-    if (!L.rec || !L.tseg || !L.hps || !L.hpc || !L.hcnt) return set_error(ctx, GBK_ENOMEM, "pr segment scratch");
+    const size_t need = seg_carve(nullptr, g->m, rt.ntiles, rt.nheavy, L.S, &L);
+    auto *mg = const_cast<gbk_graph *>(g);
+    if (mg->seg_bytes < need) {
+      if (mg->seg_mem) {
+        GBK_HIP(ctx, hipStreamSynchronize(s));
+        (void)hipFree(mg->seg_mem);
+        mg->owned.erase(std::find(mg->owned.begin(), mg->owned.end(), mg->seg_mem));
+        mg->seg_mem = nullptr;
+        mg->seg_bytes = 0;
+        mg->seg_lw = 0;
+      }
+      void *mem = nullptr;
+      if (hipMalloc(&mem, need) != hipSuccess) return set_error(ctx, GBK_ENOMEM, "pr segment layout");
+      mg->owned.push_back(mem);
+      mg->seg_mem = mem;
+      mg->seg_bytes = need;
+    }
+    seg_carve(static_cast<char *>(mg->seg_mem), g->m, rt.ntiles, rt.nheavy, L.S, &L);
+    L.build = mg->seg_lw != L.lw;
+    mg->seg_lw = 0;  // set again once the build is queued
   }
   const PrPlan p = pr_plan(ctx, rt, n, gs, true, L.lw != 0);
   PrBufs b;
@@ -894,6 +954,7 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
   LoopGraph loop(ctx);
   std::optional<EventTimer> timer;
   if (int rc = pr_enqueue(ctx, g, rt, p, b, gs, max_iters, tol, fixed_work, &loop, &timer, &L)) return rc;
+  if (L.lw) const_cast<gbk_graph *>(g)->seg_lw = L.lw;
   const double ms = timer->stop_ms();
   PrState h{};
   GBK_HIP(ctx, hipMemcpyAsync(&h, b.st, sizeof(h), hipMemcpyDeviceToHost, s));

@@ -134,3 +134,28 @@ def test_segmented_most_segments_and_knob(ctx):
     with params(ctx, pr_segment=12):
         assert ctx.get_param("pr_segment") == 12
     assert ctx.get_param("pr_segment") == 0
+
+
+def test_layout_kept_with_the_graph(ctx):
+    """The first call builds the layout and later calls reuse it: repeated
+    calls give the same scores (to 1e-6: the block sweeps fix what each row
+    reads, but heavy rows' chunk sums are added by float atomics in any
+    order), and a call with another segment width rebuilds it, after which
+    the first width's scores come back."""
+    g = O.kron(14)
+    with params(ctx, heavy_degree=64):
+        dg = ctx.upload_graph(g)
+    try:
+        with params(ctx, pr_blocks=4):
+            a, _ = _run(ctx, dg, gbk.PR_GS, 9, 6)
+            b, _ = _run(ctx, dg, gbk.PR_GS, 9, 6)
+            np.testing.assert_allclose(b, a, rtol=1e-6, atol=1e-12)
+            c, _ = _run(ctx, dg, gbk.PR_GS, 11, 6)
+            np.testing.assert_allclose(c, a, rtol=1e-5, atol=1e-11)
+            d, _ = _run(ctx, dg, gbk.PR_GS, 9, 6)
+            np.testing.assert_allclose(d, a, rtol=1e-6, atol=1e-12)
+        exp, _, _ = O.pr_spmv(g, max_iters=6, tol=0.0, fixed=True)
+        got, _ = _run(ctx, dg, gbk.PR_SPMV, 9, 6)
+        np.testing.assert_allclose(got, exp, rtol=1e-4, atol=1e-10)
+    finally:
+        dg.free()
