@@ -107,6 +107,59 @@ __device__ __forceinline__ T wave_min(T v) {
   return v;
 }
 
+// The value of x held by lane (lane ^ J), J in {1, 2, 4, 8, 16, 32}, moved
+// by the VALU instead of the LDS crossbar (ds_bpermute): DPP quad permutes
+// for 1 and 2, DPP row shifts for 4 and 8 (a lane takes the shift toward
+// its partner), gfx950's v_permlane16_swap / v_permlane32_swap for 16 and
+// 32 (with both operands x, the swapped halves hold every lane's partner).
+// Every lane of the wave must be active.
+template <int J>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x) {
+  static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32, "lane_xor stride");
+  if constexpr (J == 1) {
+    return __builtin_amdgcn_update_dpp(0u, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (J == 2) {
+    return __builtin_amdgcn_update_dpp(0u, x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (J == 4 || J == 8) {
+    const uint32_t from_up = __builtin_amdgcn_update_dpp(0u, x, 0x100 + J, 0xF, 0xF, false);  // row_shl:J
+    const uint32_t from_dn = __builtin_amdgcn_update_dpp(0u, x, 0x110 + J, 0xF, 0xF, false);  // row_shr:J
+    return (__lane_id() & J) ? from_dn : from_up;
+  } else if constexpr (J == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return (__lane_id() & 16) ? r[0] : r[1];
+  } else {
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return (__lane_id() & 32) ? r[0] : r[1];
+  }
+}
+
+template <int J, typename T>
+__device__ __forceinline__ T lane_xor_t(T x) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "lane_xor_t width");
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = static_cast<uint64_t>(x);
+    const uint32_t lo = lane_xor<J>(static_cast<uint32_t>(u));
+    const uint32_t hi = lane_xor<J>(static_cast<uint32_t>(u >> 32));
+    return static_cast<T>((static_cast<uint64_t>(hi) << 32) | lo);
+  } else {
+    return static_cast<T>(lane_xor<J>(static_cast<uint32_t>(x)));
+  }
+}
+
+// lane_xor_t for a stride known only after unrolling (j a power of two
+// below 64).
+template <typename T>
+__device__ __forceinline__ T lane_xor_rt(T x, int j) {
+  switch (j) {
+    case 1: return lane_xor_t<1>(x);
+    case 2: return lane_xor_t<2>(x);
+    case 4: return lane_xor_t<4>(x);
+    case 8: return lane_xor_t<8>(x);
+    case 16: return lane_xor_t<16>(x);
+    default: return lane_xor_t<32>(x);
+  }
+}
+
 // Inclusive scan across the wave.
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
@@ -210,7 +263,7 @@ struct gbk_ctx {
     void *p = nullptr;
     size_t bytes = 0;
   };
-  Slot slot[16];
+  Slot slot[18];
   void *pinned = nullptr;      // small pinned staging for counters
   size_t pinned_bytes = 0;
   // Per-launch timing of one selected kernel (GBK_KT_*): a start/stop event
@@ -233,7 +286,7 @@ namespace gbk {
 
 // Scratch slot ids (each kernel family documents which it uses).
 enum SlotId {
-  kS0 = 0, kS1, kS2, kS3, kS4, kS5, kS6, kS7, kS8, kS9, kS10, kS11, kS12, kS13, kS14, kS15
+  kS0 = 0, kS1, kS2, kS3, kS4, kS5, kS6, kS7, kS8, kS9, kS10, kS11, kS12, kS13, kS14, kS15, kS16, kS17
 };
 
 int set_error(gbk_ctx *ctx, int code, const std::string &msg);

@@ -10,13 +10,13 @@
 // and each bin is gathered, remapped and sorted in one fused kernel, so the
 // neighbour array is read once and the relabelled array written once:
 //   deg <= 64    a group of 8/16/32/64 lanes per row, one entry per lane,
-//                bitonic network over shuffles (registers only)
-//   deg <= 512   one wave per row, 2/4/8 entries per lane in registers,
+//                bitonic network over lane exchanges (registers only)
+//   deg <= 1024  one wave per row, 2..16 entries per lane in registers,
 //                bitonic network (in-lane strides in registers, the others
-//                over shuffles; no LDS, no barriers)
-//   deg <= 2048  one row per workgroup, bitonic sort in LDS
-//   larger       2048-element chunks sorted in LDS, then merge-path passes
-//                over a compact staging buffer, then copied into place.
+//                over lane exchanges; no LDS, no barriers)
+//   larger       an MSD bucket partition of each row on its remapped keys
+//                (buckets of a few hundred entries), then one wave sorts
+//                each bucket in registers straight into place.
 // Keys compare as (neighbour, weight), the order of NodeWeight::operator<
 // (gapbs/graph.h:136-140).
 #include <algorithm>
@@ -50,8 +50,8 @@ __global__ void relabel_invert(const int64_t *off, const int32_t *new_ids, const
 }
 
 // Degree classes: <=8, <=16, <=32, <=64 (lane-group register sorts), <=128,
-// <=256, <=512, <=1024, <=2048 (wave register sorts), larger (chunk sort +
-// merge).  Empty rows need no work.  The rows with entries are listed class
+// <=256, <=512, <=1024 (wave register sorts), class 8 unused, larger (the
+// bucket path).  Empty rows need no work.  The rows with entries are listed class
 // by class in one array, ascending within a class: a count per class and
 // tile of rows, one scan of those counts, and a scatter that ranks rows by
 // ballots (no global atomics; per-wave queues flushing into one counter line
@@ -62,31 +62,41 @@ struct RelabelLists {
   int32_t *l[kClasses];
 };
 
+// Rows above kBigRow entries take the bucket path: sorting 1025-2048
+// entries as one 32-entry-a-lane network cost 3x a bucketed row.
+constexpr int64_t kBigRow = 1024;
 __device__ __forceinline__ int bin_class(int64_t d) {
   return d == 0 ? -1 : d <= 8 ? 0 : d <= 16 ? 1 : d <= 32 ? 2 : d <= 64 ? 3 : d <= 128 ? 4
-       : d <= 256 ? 5 : d <= 512 ? 6 : d <= 1024 ? 7 : d <= kSortCap ? 8 : 9;
+       : d <= 256 ? 5 : d <= 512 ? 6 : d <= kBigRow ? 7 : kClassBig;
 }
 
 // Workgroup t counts the rows of each class in rows [t * tile, (t+1) * tile):
-// cnt[c * gridDim.x + t].
+// cnt[c * gridDim.x + t]; the entries of the big rows are summed into
+// *big_total.
 __global__ void __launch_bounds__(kBlock)
-relabel_bin_count(const int64_t *deg_new, int64_t n, int64_t tile, uint32_t *cnt) {
+relabel_bin_count(const int64_t *deg_new, int64_t n, int64_t tile, uint32_t *cnt,
+                  unsigned long long *big_total) {
   __shared__ uint32_t tot[kClasses];
   if (threadIdx.x < kClasses) tot[threadIdx.x] = 0;
   __syncthreads();
   const int64_t lo = static_cast<int64_t>(blockIdx.x) * tile;
   const int64_t hi = min(n, lo + tile);
   uint32_t mine[kClasses] = {};
+  unsigned long long big = 0;
   for (int64_t r = lo + threadIdx.x; r < hi; r += kBlock) {
-    const int cls = bin_class(deg_new[r]);
+    const int64_t d = deg_new[r];
+    const int cls = bin_class(d);
 #pragma unroll
     for (int c = 0; c < kClasses; c++) mine[c] += cls == c;
+    if (cls == kClassBig) big += static_cast<unsigned long long>(d);
   }
 #pragma unroll
   for (int c = 0; c < kClasses; c++) {
     const uint32_t w = wave_sum(mine[c]);
     if (lane_id() == 0 && w) atomicAdd(&tot[c], w);
   }
+  big = wave_sum(big);
+  if (lane_id() == 0 && big) atomicAdd(big_total, big);
   __syncthreads();
   if (threadIdx.x < kClasses) cnt[static_cast<int64_t>(threadIdx.x) * gridDim.x + blockIdx.x] = tot[threadIdx.x];
 }
@@ -162,23 +172,61 @@ relabel_bin_scatter(const int64_t *deg_new, int64_t n, int64_t tile, const uint3
   }
 }
 
-// Rows of at most G entries: G lanes hold one entry each and run a bitonic
-// network over shuffles (no LDS, no barriers).  Keys compare as (v, w).
-template <bool W, int G>
+// Bitonic network over groups of G lanes, one key a lane (lane exchanges
+// only: no LDS, no barriers).
+template <typename K, int G>
+__device__ __forceinline__ K group_net(K key, int glane) {
+#pragma unroll
+  for (int k = 2; k <= G; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const K other = sizeof(K) == 4 ? lane_xor_rt(key, j) : __shfl_xor(key, j);
+      const bool up = (glane & k) == 0;
+      const bool lower = (glane & j) == 0;
+      key = (lower == up) ? (other < key ? other : key) : (other > key ? other : key);
+    }
+  }
+  return key;
+}
+
+// Rows of at most G entries: G lanes hold one entry each.  Keys compare as
+// (v, w).  NW (weighted rows, ids below 2^(32 - log2 G)): the 32-bit
+// composite (v << log2 G) | entry index sorts and the weight follows through
+// the index; a group that finds one v twice lists its row in 'redo' for the
+// 64-bit sort instead of writing it.
+template <bool W, int G, bool NW = false>
 __global__ void __launch_bounds__(kBlock)
 relabel_group_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const int64_t *off,
                    const int32_t *nbr, const int32_t *wgt, const int32_t *new_ids,
-                   const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out) {
+                   const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out, int32_t *redo = nullptr,
+                   unsigned long long *redo_count = nullptr) {
   using K = typename std::conditional<W, uint64_t, uint32_t>::type;
   const int glane = threadIdx.x % G;
   const int64_t ri = static_cast<int64_t>(blockIdx.x) * (kBlock / G) + threadIdx.x / G;
   int64_t len = 0, src = 0, dst = 0;
+  int32_t r = 0;
   if (ri < nrows) {
-    const int32_t r = rows[ri];
+    r = rows[ri];
     const int32_t u = inv[r];
     src = off[u];
     len = off[u + 1] - src;
     dst = new_off[r];
+  }
+  if constexpr (W && NW) {
+    constexpr int IB = G == 8 ? 3 : G == 16 ? 4 : G == 32 ? 5 : 6;
+    uint32_t key = ~0u;
+    if (glane < len) key = (static_cast<uint32_t>(new_ids[nbr[src + glane]]) << IB) | static_cast<uint32_t>(glane);
+    key = group_net<uint32_t, G>(key, glane);
+    const uint32_t succ = __shfl(key, (lane_id() + 1) & (kWave - 1));
+    const bool dup = glane + 1 < len && (succ >> IB) == (key >> IB);
+    const uint64_t gmask = G == kWave ? ~0ull : ((1ull << G) - 1) << (lane_id() / G * G);
+    if (__ballot(dup) & gmask) {
+      if (glane == 0) redo[atomicAdd(redo_count, 1ull)] = r;
+    } else if (glane < len) {
+      nbr_out[dst + glane] = static_cast<int32_t>(key >> IB);
+      wgt_out[dst + glane] = wgt[src + (key & (G - 1))];
+    }
+    return;
   }
   K key = static_cast<K>(~static_cast<K>(0));
   if (glane < len) {
@@ -189,16 +237,7 @@ relabel_group_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const
     else
       key = static_cast<uint32_t>(v);
   }
-#pragma unroll
-  for (int k = 2; k <= G; k <<= 1) {
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      const K other = __shfl_xor(key, j);
-      const bool up = (glane & k) == 0;
-      const bool lower = (glane & j) == 0;
-      key = (lower == up) ? (other < key ? other : key) : (other > key ? other : key);
-    }
-  }
+  key = group_net<K, G>(key, glane);
   if (glane < len) {
     if constexpr (W) {
       nbr_out[dst + glane] = static_cast<int32_t>(key >> 32);
@@ -228,17 +267,13 @@ struct RowKey {
 
 // One wave sorts up to 64*E keys: load(i) gives key i (i < len), store(i,
 // key) takes the i-th smallest.  Padding keys are all ones.
-template <bool W, int E, class Load, class Store>
-__device__ __forceinline__ void wave_sort(int64_t len, Load load, Store store) {
-  using K = typename RowKey<W>::T;
+// Bitonic network over 64*E keys held by one wave: entry i in lane i % 64,
+// register i / 64.  Strides below 64 pair lanes, larger strides pair
+// registers of one lane.
+template <typename K, int E>
+__device__ __forceinline__ void sort_regs(K (&key)[E]) {
   constexpr int N = kWave * E;
   const int lane = lane_id();
-  K key[E];
-#pragma unroll
-  for (int e = 0; e < E; e++) {
-    const int64_t i = static_cast<int64_t>(e) * kWave + lane;
-    key[e] = i < len ? load(i) : static_cast<K>(~static_cast<K>(0));
-  }
 #pragma unroll
   for (int k = 2; k <= N; k <<= 1) {
 #pragma unroll
@@ -246,7 +281,10 @@ __device__ __forceinline__ void wave_sort(int64_t len, Load load, Store store) {
       if (j < kWave) {
 #pragma unroll
         for (int e = 0; e < E; e++) {
-          const K other = __shfl_xor(key[e], j);
+          // 32-bit keys up to 16 a lane exchange through DPP / permlane
+          // swaps (VALU); wider rows and 64-bit keys through ds_bpermute,
+          // which needs fewer registers (measured faster there)
+          const K other = sizeof(K) == 4 && E <= 16 ? lane_xor_rt(key[e], j) : __shfl_xor(key[e], j);
           const int i = e * kWave + lane;
           const bool up = (i & k) == 0;
           const bool lower = (i & j) == 0;
@@ -270,6 +308,21 @@ __device__ __forceinline__ void wave_sort(int64_t len, Load load, Store store) {
       }
     }
   }
+}
+
+// One wave sorts up to 64*E keys: load(i) gives key i (i < len), store(i,
+// key) takes the i-th smallest.  Padding keys are all ones.
+template <bool W, int E, class Load, class Store>
+__device__ __forceinline__ void wave_sort(int64_t len, Load load, Store store) {
+  using K = typename RowKey<W>::T;
+  const int lane = lane_id();
+  K key[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int64_t i = static_cast<int64_t>(e) * kWave + lane;
+    key[e] = i < len ? load(i) : static_cast<K>(~static_cast<K>(0));
+  }
+  sort_regs<K, E>(key);
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int64_t i = static_cast<int64_t>(e) * kWave + lane;
@@ -277,11 +330,49 @@ __device__ __forceinline__ void wave_sort(int64_t len, Load load, Store store) {
   }
 }
 
-template <bool W, int E>
+// Weighted rows with narrow ids: a row's keys (v, w) sort as the 32-bit
+// composite (v << IB) | i, i the entry's index in its row (IB bits, so
+// nb + IB <= 32 for ids below 2^nb), and the weight is fetched through i
+// afterwards.  This is the (v, w) order whenever the row's v are distinct,
+// which squished graphs guarantee; a row holding one v twice is detected
+// after the sort (its sorted neighbour has the same v) and listed for a
+// 64-bit redo instead of being written.  Returns true when the wave wrote
+// the row.
+template <int E, class LoadV, class LoadW, class Store>
+__device__ __forceinline__ bool wave_sort_narrow(int64_t len, LoadV load_v, LoadW load_w, Store store) {
+  constexpr int IB = 6 + (E == 1 ? 0 : E == 2 ? 1 : E == 4 ? 2 : E == 8 ? 3 : E == 16 ? 4 : 5);
+  const int lane = lane_id();
+  uint32_t key[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = e * kWave + lane;
+    key[e] = i < len ? (static_cast<uint32_t>(load_v(i)) << IB) | static_cast<uint32_t>(i) : ~0u;
+  }
+  sort_regs<uint32_t, E>(key);
+  bool dup = false;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint32_t down = __shfl(key[e], (lane + 1) & (kWave - 1));
+    const uint32_t next_first = e + 1 < E ? __shfl(key[e + 1 < E ? e + 1 : e], 0) : ~0u;
+    const uint32_t succ = lane == kWave - 1 ? next_first : down;
+    const int i = e * kWave + lane;
+    dup |= i + 1 < len && (succ >> IB) == (key[e] >> IB);
+  }
+  if (__ballot(dup)) return false;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = e * kWave + lane;
+    if (i < len) store(i, static_cast<int32_t>(key[e] >> IB), load_w(static_cast<int>(key[e] & ((1u << IB) - 1))));
+  }
+  return true;
+}
+
+template <bool W, int E, bool NW = false>
 __global__ void __launch_bounds__(kBlock)
 relabel_wave_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const int64_t *off,
                   const int32_t *nbr, const int32_t *wgt, const int32_t *new_ids,
-                  const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out) {
+                  const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out, int32_t *redo = nullptr,
+                  unsigned long long *redo_count = nullptr) {
   using K = typename RowKey<W>::T;
   const int64_t ri = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
   if (ri >= nrows) return;  // wave-uniform
@@ -290,6 +381,16 @@ relabel_wave_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const 
   const int64_t src = off[u];
   const int64_t len = off[u + 1] - src;
   const int64_t dst = new_off[r];
+  if constexpr (W && NW) {
+    const bool wrote = wave_sort_narrow<E>(
+        len, [&](int i) { return new_ids[nbr[src + i]]; }, [&](int i) { return wgt[src + i]; },
+        [&](int i, int32_t v, int32_t w) {
+          nbr_out[dst + i] = v;
+          wgt_out[dst + i] = w;
+        });
+    if (!wrote && lane_id() == 0) redo[atomicAdd(redo_count, 1ull)] = r;
+    return;
+  }
   wave_sort<W, E>(
       len, [&](int64_t i) { return RowKey<W>::make(new_ids[nbr[src + i]], W ? wgt[src + i] : 0); },
       [&](int64_t i, K key) {
@@ -302,48 +403,28 @@ relabel_wave_sort(const int32_t *rows, int64_t nrows, const int32_t *inv, const 
       });
 }
 
-// Big rows: composite key so a plain integer compare gives (v, w) order.
-template <bool W>
-struct BigKey {
-  using T = typename std::conditional<W, uint64_t, uint32_t>::type;
-  __device__ static T make(int32_t v, int32_t w) {
-    if constexpr (W)
-      return (static_cast<uint64_t>(static_cast<uint32_t>(v)) << 32) |
-             (static_cast<uint32_t>(w) ^ 0x80000000u);
-    else
-      return static_cast<uint32_t>(v);
-  }
-  __device__ static int32_t v(T k) {
-    if constexpr (W) return static_cast<int32_t>(k >> 32);
-    else return static_cast<int32_t>(k);
-  }
-  __device__ static int32_t w(T k) {
-    if constexpr (W) return static_cast<int32_t>(static_cast<uint32_t>(k) ^ 0x80000000u);
-    else return 1;
-  }
-};
-
-// Chunk sort: chunk c covers elements [cstart, cstart+len) of big row 'row'
-// (staging coordinates: row i of the big list starts at stage_off[i]); one
-// wave sorts a chunk of up to kSortCap keys in registers.
-template <bool W>
+// Rows a narrow sort listed (one neighbour twice): sorted again on the full
+// (v, w) keys, one wave each, grid-stride over the list (rows of at most
+// 1024 entries).
 __global__ void __launch_bounds__(kBlock)
-relabel_big_chunks(const int32_t *big_rows, const int64_t *stage_off, const int32_t *chunk_row,
-                   const int64_t *chunk_start, int64_t nchunks, const int32_t *inv, const int64_t *off,
-                   const int32_t *nbr, const int32_t *wgt, const int32_t *new_ids,
-                   typename BigKey<W>::T *stage) {
-  using K = typename BigKey<W>::T;
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  if (c >= nchunks) return;  // wave-uniform
-  const int bi = chunk_row[c];
-  const int64_t cs = chunk_start[c];
-  const int32_t u = inv[big_rows[bi]];
-  const int64_t first = off[u] + cs;
-  const int64_t len = min(static_cast<int64_t>(kSortCap), off[u + 1] - first);
-  K *out = stage + stage_off[bi] + cs;
-  wave_sort<W, kSortCap / kWave>(
-      len, [&](int64_t i) { return BigKey<W>::make(new_ids[nbr[first + i]], W ? wgt[first + i] : 1); },
-      [&](int64_t i, K key) { out[i] = key; });
+relabel_redo_rows(const int32_t *redo, const unsigned long long *count, const int32_t *inv, const int64_t *off,
+                  const int32_t *nbr, const int32_t *wgt, const int32_t *new_ids, const int64_t *new_off,
+                  int32_t *nbr_out, int32_t *wgt_out) {
+  const int64_t nr = static_cast<int64_t>(*count);
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t s = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave; s < nr; s += waves) {
+    const int32_t r = redo[s];
+    const int32_t u = inv[r];
+    const int64_t src = off[u];
+    const int64_t len = off[u + 1] - src;
+    const int64_t dst = new_off[r];
+    wave_sort<true, 16>(
+        len, [&](int64_t i) { return RowKey<true>::make(new_ids[nbr[src + i]], wgt[src + i]); },
+        [&](int64_t i, uint64_t key) {
+          nbr_out[dst + i] = static_cast<int32_t>(key >> 32);
+          wgt_out[dst + i] = static_cast<int32_t>(static_cast<uint32_t>(key) ^ 0x80000000u);
+        });
+  }
 }
 
 // One merge pass of run width 'width': element t of the staging layout is
@@ -399,35 +480,549 @@ relabel_merge_pass(const K *src, K *dst, const int64_t *stage_off, int nbig, int
   }
 }
 
+// ------------------------------------------------------------- big rows
+// Rows above kBigRow entries: an MSD bucket partition on the remapped keys,
+// then one wave per bucket sorts it in registers straight into place.
+//   level 1  workgroups of kBigChunk entries gather the keys once into a
+//            staging buffer and count them per (row, bucket): a row of d
+//            entries gets 2^b buckets on the top b key bits, b chosen so a
+//            bucket holds about kSegTarget entries; a device scan of the
+//            counts gives each bucket its staging range, a second pass over
+//            the staged keys scatters them there (order inside a bucket is
+//            whatever the LDS atomics give: the bucket is sorted next)
+//   level k  buckets still above kSortCap entries (keys crowded in a narrow
+//            range, e.g. hub rows under a degree order) are split again on
+//            their next key bits, one workgroup each, ping-ponging between
+//            the two staging buffers; a bucket whose key bits are used up
+//            holds one key repeated and is copied as it stands
+//   sort     every bucket of at most kSortCap entries is sorted by one wave
+//            (4/8/16/32 entries a lane by its size) into the output row.
+// No host work per row and no merge passes: each entry is gathered once,
+// written to staging twice and to the output once.
+constexpr int kSegTarget = 384;                     // bucket size a level aims at
+constexpr int kSegMaxBits = 12;                     // at most 4096 buckets a segment and level
+constexpr int kSegMaxBuckets = 1 << kSegMaxBits;
+constexpr int kBigChunk = 4096;                     // entries a level-1 workgroup stages
+constexpr int kChunkPer = kBigChunk / kBlock;       // entries a lane of it holds
+constexpr int kSegLists = 6;                        // sort classes 0..3, copy, overflow flag
+constexpr int kSegCopy = 4;
+
+// A bucket still to be partitioned or sorted: staging position and length;
+// hi_buf = (key bits not yet partitioned) << 1 | (staging buffer holding it).
+struct SegRec {
+  int64_t pos;
+  int32_t len;
+  int32_t hi_buf;
+};
+
 template <bool W>
-__global__ void relabel_big_scatter(const typename BigKey<W>::T *stage0, const typename BigKey<W>::T *stage1,
-                                    const int64_t *stage_off, const int32_t *big_rows, int nbig,
-                                    int64_t total, const int64_t *new_off, int32_t *nbr_out,
-                                    int32_t *wgt_out) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
-  int lo = 0, hi = nbig;
+__device__ __forceinline__ int32_t key_v(typename RowKey<W>::T k) {
+  if constexpr (W) return static_cast<int32_t>(k >> 32);
+  else return static_cast<int32_t>(k);
+}
+template <bool W>
+__device__ __forceinline__ int32_t key_w(typename RowKey<W>::T k) {
+  if constexpr (W) return static_cast<int32_t>(static_cast<uint32_t>(k) ^ 0x80000000u);
+  else return 0;
+}
+
+// Buckets of a segment of 'len' entries with 'hi' key bits left: about
+// kSegTarget entries each, at least 2, at most 2^kSegMaxBits, at most 2^hi.
+__device__ __forceinline__ int seg_bits(int64_t len, int hi) {
+  const int64_t want = (len + kSegTarget - 1) / kSegTarget;
+  int b = want > 1 ? 64 - __clzll(static_cast<unsigned long long>(want - 1)) : 1;
+  b = max(b, 1);
+  b = min(b, kSegMaxBits);
+  return min(b, hi);
+}
+
+// Largest i in [0, count) with a[i] <= x (a ascending, a[0] <= x).
+__device__ __forceinline__ int64_t find_le(const int64_t *a, int64_t count, int64_t x) {
+  int64_t lo = 0, hi = count;
   while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (stage_off[mid] <= t) lo = mid;
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= x) lo = mid;
     else hi = mid;
   }
-  // a row of length L took one merge pass per width kSortCap * 2^j below L,
-  // ping-ponging from stage0, so its sorted data ends in the buffer of that
-  // count's parity
-  const int64_t len = stage_off[lo + 1] - stage_off[lo];
-  int passes = 0;
-  for (int64_t w = kSortCap; w < len; w *= 2) passes++;
-  const typename BigKey<W>::T key = (passes & 1) ? stage1[t] : stage0[t];
-  const int64_t dst = new_off[big_rows[lo]] + (t - stage_off[lo]);
-  nbr_out[dst] = BigKey<W>::v(key);
-  if (W) wgt_out[dst] = BigKey<W>::w(key);
+  return lo;
+}
+
+// One workgroup of 1024 lanes: for the big rows (new ids big[0..nbig)) the
+// exclusive prefixes of their lengths (staging offsets), level-1 bucket
+// counts and kBigChunk chunk counts; entry [nbig] holds each total.
+__global__ void __launch_bounds__(1024)
+big_layout(const int32_t *big, int64_t nbig, const int64_t *deg_new, int key_bits, int64_t *stage_off,
+           int64_t *bucket_off, int64_t *chunk_off) {
+  __shared__ int64_t wsum[3][1024 / kWave];
+  const int t = threadIdx.x;
+  const int64_t per = (nbig + 1023) / 1024;
+  const int64_t b = min(nbig, t * per), e = min(nbig, b + per);
+  int64_t s0 = 0, s1 = 0, s2 = 0;
+  for (int64_t i = b; i < e; i++) {
+    const int64_t d = deg_new[big[i]];
+    s0 += d;
+    s1 += int64_t{1} << seg_bits(d, key_bits);
+    s2 += (d + kBigChunk - 1) / kBigChunk;
+  }
+  const int64_t i0 = wave_incl_scan(s0), i1 = wave_incl_scan(s1), i2 = wave_incl_scan(s2);
+  const int w = t / kWave;
+  if (lane_id() == kWave - 1) {
+    wsum[0][w] = i0;
+    wsum[1][w] = i1;
+    wsum[2][w] = i2;
+  }
+  __syncthreads();
+  if (w == 0) {
+    for (int q = 0; q < 3; q++) {
+      int64_t x = lane_id() < 1024 / kWave ? wsum[q][lane_id()] : 0;
+      x = wave_incl_scan(x);
+      if (lane_id() < 1024 / kWave) wsum[q][lane_id()] = x;
+    }
+  }
+  __syncthreads();
+  int64_t r0 = i0 - s0 + (w > 0 ? wsum[0][w - 1] : 0);
+  int64_t r1 = i1 - s1 + (w > 0 ? wsum[1][w - 1] : 0);
+  int64_t r2 = i2 - s2 + (w > 0 ? wsum[2][w - 1] : 0);
+  for (int64_t i = b; i < e; i++) {
+    stage_off[i] = r0;
+    bucket_off[i] = r1;
+    chunk_off[i] = r2;
+    const int64_t d = deg_new[big[i]];
+    r0 += d;
+    r1 += int64_t{1} << seg_bits(d, key_bits);
+    r2 += (d + kBigChunk - 1) / kBigChunk;
+  }
+  if (t == 1023) {
+    stage_off[nbig] = r0;
+    bucket_off[nbig] = r1;
+    chunk_off[nbig] = r2;
   }
 }
 
-__global__ void gather_i64(const int64_t *src, const int32_t *idx, int64_t n, int64_t *out) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = src[idx[i]];
+// Level 1, pass 1: workgroup c stages chunk c of its big row (keys gathered
+// through new_ids) and adds its per-bucket counts to cnt.
+template <bool W>
+__global__ void __launch_bounds__(kBlock)
+big_count(const int32_t *big, int64_t nbig, const int64_t *stage_off, const int64_t *bucket_off,
+          const int64_t *chunk_off, const int32_t *inv, const int64_t *off, const int32_t *nbr,
+          const int32_t *wgt, const int32_t *new_ids, int key_bits, typename RowKey<W>::T *stage,
+          unsigned long long *cnt) {
+  using K = typename RowKey<W>::T;
+  __shared__ uint32_t hist[kSegMaxBuckets];
+  const int64_t c = blockIdx.x;
+  if (c >= chunk_off[nbig]) return;  // workgroup-uniform
+  const int64_t i = find_le(chunk_off, nbig, c);
+  const int32_t u = inv[big[i]];
+  const int64_t d = stage_off[i + 1] - stage_off[i];
+  const int b = seg_bits(d, key_bits);
+  const int shift = key_bits - b;
+  for (int k = threadIdx.x; k < (1 << b); k += kBlock) hist[k] = 0;
+  __syncthreads();
+  const int64_t cs = (c - chunk_off[i]) * kBigChunk;
+  const int len = static_cast<int>(min<int64_t>(kBigChunk, d - cs));
+  const int64_t src = off[u] + cs;
+  K *out = stage + stage_off[i] + cs;
+  int32_t v[kChunkPer], w[kChunkPer];
+#pragma unroll
+  for (int e = 0; e < kChunkPer; e++) {
+    const int j = e * kBlock + threadIdx.x;
+    v[e] = j < len ? nbr[src + j] : 0;
+    w[e] = W && j < len ? wgt[src + j] : 0;
+  }
+#pragma unroll
+  for (int e = 0; e < kChunkPer; e++) {
+    const int j = e * kBlock + threadIdx.x;
+    if (j < len) v[e] = new_ids[v[e]];
+  }
+#pragma unroll
+  for (int e = 0; e < kChunkPer; e++) {
+    const int j = e * kBlock + threadIdx.x;
+    if (j < len) {
+      const K key = RowKey<W>::make(v[e], w[e]);
+      out[j] = key;
+      atomicAdd(&hist[static_cast<uint32_t>(key >> shift)], 1u);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < (1 << b); k += kBlock)
+    if (hist[k]) atomicAdd(&cnt[bucket_off[i] + k], static_cast<unsigned long long>(hist[k]));
+}
+
+// Level 1, pass 2: workgroup c moves its staged chunk into the buckets; cur
+// holds each bucket's next free staging slot (starts as the scan of cnt).
+template <bool W>
+__global__ void __launch_bounds__(kBlock)
+big_scatter(const int64_t *stage_off, const int64_t *bucket_off, const int64_t *chunk_off, int64_t nbig,
+            int key_bits, const typename RowKey<W>::T *stage, typename RowKey<W>::T *stage2,
+            unsigned long long *cur) {
+  using K = typename RowKey<W>::T;
+  __shared__ uint32_t hist[kSegMaxBuckets];
+  __shared__ unsigned long long base[kSegMaxBuckets];
+  const int64_t c = blockIdx.x;
+  if (c >= chunk_off[nbig]) return;  // workgroup-uniform
+  const int64_t i = find_le(chunk_off, nbig, c);
+  const int64_t d = stage_off[i + 1] - stage_off[i];
+  const int b = seg_bits(d, key_bits);
+  const int shift = key_bits - b;
+  for (int k = threadIdx.x; k < (1 << b); k += kBlock) hist[k] = 0;
+  __syncthreads();
+  const int64_t cs = (c - chunk_off[i]) * kBigChunk;
+  const int len = static_cast<int>(min<int64_t>(kBigChunk, d - cs));
+  const K *in = stage + stage_off[i] + cs;
+  K key[kChunkPer];
+  uint32_t rank[kChunkPer];
+#pragma unroll
+  for (int e = 0; e < kChunkPer; e++) {
+    const int j = e * kBlock + threadIdx.x;
+    key[e] = j < len ? in[j] : K{0};
+  }
+#pragma unroll
+  for (int e = 0; e < kChunkPer; e++) {
+    const int j = e * kBlock + threadIdx.x;
+    rank[e] = j < len ? atomicAdd(&hist[static_cast<uint32_t>(key[e] >> shift)], 1u) : 0u;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < (1 << b); k += kBlock)
+    if (hist[k]) base[k] = atomicAdd(&cur[bucket_off[i] + k], static_cast<unsigned long long>(hist[k]));
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < kChunkPer; e++) {
+    const int j = e * kBlock + threadIdx.x;
+    if (j < len) stage2[base[static_cast<uint32_t>(key[e] >> shift)] + rank[e]] = key[e];
+  }
+}
+
+// Appends 'rec' to the list its length selects: sort classes of at most
+// 256/512/1024/2048 entries, the copy list (key bits used up), or the next
+// level's list; lists hold 'cap' records, an append past that sets
+// counters[kSegLists - 1].  One atomic per wave and list.
+__device__ __forceinline__ void seg_emit(bool valid, const SegRec &rec, SegRec *lists, int64_t cap,
+                                         unsigned long long *counters, SegRec *next, int64_t next_cap,
+                                         unsigned long long *next_count) {
+  int cls = -1;
+  if (valid) {
+    const int hi = rec.hi_buf >> 1;
+    cls = rec.len <= 256 ? 0 : rec.len <= 512 ? 1 : rec.len <= 1024 ? 2 : rec.len <= kSortCap ? 3
+        : hi == 0 ? kSegCopy : kSegLists;  // kSegLists: the next level
+  }
+  for (int c = 0; c <= kSegLists; c++) {
+    if (c == kSegLists - 1) continue;
+    unsigned long long *ctr = c == kSegLists ? next_count : &counters[c];
+    const int64_t slot = wave_append(ctr, cls == c);
+    if (cls == c) {
+      if (slot < (c == kSegLists ? next_cap : cap)) (c == kSegLists ? next : lists + c * cap)[slot] = rec;
+      else atomicOr(&counters[kSegLists - 1], 1ull);
+    }
+  }
+}
+
+// Level 1, pass 3: bucket k's record (its staging range in buffer 1).
+__global__ void big_segments(const int64_t *bucket_off, const int64_t *stage_off, int64_t nbig, int key_bits,
+                             const unsigned long long *cnt, const unsigned long long *cur, SegRec *lists,
+                             int64_t cap, unsigned long long *counters, SegRec *next, int64_t next_cap,
+                             unsigned long long *next_count) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nb = bucket_off[nbig];
+  SegRec rec{};
+  bool valid = false;
+  if (k < nb) {
+    const int64_t i = find_le(bucket_off, nbig, k);
+    const int64_t len = static_cast<int64_t>(cnt[k]);
+    if (len > 0) {
+      const int b = seg_bits(stage_off[i + 1] - stage_off[i], key_bits);
+      rec.pos = static_cast<int64_t>(cur[k]) - len;
+      rec.len = static_cast<int32_t>(len);
+      rec.hi_buf = ((key_bits - b) << 1) | 1;
+      valid = true;
+    }
+  }
+  seg_emit(valid, rec, lists, cap, counters, next, next_cap, next_count);
+}
+
+// Level >= 2: workgroup s splits segment in[s] on its next key bits into the
+// other staging buffer (LDS histogram, scan and cursors) and emits the parts.
+template <bool W>
+__global__ void __launch_bounds__(1024)
+seg_split(const SegRec *in, typename RowKey<W>::T *buf0, typename RowKey<W>::T *buf1, SegRec *lists,
+          int64_t cap, unsigned long long *counters, SegRec *next, int64_t next_cap,
+          unsigned long long *next_count) {
+  using K = typename RowKey<W>::T;
+  __shared__ uint32_t hist[kSegMaxBuckets];
+  __shared__ uint32_t start[kSegMaxBuckets];
+  __shared__ uint32_t wtot[1024 / kWave];
+  __shared__ K wlo[1024 / kWave], whi[1024 / kWave];
+  const SegRec rec = in[blockIdx.x];
+  const K *src = (rec.hi_buf & 1) ? buf1 : buf0;
+  K *dst = (rec.hi_buf & 1) ? buf0 : buf1;
+  // the keys agree on every bit above the highest bit where their minimum
+  // and maximum differ: partition just below it
+  K lo = static_cast<K>(~K{0}), hi_key = 0;
+  for (int64_t j = threadIdx.x; j < rec.len; j += 1024) {
+    const K key = src[rec.pos + j];
+    lo = key < lo ? key : lo;
+    hi_key = key > hi_key ? key : hi_key;
+  }
+  lo = wave_min(lo);
+  hi_key = wave_max(hi_key);
+  if (lane_id() == 0) {
+    wlo[threadIdx.x / kWave] = lo;
+    whi[threadIdx.x / kWave] = hi_key;
+  }
+  for (int k = threadIdx.x; k < kSegMaxBuckets; k += 1024) hist[k] = 0;
+  __syncthreads();
+  for (int w = 0; w < 1024 / kWave; w++) {
+    lo = wlo[w] < lo ? wlo[w] : lo;
+    hi_key = whi[w] > hi_key ? whi[w] : hi_key;
+  }
+  const K diff = lo ^ hi_key;
+  int hi = 0;
+  if constexpr (W) hi = diff == 0 ? 0 : 64 - __clzll(diff);
+  else hi = diff == 0 ? 0 : 32 - __clz(diff);
+  if (hi == 0) {  // one key repeated: copied as it stands (workgroup-uniform)
+    SegRec same = rec;
+    same.hi_buf = rec.hi_buf & 1;
+    if (threadIdx.x < kWave) seg_emit(threadIdx.x == 0, same, lists, cap, counters, next, next_cap, next_count);
+    return;
+  }
+  const int b = seg_bits(rec.len, hi);
+  const int shift = hi - b;
+  const uint32_t mask = (1u << b) - 1;
+  for (int64_t j = threadIdx.x; j < rec.len; j += 1024)
+    atomicAdd(&hist[static_cast<uint32_t>(src[rec.pos + j] >> shift) & mask], 1u);
+  __syncthreads();
+  // exclusive scan of the 4096 counts: 4 per lane
+  uint32_t c4[4], s = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    c4[q] = hist[threadIdx.x * 4 + q];
+    s += c4[q];
+  }
+  const uint32_t incl = wave_incl_scan(s);
+  if (lane_id() == kWave - 1) wtot[threadIdx.x / kWave] = incl;
+  __syncthreads();
+  uint32_t run = incl - s;
+  for (int w = 0; w < static_cast<int>(threadIdx.x / kWave); w++) run += wtot[w];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    start[threadIdx.x * 4 + q] = run;
+    run += c4[q];
+  }
+  __syncthreads();
+  // cursors: hist becomes each bucket's next slot
+#pragma unroll
+  for (int q = 0; q < 4; q++) hist[threadIdx.x * 4 + q] = start[threadIdx.x * 4 + q];
+  __syncthreads();
+  for (int64_t j = threadIdx.x; j < rec.len; j += 1024) {
+    const K key = src[rec.pos + j];
+    const uint32_t p = atomicAdd(&hist[static_cast<uint32_t>(key >> shift) & mask], 1u);
+    dst[rec.pos + p] = key;
+  }
+  __syncthreads();
+  for (int k0 = 0; k0 < kSegMaxBuckets; k0 += 1024) {
+    const int k = k0 + threadIdx.x;
+    const uint32_t len = k < (1 << b) ? hist[k] - start[k] : 0;
+    SegRec part{};
+    part.pos = rec.pos + start[k];
+    part.len = static_cast<int32_t>(len);
+    part.hi_buf = (shift << 1) | ((rec.hi_buf & 1) ^ 1);
+    seg_emit(len > 0, part, lists, cap, counters, next, next_cap, next_count);
+  }
+}
+
+// Where staging position 'pos' lands in the output: its big row's offset
+// plus its place in the row.
+__device__ __forceinline__ int64_t seg_dst(int64_t pos, const int64_t *stage_off, int64_t nbig,
+                                           const int32_t *big, const int64_t *new_off) {
+  const int64_t i = find_le(stage_off, nbig, pos);
+  return new_off[big[i]] + (pos - stage_off[i]);
+}
+
+// Sorts the buckets of one class (at most 64*E entries) one wave each into
+// the output rows; grid-stride over the class's records.  NW: weighted
+// buckets sort as 32-bit composites (see wave_sort_narrow); a bucket with one
+// v twice goes to 'redo' instead.
+template <bool W, int E, bool NW = false>
+__global__ void __launch_bounds__(kBlock)
+seg_sort(const SegRec *list, const unsigned long long *count, int64_t cap, const typename RowKey<W>::T *buf0,
+         const typename RowKey<W>::T *buf1, const int64_t *stage_off, int64_t nbig, const int32_t *big,
+         const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out, SegRec *redo = nullptr,
+         unsigned long long *redo_count = nullptr) {
+  using K = typename RowKey<W>::T;
+  const int64_t nseg = min(static_cast<int64_t>(*count), cap);
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t s = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave; s < nseg; s += waves) {
+    const SegRec rec = list[s];
+    const K *src = ((rec.hi_buf & 1) ? buf1 : buf0) + rec.pos;
+    const int64_t dst = seg_dst(rec.pos, stage_off, nbig, big, new_off);
+    if constexpr (W && NW) {
+      const bool wrote = wave_sort_narrow<E>(
+          rec.len, [&](int i) { return key_v<W>(src[i]); }, [&](int i) { return key_w<W>(src[i]); },
+          [&](int i, int32_t v, int32_t w) {
+            nbr_out[dst + i] = v;
+            wgt_out[dst + i] = w;
+          });
+      if (!wrote && lane_id() == 0) redo[atomicAdd(redo_count, 1ull)] = rec;
+    } else {
+      wave_sort<W, E>(
+          rec.len, [&](int64_t i) { return src[i]; },
+          [&](int64_t i, K key) {
+            nbr_out[dst + i] = key_v<W>(key);
+            if constexpr (W) wgt_out[dst + i] = key_w<W>(key);
+          });
+    }
+  }
+}
+
+// Weighted buckets a narrow sort listed (one v twice): sorted on the full
+// (v, w) keys (buckets of at most kSortCap entries).
+__global__ void __launch_bounds__(kBlock)
+seg_redo(const SegRec *list, const unsigned long long *count, const uint64_t *buf0, const uint64_t *buf1,
+         const int64_t *stage_off, int64_t nbig, const int32_t *big, const int64_t *new_off, int32_t *nbr_out,
+         int32_t *wgt_out) {
+  const int64_t nseg = static_cast<int64_t>(*count);
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t s = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave; s < nseg; s += waves) {
+    const SegRec rec = list[s];
+    const uint64_t *src = ((rec.hi_buf & 1) ? buf1 : buf0) + rec.pos;
+    const int64_t dst = seg_dst(rec.pos, stage_off, nbig, big, new_off);
+    wave_sort<true, kSortCap / kWave>(
+        rec.len, [&](int64_t i) { return src[i]; },
+        [&](int64_t i, uint64_t key) {
+          nbr_out[dst + i] = key_v<true>(key);
+          wgt_out[dst + i] = key_w<true>(key);
+        });
+  }
+}
+
+// Buckets whose key bits are used up hold one key repeated: copied as they are.
+template <bool W>
+__global__ void __launch_bounds__(kBlock)
+seg_copy(const SegRec *list, const unsigned long long *count, int64_t cap, const typename RowKey<W>::T *buf0,
+         const typename RowKey<W>::T *buf1, const int64_t *stage_off, int64_t nbig, const int32_t *big,
+         const int64_t *new_off, int32_t *nbr_out, int32_t *wgt_out) {
+  const int64_t nseg = min(static_cast<int64_t>(*count), cap);
+  for (int64_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+    const SegRec rec = list[s];
+    const auto *src = ((rec.hi_buf & 1) ? buf1 : buf0) + rec.pos;
+    const int64_t dst = seg_dst(rec.pos, stage_off, nbig, big, new_off);
+    for (int64_t i = threadIdx.x; i < rec.len; i += kBlock) {
+      nbr_out[dst + i] = key_v<W>(src[i]);
+      if constexpr (W) wgt_out[dst + i] = key_w<W>(src[i]);
+    }
+  }
+}
+
+// Big rows (new ids big[0..nbig), 'total' entries): the bucket partition
+// above.  The host reads the list counters once after each level (usually
+// one level); a list that would overflow (keys crowded far beyond what the
+// capacity allows) restarts the path with four times the capacity.
+template <bool W>
+static int relabel_big(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_t *nbr, const int32_t *wgt,
+                       const int32_t *new_ids, const int32_t *inv, const int64_t *deg_new,
+                       const int64_t *new_off, const int32_t *big, int64_t nbig, int64_t total,
+                       int32_t *nbr_out, int32_t *wgt_out) {
+  using K = typename RowKey<W>::T;
+  hipStream_t s = ctx->stream;
+  int nb = 1;
+  while (nb < 31 && (int64_t{1} << nb) < n) nb++;  // every id is below 2^nb
+  const int key_bits = nb + (W ? 32 : 0);
+  const int64_t chunk_bound = total / kBigChunk + nbig + 1;
+  const int64_t bucket_bound = 2 * (total / kSegTarget) + 2 * nbig + 1;
+  const int64_t next_cap = total / kSortCap + 16;  // segments above kSortCap entries are disjoint
+  auto *lay = static_cast<int64_t *>(scratch(ctx, kS9, sizeof(int64_t) * 3 * (nbig + 1)));
+  auto *buf0 = static_cast<K *>(scratch(ctx, kS10, sizeof(K) * total));
+  auto *buf1 = static_cast<K *>(scratch(ctx, kS11, sizeof(K) * total));
+  auto *cnt = static_cast<unsigned long long *>(scratch(ctx, kS6, sizeof(int64_t) * (bucket_bound + 1)));
+  auto *cur = static_cast<unsigned long long *>(scratch(ctx, kS7, sizeof(int64_t) * (bucket_bound + 1)));
+  auto *next = static_cast<SegRec *>(scratch(ctx, kS13, sizeof(SegRec) * 2 * next_cap));
+  auto *ctr = static_cast<unsigned long long *>(scratch(ctx, kS14, 16 * sizeof(unsigned long long)));
+  auto *hc = static_cast<unsigned long long *>(pinned(ctx, 8 * sizeof(unsigned long long)));
+  if (!lay || !buf0 || !buf1 || !cnt || !cur || !next || !ctr || !hc)
+    return set_error(ctx, GBK_ENOMEM, "relabel big-row scratch");
+  int64_t *stage_off = lay, *bucket_off = lay + (nbig + 1), *chunk_off = lay + 2 * (nbig + 1);
+  const unsigned chunk_grid = static_cast<unsigned>(std::min<int64_t>(chunk_bound, INT32_MAX));
+  for (int64_t cap = bucket_bound + bucket_bound / 2 + 4096;; cap *= 4) {
+    auto *lists = static_cast<SegRec *>(scratch(ctx, kS12, sizeof(SegRec) * (kSegCopy + 1) * cap));
+    if (!lists) return set_error(ctx, GBK_ENOMEM, "relabel bucket lists");
+    // counters: [0..4] sort classes and copy list, [5] overflow, [6]/[7] the
+    // two ping-pong lists of segments to split at the next level, [8] the
+    // weighted buckets listed for a 64-bit redo
+    fill<unsigned long long>(ctx, ctr, 0ull, 9);
+    fill<unsigned long long>(ctx, cnt, 0ull, bucket_bound + 1);
+    big_layout<<<1, 1024, 0, s>>>(big, nbig, deg_new, key_bits, stage_off, bucket_off, chunk_off);
+    big_count<W><<<chunk_grid, kBlock, 0, s>>>(big, nbig, stage_off, bucket_off, chunk_off, inv, off, nbr, wgt,
+                                               new_ids, key_bits, buf0, cnt);
+    GBK_LAUNCH_CHECK(ctx, "big_count");
+    int rc = exclusive_scan_i64(ctx, reinterpret_cast<const int64_t *>(cnt), reinterpret_cast<int64_t *>(cur),
+                                bucket_bound);
+    if (rc) return rc;
+    big_scatter<W><<<chunk_grid, kBlock, 0, s>>>(stage_off, bucket_off, chunk_off, nbig, key_bits, buf0, buf1,
+                                                 cur);
+    big_segments<<<grid_for(bucket_bound), kBlock, 0, s>>>(bucket_off, stage_off, nbig, key_bits, cnt, cur,
+                                                           lists, cap, ctr, next, next_cap, &ctr[6]);
+    GBK_LAUNCH_CHECK(ctx, "big_segments");
+    int side = 0;  // next[side * next_cap ..] holds the segments of the coming level
+    bool overflow = false;
+    for (;;) {
+      GBK_HIP(ctx, hipMemcpyAsync(hc, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+      GBK_HIP(ctx, hipStreamSynchronize(s));
+      if (hc[kSegLists - 1]) {
+        overflow = true;
+        break;
+      }
+      const int64_t nsplit = static_cast<int64_t>(hc[6 + side]);
+      if (nsplit == 0) break;
+      if (nsplit > next_cap) return set_error(ctx, GBK_EINVAL, "relabel: bucket split list overflow");
+      fill<unsigned long long>(ctx, &ctr[6 + (side ^ 1)], 0ull, 1);
+      seg_split<W><<<static_cast<unsigned>(nsplit), 1024, 0, s>>>(next + side * next_cap, buf0, buf1, lists, cap,
+                                                                  ctr, next + (side ^ 1) * next_cap, next_cap,
+                                                                  &ctr[6 + (side ^ 1)]);
+      GBK_LAUNCH_CHECK(ctx, "seg_split");
+      side ^= 1;
+    }
+    if (overflow) continue;
+    const unsigned long long *cts = ctr;
+    auto sort_grid = [&](int c) {
+      const int64_t k = static_cast<int64_t>(hc[c]);
+      return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((k + kWavesPerBlock - 1) / kWavesPerBlock,
+                                                                          8192)));
+    };
+    // narrow weighted sorts where nb + index bits (8..11) <= 32
+    // (room for every record of the four sort lists: a bucket is listed once)
+    auto *redo = static_cast<SegRec *>(scratch(ctx, kS17, sizeof(SegRec) * 4 * cap));
+    if (!redo) return set_error(ctx, GBK_ENOMEM, "relabel bucket redo list");
+    unsigned long long *redo_count = &ctr[8];
+    bool any_narrow = false;
+#define GBK_SEG(c, E, IB)                                                                                   \
+  if (hc[c]) {                                                                                              \
+    if (W && nb + IB <= 32) {                                                                               \
+      any_narrow = true;                                                                                    \
+      seg_sort<W, E, true><<<sort_grid(c), kBlock, 0, s>>>(lists + c * cap, cts + c, cap, buf0, buf1,       \
+                                                           stage_off, nbig, big, new_off, nbr_out, wgt_out, \
+                                                           redo, redo_count);                               \
+    } else {                                                                                                \
+      seg_sort<W, E><<<sort_grid(c), kBlock, 0, s>>>(lists + c * cap, cts + c, cap, buf0, buf1, stage_off,  \
+                                                     nbig, big, new_off, nbr_out, wgt_out);                 \
+    }                                                                                                       \
+  }
+    GBK_SEG(0, 4, 8)
+    GBK_SEG(1, 8, 9)
+    GBK_SEG(2, 16, 10)
+    GBK_SEG(3, 32, 11)
+#undef GBK_SEG
+    if (hc[kSegCopy])
+      seg_copy<W><<<static_cast<unsigned>(std::min<unsigned long long>(hc[kSegCopy], 4096)), kBlock, 0, s>>>(
+          lists + kSegCopy * cap, cts + kSegCopy, cap, buf0, buf1, stage_off, nbig, big, new_off, nbr_out, wgt_out);
+    GBK_LAUNCH_CHECK(ctx, "seg_sort");
+    if constexpr (W) {
+      if (any_narrow) {
+        seg_redo<<<256, kBlock, 0, s>>>(redo, redo_count, buf0, buf1, stage_off, nbig, big, new_off, nbr_out,
+                                        wgt_out);
+        GBK_LAUNCH_CHECK(ctx, "seg_redo");
+      }
+    }
+    return GBK_OK;
+  }
 }
 
 // Relabels one CSR (out or in) of 'g' into (off_out, nbr_out, wgt_out).
@@ -447,16 +1042,19 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
   const int64_t ntiles = (n + tile - 1) / tile;
   auto *list = static_cast<int32_t *>(scratch(ctx, kS4, sizeof(int32_t) * (n + 1)));
   auto *tile_cnt = static_cast<uint32_t *>(scratch(ctx, kS5, sizeof(uint32_t) * kClasses * (ntiles + 1)));
+  // [0..kClasses) rows per class, [kClasses] big rows' entries, [kClasses+1]
+  // rows listed for a redo
   auto *counts_d = static_cast<unsigned long long *>(scratch(ctx, kS8, 128));
   if (!list || !tile_cnt || !counts_d) return set_error(ctx, GBK_ENOMEM, "relabel bins");
-  fill<unsigned long long>(ctx, counts_d, 0ull, kClasses);
+  fill<unsigned long long>(ctx, counts_d, 0ull, kClasses + 2);
   if (n > 0) {
-    relabel_bin_count<<<static_cast<unsigned>(ntiles), kBlock, 0, s>>>(deg_new, n, tile, tile_cnt);
+    relabel_bin_count<<<static_cast<unsigned>(ntiles), kBlock, 0, s>>>(deg_new, n, tile, tile_cnt,
+                                                                       counts_d + kClasses);
     relabel_bin_offsets<<<1, 1024, 0, s>>>(tile_cnt, ntiles, counts_d);
     relabel_bin_scatter<<<static_cast<unsigned>(ntiles), kBlock, 0, s>>>(deg_new, n, tile, tile_cnt, list);
   }
   GBK_LAUNCH_CHECK(ctx, "relabel bins");
-  unsigned long long counts[kClasses];
+  unsigned long long counts[kClasses + 1];  // rows per class, then the big rows' entries
   GBK_HIP(ctx, hipMemcpyAsync(counts, counts_d, sizeof(counts), hipMemcpyDeviceToHost, s));
   GBK_HIP(ctx, hipStreamSynchronize(s));
   {
@@ -466,116 +1064,76 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
       at += static_cast<int64_t>(counts[c]);
     }
   }
+  // Weighted rows sort as 32-bit composites where the ids leave room for the
+  // entry index (nb + index bits <= 32); rows with a repeated neighbour are
+  // listed in 'redo' and sorted again on 64-bit keys after the classes.
+  int nb = 1;
+  while (nb < 31 && (int64_t{1} << nb) < n) nb++;
+  auto fits = [&](int index_bits) { return W && nb + index_bits <= 32; };
+  int32_t *redo = nullptr;
+  unsigned long long *redo_count = counts_d + kClasses + 1;
+  if (W) {
+    redo = static_cast<int32_t *>(scratch(ctx, kS16, sizeof(int32_t) * (n + 1)));
+    if (!redo) return set_error(ctx, GBK_ENOMEM, "relabel redo list");
+  }
+  bool any_narrow = false;
   // rows of <= 64 entries: lane groups sort in registers
   const int groups[4] = {8, 16, 32, 64};
   for (int c = 0; c < 4; c++) {
     const int64_t cnt = static_cast<int64_t>(counts[c]);
     if (cnt == 0) continue;
     const int64_t blocks = (cnt + kBlock / groups[c] - 1) / (kBlock / groups[c]);
+    const bool nw = fits(3 + c);
+    any_narrow |= nw;
+#define GBK_GROUP(G)                                                                                        \
+  if (nw)                                                                                                   \
+    relabel_group_sort<W, G, true><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d,  \
+                                                             off_out, nbr_out, wgt_out, redo, redo_count);  \
+  else                                                                                                      \
+    relabel_group_sort<W, G><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d,       \
+                                                       off_out, nbr_out, wgt_out);
     switch (groups[c]) {
-      case 8: relabel_group_sort<W, 8><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
-      case 16: relabel_group_sort<W, 16><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
-      case 32: relabel_group_sort<W, 32><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
-      default: relabel_group_sort<W, 64><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+      case 8: GBK_GROUP(8) break;
+      case 16: GBK_GROUP(16) break;
+      case 32: GBK_GROUP(32) break;
+      default: GBK_GROUP(64) break;
     }
+#undef GBK_GROUP
     GBK_LAUNCH_CHECK(ctx, "relabel_group_sort");
   }
-  // rows of <= 2048 entries: one wave each, in registers
+  // rows of <= 1024 entries: one wave each, in registers
   for (int c = 4; c < kClassBig; c++) {
     const int64_t cnt = static_cast<int64_t>(counts[c]);
     if (cnt == 0) continue;
     const unsigned blocks = static_cast<unsigned>((cnt + kWavesPerBlock - 1) / kWavesPerBlock);
+    const bool nw = fits(3 + c);  // 64 * E entries: 7..10 index bits
+    any_narrow |= nw;
+#define GBK_WAVE(E)                                                                                         \
+  if (nw)                                                                                                   \
+    relabel_wave_sort<W, E, true><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d,   \
+                                                            off_out, nbr_out, wgt_out, redo, redo_count);   \
+  else                                                                                                      \
+    relabel_wave_sort<W, E><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d,        \
+                                                      off_out, nbr_out, wgt_out);
     switch (c) {
-      case 4: relabel_wave_sort<W, 2><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
-      case 5: relabel_wave_sort<W, 4><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
-      case 6: relabel_wave_sort<W, 8><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
-      case 7: relabel_wave_sort<W, 16><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
-      default: relabel_wave_sort<W, 32><<<blocks, kBlock, 0, s>>>(lists[c], cnt, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out, wgt_out); break;
+      case 4: GBK_WAVE(2) break;
+      case 5: GBK_WAVE(4) break;
+      case 6: GBK_WAVE(8) break;
+      case 7: GBK_WAVE(16) break;
+      default: GBK_WAVE(32) break;
     }
+#undef GBK_WAVE
     GBK_LAUNCH_CHECK(ctx, "relabel_wave_sort");
   }
-  const int nbig = static_cast<int>(counts[kClassBig]);
-  int32_t *big_list = lists[kClassBig];
+  if (any_narrow) {
+    relabel_redo_rows<<<256, kBlock, 0, s>>>(redo, redo_count, inv, off, nbr, wgt, new_ids_d, off_out, nbr_out,
+                                             wgt_out);
+    GBK_LAUNCH_CHECK(ctx, "relabel_redo_rows");
+  }
+  const int64_t nbig = static_cast<int64_t>(counts[kClassBig]);
   if (nbig == 0) return GBK_OK;
-  // Big rows: host builds the staging layout and chunk table (few rows).
-  std::vector<int32_t> big(nbig);
-  std::vector<int64_t> bdeg(nbig);
-  GBK_HIP(ctx, hipMemcpyAsync(big.data(), big_list, sizeof(int32_t) * nbig, hipMemcpyDeviceToHost, s));
-  GBK_HIP(ctx, hipStreamSynchronize(s));
-  {
-    // degrees of the big rows: deg_new[row], gathered on the device
-    auto *tmp = static_cast<int64_t *>(scratch(ctx, kS9, sizeof(int64_t) * nbig));
-    if (!tmp) return set_error(ctx, GBK_ENOMEM, "relabel big degrees");
-    gather_i64<<<grid_for(nbig), kBlock, 0, s>>>(deg_new, big_list, nbig, tmp);
-    GBK_LAUNCH_CHECK(ctx, "gather_i64");
-    GBK_HIP(ctx, hipMemcpyAsync(bdeg.data(), tmp, sizeof(int64_t) * nbig, hipMemcpyDeviceToHost, s));
-    GBK_HIP(ctx, hipStreamSynchronize(s));
-  }
-  {
-    // longest rows first: a merge pass of width w then covers a prefix of
-    // the rows (those longer than w) instead of copying every big row again
-    std::vector<int> order(nbig);
-    for (int i = 0; i < nbig; i++) order[i] = i;
-    std::sort(order.begin(), order.end(), [&](int a, int b) {
-      return bdeg[a] != bdeg[b] ? bdeg[a] > bdeg[b] : big[a] < big[b];
-    });
-    std::vector<int32_t> b2(nbig);
-    std::vector<int64_t> d2(nbig);
-    for (int i = 0; i < nbig; i++) {
-      b2[i] = big[order[i]];
-      d2[i] = bdeg[order[i]];
-    }
-    big.swap(b2);
-    bdeg.swap(d2);
-  }
-  std::vector<int64_t> soff(nbig + 1, 0);
-  std::vector<int32_t> crow;
-  std::vector<int64_t> cstart;
-  int64_t maxdeg = 0;
-  for (int i = 0; i < nbig; i++) {
-    soff[i + 1] = soff[i] + bdeg[i];
-    maxdeg = std::max(maxdeg, bdeg[i]);
-    for (int64_t c = 0; c < bdeg[i]; c += kSortCap) {
-      crow.push_back(i);
-      cstart.push_back(c);
-    }
-  }
-  const int64_t total = soff[nbig];
-  using K = typename BigKey<W>::T;
-  const size_t meta_bytes = sizeof(int32_t) * nbig + sizeof(int64_t) * (nbig + 1) +
-                            sizeof(int32_t) * crow.size() + sizeof(int64_t) * cstart.size() + 64;
-  char *meta = static_cast<char *>(scratch(ctx, kS9, meta_bytes));
-  K *st0 = static_cast<K *>(scratch(ctx, kS10, sizeof(K) * total));
-  K *st1 = static_cast<K *>(scratch(ctx, kS11, sizeof(K) * total));
-  if (!meta || !st0 || !st1) return set_error(ctx, GBK_ENOMEM, "relabel staging");
-  auto *big_d = reinterpret_cast<int32_t *>(meta);
-  auto *soff_d = reinterpret_cast<int64_t *>(meta + ((sizeof(int32_t) * nbig + 7) / 8) * 8);
-  auto *crow_d = reinterpret_cast<int32_t *>(soff_d + nbig + 1);
-  auto *cstart_d = reinterpret_cast<int64_t *>(
-      reinterpret_cast<char *>(crow_d) + ((sizeof(int32_t) * crow.size() + 7) / 8) * 8);
-  GBK_HIP(ctx, hipMemcpyAsync(big_d, big.data(), sizeof(int32_t) * nbig, hipMemcpyHostToDevice, s));
-  GBK_HIP(ctx, hipMemcpyAsync(soff_d, soff.data(), sizeof(int64_t) * (nbig + 1), hipMemcpyHostToDevice, s));
-  GBK_HIP(ctx, hipMemcpyAsync(crow_d, crow.data(), sizeof(int32_t) * crow.size(), hipMemcpyHostToDevice, s));
-  GBK_HIP(ctx, hipMemcpyAsync(cstart_d, cstart.data(), sizeof(int64_t) * cstart.size(), hipMemcpyHostToDevice, s));
-  const int64_t nchunks = static_cast<int64_t>(crow.size());
-  relabel_big_chunks<W><<<static_cast<unsigned>((nchunks + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, s>>>(
-      big_d, soff_d, crow_d, cstart_d, nchunks, inv, off, nbr, wgt, new_ids_d, st0);
-  GBK_LAUNCH_CHECK(ctx, "relabel_big_chunks");
-  K *src = st0, *dst = st1;
-  int active = nbig;  // rows longer than the current width: a prefix
-  for (int64_t width = kSortCap; width < maxdeg; width *= 2) {
-    while (active > 0 && bdeg[active - 1] <= width) active--;
-    const int64_t span = soff[active];
-    const int64_t threads = (span + kMergeItems - 1) / kMergeItems;
-    relabel_merge_pass<K><<<grid_for(threads), kBlock, 0, s>>>(src, dst, soff_d, active, span, width);
-    GBK_LAUNCH_CHECK(ctx, "relabel_merge_pass");
-    std::swap(src, dst);
-  }
-  relabel_big_scatter<W><<<grid_for(total), kBlock, 0, s>>>(st0, st1, soff_d, big_d, nbig, total, off_out,
-                                                            nbr_out, wgt_out);
-  GBK_LAUNCH_CHECK(ctx, "relabel_big_scatter");
-  GBK_HIP(ctx, hipStreamSynchronize(s));  // staging buffers are reused by the next CSR
-  return GBK_OK;
+  return relabel_big<W>(ctx, n, off, nbr, wgt, new_ids_d, inv, deg_new, off_out, lists[kClassBig], nbig,
+                        static_cast<int64_t>(counts[kClasses]), nbr_out, wgt_out);
 }
 
 __global__ void degrees_through_inverse(const int64_t *off, const int32_t *inv, int64_t n,

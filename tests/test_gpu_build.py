@@ -54,6 +54,21 @@ def test_build_duplicates_keep_smallest_weight(ctx):
         _same(dg, O.make_graph(u, v, w, symmetrize=sym))
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_build_thousands_of_copies(ctx, weighted):
+    """Raw rows holding thousands of copies of one edge (sorted as buckets of
+    one repeated key) squish to the reference's CSR."""
+    rng = np.random.default_rng(9)
+    bu = rng.integers(0, 5000, 30000)
+    bv = rng.integers(0, 5000, 30000)
+    u = np.concatenate([np.zeros(6000), np.full(2500, 7), bu]).astype(np.int32)
+    v = np.concatenate([np.ones(6000), np.full(2500, 4999), bv]).astype(np.int32)
+    w = rng.integers(1, 4, len(u)).astype(np.int32) if weighted else None
+    for sym in (True, False):
+        dg, _ = ctx.build(u, v, w, symmetrize=sym)
+        _same(dg, O.make_graph(u, v, w, symmetrize=sym))
+
+
 @pytest.mark.parametrize("case", ["bad_id", "negative_id", "decreasing", "bad_in_id"])
 def test_upload_rejects_malformed_csr(ctx, case):
     """gbk_graph_upload checks a caller's CSR on the host before any kernel

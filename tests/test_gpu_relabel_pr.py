@@ -58,7 +58,22 @@ def _class_rows(weighted=True):
     return O.make_graph(u, v, w, symmetrize=False, n=n)
 
 
+def _crowded(weighted=False):
+    """Big rows whose new ids crowd into a narrow part of [0, n): under the
+    identity, row 0's 3001 neighbours are 1..3000 and 32000 (the row's first
+    bucket split twice more), row 2^17's 6000 neighbours are random."""
+    n = 1 << 18
+    rng = np.random.default_rng(5)
+    hub2 = rng.choice(np.arange(2, n), 6000, replace=False)
+    u = np.concatenate([np.zeros(3001), np.full(6000, 1 << 17)]).astype(np.int32)
+    v = np.concatenate([np.arange(1, 3001), [32000], hub2]).astype(np.int32)
+    w = rng.integers(1, 256, len(u)).astype(np.int32) if weighted else None
+    return O.make_graph(u, v, w, n=n)
+
+
 GRAPHS = {
+    "crowded": lambda: _crowded(),
+    "crowded_w": lambda: _crowded(True),
     "classes_w": lambda: _class_rows(),
     "classes": lambda: _class_rows(False),
     "kron12": lambda: O.kron(12),
