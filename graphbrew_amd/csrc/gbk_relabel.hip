@@ -536,6 +536,13 @@ __device__ __forceinline__ int seg_bits(int64_t len, int hi) {
   return min(b, hi);
 }
 
+// Level-1 buckets per big row: as seg_bits, at most 2^kL1MaxBits so a
+// workgroup's counts, offsets and reordered chunk fit in LDS (rows beyond
+// kSegTarget * 2^kL1MaxBits entries get their big buckets split at level 2).
+constexpr int kL1MaxBits = 10;
+constexpr int kL1MaxBuckets = 1 << kL1MaxBits;
+__device__ __forceinline__ int l1_bits(int64_t d, int key_bits) { return min(seg_bits(d, key_bits), kL1MaxBits); }
+
 // Largest i in [0, count) with a[i] <= x (a ascending, a[0] <= x).
 __device__ __forceinline__ int64_t find_le(const int64_t *a, int64_t count, int64_t x) {
   int64_t lo = 0, hi = count;
@@ -547,30 +554,40 @@ __device__ __forceinline__ int64_t find_le(const int64_t *a, int64_t count, int6
   return lo;
 }
 
-// One workgroup of 1024 lanes: for the big rows (new ids big[0..nbig)) the
-// exclusive prefixes of their lengths (staging offsets), level-1 bucket
-// counts and kBigChunk chunk counts; entry [nbig] holds each total.
+// Per big row i (new id big[i]): its length, level-1 bucket count and
+// kBigChunk chunk count, the inputs of big_layout's scans.
+__global__ void big_row_sizes(const int32_t *big, int64_t nbig, const int64_t *deg_new, int key_bits,
+                              int64_t *len, int64_t *buckets, int64_t *chunks) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nbig) return;
+  const int64_t d = deg_new[big[i]];
+  len[i] = d;
+  buckets[i] = int64_t{1} << l1_bits(d, key_bits);
+  chunks[i] = (d + kBigChunk - 1) / kBigChunk;
+}
+
+// One workgroup of 1024 lanes turns the three arrays (nbig + 1 entries
+// each) into their exclusive prefixes in place (staging offsets, level-1
+// bucket offsets, chunk offsets); entry [nbig] receives each total.  Lane t
+// takes a contiguous run of rows, loads it whole (independent loads) and
+// carries its sums through one block scan.
 __global__ void __launch_bounds__(1024)
-big_layout(const int32_t *big, int64_t nbig, const int64_t *deg_new, int key_bits, int64_t *stage_off,
-           int64_t *bucket_off, int64_t *chunk_off) {
+big_layout(int64_t nbig, int64_t *stage_off, int64_t *bucket_off, int64_t *chunk_off) {
   __shared__ int64_t wsum[3][1024 / kWave];
+  int64_t *arr[3] = {stage_off, bucket_off, chunk_off};
   const int t = threadIdx.x;
   const int64_t per = (nbig + 1023) / 1024;
   const int64_t b = min(nbig, t * per), e = min(nbig, b + per);
-  int64_t s0 = 0, s1 = 0, s2 = 0;
-  for (int64_t i = b; i < e; i++) {
-    const int64_t d = deg_new[big[i]];
-    s0 += d;
-    s1 += int64_t{1} << seg_bits(d, key_bits);
-    s2 += (d + kBigChunk - 1) / kBigChunk;
+  int64_t s[3] = {0, 0, 0};
+  for (int q = 0; q < 3; q++) {
+#pragma unroll 8
+    for (int64_t i = b; i < e; i++) s[q] += arr[q][i];
   }
-  const int64_t i0 = wave_incl_scan(s0), i1 = wave_incl_scan(s1), i2 = wave_incl_scan(s2);
+  int64_t incl[3];
+  for (int q = 0; q < 3; q++) incl[q] = wave_incl_scan(s[q]);
   const int w = t / kWave;
-  if (lane_id() == kWave - 1) {
-    wsum[0][w] = i0;
-    wsum[1][w] = i1;
-    wsum[2][w] = i2;
-  }
+  if (lane_id() == kWave - 1)
+    for (int q = 0; q < 3; q++) wsum[q][w] = incl[q];
   __syncthreads();
   if (w == 0) {
     for (int q = 0; q < 3; q++) {
@@ -580,22 +597,15 @@ big_layout(const int32_t *big, int64_t nbig, const int64_t *deg_new, int key_bit
     }
   }
   __syncthreads();
-  int64_t r0 = i0 - s0 + (w > 0 ? wsum[0][w - 1] : 0);
-  int64_t r1 = i1 - s1 + (w > 0 ? wsum[1][w - 1] : 0);
-  int64_t r2 = i2 - s2 + (w > 0 ? wsum[2][w - 1] : 0);
-  for (int64_t i = b; i < e; i++) {
-    stage_off[i] = r0;
-    bucket_off[i] = r1;
-    chunk_off[i] = r2;
-    const int64_t d = deg_new[big[i]];
-    r0 += d;
-    r1 += int64_t{1} << seg_bits(d, key_bits);
-    r2 += (d + kBigChunk - 1) / kBigChunk;
-  }
-  if (t == 1023) {
-    stage_off[nbig] = r0;
-    bucket_off[nbig] = r1;
-    chunk_off[nbig] = r2;
+  for (int q = 0; q < 3; q++) {
+    int64_t run = incl[q] - s[q] + (w > 0 ? wsum[q][w - 1] : 0);
+#pragma unroll 8
+    for (int64_t i = b; i < e; i++) {
+      const int64_t x = arr[q][i];
+      arr[q][i] = run;
+      run += x;
+    }
+    if (t == 1023) arr[q][nbig] = run;
   }
 }
 
@@ -608,13 +618,13 @@ big_count(const int32_t *big, int64_t nbig, const int64_t *stage_off, const int6
           const int32_t *wgt, const int32_t *new_ids, int key_bits, typename RowKey<W>::T *stage,
           unsigned long long *cnt) {
   using K = typename RowKey<W>::T;
-  __shared__ uint32_t hist[kSegMaxBuckets];
+  __shared__ uint32_t hist[kL1MaxBuckets];
   const int64_t c = blockIdx.x;
   if (c >= chunk_off[nbig]) return;  // workgroup-uniform
   const int64_t i = find_le(chunk_off, nbig, c);
   const int32_t u = inv[big[i]];
   const int64_t d = stage_off[i + 1] - stage_off[i];
-  const int b = seg_bits(d, key_bits);
+  const int b = l1_bits(d, key_bits);
   const int shift = key_bits - b;
   for (int k = threadIdx.x; k < (1 << b); k += kBlock) hist[k] = 0;
   __syncthreads();
@@ -650,19 +660,21 @@ big_count(const int32_t *big, int64_t nbig, const int64_t *stage_off, const int6
 
 // Level 1, pass 2: workgroup c moves its staged chunk into the buckets; cur
 // holds each bucket's next free staging slot (starts as the scan of cnt).
+// (Reordering the chunk by bucket in LDS first, so each bucket's share
+// leaves as one run, measured slower: 254 -> 284 us at Kronecker scale 22.)
 template <bool W>
 __global__ void __launch_bounds__(kBlock)
 big_scatter(const int64_t *stage_off, const int64_t *bucket_off, const int64_t *chunk_off, int64_t nbig,
             int key_bits, const typename RowKey<W>::T *stage, typename RowKey<W>::T *stage2,
             unsigned long long *cur) {
   using K = typename RowKey<W>::T;
-  __shared__ uint32_t hist[kSegMaxBuckets];
-  __shared__ unsigned long long base[kSegMaxBuckets];
+  __shared__ uint32_t hist[kL1MaxBuckets];
+  __shared__ unsigned long long base[kL1MaxBuckets];
   const int64_t c = blockIdx.x;
   if (c >= chunk_off[nbig]) return;  // workgroup-uniform
   const int64_t i = find_le(chunk_off, nbig, c);
   const int64_t d = stage_off[i + 1] - stage_off[i];
-  const int b = seg_bits(d, key_bits);
+  const int b = l1_bits(d, key_bits);
   const int shift = key_bits - b;
   for (int k = threadIdx.x; k < (1 << b); k += kBlock) hist[k] = 0;
   __syncthreads();
@@ -729,7 +741,7 @@ __global__ void big_segments(const int64_t *bucket_off, const int64_t *stage_off
     const int64_t i = find_le(bucket_off, nbig, k);
     const int64_t len = static_cast<int64_t>(cnt[k]);
     if (len > 0) {
-      const int b = seg_bits(stage_off[i + 1] - stage_off[i], key_bits);
+      const int b = l1_bits(stage_off[i + 1] - stage_off[i], key_bits);
       rec.pos = static_cast<int64_t>(cur[k]) - len;
       rec.len = static_cast<int32_t>(len);
       rec.hi_buf = ((key_bits - b) << 1) | 1;
@@ -949,7 +961,9 @@ static int relabel_big(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
     // weighted buckets listed for a 64-bit redo
     fill<unsigned long long>(ctx, ctr, 0ull, 9);
     fill<unsigned long long>(ctx, cnt, 0ull, bucket_bound + 1);
-    big_layout<<<1, 1024, 0, s>>>(big, nbig, deg_new, key_bits, stage_off, bucket_off, chunk_off);
+    big_row_sizes<<<grid_for(nbig), kBlock, 0, s>>>(big, nbig, deg_new, key_bits, stage_off, bucket_off,
+                                                    chunk_off);
+    big_layout<<<1, 1024, 0, s>>>(nbig, stage_off, bucket_off, chunk_off);
     big_count<W><<<chunk_grid, kBlock, 0, s>>>(big, nbig, stage_off, bucket_off, chunk_off, inv, off, nbr, wgt,
                                                new_ids, key_bits, buf0, cnt);
     GBK_LAUNCH_CHECK(ctx, "big_count");
