@@ -352,19 +352,30 @@ __device__ __forceinline__ int tile_owner32(int32_t excl, int32_t e) {
   return lo;
 }
 
-// One wave per tile.  Each lane counts its row's light in-edges per segment
-// by binary search over the row's ascending sources, the segments are laid
-// out, then one coalesced pass writes the records, each edge also compared
-// with its predecessor in the row.  A tile with a row whose sources are not
-// ascending (an uploaded CSR need not be sorted) is then redone lane by
-// lane: counted and scattered by each lane walking its own row in order.
-// Writes stay inside the tile's record range in every case.
+// One wave per tile: the tile's light in-edges (rows of at most heavy_deg
+// entries), in row order, are counting-sorted by source segment, stably, so
+// the records come out ordered by (segment, row, position) whatever the
+// order of the sources inside a row.  Pass 1 counts each segment's edges (a
+// ballot per segment and 64 edges; lane s keeps segment s's count), their
+// exclusive scan gives each segment's first record, pass 2 ranks every edge
+// among its batch's edges of the same segment (ballot + mbcnt) and writes
+// its record.  kSegBuildU batches of 64 edges are in flight per wave.
+// Records stay inside the tile's edge range.
+__device__ __forceinline__ void seg_batch_load(const int32_t *in_nbr, int64_t tbase, int32_t excl, int32_t rel,
+                                               int32_t total, int32_t e, int lw, int &own, int32_t &v, int &sg) {
+  const int lo = tile_owner32(excl, e);
+  // shuffles in uniform control flow: the owner lane may be past the end
+  const int32_t k = e - __shfl(excl, lo);
+  const int32_t orel = __shfl(rel, lo);
+  own = lo;
+  v = e < total ? in_nbr[tbase + orel + k] : 0;
+  sg = e < total ? (v >> lw) : -1;
+}
+
 __global__ void __launch_bounds__(kBlock)
 seg_records(RowTiles rt, const int32_t *in_nbr, int S, int lw, int32_t *rec, int32_t *tseg) {
-  __shared__ int32_t cnt_all[kWavesPerBlock][kSegMax * kWave];
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
-  int32_t *cnt = cnt_all[wv];
   const int32_t mask = (1 << lw) - 1;
   for (int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wv; t < rt.ntiles;
        t += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
@@ -377,83 +388,50 @@ seg_records(RowTiles rt, const int32_t *in_nbr, int S, int lw, int32_t *rec, int
       len = rt.off[u + 1] - start;
     }
     if (len > rt.heavy_deg) len = 0;
-    const int32_t *row = in_nbr + start;
-    // per-segment counts of this lane's row, if its sources ascend
-    int64_t prev = 0;
-    for (int s = 0; s < S; s++) {
-      const int64_t nb = s + 1 < S ? lower_edge(row, prev, len, static_cast<int64_t>(s + 1) << lw) : len;
-      cnt[s * kWave + lane] = static_cast<int32_t>(nb - prev);
-      prev = nb;
-    }
     const int32_t len32 = static_cast<int32_t>(len);
     const int32_t rel = static_cast<int32_t>(start - tbase);
     const int32_t incl = wave_incl_scan(len32);
     const int32_t excl = incl - len32;
     const int32_t total = __shfl(incl, kWave - 1);
-    bool sorted = true;
-    for (int pass = 0; pass < 2; pass++) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      // segment s's records start at 'at'; within it the rows in order.  A
-      // sorted row's segment-s edges are contiguous from position 'rowpre'
-      // of the row, so its slot holds first - rowpre (add the position);
-      // otherwise the slot holds the next free record of (row, segment).
-      int32_t at = 0, rowpre = 0;
-      for (int s = 0; s < S; s++) {
-        const int32_t cs = cnt[s * kWave + lane];
-        const int32_t ci = wave_incl_scan(cs);
-        const int32_t first = at + ci - cs;
-        cnt[s * kWave + lane] = sorted ? first - rowpre : first;
-        if (lane == 0) tseg[t * (S + 1) + s] = at;
-        rowpre += cs;
-        at += __shfl(ci, kWave - 1);
-      }
-      if (lane == 0) tseg[t * (S + 1) + S] = at;
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      int32_t *out = rec + tbase;
-      if (sorted) {
-        bool bad = false;
-        for (int32_t c = 0; c < total; c += kSegBuildU * kWave) {
-          int32_t v[kSegBuildU], pv[kSegBuildU], k[kSegBuildU];
-          int own[kSegBuildU];
+    // pass 1: lane s counts segment s
+    int32_t mine = 0;
+    for (int32_t c = 0; c < total; c += kSegBuildU * kWave) {
+      int own[kSegBuildU], sg[kSegBuildU];
+      int32_t v[kSegBuildU];
 #pragma unroll
-          for (int j = 0; j < kSegBuildU; j++) {
-            const int32_t e = c + j * kWave + lane;
-            const int lo = tile_owner32(excl, e);
-            // shuffles in uniform control flow: the owner lane may be past the end
-            k[j] = e - __shfl(excl, lo);
-            const int32_t orel = __shfl(rel, lo);
-            own[j] = e < total ? lo : kWave;
-            v[j] = e < total ? in_nbr[tbase + orel + k[j]] : 0;
-            pv[j] = e < total && k[j] > 0 ? in_nbr[tbase + orel + k[j] - 1] : 0;
-          }
+      for (int j = 0; j < kSegBuildU; j++)
+        seg_batch_load(in_nbr, tbase, excl, rel, total, c + j * kWave + lane, lw, own[j], v[j], sg[j]);
 #pragma unroll
-          for (int j = 0; j < kSegBuildU; j++) {
-            if (own[j] < kWave) {
-              bad |= v[j] < pv[j];
-              const int32_t dst = cnt[(v[j] >> lw) * kWave + own[j]] + k[j];
-              if (dst >= 0 && dst < total) out[dst] = seg_record(own[j], v[j] & mask);
-            }
-          }
+      for (int j = 0; j < kSegBuildU; j++)
+        for (int q = 0; q < S; q++) {
+          const uint64_t m = __ballot(sg[j] == q);
+          if (lane == q) mine += __popcll(m);
         }
-        if (__ballot(bad) == 0) break;
-        // some row is not ascending: count again lane by lane and redo
-        sorted = false;
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        for (int s = 0; s < S; s++) cnt[s * kWave + lane] = 0;
-        for (int64_t q = 0; q < len; q++) cnt[(row[q] >> lw) * kWave + lane]++;
-      } else {
-        for (int64_t q = 0; q < len; q++) {
-          const int32_t v = row[q];
-          const int32_t pos = cnt[(v >> lw) * kWave + lane]++;
-          out[pos] = seg_record(lane, v & mask);
+    }
+    const int32_t first = wave_incl_scan(mine) - mine;
+    if (lane < S) tseg[t * (S + 1) + lane] = first;
+    if (lane == 0) tseg[t * (S + 1) + S] = total;
+    // pass 2: lane s holds segment s's next record
+    int32_t run = first;
+    int32_t *out = rec + tbase;
+    for (int32_t c = 0; c < total; c += kSegBuildU * kWave) {
+      int own[kSegBuildU], sg[kSegBuildU];
+      int32_t v[kSegBuildU];
+#pragma unroll
+      for (int j = 0; j < kSegBuildU; j++)
+        seg_batch_load(in_nbr, tbase, excl, rel, total, c + j * kWave + lane, lw, own[j], v[j], sg[j]);
+#pragma unroll
+      for (int j = 0; j < kSegBuildU; j++) {
+        int32_t pos = 0;
+        for (int q = 0; q < S; q++) {
+          const uint64_t m = __ballot(sg[j] == q);
+          const int32_t at = __shfl(run, q);
+          if (sg[j] == q) pos = at + lanemask_lt_count(m);
+          if (lane == q) run += __popcll(m);
         }
+        if (sg[j] >= 0 && pos >= 0 && pos < total) out[pos] = seg_record(own[j], v[j] & mask);
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -469,6 +447,7 @@ seg_heavy_pieces(RowTiles rt, const int32_t *in_nbr, int S, int lw, int64_t *hps
     if (threadIdx.x == 0) bad = 0;
     __syncthreads();
     bool mine = false;
+#pragma unroll 8
     for (int64_t k = a + 1 + threadIdx.x; k < b; k += blockDim.x) mine |= in_nbr[k] < in_nbr[k - 1];
     if (mine) atomicOr(&bad, 1);
     __syncthreads();

@@ -932,7 +932,7 @@ template <bool W>
 static int relabel_big(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_t *nbr, const int32_t *wgt,
                        const int32_t *new_ids, const int32_t *inv, const int64_t *deg_new,
                        const int64_t *new_off, const int32_t *big, int64_t nbig, int64_t total,
-                       int32_t *nbr_out, int32_t *wgt_out) {
+                       int32_t *nbr_out, int32_t *wgt_out, bool narrow) {
   using K = typename RowKey<W>::T;
   hipStream_t s = ctx->stream;
   int nb = 1;
@@ -1009,7 +1009,7 @@ static int relabel_big(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
     bool any_narrow = false;
 #define GBK_SEG(c, E, IB)                                                                                   \
   if (hc[c]) {                                                                                              \
-    if (W && nb + IB <= 32) {                                                                               \
+    if (W && narrow && nb + IB <= 32) {                                                                     \
       any_narrow = true;                                                                                    \
       seg_sort<W, E, true><<<sort_grid(c), kBlock, 0, s>>>(lists + c * cap, cts + c, cap, buf0, buf1,       \
                                                            stage_off, nbig, big, new_off, nbr_out, wgt_out, \
@@ -1040,11 +1040,14 @@ static int relabel_big(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
 }
 
 // Relabels one CSR (out or in) of 'g' into (off_out, nbr_out, wgt_out).
+// narrow: weighted rows may sort as 32-bit composites (rows with distinct
+// neighbours, as squished graphs have; the builder's raw rows repeat
+// neighbours and sort on 64-bit keys directly).
 template <bool W>
 static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_t *nbr,
                        const int32_t *wgt, const int32_t *new_ids_d, const int32_t *inv,
                        int64_t *deg_new, int64_t *off_out, int32_t *nbr_out, int32_t *wgt_out,
-                       bool degrees_ready) {
+                       bool degrees_ready, bool narrow = true) {
   hipStream_t s = ctx->stream;
   if (!degrees_ready) return set_error(ctx, GBK_EINVAL, "relabel: degrees not computed");
   int rc = exclusive_scan_i64(ctx, deg_new, off_out, n);
@@ -1083,7 +1086,7 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
   // listed in 'redo' and sorted again on 64-bit keys after the classes.
   int nb = 1;
   while (nb < 31 && (int64_t{1} << nb) < n) nb++;
-  auto fits = [&](int index_bits) { return W && nb + index_bits <= 32; };
+  auto fits = [&](int index_bits) { return W && narrow && nb + index_bits <= 32; };
   int32_t *redo = nullptr;
   unsigned long long *redo_count = counts_d + kClasses + 1;
   if (W) {
@@ -1147,7 +1150,7 @@ static int relabel_csr(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_
   const int64_t nbig = static_cast<int64_t>(counts[kClassBig]);
   if (nbig == 0) return GBK_OK;
   return relabel_big<W>(ctx, n, off, nbr, wgt, new_ids_d, inv, deg_new, off_out, lists[kClassBig], nbig,
-                        static_cast<int64_t>(counts[kClasses]), nbr_out, wgt_out);
+                        static_cast<int64_t>(counts[kClasses]), nbr_out, wgt_out, narrow);
 }
 
 __global__ void degrees_through_inverse(const int64_t *off, const int32_t *inv, int64_t n,
@@ -1164,8 +1167,8 @@ __global__ void degrees_through_inverse(const int64_t *off, const int32_t *inv, 
 int sort_rows(gbk_ctx *ctx, int64_t n, const int64_t *off, const int32_t *nbr, const int32_t *wgt,
               const int32_t *iota, int64_t *deg, int64_t *off_out, int32_t *nbr_out,
               int32_t *wgt_out) {
-  if (wgt) return relabel_csr<true>(ctx, n, off, nbr, wgt, iota, iota, deg, off_out, nbr_out, wgt_out, true);
-  return relabel_csr<false>(ctx, n, off, nbr, wgt, iota, iota, deg, off_out, nbr_out, wgt_out, true);
+  if (wgt) return relabel_csr<true>(ctx, n, off, nbr, wgt, iota, iota, deg, off_out, nbr_out, wgt_out, true, false);
+  return relabel_csr<false>(ctx, n, off, nbr, wgt, iota, iota, deg, off_out, nbr_out, wgt_out, true, false);
 }
 
 }  // namespace gbk

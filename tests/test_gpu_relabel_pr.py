@@ -109,6 +109,46 @@ def test_relabel_directed_weighted(ctx):
         _assert_same_graph(r, O.relabel(g, ids))
 
 
+def _multi_edge_csr(n, edges):
+    """CSR of the (u, v, w) edge list as given: repeated (u, v) pairs with
+    different weights stay (an uploaded CSR need not be squished)."""
+    u, v, w = (np.asarray(x, np.int32) for x in edges)
+    order = np.lexsort((w, v, u))
+    u, v, w = u[order], v[order], w[order]
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, u + 1, 1)
+    return np.cumsum(off), v.copy(), w.copy()
+
+
+def test_relabel_weighted_repeated_neighbours(ctx):
+    """Rows that hold one neighbour several times (weights differ) sort by
+    (neighbour, weight) in every row class: the 32-bit (id, index) sorts
+    find the repeats and hand those rows to the 64-bit redo."""
+    rng = np.random.default_rng(17)
+    n = 5000
+    us, vs, ws = [], [], []
+    for row, k, reps in [(0, 3000, 3), (1, 500, 2), (2, 40, 4), (3, 9, 2), (4, 1500, 1), (5, 200, 1)]:
+        base = rng.choice(n, k // reps, replace=False)
+        vv = np.repeat(base, reps)[:k]
+        us.append(np.full(len(vv), row))
+        vs.append(vv)
+        ws.append(rng.permutation(len(vv)) + 1)
+    bu, bv = rng.integers(0, n, 20000), rng.integers(0, n, 20000)
+    us.append(bu)
+    vs.append(bv)
+    ws.append(rng.integers(1, 256, 20000))
+    edges = (np.concatenate(us), np.concatenate(vs), np.concatenate(ws))
+    off, nbr, wgt = _multi_edge_csr(n, edges)
+    in_off, in_nbr, in_wgt = _multi_edge_csr(n, (edges[1], edges[0], edges[2]))
+    g = O.Graph(n=n, off=off, nbr=nbr, wgt=wgt, in_off=in_off, in_nbr=in_nbr, in_wgt=in_wgt, directed=True)
+    dg = ctx.upload_graph(g)
+    for mname, ids in _mappings(g).items():
+        print(f"relabel repeated neighbours {mname}", flush=True)
+        r, _ = ctx.relabel(dg, ids)
+        _assert_same_graph(r, O.relabel(g, ids))
+        r.free()
+
+
 def test_relabel_chain_composes_org_ids(ctx):
     g = O.kron(11)
     a, b = O.mapping_random(g.n), O.mapping_sort(g)
