@@ -322,6 +322,10 @@ int compact_flags(gbk_ctx *ctx, const int32_t *flags, int64_t n, int32_t *out, i
 // The reference's random mapping (SplitMix64 Fisher-Yates, reorder_basic.h:
 // 104-135) of n vertices into new_ids (device), gbk_shuffle.hip.
 int random_mapping(gbk_ctx *ctx, int64_t n, int32_t *new_ids);
+// Smallest and largest edge weight of a weighted graph, found once and kept
+// with the graph (g->wmin, g->wmax; 0 when unweighted or empty); 'partial'
+// holds 2048 entries of device scratch.
+int graph_weight_range(gbk_ctx *ctx, const gbk_graph *g, long long *partial);
 // Heavy-row list for a CSR.
 int build_heavy_list(gbk_ctx *ctx, const int64_t *off, int64_t n, int64_t threshold,
                      int32_t **list, int64_t *count, int64_t *max_degree,

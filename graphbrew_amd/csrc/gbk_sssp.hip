@@ -464,6 +464,30 @@ __global__ void weight_range(const int32_t *w, int64_t m, long long *partial) {
   }
 }
 
+int graph_weight_range(gbk_ctx *ctx, const gbk_graph *g, long long *partial) {
+  if (g->wmin >= 0) return GBK_OK;
+  constexpr unsigned kRangeGrid = 1024;
+  hipStream_t s = ctx->stream;
+  std::vector<long long> h(2 * kRangeGrid);
+  int64_t lo = 0, hi = 0;
+  if (g->m > 0 && g->wgt) {
+    weight_range<<<kRangeGrid, kBlock, 0, s>>>(g->wgt, g->m, partial);
+    GBK_LAUNCH_CHECK(ctx, "weight range");
+    GBK_HIP(ctx, hipMemcpyAsync(h.data(), partial, sizeof(long long) * h.size(), hipMemcpyDeviceToHost, s));
+    GBK_HIP(ctx, hipStreamSynchronize(s));
+    lo = LLONG_MAX;
+    hi = LLONG_MIN;
+    for (unsigned i = 0; i < kRangeGrid; i++) {
+      lo = std::min<int64_t>(lo, h[2 * i]);
+      hi = std::max<int64_t>(hi, h[2 * i + 1]);
+    }
+  }
+  auto *mg = const_cast<gbk_graph *>(g);
+  mg->wmin = std::max<int64_t>(lo, 0);
+  mg->wmax = std::max<int64_t>(hi, 0);
+  return GBK_OK;
+}
+
 // Smallest bucket above the current one that holds a reached vertex.
 __global__ void sssp_min_bucket(const unsigned long long *word, int64_t n, int64_t delta,
                                 const SsspState *st, unsigned long long *partial) {
@@ -582,29 +606,8 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
     return set_error(ctx, GBK_ENOMEM, "sssp scratch");
   hq.overflow = &st->overflow;
   int32_t *near0 = q, *near1 = q + cap, *settled = q + 2 * cap, *stamp = q + 3 * cap;
-  if (g->wmin < 0) {
-    // the graph's weight range, once: which edge classes a delta leaves
-    constexpr unsigned kRangeGrid = 1024;
-    static_assert(2 * kRangeGrid <= 2 * kGridCap, "weight range partials fit the partial buffer");
-    auto *rp = reinterpret_cast<long long *>(partial);
-    std::vector<long long> h(2 * kRangeGrid);
-    int64_t lo = 0, hi = 0;
-    if (g->m > 0) {
-      weight_range<<<kRangeGrid, kBlock, 0, s>>>(g->wgt, g->m, rp);
-      GBK_LAUNCH_CHECK(ctx, "sssp weight range");
-      GBK_HIP(ctx, hipMemcpyAsync(h.data(), rp, sizeof(long long) * h.size(), hipMemcpyDeviceToHost, s));
-      GBK_HIP(ctx, hipStreamSynchronize(s));
-      lo = LLONG_MAX;
-      hi = LLONG_MIN;
-      for (unsigned i = 0; i < kRangeGrid; i++) {
-        lo = std::min<int64_t>(lo, h[2 * i]);
-        hi = std::max<int64_t>(hi, h[2 * i + 1]);
-      }
-    }
-    auto *mg = const_cast<gbk_graph *>(g);
-    mg->wmin = std::max<int64_t>(lo, 0);
-    mg->wmax = std::max<int64_t>(hi, 0);
-  }
+  // the graph's weight range (found once): which edge classes a delta leaves
+  if (int rc = graph_weight_range(ctx, g, reinterpret_cast<long long *>(partial))) return rc;
   // heavy edges (weight >= delta) exist
   const int has_heavy = g->m > 0 && g->wmax >= delta;
   const int64_t H = ctx->tune.heavy_degree;
