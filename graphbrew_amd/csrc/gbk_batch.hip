@@ -584,7 +584,8 @@ struct GroupRelax {
         for (int j = 0; j < kRelaxUnroll; j++) {
           const int64_t cand = du + w[j];
           // judged against the loaded distance, as in walk()
-          const bool imp = j0 + j < cnt && act && cand < dv[j];
+          // (non-short-circuit: the loads stay unconditional, not sunk under act)
+          const bool imp = (j0 + j < cnt) & act & (cand < dv[j]);
           const uint64_t im = __ballot(imp);
           if (im == 0) continue;
           if (imp) atomicMin(&dist[static_cast<int64_t>(x[j]) * Kp + k], static_cast<int32_t>(cand));
