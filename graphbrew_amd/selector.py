@@ -108,6 +108,19 @@ def powerlaw(n, gamma, avg_degree, seed, permute=True):
     return _finish(stubs[0::2], stubs[1::2], n, rng, permute), n
 
 
+def kron_specs() -> List[dict]:
+    """The reference's own Kronecker generator (gapbs/generator.h, fixed
+    seed) at scales 19-23 and degrees 8/16/32, without the bench graph
+    (scale 22, degree 16): the family the bench graph comes from."""
+    out = []
+    for scale in (19, 20, 21, 22, 23):
+        for ef in (8, 16, 32):
+            if (scale, ef) == (22, 16) or (scale == 23 and ef != 16):
+                continue
+            out.append(dict(family="kron", scale=scale, ef=ef))
+    return out
+
+
 def corpus(count: int, seed0: int = 1000, min_scale: int = 12, max_scale: int = 16,
            skewed: bool = False, families: Sequence[str] = ("rmat", "sbm", "lattice", "powerlaw")) -> List[dict]:
     """Seeded graph specifications, cycling through the four families.
@@ -142,6 +155,9 @@ def corpus(count: int, seed0: int = 1000, min_scale: int = 12, max_scale: int = 
 
 def make_edges(spec: dict):
     f = spec["family"]
+    if f == "kron":
+        u, v = H.generate_el(spec["scale"], spec["ef"], False)
+        return (u, v), 1 << spec["scale"]
     if f == "rmat":
         return rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"], spec["seed"], spec["permute"])
     if f == "sbm":
@@ -239,6 +255,39 @@ def remap(ctx: gbk.Context, data: Sequence[dict], log=None) -> None:
         g.free()
         if log:
             log(f"remap {i + 1}/{len(data)} n={row['n']} m={row['m']}")
+
+
+def retime_relabel(ctx: gbk.Context, data: Sequence[dict], log=None) -> None:
+    """Rebuilds every stored graph on the device and, in place, replaces the
+    relabel times of each ordering (unweighted and weighted CSR, device
+    time) by those of the current relabel kernels; the mapping and kernel
+    times are kept (the kernels a row times are not the relabel)."""
+    for i, row in enumerate(data):
+        (u, v), n = make_edges(row["spec"])
+        w = ((np.arange(len(u), dtype=np.int64) * 2654435761) % 255 + 1).astype(np.int32)
+        g, _ = ctx.build(u, v, None, n, True)
+        gw, _ = ctx.build(u, v, w, n, True)
+        if g.n != row["n"] or g.m != row["m"]:
+            raise RuntimeError(f"graph {i} rebuilt with different size")
+        off = g.download()["off"]
+        for a in ORDERINGS:
+            if a == 0:
+                continue
+            ids = mapping_ms(ctx, g, off, a, repeats=1)[0]
+            rel, rel_w = [], []
+            for _ in range(3):
+                rg, st = ctx.relabel(g, ids)
+                rgw, st2 = ctx.relabel(gw, ids)
+                rel.append(st.device_ms)
+                rel_w.append(st2.device_ms)
+                rg.free()
+                rgw.free()
+            o = row["orderings"][str(a)]
+            o["relabel_ms"], o["relabel_w_ms"] = _median(rel), _median(rel_w)
+        g.free()
+        gw.free()
+        if log:
+            log(f"relabel {i + 1}/{len(data)} n={row['n']} m={row['m']}")
 
 
 def rule_decision(row: dict, kernel: str, reuse: int, llc: int = 0) -> bool:
@@ -438,26 +487,29 @@ def main(argv=None):
     import argparse
     import sys
     p = argparse.ArgumentParser(description="collect timings / train / evaluate the ordering selector")
-    p.add_argument("cmd", choices=["collect", "remeasure", "remap", "train", "evaluate"])
+    p.add_argument("cmd", choices=["collect", "remeasure", "remap", "relabel", "train", "evaluate"])
     p.add_argument("--graphs", type=int, default=96)
     p.add_argument("--min-scale", type=int, default=12)
     p.add_argument("--max-scale", type=int, default=15)
     p.add_argument("--seed0", type=int, default=1000)
     p.add_argument("--skewed", action="store_true", help="collect: R-MAT b = c near (1 - a) / 2")
-    p.add_argument("--families", default="rmat,sbm,lattice,powerlaw", help="collect: families to cycle through")
+    p.add_argument("--families", default="rmat,sbm,lattice,powerlaw",
+                   help="collect: families to cycle through, or 'kron' for the reference generator's graphs")
     p.add_argument("--corpus", default=os.path.join(os.path.dirname(MODEL_PATH), "selector_corpus.json"),
                    help="output of collect; for train / evaluate a comma-separated list to merge")
     p.add_argument("--model", default=MODEL_PATH)
     p.add_argument("--epochs", type=int, default=40)
-    p.add_argument("--start", type=int, default=0, help="remeasure: first graph of the corpus")
-    p.add_argument("--count", type=int, default=-1, help="remeasure: graphs (-1: to the end)")
-    p.add_argument("--out", default="", help="remeasure: output file")
+    p.add_argument("--start", type=int, default=0, help="remeasure / remap / relabel: first graph of the corpus")
+    p.add_argument("--count", type=int, default=-1, help="remeasure / remap / relabel: graphs (-1: to the end)")
+    p.add_argument("--out", default="", help="remeasure / remap / relabel: output file")
     a = p.parse_args(argv)
     ctx = gbk.Context(0)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
     if a.cmd == "collect":
-        collect(ctx, corpus(a.graphs, seed0=a.seed0, min_scale=a.min_scale, max_scale=a.max_scale,
-                            skewed=a.skewed, families=a.families.split(",")), a.corpus, log)
+        specs = kron_specs() if a.families == "kron" else corpus(
+            a.graphs, seed0=a.seed0, min_scale=a.min_scale, max_scale=a.max_scale, skewed=a.skewed,
+            families=a.families.split(","))
+        collect(ctx, specs, a.corpus, log)
         return 0
     if a.cmd == "remeasure":
         # the same graphs (their specs are stored with the timings) timed again
@@ -467,10 +519,11 @@ def main(argv=None):
         end = len(specs) if a.count < 0 else min(len(specs), a.start + a.count)
         collect(ctx, specs[a.start:end], a.out or a.corpus, log)
         return 0
-    if a.cmd == "remap":
+    if a.cmd in ("remap", "relabel"):
         with open(a.corpus) as f:
             data = json.load(f)
-        remap(ctx, data, log)
+        end = len(data) if a.count < 0 else min(len(data), a.start + a.count)
+        (remap if a.cmd == "remap" else retime_relabel)(ctx, data[a.start:end], log)
         with open(a.out or a.corpus, "w") as f:
             json.dump(data, f)
         return 0

@@ -212,7 +212,11 @@ def test_selector_corpus_is_complete_and_split_by_family():
         assert g["orderings"]["0"]["map_ms"] == 0 and g["orderings"]["0"]["relabel_ms"] == 0
     train, test = S.split(data)
     fams = lambda part: {g["spec"]["family"] for g in part}  # noqa: E731
-    assert fams(train) == fams(test) == {"rmat", "sbm", "lattice", "powerlaw"}
+    # the four seeded families, and the reference generator's Kronecker graphs
+    # at other scales and degrees than the bench graph's
+    assert fams(train) == fams(test) == {"rmat", "sbm", "lattice", "powerlaw", "kron"}
+    assert not any(g["spec"]["family"] == "kron" and (g["spec"]["scale"], g["spec"]["ef"]) == (22, 16)
+                   for g in data)
     assert len(test) == 4 * (len(data) // 16)
     assert len(train) + len(test) == len(data)
 
