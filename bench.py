@@ -72,33 +72,43 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-# Algorithmic HBM bytes (compulsory traffic; o = offset width = 8 B here).
+# Algorithmic HBM bytes (compulsory traffic).  o is the offset width of the
+# definition: 4 B while the graph has fewer than 2^31 edges, else 8 B (the
+# kernels read 8-byte offsets either way, so below 2^31 edges the figures do
+# not credit the wider reads).  Work sets it from the bench graph.
+OFF_W = 8
+
+
+def offset_width(m):
+    return 4 if m < 2 ** 31 else 8
+
+
 def bytes_relabel(n, m, weighted=False):
-    return 8 * m + (2 * 8 + 8) * n + (8 * m if weighted else 0)
+    return 8 * m + (2 * OFF_W + 8) * n + (8 * m if weighted else 0)
 
 
 def bytes_pr_iter(n, m):
-    return 4 * m + (8 + 16) * n
+    return 4 * m + (OFF_W + 16) * n
 
 
 def bytes_bfs(n, m_r):
-    return 4 * m_r + (8 + 8) * n + n / 4
+    return 4 * m_r + (OFF_W + 8) * n + n / 4
 
 
 def bytes_cc(n, m):
-    return 4 * m + (8 + 8) * n + 8 * n
+    return 4 * m + (OFF_W + 16) * n
 
 
 def bytes_sv_iter(n, m):
-    return 4 * m + (8 + 8) * n
+    return 4 * m + (OFF_W + 8) * n
 
 
 def bytes_sssp(n, m_r):
-    return 8 * m_r + (8 + 8) * n
+    return 8 * m_r + (OFF_W + 8) * n
 
 
 def bytes_bc(n, m_r):
-    return 8 * m_r + m_r / 4 + (2 * 8 + 40) * n
+    return 8 * m_r + m_r / 4 + (2 * OFF_W + 40) * n
 
 
 class Workload:
@@ -108,6 +118,8 @@ class Workload:
         u, v = H.generate_el(args.scale, args.degree, args.uniform)
         log(f"generated {len(u)} edges in {time.time() - t0:.1f}s")
         self.g, ms = ctx.build(u, v, None, -1, True)
+        global OFF_W
+        OFF_W = offset_width(self.g.m)
         w = H.insert_weights(len(u))
         self.gw, ms2 = ctx.build(u, v, w, self.g.n, True)
         del u, v, w
@@ -662,6 +674,7 @@ def main():
         "data": "synthetic (seeded Kronecker generator of the reference, weights U[1,255])",
         "config": {"workload": workload,
                    "n": n, "m_directed": m, "order": args.order, "mapping_fingerprint": work.fingerprint,
+                   "offset_width_bytes": OFF_W,
                    "delta": args.delta if args.single_source else args.batch_delta,
                    "batched": not args.single_source, "autotuned": tuned,
                    "params": {k: ctx.get_param(k) for k in ("heavy_degree", "bu_group", "bfs_alpha", "bfs_beta",
