@@ -95,12 +95,14 @@ cc_final_link(RowTiles rt, const int32_t *nbr, int64_t skip, int32_t c, int32_t 
 // vertex id -- are the reference's.
 // The labels are fully compressed between sweeps, so every label is a
 // root and the reference's root test always holds.  An edge offers its
-// higher label's root the lower one; on an undirected graph each edge is
-// seen from both rows, so it is taken from the row whose own label is the
-// higher: all of a row's offers then go to the same root and are reduced to
-// their minimum in registers, one atomic per row run.  A directed graph's
-// edge is seen from one row only, so its offers to the other end's root
-// are sent edge by edge.
+// higher label's root the lower one.  Each edge is read once: a directed
+// graph stores it in one row; an undirected graph stores it in both, and it
+// is taken from the row of its higher endpoint (neighbour id below the
+// row's), so a sweep gathers half the neighbour labels and makes the same
+// offers.  Offers to the row's own root (a neighbour's label below the
+// row's) are reduced to their minimum in registers, one atomic per row run
+// (in the first sweep, where every label is its vertex, all offers are of
+// that kind); offers to a neighbour's root are sent edge by edge.
 __device__ __forceinline__ int32_t seg_run_min(int32_t x, int32_t key, bool &tail) {
   const int lane = lane_id();
 #pragma unroll
@@ -129,8 +131,9 @@ struct HookEdge {
   // Heavy rows: every lane of the wave is on the same row.
   __device__ void operator()(int64_t u, int64_t k, bool active) {
     const int32_t cu = comp[u];
-    const int32_t cv = active ? comp[nbr[k]] : cu;
-    if (directed && cv > cu) offer(cv, cu);
+    const int32_t v = active ? nbr[k] : 0;
+    const int32_t cv = active && (directed || v < u) ? comp[v] : cu;
+    if (cv > cu) offer(cv, cu);
     const int32_t m = wave_min(cv < cu ? cv : INT32_MAX);
     if (lane_id() == 0 && m < cu) offer(cu, m);
   }
@@ -143,11 +146,11 @@ struct HookEdge {
 #pragma unroll
     for (int j = 0; j < U; j++) {
       cu[j] = comp[u[j]];
-      cv[j] = act[j] ? comp[v[j]] : cu[j];
+      cv[j] = act[j] && (directed || v[j] < u[j]) ? comp[v[j]] : cu[j];
     }
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      if (directed && cv[j] > cu[j]) offer(cv[j], cu[j]);
+      if (cv[j] > cu[j]) offer(cv[j], cu[j]);
       bool tail;
       const int32_t m = seg_run_min(cv[j] < cu[j] ? cv[j] : INT32_MAX, static_cast<int32_t>(u[j]), tail);
       if (tail && m < cu[j]) offer(cu[j], m);
