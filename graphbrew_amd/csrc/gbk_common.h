@@ -280,6 +280,15 @@ struct gbk_ctx {
   // resident workgroups per kernel (occupancy x compute units), cached
   int cu_count = 0;
   std::unordered_map<const void *, unsigned> resident;
+  // Graph arrays (gbk_graph_alloc) of freed graphs, kept for the next graphs
+  // of about the same size instead of hipFree / hipMalloc (0.5-1 GB arrays
+  // took 1-3 ms of host time per relabel call); released with the context.
+  struct PoolBlock {
+    void *p;
+    size_t bytes;
+  };
+  std::vector<PoolBlock> gpool;
+  std::unordered_map<void *, size_t> gsize;  // live and cached graph arrays
 };
 
 namespace gbk {
@@ -322,6 +331,13 @@ int compact_flags(gbk_ctx *ctx, const int32_t *flags, int64_t n, int32_t *out, i
 // The reference's random mapping (SplitMix64 Fisher-Yates, reorder_basic.h:
 // 104-135) of n vertices into new_ids (device), gbk_shuffle.hip.
 int random_mapping(gbk_ctx *ctx, int64_t n, int32_t *new_ids);
+// Graph arrays from the context's cache of freed graph arrays (best fit
+// within 1/8 above the request) or hipMalloc; graph_mem_free returns them to
+// the cache (pointers not from graph_mem_alloc are hipFree'd).  Work on the
+// context's stream is ordered, so a cached array is reused only after the
+// work of its former graph.
+void *graph_mem_alloc(gbk_ctx *ctx, size_t bytes);
+void graph_mem_free(gbk_ctx *ctx, void *p);
 // Smallest and largest edge weight of a weighted graph, found once and kept
 // with the graph (g->wmin, g->wmax; 0 when unweighted or empty); 'partial'
 // holds 2048 entries of device scratch.

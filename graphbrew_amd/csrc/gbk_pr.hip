@@ -902,14 +902,14 @@ extern "C" int gbk_pr(gbk_ctx *ctx, const gbk_graph *g, int variant, int max_ite
     if (mg->seg_bytes < need) {
       if (mg->seg_mem) {
         GBK_HIP(ctx, hipStreamSynchronize(s));
-        (void)hipFree(mg->seg_mem);
+        graph_mem_free(ctx, mg->seg_mem);
         mg->owned.erase(std::find(mg->owned.begin(), mg->owned.end(), mg->seg_mem));
         mg->seg_mem = nullptr;
         mg->seg_bytes = 0;
         mg->seg_lw = 0;
       }
-      void *mem = nullptr;
-      if (hipMalloc(&mem, need) != hipSuccess) return set_error(ctx, GBK_ENOMEM, "pr segment layout");
+      void *mem = graph_mem_alloc(ctx, need);
+      if (!mem) return set_error(ctx, GBK_ENOMEM, "pr segment layout");
       mg->owned.push_back(mem);
       mg->seg_mem = mem;
       mg->seg_bytes = need;

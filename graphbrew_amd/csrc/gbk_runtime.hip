@@ -18,6 +18,16 @@ int check_hip(gbk_ctx *ctx, hipError_t e, const char *what) {
   return set_error(ctx, code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Frees the cached arrays of freed graphs (graph_mem_alloc's cache).
+static void drop_graph_cache(gbk_ctx *ctx) {
+  hipStreamSynchronize(ctx->stream);
+  for (auto &b : ctx->gpool) {
+    ctx->gsize.erase(b.p);
+    hipFree(b.p);
+  }
+  ctx->gpool.clear();
+}
+
 void *scratch(gbk_ctx *ctx, int slot, size_t bytes) {
   auto &s = ctx->slot[slot];
   if (bytes == 0) bytes = 1;
@@ -31,11 +41,65 @@ void *scratch(gbk_ctx *ctx, int slot, size_t bytes) {
   // Round up so small growth steps do not reallocate every call.
   size_t want = bytes + bytes / 8 + 4096;
   if (hipMalloc(&s.p, want) != hipSuccess) {
-    s.p = nullptr;
-    return nullptr;
+    // the cache of freed graph arrays may hold the memory: release it once
+    (void)hipGetLastError();
+    drop_graph_cache(ctx);
+    if (hipMalloc(&s.p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      s.p = nullptr;
+      return nullptr;
+    }
   }
   s.bytes = want;
   return s.p;
+}
+
+constexpr size_t kPoolMaxBlocks = 24;
+constexpr size_t kPoolMaxBytes = size_t{64} << 30;
+
+void *graph_mem_alloc(gbk_ctx *ctx, size_t bytes) {
+  if (bytes == 0) bytes = 4;
+  size_t best = ctx->gpool.size();
+  for (size_t i = 0; i < ctx->gpool.size(); i++) {
+    const size_t b = ctx->gpool[i].bytes;
+    if (b >= bytes && b <= bytes + bytes / 8 + (size_t{1} << 20) &&
+        (best == ctx->gpool.size() || b < ctx->gpool[best].bytes))
+      best = i;
+  }
+  if (best < ctx->gpool.size()) {
+    void *p = ctx->gpool[best].p;
+    ctx->gpool.erase(ctx->gpool.begin() + static_cast<std::ptrdiff_t>(best));
+    return p;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    // the cache may hold the memory: give it back and try once more
+    (void)hipGetLastError();
+    drop_graph_cache(ctx);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  ctx->gsize[p] = bytes;
+  return p;
+}
+
+void graph_mem_free(gbk_ctx *ctx, void *p) {
+  const auto it = ctx->gsize.find(p);
+  if (it == ctx->gsize.end()) {
+    hipFree(p);
+    return;
+  }
+  ctx->gpool.push_back({p, it->second});
+  size_t total = 0;
+  for (const auto &b : ctx->gpool) total += b.bytes;
+  while (!ctx->gpool.empty() && (ctx->gpool.size() > kPoolMaxBlocks || total > kPoolMaxBytes)) {
+    total -= ctx->gpool.front().bytes;
+    ctx->gsize.erase(ctx->gpool.front().p);
+    hipFree(ctx->gpool.front().p);
+    ctx->gpool.erase(ctx->gpool.begin());
+  }
 }
 
 unsigned resident_grid(gbk_ctx *ctx, const void *kernel) {
@@ -377,6 +441,7 @@ int gbk_ctx_destroy(gbk_ctx *ctx) {
   hipStreamSynchronize(ctx->stream);
   for (auto &s : ctx->slot)
     if (s.p) hipFree(s.p);
+  for (auto &b : ctx->gpool) hipFree(b.p);
   if (ctx->pinned) hipHostFree(ctx->pinned);
   if (ctx->kt_dev) hipFree(ctx->kt_dev);
   for (hipEvent_t e : ctx->kt_ev) hipEventDestroy(e);
@@ -493,7 +558,7 @@ int gbk_graph_info(const gbk_graph *g, int64_t *n, int64_t *m, int *directed, in
 int gbk_graph_free(gbk_ctx *ctx, gbk_graph *g) {
   if (!ctx || !g) return GBK_EINVAL;
   hipStreamSynchronize(ctx->stream);
-  for (void *p : g->owned) hipFree(p);
+  for (void *p : g->owned) graph_mem_free(ctx, p);
   delete g;
   return GBK_OK;
 }
@@ -537,8 +602,8 @@ int gbk_graph_alloc(gbk_ctx *ctx, int64_t n, int64_t m, int64_t m_in, int direct
   g->directed = directed;
   g->weighted = weighted;
   auto alloc = [&](void **p, size_t bytes) {
-    if (bytes == 0) bytes = 4;
-    if (hipMalloc(p, bytes) != hipSuccess) return false;
+    *p = graph_mem_alloc(ctx, bytes);
+    if (!*p) return false;
     g->owned.push_back(*p);
     return true;
   };
@@ -556,7 +621,7 @@ int gbk_graph_alloc(gbk_ctx *ctx, int64_t n, int64_t m, int64_t m_in, int direct
     g->in_wgt = g->wgt;
   }
   if (!ok) {
-    for (void *p : g->owned) hipFree(p);
+    for (void *p : g->owned) graph_mem_free(ctx, p);
     delete g;
     return set_error(ctx, GBK_ENOMEM, "graph allocation");
   }

@@ -620,13 +620,12 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
     EventTimer layout_timer(ctx);
     auto *mg = const_cast<gbk_graph *>(g);
     if (!mg->pnbr) {
-      void *a = nullptr, *b = nullptr, *c = nullptr;
-      if (hipMalloc(&a, sizeof(int32_t) * std::max<int64_t>(g->m, 1)) != hipSuccess ||
-          hipMalloc(&b, sizeof(int32_t) * std::max<int64_t>(g->m, 1)) != hipSuccess ||
-          hipMalloc(&c, sizeof(int32_t) * std::max<int64_t>(n, 1)) != hipSuccess) {
-        hipFree(a);
-        hipFree(b);
-        hipFree(c);
+      void *a = graph_mem_alloc(ctx, sizeof(int32_t) * std::max<int64_t>(g->m, 1));
+      void *b = a ? graph_mem_alloc(ctx, sizeof(int32_t) * std::max<int64_t>(g->m, 1)) : nullptr;
+      void *c = b ? graph_mem_alloc(ctx, sizeof(int32_t) * std::max<int64_t>(n, 1)) : nullptr;
+      if (!c) {
+        if (a) graph_mem_free(ctx, a);
+        if (b) graph_mem_free(ctx, b);
         return set_error(ctx, GBK_ENOMEM, "sssp light/heavy layout");
       }
       mg->owned.push_back(a);
