@@ -257,11 +257,11 @@ def remap(ctx: gbk.Context, data: Sequence[dict], log=None) -> None:
             log(f"remap {i + 1}/{len(data)} n={row['n']} m={row['m']}")
 
 
-def retime_relabel(ctx: gbk.Context, data: Sequence[dict], log=None) -> None:
+def retime_relabel(ctx: gbk.Context, data: Sequence[dict], log=None, kernels: Sequence[str] = ()) -> None:
     """Rebuilds every stored graph on the device and, in place, replaces the
     relabel times of each ordering (unweighted and weighted CSR, device
-    time) by those of the current relabel kernels; the mapping and kernel
-    times are kept (the kernels a row times are not the relabel)."""
+    time) by those of the current relabel kernels, and the times of the
+    listed kernels by the current ones; the other times are kept."""
     for i, row in enumerate(data):
         (u, v), n = make_edges(row["spec"])
         w = ((np.arange(len(u), dtype=np.int64) * 2654435761) % 255 + 1).astype(np.int32)
@@ -270,20 +270,28 @@ def retime_relabel(ctx: gbk.Context, data: Sequence[dict], log=None) -> None:
         if g.n != row["n"] or g.m != row["m"]:
             raise RuntimeError(f"graph {i} rebuilt with different size")
         off = g.download()["off"]
+        src_orig = int(H.source_picker(off, 1)[0][0])
         for a in ORDERINGS:
+            o = row["orderings"][str(a)]
             if a == 0:
+                for k in kernels:
+                    o["kernel_ms"][k] = kernel_ms(ctx, k, g, gw, src_orig)
                 continue
             ids = mapping_ms(ctx, g, off, a, repeats=1)[0]
             rel, rel_w = [], []
-            for _ in range(3):
+            for rep in range(3):
                 rg, st = ctx.relabel(g, ids)
                 rgw, st2 = ctx.relabel(gw, ids)
                 rel.append(st.device_ms)
                 rel_w.append(st2.device_ms)
-                rg.free()
-                rgw.free()
-            o = row["orderings"][str(a)]
+                if rep < 2:  # the last relabelled pair is kept for the kernels
+                    rg.free()
+                    rgw.free()
             o["relabel_ms"], o["relabel_w_ms"] = _median(rel), _median(rel_w)
+            for k in kernels:
+                o["kernel_ms"][k] = kernel_ms(ctx, k, rg, rgw, int(ids[src_orig]))
+            rg.free()
+            rgw.free()
         g.free()
         gw.free()
         if log:
@@ -502,6 +510,7 @@ def main(argv=None):
     p.add_argument("--start", type=int, default=0, help="remeasure / remap / relabel: first graph of the corpus")
     p.add_argument("--count", type=int, default=-1, help="remeasure / remap / relabel: graphs (-1: to the end)")
     p.add_argument("--out", default="", help="remeasure / remap / relabel: output file")
+    p.add_argument("--kernels", default="", help="relabel: kernels to re-time as well (comma list)")
     a = p.parse_args(argv)
     ctx = gbk.Context(0)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
@@ -523,7 +532,10 @@ def main(argv=None):
         with open(a.corpus) as f:
             data = json.load(f)
         end = len(data) if a.count < 0 else min(len(data), a.start + a.count)
-        (remap if a.cmd == "remap" else retime_relabel)(ctx, data[a.start:end], log)
+        if a.cmd == "remap":
+            remap(ctx, data[a.start:end], log)
+        else:
+            retime_relabel(ctx, data[a.start:end], log, [k for k in a.kernels.split(",") if k])
         with open(a.out or a.corpus, "w") as f:
             json.dump(data, f)
         return 0
