@@ -46,6 +46,11 @@ struct Tuning {
                                 // frontier has more out-edges than the unlisted vertices, 2 bottom-up
   int bc_pull_ratio = 1;        // a frontier edge weighs this many open edges in that choice
   int sssp_wide = 1;            // SSSP batches of 17..32 sources on 64-lane rows
+  int64_t sssp_heavy = 512;     // one-source SSSP: rows with more edges to walk go to the
+                                // workgroup-wide heavy kernel
+  int sssp_pull = 1;            // one-source SSSP heavy pass: 0 pushed from the bucket's vertices,
+                                // 1 pulled into the unsettled vertices when that reads fewer
+                                // edges, 2 always pulled
   int msbfs_group = 32;         // lanes per vertex in the BFS batch's bottom-up step (1..64)
   int pr_segment = 0;           // PageRank segmented sweep: 0 by size, -1 never, k > 0 smallest
                                 // segment width 2^k tried (6..26)
@@ -289,6 +294,11 @@ struct gbk_ctx {
   };
   std::vector<PoolBlock> gpool;
   std::unordered_map<void *, size_t> gsize;  // live and cached graph arrays
+  // Steps the last one-source SSSP call took, with its graph and delta: the
+  // next call on them queues that many before its first look at the state.
+  const gbk_graph *sssp_hint_graph = nullptr;
+  int64_t sssp_hint_delta = 0;
+  int64_t sssp_hint_steps = 0;
 };
 
 namespace gbk {

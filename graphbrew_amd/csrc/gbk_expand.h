@@ -183,10 +183,11 @@ constexpr int64_t kHeavyChunk = 1024;
 constexpr unsigned kHeavyGrid = 512;
 
 // Device storage for a HeavyQueue in scratch slot 'slot': at most
-// m / heavy_deg rows can exceed heavy_deg.  Returns false on allocation
-// failure.  The count and overflow words are cleared on the stream.
-inline bool make_heavy_queue(gbk_ctx *ctx, const gbk_graph *g, int slot, HeavyQueue *hq) {
-  const int64_t H = ctx->tune.heavy_degree;
+// m / heavy_deg rows can exceed heavy_deg (the context's heavy degree unless
+// 'heavy_deg' is given).  Returns false on allocation failure.  The count and
+// overflow words are cleared on the stream.
+inline bool make_heavy_queue(gbk_ctx *ctx, const gbk_graph *g, int slot, HeavyQueue *hq, int64_t heavy_deg = 0) {
+  const int64_t H = heavy_deg > 0 ? heavy_deg : ctx->tune.heavy_degree;
   const int64_t cap = std::min<int64_t>(g->n, g->m / (H > 0 ? H : 1) + 1) + 1;
   // a level's chunk total is counted in 32 bits (chunks are >= 64 edges)
   if (g->m / 64 + cap >= (int64_t{1} << 32)) return false;

@@ -483,6 +483,8 @@ int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value) {
   else if (k == "bc_pull" && value >= 0 && value <= 2) t.bc_pull = static_cast<int>(value);
   else if (k == "bc_pull_ratio" && value >= 1 && value <= 1024) t.bc_pull_ratio = static_cast<int>(value);
   else if (k == "sssp_wide" && (value == 0 || value == 1)) t.sssp_wide = static_cast<int>(value);
+  else if (k == "sssp_pull" && value >= 0 && value <= 2) t.sssp_pull = static_cast<int>(value);
+  else if (k == "sssp_heavy" && value >= 64) t.sssp_heavy = value;
   else if (k == "msbfs_group" && value >= 1 && value <= 64 && (value & (value - 1)) == 0)
     t.msbfs_group = static_cast<int>(value);
   else if (k == "pr_segment" && (value == -1 || value == 0 || (value >= 6 && value <= 26)))
@@ -520,6 +522,8 @@ int gbk_ctx_get_param(gbk_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "bc_pull") *value = t.bc_pull;
   else if (k == "bc_pull_ratio") *value = t.bc_pull_ratio;
   else if (k == "sssp_wide") *value = t.sssp_wide;
+  else if (k == "sssp_pull") *value = t.sssp_pull;
+  else if (k == "sssp_heavy") *value = t.sssp_heavy;
   else if (k == "msbfs_group") *value = t.msbfs_group;
   else if (k == "pr_segment") *value = t.pr_segment;
   else if (k == "ktime") *value = ctx->ktime;
@@ -559,6 +563,7 @@ int gbk_graph_free(gbk_ctx *ctx, gbk_graph *g) {
   if (!ctx || !g) return GBK_EINVAL;
   hipStreamSynchronize(ctx->stream);
   for (void *p : g->owned) graph_mem_free(ctx, p);
+  if (ctx->sssp_hint_graph == g) ctx->sssp_hint_graph = nullptr;
   delete g;
   return GBK_OK;
 }

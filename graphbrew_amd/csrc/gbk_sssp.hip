@@ -37,13 +37,15 @@
 // Each lane relaxes four edges per trip (independent loads and atomics in
 // flight together): the path is bound by the latency of dependent random
 // accesses, not by bandwidth.
-// Kernels of one step: relax (a light round in phase 0, the heavy pass in
-// phase 3), then min-bucket and split (each acting only in its phase), each
-// followed by a one-workgroup kernel that advances the device state block.  The host queues steps without waiting and checks the
+// Kernels of one step: relax and relax-heavy (a light round in phase 0, the
+// heavy pass pushed in phase 3 or pulled in phase 4), then min-bucket and
+// split (each acting only in its phase); the last workgroup of each kernel
+// advances the device state block.  The host queues steps without waiting and checks the
 // 'done' flag after 2, 4, ... 64 steps.  Distances are integers, so the
 // result is exact whatever the order of relaxations.
 #include <algorithm>
 #include <climits>
+#include <cstddef>
 #include <vector>
 
 #include "gbk_expand.h"
@@ -65,15 +67,47 @@ struct SsspState {
   long long bucket;
   long long relax_rounds;
   long long buckets;
+  long long steps;  // steps that found the search still open
   int cur, phase, done, overflow;
 };
 
+// Device scratch beside the state that every workgroup of a launch updates.
+// Same-address atomics from thousands of workgroups serialize (about 11 ns
+// each on this GPU), so such counters are spread over kSpread slots 256
+// bytes apart (slot = workgroup id mod kSpread) and combined by one
+// workgroup, and per-bucket edge counts are per-workgroup partials.
+constexpr int kSpread = 32;
+constexpr int kSlotWords = 64;  // 32-bit words per 256-byte slot
+struct SsspAux {
+  // arrivals at the end of the relax-heavy, min and split kernels: a counter
+  // per slot group, then one for the groups; the last arrival closes the phase
+  unsigned int arrive[3][kSpread * kSlotWords];
+  unsigned int arrive_top[3][kSlotWords];
+  // smallest distance above the current bucket found by the pass that looks
+  // for the next bucket (ULLONG_MAX: none), slot s at next_min[s * 32]
+  unsigned long long next_min[kSpread * kSlotWords / 2];
+  // per relax workgroup: heavy edges of the vertices the current bucket
+  // listed (a push reads these) and in-edges of every vertex listed so far
+  // (a pull reads the in-edges of the others)
+  unsigned long long push_part[kGridCapRuntime];
+  unsigned long long in_part[kGridCapRuntime];
+};
+
 __global__ void sssp_init(int64_t n, int32_t source, unsigned long long *word, int32_t *stamp, int32_t *near0,
-                          SsspState *st) {
+                          SsspState *st, SsspAux *aux) {
   const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v < n) {
     word[v] = make_word(v == source ? 0 : kDistInf, 0);
     stamp[v] = -1;
+  }
+  {
+    // counters and partials 0, the spread minimum all ones (ULLONG_MAX)
+    unsigned int *w = reinterpret_cast<unsigned int *>(aux);
+    const int64_t words = sizeof(SsspAux) / sizeof(unsigned int);
+    const int64_t nm0 = offsetof(SsspAux, next_min) / sizeof(unsigned int);
+    const int64_t nm1 = nm0 + sizeof(aux->next_min) / sizeof(unsigned int);
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = v; i < words; i += stride) w[i] = i >= nm0 && i < nm1 ? 0xffffffffu : 0u;
   }
   if (v == 0) {
     near0[0] = source;
@@ -84,6 +118,7 @@ __global__ void sssp_init(int64_t n, int32_t source, unsigned long long *word, i
     st->bucket = 0;
     st->relax_rounds = 0;
     st->buckets = 0;
+    st->steps = 0;
     st->cur = 0;
     st->phase = 0;
     st->done = 0;
@@ -136,6 +171,114 @@ struct RelaxEdge {
   }
 };
 
+// The in-CSR (the out-CSR itself for undirected graphs) and its row count.
+struct SsspIn {
+  const int64_t *off;
+  const int32_t *nbr;
+  const int32_t *wgt;
+  int64_t n;
+};
+
+// The heavy pass as a pull: each unsettled vertex v takes the minimum of
+// dist[u] + w over all its in-edges (u, v).  Every distance is the length of
+// a path, so relaxing from any vertex is safe; the edges out of the settled
+// bucket are among those read, so the pass does what the push does.
+// Candidates are at least the bucket's upper end, so nothing is queued.
+struct PullEdge {
+  const int32_t *nbr;  // in-CSR
+  const int32_t *wgt;
+  unsigned long long *word;
+  int64_t best;        // smallest finite candidate or unsettled distance seen
+
+  template <int U>
+  __device__ __forceinline__ void batch(const int64_t (&v)[U], const int64_t (&k)[U], const bool (&act)[U]) {
+    int32_t u[U];
+    int64_t cand[U], dv[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      u[j] = act[j] ? nbr[k[j]] : 0;
+      cand[j] = act[j] ? wgt[k[j]] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      cand[j] += act[j] ? word_dist(word[u[j]]) : kDistInf;
+      dv[j] = act[j] ? word_dist(word[v[j]]) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      if (act[j] && cand[j] < dv[j]) atomicMin(&word[v[j]], make_word(cand[j], 0u));
+      if (act[j] && cand[j] < best) best = cand[j];
+    }
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ T agent(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// True in every thread of the last workgroup of the launch to arrive (the
+// same answer in all threads of a workgroup).  Each thread first waits for its
+// own memory operations (the barrier alone does not wait for atomics that
+// return nothing), so every workgroup's atomics are performed before its
+// arrival counts; the closing workgroup reads such values with agent-scope
+// atomic loads.  What the workgroups stored plainly is read only by later
+// kernels, so no release fence (a write-back of the L2 on this GPU) is
+// needed.  The counters are left at 0.
+__device__ __forceinline__ bool last_to_arrive(unsigned int *groups, unsigned int *top) {
+  __shared__ bool last;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned g = blockIdx.x % kSpread;
+    const unsigned members = (gridDim.x - 1 - g) / kSpread + 1;
+    const unsigned ngroups = gridDim.x < kSpread ? gridDim.x : kSpread;
+    bool l = false;
+    if (atomicAdd(&groups[g * kSlotWords], 1u) == members - 1) {
+      atomicExch(&groups[g * kSlotWords], 0u);
+      if (atomicAdd(top, 1u) == ngroups - 1) {
+        atomicExch(top, 0u);
+        l = true;
+      }
+    }
+    last = l;
+  }
+  __syncthreads();
+  return last;
+}
+
+// Workgroup minimum of 'x' into this workgroup's slot of a spread minimum
+// (one atomic per workgroup, slots 256 bytes apart); every thread calls it.
+__device__ __forceinline__ void spread_min(int64_t x, unsigned long long *slots) {
+  __shared__ int64_t red[kWavesPerBlock];
+  x = wave_min(x);
+  __syncthreads();
+  if (lane_id() == 0) red[threadIdx.x / kWave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kWavesPerBlock; i++) x = min(x, red[i]);
+    if (x < kDistInf)
+      atomicMin(&slots[(blockIdx.x % kSpread) * (kSlotWords / 2)], static_cast<unsigned long long>(x));
+  }
+}
+
+// Minimum over the slots, which are reset to ULLONG_MAX; called by every
+// thread of one workgroup, returns the same value in all of them.
+__device__ __forceinline__ unsigned long long spread_take_min(unsigned long long *slots) {
+  __shared__ unsigned long long res;
+  if (threadIdx.x < kWave) {
+    unsigned long long x = ULLONG_MAX;
+    if (threadIdx.x < kSpread) {
+      x = agent(&slots[threadIdx.x * (kSlotWords / 2)]);
+      atomicExch(&slots[threadIdx.x * (kSlotWords / 2)], ULLONG_MAX);
+    }
+    x = wave_min(x);
+    if (threadIdx.x == 0) res = x;
+  }
+  __syncthreads();
+  return res;
+}
+
 __device__ __forceinline__ void relax_setup(RelaxEdge &f, int32_t *seg, const int32_t *nbr,
                                             const int32_t *wgt, unsigned long long *word,
                                             int32_t *near0, int32_t *near1, int64_t cap,
@@ -160,11 +303,35 @@ __device__ __forceinline__ void relax_setup(RelaxEdge &f, int32_t *seg, const in
 // walks a row's light edges, the heavy pass its heavy ones.  Without heavy
 // edges in the graph no vertex is listed.
 __global__ void __launch_bounds__(kBlock)
-sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, const int32_t *lc,
+sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, const int32_t *lc, SsspIn in,
            unsigned long long *word, int32_t *near0, int32_t *near1, int32_t *settled, int32_t *stamp,
-           int64_t cap, int64_t delta, int64_t heavy_deg, int has_heavy, SsspState *st, HeavyQueue hq) {
+           int64_t cap, int64_t delta, int64_t heavy_deg, int has_heavy, int pull_mode, SsspState *st,
+           SsspAux *aux, HeavyQueue hq) {
   const volatile SsspState *vs = st;
-  if (vs->done || (vs->phase != 0 && vs->phase != 3)) return;
+  const int phase = vs->phase;
+  if (vs->done || (phase != 0 && phase != 3 && phase != 4)) return;
+  if (phase == 4) {
+    // the pull: rows are the vertices above the bucket, in id order
+    const int64_t upper = delta * (vs->bucket + 1);
+    PullEdge pf{in.nbr, in.wgt, word, kDistInf};
+    for (int64_t tile = blockIdx.x; tile * kBlock < in.n; tile += gridDim.x) {
+      const int64_t v = tile * kBlock + threadIdx.x;
+      int64_t start = 0, len = 0;
+      const int64_t dv = v < in.n ? word_dist(word[v]) : 0;
+      if (dv >= upper) {
+        pf.best = min(pf.best, dv);
+        start = in.off[v];
+        len = in.off[v + 1] - start;
+        if (len > heavy_deg) {
+          heavy_push(hq, static_cast<int32_t>(v), start, len);
+          len = 0;
+        }
+      }
+      wave_walk_batch<kRelaxBatch>(v, start, len, pf);
+    }
+    spread_min(pf.best, aux->next_min);
+    return;
+  }
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   __shared__ int32_t sseg[kWavesPerBlock * kSeg];
   const int cur = vs->cur;
@@ -178,6 +345,7 @@ sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, const int
   sq.init(sseg + (threadIdx.x / kWave) * kSeg, &st->settled_cnt, settled, cap, &st->overflow);
   const int64_t lower = delta * f.bucket;
   const int32_t bucket = static_cast<int32_t>(f.bucket);
+  unsigned long long push_edges = 0, settled_in = 0;
   for (int64_t tile = blockIdx.x; tile * kBlock < count; tile += gridDim.x) {
     const int64_t i = tile * kBlock + threadIdx.x;
     int64_t row = 0, start = 0, len = 0;
@@ -193,6 +361,10 @@ sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, const int
         start = off[row];
         len = nl;
         if (has_heavy && deg > nl) first = atomicExch(&stamp[row], bucket) != bucket;
+        if (first && pull_mode == 1) {
+          push_edges += static_cast<unsigned long long>(deg - nl);
+          settled_in += static_cast<unsigned long long>(in.off[row + 1] - in.off[row]);
+        }
       }
       if (len > heavy_deg) {
         heavy_push(hq, static_cast<int32_t>(row), start, len);
@@ -204,30 +376,75 @@ sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, const int
   }
   f.nearq.flush_block();
   sq.flush_block();
+  if (pull_mode == 1) {
+    __shared__ unsigned long long red[2][kWavesPerBlock];
+    push_edges = wave_sum(push_edges);
+    settled_in = wave_sum(settled_in);
+    if (lane_id() == 0) {
+      red[0][threadIdx.x / kWave] = push_edges;
+      red[1][threadIdx.x / kWave] = settled_in;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 1; i < kWavesPerBlock; i++) {
+        push_edges += red[0][i];
+        settled_in += red[1][i];
+      }
+      if (push_edges) {
+        aux->push_part[blockIdx.x] += push_edges;
+        aux->in_part[blockIdx.x] += settled_in;
+      }
+    }
+  }
 }
 
 // Heavy rows of the round's list, spread over all workgroups.
+__device__ void relax_close(SsspState *st, SsspAux *aux, int phase, int cur, unsigned long long *heavy_count,
+                            int has_heavy, int pull_mode, int relax_blocks, int64_t n, int64_t m_in, int64_t delta);
+
+// The last workgroup to finish closes the round (relax_close).
 __global__ void __launch_bounds__(kBlock)
-sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, unsigned long long *word, int32_t *near0,
-                 int32_t *near1, int64_t cap, int64_t delta, SsspState *st, HeavyQueue hq) {
+sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, SsspIn in, unsigned long long *word, int32_t *near0,
+                 int32_t *near1, int64_t cap, int64_t delta, int has_heavy, int pull_mode, int relax_blocks,
+                 int64_t m_in, SsspState *st, SsspAux *aux, HeavyQueue hq) {
   const volatile SsspState *vs = st;
-  if (vs->done || (vs->phase != 0 && vs->phase != 3)) return;
-  __shared__ int32_t seg[kWavesPerBlock * kSeg];
-  RelaxEdge f;
-  relax_setup(f, seg, nbr, wgt, word, near0, near1, cap, delta, st);
-  expand_heavy_batch<kRelaxBatch>(hq, f);
-  f.nearq.flush_block();
+  const int phase = vs->phase;
+  const int cur = vs->cur;
+  if (vs->done || (phase != 0 && phase != 3 && phase != 4)) return;
+  if (phase == 4) {
+    PullEdge pf{in.nbr, in.wgt, word, kDistInf};
+    expand_heavy_batch<kRelaxBatch>(hq, pf);
+    spread_min(pf.best, aux->next_min);
+  } else {
+    __shared__ int32_t seg[kWavesPerBlock * kSeg];
+    RelaxEdge f;
+    relax_setup(f, seg, nbr, wgt, word, near0, near1, cap, delta, st);
+    expand_heavy_batch<kRelaxBatch>(hq, f);
+    f.nearq.flush_block();
+  }
+  if (last_to_arrive(aux->arrive[0], aux->arrive_top[0]))
+    relax_close(st, aux, phase, cur, hq.count, has_heavy, pull_mode, relax_blocks, in.n, m_in, delta);
 }
 
 // Work of the coming relaxation round, for the kernel timer only: rows of
 // the near queue that are not stale and their edges, as per-workgroup
 // partials (rows in partial[0, grid), edges in partial[grid, 2 grid)).
-__global__ void sssp_count_round(const int64_t *off, const int32_t *lc, const unsigned long long *word,
+__global__ void sssp_count_round(const int64_t *off, const int32_t *lc, SsspIn in, const unsigned long long *word,
                                  const int32_t *near0, const int32_t *near1, const int32_t *settled,
                                  int64_t delta, const SsspState *st, unsigned long long *partial) {
   const volatile SsspState *vs = st;
   unsigned long long rows = 0, edges = 0;
-  if (!vs->done && (vs->phase == 0 || vs->phase == 3)) {
+  if (!vs->done && vs->phase == 4) {
+    const int64_t upper = delta * (vs->bucket + 1);
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < in.n; v += stride) {
+      const int64_t deg = in.off[v + 1] - in.off[v];
+      if (deg > 0 && word_dist(word[v]) >= upper) {
+        rows++;
+        edges += static_cast<unsigned long long>(deg);
+      }
+    }
+  } else if (!vs->done && (vs->phase == 0 || vs->phase == 3)) {
     const int cur = vs->cur;
     const bool heavy = vs->phase == 3;
     const int32_t *q = heavy ? settled : (cur ? near1 : near0);
@@ -247,24 +464,79 @@ __global__ void sssp_count_round(const int64_t *off, const int32_t *lc, const un
   block_partial(edges, partial + gridDim.x);
 }
 
-// One workgroup: closes a relaxation round (swap the near queues; when the
-// next one is empty the bucket is done and the search for the next opens).
-// After the heavy pass the search for the next bucket opens.
-__global__ void sssp_relax_done(SsspState *st, unsigned long long *heavy_count, int has_heavy) {
-  if (threadIdx.x != 0 || st->done) return;
-  if (st->phase == 3) {
+// Closes a relaxation round (one thread, after every workgroup of the
+// round's kernels): swap the near queues; when the next one is empty the
+// bucket is done and its heavy pass opens.  The heavy pass is pulled
+// (phase 4) rather than pushed (phase 3) when the in-edges of the vertices
+// not yet listed -- an over-count of the unsettled ones' -- plus a pass over
+// all vertices (n / 8 edges' worth) are fewer than the bucket's heavy edges.
+// After a push the search for the next bucket opens (phase 1); a pull has
+// found the next bucket itself (phase 2, or done when nothing is left).
+__device__ void relax_close(SsspState *st, SsspAux *aux, int phase, int cur, unsigned long long *heavy_count,
+                            int has_heavy, int pull_mode, int relax_blocks, int64_t n, int64_t m_in, int64_t delta) {
+  // Every thread of the closing workgroup runs this.  phase and cur are the
+  // launch's own (read by every workgroup at its start); values other
+  // workgroups updated by atomics are read with agent loads, counters are
+  // bumped by atomics, and the state is written by thread 0.
+  if (phase == 4) {
+    const unsigned long long nm = spread_take_min(aux->next_min);
+    if (threadIdx.x == 0) {
+      if (nm == ULLONG_MAX) {
+        st->done = 1;
+      } else {
+        st->next_bucket = nm / static_cast<unsigned long long>(delta);
+        st->phase = 2;
+      }
+    }
+  }
+  if (threadIdx.x != 0 && phase != 0) return;
+  if (phase == 3 || phase == 4) {
+    __hip_atomic_fetch_add(&st->steps, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     *heavy_count = 0;
     st->settled_cnt = 0;
-    st->phase = 1;
+    if (phase == 3) st->phase = 1;
     return;
   }
-  if (st->phase != 0) return;
-  *heavy_count = 0;
-  const int cur = st->cur;
-  st->near_cnt[cur] = 0;
-  st->cur = 1 - cur;
-  st->relax_rounds += 1;
-  if (st->near_cnt[1 - cur] == 0) st->phase = has_heavy ? 3 : 1;
+  // the end of a light round
+  __shared__ int open;
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(&st->steps, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *heavy_count = 0;
+    st->near_cnt[cur] = 0;
+    st->cur = 1 - cur;
+    __hip_atomic_fetch_add(&st->relax_rounds, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    open = agent(&st->near_cnt[1 - cur]) != 0;
+  }
+  __syncthreads();
+  if (open) return;
+  // the bucket's light rounds are over: push or pull its heavy edges
+  if (has_heavy && pull_mode == 1) {
+    __shared__ unsigned long long red[2][kWavesPerBlock];
+    unsigned long long push = 0, listed = 0;
+    for (int i = threadIdx.x; i < relax_blocks; i += blockDim.x) {
+      push += aux->push_part[i];
+      listed += aux->in_part[i];
+      aux->push_part[i] = 0;
+    }
+    push = wave_sum(push);
+    listed = wave_sum(listed);
+    if (lane_id() == 0) {
+      red[0][threadIdx.x / kWave] = push;
+      red[1][threadIdx.x / kWave] = listed;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 1; i < kWavesPerBlock; i++) {
+        push += red[0][i];
+        listed += red[1][i];
+      }
+      const long long l = static_cast<long long>(listed);
+      const long long pull = (m_in > l ? m_in - l : 0) + n / 8;
+      st->phase = static_cast<unsigned long long>(pull) < push ? 4 : 3;
+    }
+    return;
+  }
+  if (threadIdx.x == 0) st->phase = !has_heavy ? 1 : pull_mode == 2 ? 4 : 3;
 }
 
 // ------------------------------------------------ light/heavy row layout
@@ -488,59 +760,38 @@ int graph_weight_range(gbk_ctx *ctx, const gbk_graph *g, long long *partial) {
   return GBK_OK;
 }
 
-// Smallest bucket above the current one that holds a reached vertex.
-__global__ void sssp_min_bucket(const unsigned long long *word, int64_t n, int64_t delta,
-                                const SsspState *st, unsigned long long *partial) {
+// Smallest distance above the current bucket (after a pushed heavy pass);
+// the last workgroup opens that bucket's split, or ends the search.
+__global__ void sssp_min_bucket(const unsigned long long *word, int64_t n, int64_t delta, SsspState *st,
+                                SsspAux *aux) {
   const volatile SsspState *vs = st;
   if (vs->done || vs->phase != 1) return;
-  const int64_t bucket = vs->bucket;
-  unsigned long long best = ULLONG_MAX;
+  const int64_t upper = delta * (vs->bucket + 1);
+  int64_t best = kDistInf;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
     const int64_t d = word_dist(word[v]);
-    if (d < kDistInf) {
-      const int64_t b = d / delta;
-      if (b > bucket && static_cast<unsigned long long>(b) < best) best = static_cast<unsigned long long>(b);
+    if (d >= upper && d < best) best = d;
+  }
+  spread_min(best, aux->next_min);
+  if (last_to_arrive(aux->arrive[1], aux->arrive_top[1])) {
+    const unsigned long long nm = spread_take_min(aux->next_min);
+    if (threadIdx.x == 0) {
+      if (nm == ULLONG_MAX) {
+        st->done = 1;
+      } else {
+        st->next_bucket = nm / static_cast<unsigned long long>(delta);
+        st->phase = 2;
+      }
     }
   }
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    const unsigned long long x = __shfl_xor(best, o);
-    best = x < best ? x : best;
-  }
-  __shared__ unsigned long long red[kWavesPerBlock];
-  if (lane_id() == 0) red[threadIdx.x / kWave] = best;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long m = red[0];
-    for (int i = 1; i < kWavesPerBlock; i++) m = red[i] < m ? red[i] : m;
-    partial[blockIdx.x] = m;
-  }
 }
 
-__global__ void sssp_min_done(const unsigned long long *partial, int count, SsspState *st) {
-  if (st->done || st->phase != 1) return;
-  __shared__ unsigned long long red[1024 / kWave];
-  unsigned long long m = ULLONG_MAX;
-  for (int i = threadIdx.x; i < count; i += blockDim.x) m = partial[i] < m ? partial[i] : m;
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    const unsigned long long x = __shfl_xor(m, o);
-    m = x < m ? x : m;
-  }
-  if (lane_id() == 0) red[threadIdx.x / kWave] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    m = red[0];
-    for (int i = 1; i < static_cast<int>(blockDim.x) / kWave; i++) m = red[i] < m ? red[i] : m;
-    st->next_bucket = m;
-    if (m == ULLONG_MAX) st->done = 1;
-    else st->phase = 2;
-  }
-}
-
-// Queues every vertex of the chosen bucket (each vertex once).
+// Queues every vertex of the chosen bucket (each vertex once); the last
+// workgroup makes it the current bucket.
 __global__ void __launch_bounds__(kBlock)
 sssp_split(const unsigned long long *word, int64_t n, int32_t *near0, int32_t *near1, int64_t cap,
-           int64_t delta, SsspState *st) {
+           int64_t delta, SsspState *st, SsspAux *aux) {
   const volatile SsspState *vs = st;
   if (vs->done || vs->phase != 2) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
@@ -560,13 +811,11 @@ sssp_split(const unsigned long long *word, int64_t n, int32_t *near0, int32_t *n
     q.push(take, static_cast<int32_t>(v));
   }
   q.flush_block();
-}
-
-__global__ void sssp_split_done(SsspState *st) {
-  if (threadIdx.x != 0 || st->done || st->phase != 2) return;
-  st->bucket = static_cast<long long>(st->next_bucket);
-  st->buckets += 1;
-  st->phase = 0;
+  if (last_to_arrive(aux->arrive[2], aux->arrive_top[2]) && threadIdx.x == 0) {
+    st->bucket = nb;
+    __hip_atomic_fetch_add(&st->buckets, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st->phase = 0;
+  }
 }
 
 // Distances out of the words, plus reached vertices and their out-edges.
@@ -598,11 +847,14 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   auto *word = static_cast<unsigned long long *>(scratch(ctx, kS5, sizeof(unsigned long long) * cap));
   auto *q = static_cast<int32_t *>(scratch(ctx, kS7, sizeof(int32_t) * cap * 4));
   auto *st = static_cast<SsspState *>(scratch(ctx, kS8, sizeof(SsspState) + 64));
+  auto *aux = static_cast<SsspAux *>(scratch(ctx, kS12, sizeof(SsspAux)));
   auto *fin = static_cast<unsigned long long *>(scratch(ctx, kS9, 64));
   auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS10, sizeof(unsigned long long) * 2 * kGridCap));
   auto *hs = static_cast<SsspState *>(pinned(ctx, sizeof(SsspState)));
   HeavyQueue hq;
-  if (!word || !q || !st || !fin || !partial || !hs || !make_heavy_queue(ctx, g, kS11, &hq))
+  // rows with more edges to walk than this go to the heavy kernel
+  const int64_t H = ctx->tune.sssp_heavy;
+  if (!word || !q || !st || !aux || !fin || !partial || !hs || !make_heavy_queue(ctx, g, kS11, &hq, H))
     return set_error(ctx, GBK_ENOMEM, "sssp scratch");
   hq.overflow = &st->overflow;
   int32_t *near0 = q, *near1 = q + cap, *settled = q + 2 * cap, *stamp = q + 3 * cap;
@@ -610,7 +862,6 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   if (int rc = graph_weight_range(ctx, g, reinterpret_cast<long long *>(partial))) return rc;
   // heavy edges (weight >= delta) exist
   const int has_heavy = g->m > 0 && g->wmax >= delta;
-  const int64_t H = ctx->tune.heavy_degree;
   // The light/heavy layout for this delta, kept with the graph; its first
   // call with a delta pays for it (timed on its own and added to the call's
   // device time).  It exists before the step loop is captured: a captured
@@ -644,6 +895,9 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
     mg->part_delta = delta;
   }
   const unsigned relax_grid = static_cast<unsigned>(ctx->tune.relax_grid);
+  // the pull reads the in-CSR's weights (the out-CSR's on undirected graphs)
+  const SsspIn in{g->in_off, g->in_nbr, g->in_wgt, n};
+  const int pull_mode = in.off && in.nbr && in.wgt ? ctx->tune.sssp_pull : 0;
   const bool count = ctx->ktime == GBK_KT_SSSP_RELAX && ctx->kt_dev;
   const unsigned count_grid = static_cast<unsigned>(kKtPartials < 256 ? kKtPartials : 256);
   const unsigned scan_grid = expand_grid(n);
@@ -652,26 +906,25 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   auto step = [&]() {
       if (count) {
         unsigned long long *part = ctx->kt_dev + 2;
-        sssp_count_round<<<count_grid, kBlock, 0, s>>>(g->off, g->plc, word, near0, near1, settled, delta, st,
-                                                       part);
+        sssp_count_round<<<count_grid, kBlock, 0, s>>>(g->off, g->plc, in, word, near0, near1, settled, delta,
+                                                       st, part);
         sum_partials<unsigned long long><<<1, 1024, 0, s>>>(part, static_cast<int>(count_grid), ctx->kt_dev, 1);
         sum_partials<unsigned long long><<<1, 1024, 0, s>>>(part + count_grid, static_cast<int>(count_grid),
                                                           ctx->kt_dev + 1, 1);
       }
       const int kt = kt_begin(ctx, GBK_KT_SSSP_RELAX);
-      sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->pnbr, g->pwgt, g->plc, word, near0, near1, settled,
-                                               stamp, cap, delta, H, has_heavy, st, hq);
-      sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->pnbr, g->pwgt, word, near0, near1, cap, delta, st,
-                                                     hq);
+      sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->pnbr, g->pwgt, g->plc, in, word, near0, near1,
+                                               settled, stamp, cap, delta, H, has_heavy, pull_mode, st, aux, hq);
+      sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->pnbr, g->pwgt, in, word, near0, near1, cap, delta,
+                                                     has_heavy, pull_mode, static_cast<int>(relax_grid), g->m,
+                                                     st, aux, hq);
       kt_end(ctx, kt);
-      sssp_relax_done<<<1, kWave, 0, s>>>(st, hq.count, has_heavy);
-      sssp_min_bucket<<<scan_grid, kBlock, 0, s>>>(word, n, delta, st, partial);
-      sssp_min_done<<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), st);
-      sssp_split<<<scan_grid, kBlock, 0, s>>>(word, n, near0, near1, cap, delta, st);
-      sssp_split_done<<<1, kWave, 0, s>>>(st);
+      sssp_min_bucket<<<scan_grid, kBlock, 0, s>>>(word, n, delta, st, aux);
+      sssp_split<<<scan_grid, kBlock, 0, s>>>(word, n, near0, near1, cap, delta, st, aux);
   };
   // steps are queued blindly in blocks of kStepsPerGraph (a captured graph)
-  // and the host reads the state after 4, 8, ... 64 of them
+  // and the host reads the state after 4, 8, ... 64 of them -- or first
+  // after as many as the last call on this graph and delta took
   constexpr int kStepsPerGraph = 4;
   auto block = [&]() {
     for (int j = 0; j < kStepsPerGraph; j++) step();
@@ -679,11 +932,14 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   LoopGraph loop(ctx);
   if (int rc = loop.capture(block)) return rc;
   EventTimer timer(ctx);
-  sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, word, stamp, near0, st);
-  int batch = kStepsPerGraph;
+  sssp_init<<<grid_for(n), kBlock, 0, s>>>(n, source, word, stamp, near0, st, aux);
+  const bool hinted = ctx->sssp_hint_graph == g && ctx->sssp_hint_delta == delta && ctx->sssp_hint_steps > 0;
+  int64_t batch = kStepsPerGraph;
+  if (hinted)
+    batch = std::min<int64_t>((ctx->sssp_hint_steps + kStepsPerGraph - 1) / kStepsPerGraph * kStepsPerGraph, 256);
   int64_t launched = 0;
   for (;;) {
-    for (int j = 0; j < batch; j += kStepsPerGraph)
+    for (int64_t j = 0; j < batch; j += kStepsPerGraph)
       if (int rc = loop.run(block)) return rc;
     launched += batch;
     GBK_LAUNCH_CHECK(ctx, "sssp step");
@@ -692,7 +948,8 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
     kt_collect(ctx);
     if (hs->overflow) return set_error(ctx, GBK_EHIP, "sssp: queue overflow");
     if (hs->done) break;
-    if (batch < 64) batch *= 2;
+    if (hinted && launched == batch) batch = kStepsPerGraph;
+    else if (batch < 64) batch *= 2;
     // every step either settles a bucket or shrinks the work of the current
     // one; more steps than buckets times rounds would mean a broken state
     if (launched > 4 * (n + 16) + 1024) return set_error(ctx, GBK_EHIP, "sssp: no convergence");
@@ -702,6 +959,9 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   sum_partials<unsigned long long><<<1, 1024, 0, s>>>(partial + kGridCap, static_cast<int>(scan_grid), fin + 1, 0);
   GBK_LAUNCH_CHECK(ctx, "sssp_finish");
   const double ms = timer.stop_ms() + layout_ms;
+  ctx->sssp_hint_graph = g;
+  ctx->sssp_hint_delta = delta;
+  ctx->sssp_hint_steps = hs->steps;
   unsigned long long hf[2] = {0, 0};
   GBK_HIP(ctx, hipMemcpyAsync(hf, fin, sizeof(hf), hipMemcpyDeviceToHost, s));
   GBK_HIP(ctx, hipStreamSynchronize(s));

@@ -118,7 +118,11 @@ int gbk_ctx_sync(gbk_ctx *ctx);
  * top-down only, 1 (default) bottom-up from in-neighbours when the frontier's out-edges outweigh
  * those of the vertices some source has not reached, 2 bottom-up at every level); "bc_pull_ratio"
  * (1..1024, default 1: how many such edges one frontier edge weighs in that choice); "sssp_wide"
- * (1, default: SSSP batches of 17..32 sources run on 64-lane rows; 0: 32-lane rows); "msbfs_group"
+ * (1, default: SSSP batches of 17..32 sources run on 64-lane rows; 0: 32-lane rows); "sssp_pull"
+ * (one-source SSSP's pass over a settled bucket's heavy edges: 0 pushed from the bucket's
+ * vertices, 1 (default) pulled into the unsettled vertices when their edges are fewer, 2 always
+ * pulled); "sssp_heavy" (one-source SSSP: rows with more edges to walk than this, default 512, go to
+ * a kernel that spreads them over all workgroups); "msbfs_group"
  * (lanes per vertex in the BFS batch's bottom-up step: 1, 2, 4, ..., 64; default 32); and
  * "ktime" (a gbk_ktime_kernel) selecting the kernel whose
  * launches are timed one by one with HIP events (loops then run eagerly).  Returns

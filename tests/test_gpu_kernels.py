@@ -138,6 +138,32 @@ def test_sssp_parity(ctx, name):
         assert st.reached == int((ref != O.K_DIST_INF).sum())
 
 
+@pytest.mark.parametrize("name", ["kron13w", "uniform12w", "star9000w", "directed_w"])
+@pytest.mark.parametrize("sssp_heavy", [512, 64])
+def test_sssp_heavy_pass_push_and_pull(ctx, name, sssp_heavy):
+    """The heavy pass pushed from the bucket's vertices (sssp_pull 0), pulled
+    into the unsettled ones (2) and chosen by edge counts (1) give the same
+    exact distances; a row threshold of 64 sends the pull's big rows through
+    the heavy-row kernel."""
+    g = SSSP_GRAPHS[name]()
+    dg = ctx.upload_graph(g)
+    _, srcs = O.source_picker(g, 3)
+    try:
+        ctx.set_param("sssp_heavy", sssp_heavy)
+        for s in srcs:
+            s = int(s)
+            ref = O.dijkstra(g, s)
+            for mode in (0, 2, 1):
+                ctx.set_param("sssp_pull", mode)
+                for delta in (4, 32, 64):
+                    dist, st = ctx.sssp(dg, s, delta)
+                    assert np.array_equal(dist, ref), (name, s, mode, delta)
+                    assert st.reached == int((ref != O.K_DIST_INF).sum())
+    finally:
+        ctx.set_param("sssp_pull", 1)
+        ctx.set_param("sssp_heavy", 512)
+
+
 def test_sssp_weighted_path_closed_form(ctx):
     g = _fixture("weighted_path.wel", weighted=True, symmetrize=False)
     dist, _ = ctx.sssp(ctx.upload_graph(g), 0, 2)
