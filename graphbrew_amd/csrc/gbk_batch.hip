@@ -442,8 +442,13 @@ struct BatchState {
   int cur, phase, done, overflow;
 };
 
-__global__ void sb_init_state(BatchState *st, long long first) {
+// The state's scratch: the state, then (256-byte aligned) the spread counter
+// of edges walked in the current round.
+constexpr size_t kBatchStateBytes = (sizeof(BatchState) + 64 + 255) / 256 * 256;
+
+__global__ void sb_init_state(BatchState *st, long long first, unsigned long long *scan_slots) {
   if (threadIdx.x == 0) {
+    for (int i = 0; i < kSpread; i++) scan_slots[i * kSpreadStride] = 0;
     st->near_cnt[0] = static_cast<unsigned long long>(first);
     st->near_cnt[1] = 0;
     st->bucket = 0;
@@ -610,7 +615,7 @@ template <int Kp>
 __global__ void __launch_bounds__(kBlock, 8)
 sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *pend,
          uint32_t *vtag, int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, BatchState *st,
-         HeavyQueue hq) {
+         unsigned long long *scan_slots, HeavyQueue hq) {
   const volatile BatchState *vs = st;
   if (vs->done || vs->phase != 0) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
@@ -624,7 +629,10 @@ sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *di
   f.setup(seg, nbr, wgt, dist, pend, vtag, near0, near1, cap, delta, st);
   const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  unsigned long long walked = 0;
+  // edges walked, counted in LDS (a register held across the loop spills)
+  __shared__ unsigned long long walked;
+  if (threadIdx.x == 0) walked = 0;
+  __syncthreads();
   for (int64_t base = gw * G; base < count; base += waves * G) {
     const int64_t i = base + grp;
     const bool valid = i < count;
@@ -642,17 +650,18 @@ sb_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, int32_t *di
     bool act;
     f.row_lane(row, valid, !heavy, du, act);
     f.walk(s, heavy ? 0 : len, du, act);
-    if (f.leader && !heavy) walked += static_cast<unsigned long long>(len);
+    if (f.leader && !heavy) atomicAdd(&walked, static_cast<unsigned long long>(len));
   }
   f.nearq.flush_block();
-  block_add(walked, &st->edges_scanned);
+  __syncthreads();
+  if (threadIdx.x == 0 && walked) atomicAdd(&scan_slots[(blockIdx.x % kSpread) * kSpreadStride], walked);
 }
 
 template <int Kp>
 __global__ void __launch_bounds__(kBlock, 8)
 sb_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *pend, uint32_t *vtag,
                int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, BatchState *st,
-               HeavyQueue hq) {
+               unsigned long long *scan_slots, HeavyQueue hq) {
   const volatile BatchState *vs = st;
   if (vs->done || vs->phase != 0) return;
   int64_t rows, total;
@@ -665,7 +674,10 @@ sb_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *p
   f.setup(seg, nbr, wgt, dist, pend, vtag, near0, near1, cap, delta, st);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  unsigned long long walked = 0;
+  // edges walked, counted in LDS (a register held across the loop spills)
+  __shared__ unsigned long long walked;
+  if (threadIdx.x == 0) walked = 0;
+  __syncthreads();
   for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
     const int64_t c = c0 + grp;
     const bool valid = c < total;
@@ -686,14 +698,20 @@ sb_relax_heavy(const int32_t *nbr, const int32_t *wgt, int32_t *dist, int32_t *p
     bool act;
     f.row_lane(row, valid, first == 0, du, act);
     f.walk(start + first, valid ? last - first : 0, du, act);
-    if (f.leader && valid) walked += static_cast<unsigned long long>(last - first);
+    if (f.leader && valid) atomicAdd(&walked, static_cast<unsigned long long>(last - first));
   }
   f.nearq.flush_block();
-  block_add(walked, &st->edges_scanned);
+  __syncthreads();
+  if (threadIdx.x == 0 && walked) atomicAdd(&scan_slots[(blockIdx.x % kSpread) * kSpreadStride], walked);
 }
 
-__global__ void sb_relax_done(BatchState *st, unsigned long long *heavy_count) {
-  if (threadIdx.x != 0 || st->done || st->phase != 0) return;
+// One wave: closes a round (and folds the round's walked edges into the
+// state).
+__global__ void sb_relax_done(BatchState *st, unsigned long long *heavy_count, unsigned long long *scan_slots) {
+  if (st->done || st->phase != 0) return;
+  const unsigned long long walked = wave_take_spread(scan_slots);
+  if (threadIdx.x != 0) return;
+  st->edges_scanned += walked;
   *heavy_count = 0;
   const int cur = st->cur;
   st->rows_scanned += st->near_cnt[cur];
@@ -1039,7 +1057,10 @@ bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigm
   f.leader = f.k == 0;
   const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  unsigned long long walked = 0;
+  // edges walked, counted in LDS (a register held across the loop spills)
+  __shared__ unsigned long long walked;
+  if (threadIdx.x == 0) walked = 0;
+  __syncthreads();
   for (int64_t base = gw * G; base < count; base += waves * G) {
     const int64_t i = base + grp;
     const bool valid = i < count;
@@ -1054,10 +1075,11 @@ bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigm
     const bool heavy = len > kBatchHeavy;
     if (heavy && f.leader) heavy_push(hq, static_cast<int32_t>(row), s, len);
     bcb_edges<Kp>(nbr, s, heavy ? 0 : len, f);
-    if (f.leader && !heavy) walked += static_cast<unsigned long long>(len);
+    if (f.leader && !heavy) atomicAdd(&walked, static_cast<unsigned long long>(len));
   }
   f.q.flush_block();
-  block_add(walked, &c->edges_scanned);
+  __syncthreads();
+  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
 }
 
 template <int Kp>
@@ -1083,7 +1105,10 @@ bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *l
   f.leader = f.k == 0;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  unsigned long long walked = 0;
+  // edges walked, counted in LDS (a register held across the loop spills)
+  __shared__ unsigned long long walked;
+  if (threadIdx.x == 0) walked = 0;
+  __syncthreads();
   for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
     const int64_t cc = c0 + grp;
     const bool valid = cc < total;
@@ -1095,10 +1120,11 @@ bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *l
     f.act = valid && depth[row * Kp + f.k] == d;
     f.su = f.act ? sigma[row * Kp + f.k] : 0.0;
     bcb_edges<Kp>(nbr, valid ? hq.start[h] + first : 0, len, f);
-    if (f.leader) walked += static_cast<unsigned long long>(len);
+    if (f.leader) atomicAdd(&walked, static_cast<unsigned long long>(len));
   }
   f.q.flush_block();
-  block_add(walked, &c->edges_scanned);
+  __syncthreads();
+  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
 }
 
 // Bottom-up forward level d, for levels whose frontier has more out-edges
@@ -1148,7 +1174,10 @@ bcb_forward_pull(const int64_t *ioff, const int32_t *inbr, int64_t n, int K, int
   const bool leader = f.k == 0;
   const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  unsigned long long walked = 0;
+  // edges walked, counted in LDS (a register held across the loop spills)
+  __shared__ unsigned long long walked;
+  if (threadIdx.x == 0) walked = 0;
+  __syncthreads();
   for (int64_t base = gw * G; base < n; base += waves * G) {
     const int64_t v = base + grp;
     const bool valid = v < n;
@@ -1170,10 +1199,11 @@ bcb_forward_pull(const int64_t *ioff, const int32_t *inbr, int64_t n, int K, int
     }
     const bool anyf = group_any<Kp>(found);
     q.push(leader && anyf, static_cast<int32_t>(v));
-    if (leader && !heavy) walked += static_cast<unsigned long long>(len);
+    if (leader && !heavy) atomicAdd(&walked, static_cast<unsigned long long>(len));
   }
   q.flush_block();
-  block_add(walked, &c->edges_scanned);
+  __syncthreads();
+  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
 }
 
 // Heavy rows of a bottom-up level: hpart[chunk][k] = lane k's sigma sum over
@@ -1196,7 +1226,10 @@ bcb_pull_heavy(const int32_t *inbr, int K, const int32_t *depth, const double *s
   f.k = lane_id() % Kp;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  unsigned long long walked = 0;
+  // edges walked, counted in LDS (a register held across the loop spills)
+  __shared__ unsigned long long walked;
+  if (threadIdx.x == 0) walked = 0;
+  __syncthreads();
   for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
     const int64_t cc = c0 + grp;
     const bool valid = cc < total;
@@ -1209,9 +1242,10 @@ bcb_pull_heavy(const int32_t *inbr, int K, const int32_t *depth, const double *s
     f.acc = 0.0;
     bcb_edges<Kp>(inbr, valid ? hq.start[h] + first : 0, len, f);
     if (valid) hpart[cc * Kp + f.k] = f.acc;
-    if (f.k == 0) walked += static_cast<unsigned long long>(len);
+    if (f.k == 0) atomicAdd(&walked, static_cast<unsigned long long>(len));
   }
-  block_add(walked, &c->edges_scanned);
+  __syncthreads();
+  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
 }
 
 // One wave per heavy row of a bottom-up level: its chunks' sums meet as in
@@ -1320,7 +1354,10 @@ bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const
   const bool leader = f.k == 0;
   const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  unsigned long long walked = 0;
+  // edges walked, counted in LDS (a register held across the loop spills)
+  __shared__ unsigned long long walked;
+  if (threadIdx.x == 0) walked = 0;
+  __syncthreads();
   for (int64_t base = gw * G; base < count; base += waves * G) {
     const int64_t i = base + grp;
     const bool valid = i < count;
@@ -1343,9 +1380,10 @@ bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const
     }
     const double tot = group_sum_d<Kp>(delta);
     if (leader && valid && !heavy && tot != 0.0) scores[row] += static_cast<float>(tot);
-    if (leader && !heavy) walked += static_cast<unsigned long long>(len);
+    if (leader && !heavy) atomicAdd(&walked, static_cast<unsigned long long>(len));
   }
-  block_add(walked, &c->edges_scanned);
+  __syncthreads();
+  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
 }
 
 // Heavy rows of a backward level: hpart[chunk][k] = lane k's coefficient
@@ -1365,7 +1403,10 @@ bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const uint64_t *dc,
   f.k = lane_id() % Kp;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  unsigned long long walked = 0;
+  // edges walked, counted in LDS (a register held across the loop spills)
+  __shared__ unsigned long long walked;
+  if (threadIdx.x == 0) walked = 0;
+  __syncthreads();
   for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
     const int64_t cc = c0 + grp;
     const bool valid = cc < total;
@@ -1378,9 +1419,10 @@ bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const uint64_t *dc,
     f.acc = 0.0;
     bcb_edges<Kp>(nbr, valid ? hq.start[h] + first : 0, len, f);
     if (valid) hpart[cc * Kp + f.k] = f.acc;
-    if (f.k == 0) walked += static_cast<unsigned long long>(len);
+    if (f.k == 0) atomicAdd(&walked, static_cast<unsigned long long>(len));
   }
-  block_add(walked, &c->edges_scanned);
+  __syncthreads();
+  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
 }
 
 // One wave per heavy row: lane (slot, k) adds source k's partials of chunks
@@ -1434,10 +1476,11 @@ static void launch_sb_relax(gbk_ctx *ctx, const gbk_graph *g, int32_t *dist, int
                             int32_t *near0, int32_t *near1, int64_t cap, int64_t delta, BatchState *st,
                             const HeavyQueue &hq, unsigned grid) {
   hipStream_t s = ctx->stream;
+  auto *slots = reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(st) + kBatchStateBytes);
   sb_relax<Kp><<<grid, kBlock, 0, s>>>(g->off, g->nbr, g->wgt, dist, pend, vtag, near0, near1, cap, delta,
-                                       st, hq);
+                                       st, slots, hq);
   sb_relax_heavy<Kp><<<grid, kBlock, 0, s>>>(g->nbr, g->wgt, dist, pend, vtag, near0, near1, cap, delta,
-                                             st, hq);
+                                             st, slots, hq);
 }
 
 }  // namespace gbk
@@ -1582,7 +1625,8 @@ int sssp_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K
   auto *pend = static_cast<int32_t *>(scratch(ctx, kS1, sizeof(int32_t) * (n + 1)));
   auto *vtag = static_cast<uint32_t *>(scratch(ctx, kS2, sizeof(uint32_t) * (n + 1)));
   auto *q = static_cast<int32_t *>(scratch(ctx, kS4, sizeof(int32_t) * cap * 2));
-  auto *st = static_cast<BatchState *>(scratch(ctx, kS5, sizeof(BatchState) + 64));
+  auto *st = static_cast<BatchState *>(scratch(ctx, kS5, kBatchStateBytes + kSpreadBytes));
+  auto *scan_slots = reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(st) + kBatchStateBytes);
   auto *partial = static_cast<unsigned long long *>(scratch(ctx, kS6, sizeof(unsigned long long) * kGridCap));
   auto *out = dist_host ? static_cast<int32_t *>(scratch(ctx, kS8, sizeof(int32_t) * n * K + 64)) : nullptr;
   auto *hs = static_cast<BatchState *>(pinned(ctx, sizeof(BatchState)));
@@ -1626,7 +1670,7 @@ int sssp_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K
         default: launch_sb_relax<64>(ctx, g, dist, pend, vtag, near0, near1, cap, delta, st, hq, grid); break;
       }
       kt_end(ctx, kt);
-      sb_relax_done<<<1, kWave, 0, s>>>(st, hq.count);
+      sb_relax_done<<<1, kWave, 0, s>>>(st, hq.count, scan_slots);
       sb_min_pend<<<scan_grid, kBlock, 0, s>>>(pend, n, st, partial);
       sb_min_done<<<1, 1024, 0, s>>>(partial, static_cast<int>(scan_grid), st);
       sb_split<<<scan_grid, kBlock, 0, s>>>(pend, n, near0, near1, cap, st);
@@ -1646,7 +1690,7 @@ int sssp_batch_chunk(gbk_ctx *ctx, const gbk_graph *g, const int32_t *src, int K
   fill<int32_t>(ctx, pend, INT32_MAX, n);
   fill<uint32_t>(ctx, vtag, 0u, n);
   sb_sources<<<1, kBatchMax, 0, s>>>(dsrc, K, Kp, dist);
-  sb_init_state<<<1, 1, 0, s>>>(st, static_cast<long long>(uniq.size()));
+  sb_init_state<<<1, 1, 0, s>>>(st, static_cast<long long>(uniq.size()), scan_slots);
   int batch = kStepsPerGraph;
   int64_t launched = 0;
   for (;;) {

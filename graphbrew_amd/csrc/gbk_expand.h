@@ -564,6 +564,34 @@ struct WaveQueue {
   }
 };
 
+// Spread counters.  Atomics on one address from thousands of workgroups
+// serialize (about 11 ns each on this GPU, so a resident grid's end-of-kernel
+// adds cost tens of microseconds); a spread counter is kSpread slots
+// kSpreadStride 8-byte words (256 bytes) apart, workgroup b adding into slot
+// b mod kSpread, summed by one wave afterwards.
+constexpr int kSpread = 32;
+constexpr int kSpreadStride = 32;
+constexpr size_t kSpreadBytes = sizeof(unsigned long long) * kSpread * kSpreadStride;
+
+// Adds every lane's v into this workgroup's slot (every thread calls it).
+__device__ __forceinline__ void block_add_spread(unsigned long long v, unsigned long long *slots) {
+  __shared__ unsigned long long red[kWavesPerBlock];
+  v = block_sum(v, red);
+  if (threadIdx.x == 0 && v) atomicAdd(&slots[(blockIdx.x % kSpread) * kSpreadStride], v);
+}
+
+// Called by all lanes of one wave (in a kernel after the adding ones): the
+// slots' total, and the slots cleared.
+__device__ __forceinline__ unsigned long long wave_take_spread(unsigned long long *slots) {
+  unsigned long long x = 0;
+  const int l = lane_id();
+  if (l < kSpread) {
+    x = slots[l * kSpreadStride];
+    slots[l * kSpreadStride] = 0;
+  }
+  return wave_sum(x);
+}
+
 // Adds every lane's v into *total with one atomic per workgroup (every
 // thread calls it).
 __device__ __forceinline__ void block_add(unsigned long long v, unsigned long long *total) {

@@ -71,13 +71,11 @@ struct SsspState {
   int cur, phase, done, overflow;
 };
 
-// Device scratch beside the state that every workgroup of a launch updates.
-// Same-address atomics from thousands of workgroups serialize (about 11 ns
-// each on this GPU), so such counters are spread over kSpread slots 256
-// bytes apart (slot = workgroup id mod kSpread) and combined by one
-// workgroup, and per-bucket edge counts are per-workgroup partials.
-constexpr int kSpread = 32;
-constexpr int kSlotWords = 64;  // 32-bit words per 256-byte slot
+// Device scratch beside the state that every workgroup of a launch updates:
+// its counters are spread (kSpread slots 256 bytes apart, slot = workgroup id
+// mod kSpread) and combined by one workgroup, and per-bucket edge counts are
+// per-workgroup partials.
+constexpr int kSlotWords = 2 * kSpreadStride;  // 32-bit words per 256-byte slot
 struct SsspAux {
   // arrivals at the end of the relax-heavy, min and split kernels: a counter
   // per slot group, then one for the groups; the last arrival closes the phase
