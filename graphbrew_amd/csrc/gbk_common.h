@@ -246,9 +246,11 @@ struct gbk_graph {
   int64_t wmin = -1, wmax = -1;
   // One-source SSSP's light/heavy layout of the out-CSR for delta
   // 'part_delta' (-1: none yet): each row's edges lighter than delta first,
-  // then the rest, each class in CSR order; plc[u] = light edges of row u.
+  // then the rest, each class in CSR order, as (neighbour, weight) pairs (one
+  // 8-byte load per edge); plc[u] = light edges of row u.
   int64_t part_delta = -1;
-  int32_t *pnbr = nullptr, *pwgt = nullptr, *plc = nullptr;
+  int2 *pnw = nullptr;
+  int32_t *plc = nullptr;
   // PageRank's segmented layout of the in-CSR for segment width 2^seg_lw
   // (0: none yet), one allocation of seg_bytes, built by the first call.
   int seg_lw = 0;

@@ -100,6 +100,39 @@ __device__ __forceinline__ void wave_walk_batch(int64_t row, int64_t start, int6
   }
 }
 
+// wave_walk_batch that also hands each edge its row's value 'val' (one
+// per lane, e.g. the row's distance read when the row was picked), so the
+// functor needs no load per edge for it.
+template <int U, class F>
+__device__ __forceinline__ void wave_walk_batch_v(int64_t row, int64_t start, int64_t len, int64_t val, F &f) {
+  const int lane = lane_id();
+  const int64_t incl = wave_incl_scan(len);
+  const int64_t excl = incl - len;
+  const int64_t total = __shfl(incl, kWave - 1);
+  for (int64_t c = 0; c < total; c += U * kWave) {
+    int64_t rows[U], edges[U], vals[U];
+    bool act[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int64_t e = c + j * kWave + lane;
+      int lo = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {
+        const int probe = lo + step;
+        const int64_t x = __shfl(excl, probe & (kWave - 1));
+        if (probe < kWave && x <= e) lo = probe;
+      }
+      act[j] = e < total;
+      rows[j] = __shfl(row, lo);
+      vals[j] = __shfl(val, lo);
+      const int64_t ostart = __shfl(start, lo);
+      const int64_t oexcl = __shfl(excl, lo);
+      edges[j] = act[j] ? ostart + (e - oexcl) : 0;
+    }
+    f.template batch<U>(rows, edges, vals, act);
+  }
+}
+
 // Segmented inclusive sum across a 64-edge chunk whose owners ('seg', one
 // per lane, non-decreasing across lanes) form contiguous runs.  Returns the
 // run total in the last lane of each run (other lanes hold partial sums);
@@ -327,6 +360,40 @@ __device__ void expand_heavy_batch(const HeavyQueue &hq, F &f) {
         edges[j] = act[j] ? start + e : 0;
       }
       f.template batch<U>(rws, edges, act);
+    }
+  }
+}
+
+// expand_heavy_batch handing each edge its row's value, f.row_value(row),
+// read once per chunk.
+template <int U, class F>
+__device__ void expand_heavy_batch_v(const HeavyQueue &hq, F &f) {
+  int64_t rows, total;
+  heavy_extent(hq, rows, total);
+  if (rows == 0) return;
+  for (int64_t gc = blockIdx.x; gc < total; gc += gridDim.x) {
+    int64_t lo = 0, hi = rows;  // last row with prefix[row] <= gc
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (hq.prefix[mid] <= gc) lo = mid;
+      else hi = mid;
+    }
+    const int64_t len = hq.len[lo], start = hq.start[lo], row = hq.row[lo];
+    const int64_t first = (gc - hq.prefix[lo]) * hq.chunk;
+    const int64_t last = min(len, first + hq.chunk);
+    const int64_t val = f.row_value(row);
+    for (int64_t base = first; base < last; base += U * static_cast<int64_t>(blockDim.x)) {
+      int64_t rws[U], edges[U], vals[U];
+      bool act[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int64_t e = base + j * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        act[j] = e < last;
+        rws[j] = row;
+        vals[j] = val;
+        edges[j] = act[j] ? start + e : 0;
+      }
+      f.template batch<U>(rws, edges, vals, act);
     }
   }
 }

@@ -125,41 +125,39 @@ __global__ void sssp_init(int64_t n, int32_t source, unsigned long long *word, i
 }
 
 struct RelaxEdge {
-  const int32_t *nbr;
-  const int32_t *wgt;
+  const int2 *nw;  // light/heavy layout: (neighbour, weight) pairs
   unsigned long long *word;
   WaveQueue nearq;
-  int64_t delta;
-  int64_t bucket;
-  uint32_t tag;  // this round's tag, never 0
-  bool heavy;    // the heavy pass: edges of weight >= delta, nothing queued
+  int64_t upper;  // end of the current bucket: candidates below it are queued
+  uint32_t tag;   // this round's tag, never 0
+  bool heavy;     // the heavy pass: nothing queued
 
+  __device__ __forceinline__ int64_t row_value(int64_t u) const { return word_dist(word[u]); }
+
+  // du: the row's distance when it was picked (a later, lower one re-queues
+  // the row, so relaxing from the value read then is safe)
   template <int U>
-  __device__ __forceinline__ void batch(const int64_t (&u)[U], const int64_t (&k)[U], const bool (&act)[U]) {
+  __device__ __forceinline__ void batch(const int64_t (&u)[U], const int64_t (&k)[U], const int64_t (&du)[U],
+                                        const bool (&act)[U]) {
     int32_t v[U];
     int64_t cand[U];
     unsigned long long wv[U];
-    bool take[U];
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      v[j] = nbr[k[j]];
-      cand[j] = wgt[k[j]];
+      const int2 e = act[j] ? nw[k[j]] : make_int2(0, 0);
+      v[j] = e.x;
+      cand[j] = du[j] + e.y;
     }
 #pragma unroll
-    for (int j = 0; j < U; j++) {
-      // an edge of the other class reads no distance
-      take[j] = act[j] && (cand[j] >= delta) == heavy;
-      cand[j] += take[j] ? word_dist(word[u[j]]) : 0;
-      wv[j] = take[j] ? word[v[j]] : 0ull;
-    }
+    for (int j = 0; j < U; j++) wv[j] = act[j] ? word[v[j]] : 0ull;
     bool push[U];
 #pragma unroll
     for (int j = 0; j < U; j++) {
       push[j] = false;
       // test first: most relaxations do not improve, and a plain load can
       // only overstate dist[v] (values only decrease), never hide a win
-      if (take[j] && cand[j] < word_dist(wv[j])) {
-        const bool here = !heavy && cand[j] / delta == bucket;
+      if (act[j] && cand[j] < word_dist(wv[j])) {
+        const bool here = !heavy && cand[j] < upper;
         const unsigned long long old = atomicMin(&word[v[j]], make_word(cand[j], here ? tag : 0u));
         push[j] = here && cand[j] < word_dist(old) && static_cast<uint32_t>(old) != tag;
       }
@@ -188,20 +186,23 @@ struct PullEdge {
   unsigned long long *word;
   int64_t best;        // smallest finite candidate or unsettled distance seen
 
+  __device__ __forceinline__ int64_t row_value(int64_t v) const { return word_dist(word[v]); }
+
+  // dv: the row's distance when it was picked (only this pass lowers it, so
+  // a candidate below it that is not below the current value is just a
+  // spare atomic)
   template <int U>
-  __device__ __forceinline__ void batch(const int64_t (&v)[U], const int64_t (&k)[U], const bool (&act)[U]) {
+  __device__ __forceinline__ void batch(const int64_t (&v)[U], const int64_t (&k)[U], const int64_t (&dv)[U],
+                                        const bool (&act)[U]) {
     int32_t u[U];
-    int64_t cand[U], dv[U];
+    int64_t cand[U];
 #pragma unroll
     for (int j = 0; j < U; j++) {
       u[j] = act[j] ? nbr[k[j]] : 0;
       cand[j] = act[j] ? wgt[k[j]] : 0;
     }
 #pragma unroll
-    for (int j = 0; j < U; j++) {
-      cand[j] += act[j] ? word_dist(word[u[j]]) : kDistInf;
-      dv[j] = act[j] ? word_dist(word[v[j]]) : 0;
-    }
+    for (int j = 0; j < U; j++) cand[j] += act[j] ? word_dist(word[u[j]]) : kDistInf;
 #pragma unroll
     for (int j = 0; j < U; j++) {
       if (act[j] && cand[j] < dv[j]) atomicMin(&word[v[j]], make_word(cand[j], 0u));
@@ -277,31 +278,28 @@ __device__ __forceinline__ unsigned long long spread_take_min(unsigned long long
   return res;
 }
 
-__device__ __forceinline__ void relax_setup(RelaxEdge &f, int32_t *seg, const int32_t *nbr,
-                                            const int32_t *wgt, unsigned long long *word,
+__device__ __forceinline__ void relax_setup(RelaxEdge &f, int32_t *seg, const int2 *nw, unsigned long long *word,
                                             int32_t *near0, int32_t *near1, int64_t cap,
                                             int64_t delta, SsspState *st) {
   const volatile SsspState *vs = st;
   const int cur = vs->cur;
   f.heavy = vs->phase == 3;
-  f.nbr = nbr;
-  f.wgt = wgt;
+  f.nw = nw;
   f.word = word;
   f.nearq.init(seg + (threadIdx.x / kWave) * kSeg, &st->near_cnt[1 - cur], cur ? near0 : near1, cap,
                &st->overflow);
-  f.delta = delta;
-  f.bucket = vs->bucket;
+  f.upper = delta * (vs->bucket + 1);
   f.tag = static_cast<uint32_t>(vs->relax_rounds + 1);
 }
 
 // A light round (phase 0) over the near queue -- each row not already
 // listed in this bucket goes on the settled list -- or the heavy pass
 // (phase 3) over the settled list.
-// Rows are read in the light/heavy layout (nbr, wgt, lc): a light round
+// Rows are read in the light/heavy layout (pairs and lc): a light round
 // walks a row's light edges, the heavy pass its heavy ones.  Without heavy
 // edges in the graph no vertex is listed.
 __global__ void __launch_bounds__(kBlock)
-sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, const int32_t *lc, SsspIn in,
+sssp_relax(const int64_t *off, const int2 *nw, const int32_t *lc, SsspIn in,
            unsigned long long *word, int32_t *near0, int32_t *near1, int32_t *settled, int32_t *stamp,
            int64_t cap, int64_t delta, int64_t heavy_deg, int has_heavy, int pull_mode, SsspState *st,
            SsspAux *aux, HeavyQueue hq) {
@@ -325,7 +323,7 @@ sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, const int
           len = 0;
         }
       }
-      wave_walk_batch<kRelaxBatch>(v, start, len, pf);
+      wave_walk_batch_v<kRelaxBatch>(v, start, len, dv, pf);
     }
     spread_min(pf.best, aux->next_min);
     return;
@@ -338,27 +336,33 @@ sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, const int
   const int64_t count = min(static_cast<int64_t>(heavy ? vs->settled_cnt : vs->near_cnt[cur]), cap);
   const int32_t *list = heavy ? settled : (cur ? near1 : near0);
   RelaxEdge f;
-  relax_setup(f, seg, nbr, wgt, word, near0, near1, cap, delta, st);
+  relax_setup(f, seg, nw, word, near0, near1, cap, delta, st);
   WaveQueue sq;
   sq.init(sseg + (threadIdx.x / kWave) * kSeg, &st->settled_cnt, settled, cap, &st->overflow);
-  const int64_t lower = delta * f.bucket;
-  const int32_t bucket = static_cast<int32_t>(f.bucket);
+  const int64_t lower = f.upper - delta;
+  const int32_t bucket = static_cast<int32_t>(vs->bucket);
   unsigned long long push_edges = 0, settled_in = 0;
   for (int64_t tile = blockIdx.x; tile * kBlock < count; tile += gridDim.x) {
     const int64_t i = tile * kBlock + threadIdx.x;
-    int64_t row = 0, start = 0, len = 0;
+    int64_t row = 0, start = 0, len = 0, drow = 0;
     bool first = false;
     if (i < count) {
       row = list[i];
       const int64_t deg = off[row + 1] - off[row];
       const int64_t nl = lc[row];
+      drow = word_dist(word[row]);
       if (heavy) {
         start = off[row] + nl;
         len = deg - nl;
-      } else if (word_dist(word[row]) >= lower) {
+      } else if (drow >= lower) {
         start = off[row];
         len = nl;
-        if (has_heavy && deg > nl) first = atomicExch(&stamp[row], bucket) != bucket;
+        // a round's list holds a vertex at most once, so no other lane
+        // touches its stamp in this launch
+        if (has_heavy && deg > nl && stamp[row] != bucket) {
+          stamp[row] = bucket;
+          first = true;
+        }
         if (first && pull_mode == 1) {
           push_edges += static_cast<unsigned long long>(deg - nl);
           settled_in += static_cast<unsigned long long>(in.off[row + 1] - in.off[row]);
@@ -370,7 +374,7 @@ sssp_relax(const int64_t *off, const int32_t *nbr, const int32_t *wgt, const int
       }
     }
     sq.push(first, static_cast<int32_t>(row));
-    wave_walk_batch<kRelaxBatch>(row, start, len, f);
+    wave_walk_batch_v<kRelaxBatch>(row, start, len, drow, f);
   }
   f.nearq.flush_block();
   sq.flush_block();
@@ -402,7 +406,7 @@ __device__ void relax_close(SsspState *st, SsspAux *aux, int phase, int cur, uns
 
 // The last workgroup to finish closes the round (relax_close).
 __global__ void __launch_bounds__(kBlock)
-sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, SsspIn in, unsigned long long *word, int32_t *near0,
+sssp_relax_heavy(const int2 *nw, SsspIn in, unsigned long long *word, int32_t *near0,
                  int32_t *near1, int64_t cap, int64_t delta, int has_heavy, int pull_mode, int relax_blocks,
                  int64_t m_in, SsspState *st, SsspAux *aux, HeavyQueue hq) {
   const volatile SsspState *vs = st;
@@ -411,13 +415,13 @@ sssp_relax_heavy(const int32_t *nbr, const int32_t *wgt, SsspIn in, unsigned lon
   if (vs->done || (phase != 0 && phase != 3 && phase != 4)) return;
   if (phase == 4) {
     PullEdge pf{in.nbr, in.wgt, word, kDistInf};
-    expand_heavy_batch<kRelaxBatch>(hq, pf);
+    expand_heavy_batch_v<kRelaxBatch>(hq, pf);
     spread_min(pf.best, aux->next_min);
   } else {
     __shared__ int32_t seg[kWavesPerBlock * kSeg];
     RelaxEdge f;
-    relax_setup(f, seg, nbr, wgt, word, near0, near1, cap, delta, st);
-    expand_heavy_batch<kRelaxBatch>(hq, f);
+    relax_setup(f, seg, nw, word, near0, near1, cap, delta, st);
+    expand_heavy_batch_v<kRelaxBatch>(hq, f);
     f.nearq.flush_block();
   }
   if (last_to_arrive(aux->arrive[0], aux->arrive_top[0]))
@@ -604,7 +608,7 @@ part_count(RowTiles rt, const int32_t *wgt, int64_t delta, int32_t *lc) {
 // Pass 2: each edge to its place.
 __global__ void __launch_bounds__(kBlock)
 part_place(RowTiles rt, const int32_t *nbr, const int32_t *wgt, int64_t delta, const int32_t *lc,
-           int32_t *pnbr, int32_t *pwgt) {
+           int2 *pnw) {
   __shared__ int32_t carry_all[kWavesPerBlock][2 * kWave];
   __shared__ int32_t wl[kWavesPerBlock], wh[kWavesPerBlock];
   const int lane = lane_id();
@@ -651,8 +655,7 @@ part_place(RowTiles rt, const int32_t *nbr, const int32_t *wgt, int64_t delta, c
       if (act) {
         const int64_t pos = light ? ostart + cl + __popcll(lm & before)
                                   : ostart + olight + ch + __popcll(hm & before);
-        pnbr[pos] = v;
-        pwgt[pos] = w;
+        pnw[pos] = make_int2(v, w);
       }
       const bool tail = act && (lane == kWave - 1 || e + 1 >= total || nxt != lo);
       if (tail) {
@@ -697,8 +700,7 @@ part_place(RowTiles rt, const int32_t *nbr, const int32_t *wgt, int64_t delta, c
       const uint64_t lt = (uint64_t(1) << lane) - 1;
       if (act) {
         const int64_t pos = light ? a + cl + bl + __popcll(lm & lt) : a + L + ch + bh + __popcll(hm & lt);
-        pnbr[pos] = v;
-        pwgt[pos] = w;
+        pnw[pos] = make_int2(v, w);
       }
       cl += tl;
       ch += th;
@@ -868,26 +870,22 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   if (g->part_delta != delta) {
     EventTimer layout_timer(ctx);
     auto *mg = const_cast<gbk_graph *>(g);
-    if (!mg->pnbr) {
-      void *a = graph_mem_alloc(ctx, sizeof(int32_t) * std::max<int64_t>(g->m, 1));
-      void *b = a ? graph_mem_alloc(ctx, sizeof(int32_t) * std::max<int64_t>(g->m, 1)) : nullptr;
-      void *c = b ? graph_mem_alloc(ctx, sizeof(int32_t) * std::max<int64_t>(n, 1)) : nullptr;
+    if (!mg->pnw) {
+      void *a = graph_mem_alloc(ctx, sizeof(int2) * std::max<int64_t>(g->m, 1));
+      void *c = a ? graph_mem_alloc(ctx, sizeof(int32_t) * std::max<int64_t>(n, 1)) : nullptr;
       if (!c) {
         if (a) graph_mem_free(ctx, a);
-        if (b) graph_mem_free(ctx, b);
         return set_error(ctx, GBK_ENOMEM, "sssp light/heavy layout");
       }
       mg->owned.push_back(a);
-      mg->owned.push_back(b);
       mg->owned.push_back(c);
-      mg->pnbr = static_cast<int32_t *>(a);
-      mg->pwgt = static_cast<int32_t *>(b);
+      mg->pnw = static_cast<int2 *>(a);
       mg->plc = static_cast<int32_t *>(c);
     }
     const RowTiles rt = row_tiles(g, false);
     const unsigned pgrid = grid_for(std::max<int64_t>(rt.ntiles, 1), kWavesPerBlock);
     part_count<<<pgrid, kBlock, 0, s>>>(rt, g->wgt, delta, mg->plc);
-    part_place<<<pgrid, kBlock, 0, s>>>(rt, g->nbr, g->wgt, delta, mg->plc, mg->pnbr, mg->pwgt);
+    part_place<<<pgrid, kBlock, 0, s>>>(rt, g->nbr, g->wgt, delta, mg->plc, mg->pnw);
     GBK_LAUNCH_CHECK(ctx, "sssp light/heavy layout");
     layout_ms = layout_timer.stop_ms();
     mg->part_delta = delta;
@@ -911,9 +909,9 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
                                                           ctx->kt_dev + 1, 1);
       }
       const int kt = kt_begin(ctx, GBK_KT_SSSP_RELAX);
-      sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->pnbr, g->pwgt, g->plc, in, word, near0, near1,
+      sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->pnw, g->plc, in, word, near0, near1,
                                                settled, stamp, cap, delta, H, has_heavy, pull_mode, st, aux, hq);
-      sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->pnbr, g->pwgt, in, word, near0, near1, cap, delta,
+      sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->pnw, in, word, near0, near1, cap, delta,
                                                      has_heavy, pull_mode, static_cast<int>(relax_grid), g->m,
                                                      st, aux, hq);
       kt_end(ctx, kt);
