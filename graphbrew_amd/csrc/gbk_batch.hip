@@ -1057,10 +1057,7 @@ bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigm
   f.leader = f.k == 0;
   const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  // edges walked, counted in LDS (a register held across the loop spills)
-  __shared__ unsigned long long walked;
-  if (threadIdx.x == 0) walked = 0;
-  __syncthreads();
+  unsigned long long walked = 0;
   for (int64_t base = gw * G; base < count; base += waves * G) {
     const int64_t i = base + grp;
     const bool valid = i < count;
@@ -1075,11 +1072,10 @@ bcb_forward(const int64_t *off, const int32_t *nbr, int32_t *depth, double *sigm
     const bool heavy = len > kBatchHeavy;
     if (heavy && f.leader) heavy_push(hq, static_cast<int32_t>(row), s, len);
     bcb_edges<Kp>(nbr, s, heavy ? 0 : len, f);
-    if (f.leader && !heavy) atomicAdd(&walked, static_cast<unsigned long long>(len));
+    if (f.leader && !heavy) walked += static_cast<unsigned long long>(len);
   }
   f.q.flush_block();
-  __syncthreads();
-  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+  block_add(walked, &c->edges_scanned);
 }
 
 template <int Kp>
@@ -1105,10 +1101,7 @@ bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *l
   f.leader = f.k == 0;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  // edges walked, counted in LDS (a register held across the loop spills)
-  __shared__ unsigned long long walked;
-  if (threadIdx.x == 0) walked = 0;
-  __syncthreads();
+  unsigned long long walked = 0;
   for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
     const int64_t cc = c0 + grp;
     const bool valid = cc < total;
@@ -1120,11 +1113,10 @@ bcb_forward_heavy(const int32_t *nbr, int32_t *depth, double *sigma, uint32_t *l
     f.act = valid && depth[row * Kp + f.k] == d;
     f.su = f.act ? sigma[row * Kp + f.k] : 0.0;
     bcb_edges<Kp>(nbr, valid ? hq.start[h] + first : 0, len, f);
-    if (f.leader) atomicAdd(&walked, static_cast<unsigned long long>(len));
+    if (f.leader) walked += static_cast<unsigned long long>(len);
   }
   f.q.flush_block();
-  __syncthreads();
-  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+  block_add(walked, &c->edges_scanned);
 }
 
 // Bottom-up forward level d, for levels whose frontier has more out-edges
@@ -1174,10 +1166,7 @@ bcb_forward_pull(const int64_t *ioff, const int32_t *inbr, int64_t n, int K, int
   const bool leader = f.k == 0;
   const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  // edges walked, counted in LDS (a register held across the loop spills)
-  __shared__ unsigned long long walked;
-  if (threadIdx.x == 0) walked = 0;
-  __syncthreads();
+  unsigned long long walked = 0;
   for (int64_t base = gw * G; base < n; base += waves * G) {
     const int64_t v = base + grp;
     const bool valid = v < n;
@@ -1199,11 +1188,10 @@ bcb_forward_pull(const int64_t *ioff, const int32_t *inbr, int64_t n, int K, int
     }
     const bool anyf = group_any<Kp>(found);
     q.push(leader && anyf, static_cast<int32_t>(v));
-    if (leader && !heavy) atomicAdd(&walked, static_cast<unsigned long long>(len));
+    if (leader && !heavy) walked += static_cast<unsigned long long>(len);
   }
   q.flush_block();
-  __syncthreads();
-  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+  block_add(walked, &c->edges_scanned);
 }
 
 // Heavy rows of a bottom-up level: hpart[chunk][k] = lane k's sigma sum over
@@ -1226,10 +1214,7 @@ bcb_pull_heavy(const int32_t *inbr, int K, const int32_t *depth, const double *s
   f.k = lane_id() % Kp;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  // edges walked, counted in LDS (a register held across the loop spills)
-  __shared__ unsigned long long walked;
-  if (threadIdx.x == 0) walked = 0;
-  __syncthreads();
+  unsigned long long walked = 0;
   for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
     const int64_t cc = c0 + grp;
     const bool valid = cc < total;
@@ -1242,10 +1227,9 @@ bcb_pull_heavy(const int32_t *inbr, int K, const int32_t *depth, const double *s
     f.acc = 0.0;
     bcb_edges<Kp>(inbr, valid ? hq.start[h] + first : 0, len, f);
     if (valid) hpart[cc * Kp + f.k] = f.acc;
-    if (f.k == 0) atomicAdd(&walked, static_cast<unsigned long long>(len));
+    if (f.k == 0) walked += static_cast<unsigned long long>(len);
   }
-  __syncthreads();
-  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+  block_add(walked, &c->edges_scanned);
 }
 
 // One wave per heavy row of a bottom-up level: its chunks' sums meet as in
@@ -1354,10 +1338,7 @@ bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const
   const bool leader = f.k == 0;
   const int64_t waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  // edges walked, counted in LDS (a register held across the loop spills)
-  __shared__ unsigned long long walked;
-  if (threadIdx.x == 0) walked = 0;
-  __syncthreads();
+  unsigned long long walked = 0;
   for (int64_t base = gw * G; base < count; base += waves * G) {
     const int64_t i = base + grp;
     const bool valid = i < count;
@@ -1380,10 +1361,9 @@ bcb_backward(const int64_t *off, const int32_t *nbr, const int32_t *depth, const
     }
     const double tot = group_sum_d<Kp>(delta);
     if (leader && valid && !heavy && tot != 0.0) scores[row] += static_cast<float>(tot);
-    if (leader && !heavy) atomicAdd(&walked, static_cast<unsigned long long>(len));
+    if (leader && !heavy) walked += static_cast<unsigned long long>(len);
   }
-  __syncthreads();
-  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+  block_add(walked, &c->edges_scanned);
 }
 
 // Heavy rows of a backward level: hpart[chunk][k] = lane k's coefficient
@@ -1403,10 +1383,7 @@ bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const uint64_t *dc,
   f.k = lane_id() % Kp;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock * G;
   const int64_t gw = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  // edges walked, counted in LDS (a register held across the loop spills)
-  __shared__ unsigned long long walked;
-  if (threadIdx.x == 0) walked = 0;
-  __syncthreads();
+  unsigned long long walked = 0;
   for (int64_t c0 = gw * G; c0 < total; c0 += stride) {
     const int64_t cc = c0 + grp;
     const bool valid = cc < total;
@@ -1419,10 +1396,9 @@ bcb_backward_heavy(const int32_t *nbr, const int32_t *depth, const uint64_t *dc,
     f.acc = 0.0;
     bcb_edges<Kp>(nbr, valid ? hq.start[h] + first : 0, len, f);
     if (valid) hpart[cc * Kp + f.k] = f.acc;
-    if (f.k == 0) atomicAdd(&walked, static_cast<unsigned long long>(len));
+    if (f.k == 0) walked += static_cast<unsigned long long>(len);
   }
-  __syncthreads();
-  if (threadIdx.x == 0 && walked) atomicAdd(&c->edges_scanned, walked);
+  block_add(walked, &c->edges_scanned);
 }
 
 // One wave per heavy row: lane (slot, k) adds source k's partials of chunks
