@@ -6,7 +6,9 @@ keeps the fastest candidate whose output equals the default's (integer
 results: bit-exact; PR: 1e-4 relative).  Swept, per kernel:
 
   sssp   delta in {1, 8, 32, 64, 128, 256, 512}           (CLDelta -d)
-         then relax_grid in {256, 512, 1024, 2048} workgroups
+         then relax_grid in {256, 512, 1024, 2048} workgroups, then the
+         rows handed to the all-workgroup kernel, sssp_heavy in {256, 512,
+         1024, 2048} edges
   sssp batch  bucket width in {64, 256, 1024, 4096, 2^20}
   bfs    bu_group in {1, 2, 4, 8, 16} x (alpha, beta) grid (bfs.cc:165)
   pr     Gauss-Seidel row blocks in {1, 2, 4, 8, 16} x in place / published
@@ -37,6 +39,7 @@ MSBFS_GROUPS = (4, 8, 16, 32, 64)
 ALPHA_BETA = ((15, 18), (8, 18), (30, 18), (15, 9), (15, 36))
 HEAVY = (1024, 2048, 4096, 8192)
 GRIDS = (256, 512, 1024, 2048)
+SSSP_HEAVY = (256, 512, 1024, 2048)
 BATCH_DELTAS = (64, 256, 1024, 4096, 1 << 20)
 PR_BLOCKS = (1, 2, 4, 8, 16)
 BC_PULL = ((0, 1), (2, 1), (1, 1), (1, 2), (1, 4))   # (bc_pull, bc_pull_ratio)
@@ -87,24 +90,31 @@ class Autotuner:
             self.log.append({"kernel": "sssp", "delta": d, "ms": ms, "wall": time.time() - t0})
             if best is None or ms < best["ms"]:
                 best = {"delta": d, "ms": ms}
-        # then the launch width of the relaxation kernel at the chosen delta
-        saved = self.ctx.get_param("relax_grid")
-        best["relax_grid"] = saved
-        for grid in GRIDS:
-            if grid == saved:
-                continue
-            self.ctx.set_param("relax_grid", grid)
-            ms = 0.0
-            for s, r in zip(srcs, ref):
-                out, _ = self.ctx.sssp(g, s, best["delta"])
-                if not np.array_equal(out, r):
-                    raise AssertionError(f"sssp relax_grid={grid} changed distances from source {s}")
-                ms += _median([self.ctx.sssp(g, s, best["delta"], download=False)[1].device_ms
-                               for _ in range(self.repeats)])
-            self.log.append({"kernel": "sssp", "delta": best["delta"], "relax_grid": grid, "ms": ms})
-            if ms < best["ms"]:
-                best = {"delta": best["delta"], "relax_grid": grid, "ms": ms}
-        self.ctx.set_param("relax_grid", saved)
+        # then, one at a time at the chosen delta, the launch width of the
+        # relaxation kernel and the rows handed to the all-workgroup kernel
+        sweeps = (("relax_grid", GRIDS), ("sssp_heavy", SSSP_HEAVY))
+        restore = {name: self.ctx.get_param(name) for name, _ in sweeps}
+        for name, values in sweeps:
+            saved = restore[name]
+            best[name] = saved
+            for value in values:
+                if value == saved:
+                    continue
+                self.ctx.set_param(name, value)
+                ms = 0.0
+                for s, r in zip(srcs, ref):
+                    out, _ = self.ctx.sssp(g, s, best["delta"])
+                    if not np.array_equal(out, r):
+                        raise AssertionError(f"sssp {name}={value} changed distances from source {s}")
+                    ms += _median([self.ctx.sssp(g, s, best["delta"], download=False)[1].device_ms
+                                   for _ in range(self.repeats)])
+                self.log.append({"kernel": "sssp", "delta": best["delta"], name: value, "ms": ms})
+                if ms < best["ms"]:
+                    best = dict(best, **{name: value, "ms": ms})
+            # the next sweep runs at the best value found so far
+            self.ctx.set_param(name, best[name])
+        for name, value in restore.items():
+            self.ctx.set_param(name, value)
         self.cache[k] = best
         self._save()
         return best
@@ -267,7 +277,7 @@ class Autotuner:
         return best
 
     def apply(self, tuned: dict) -> None:
-        for name in ("bu_group", "heavy_degree", "relax_grid", "pr_blocks", "pr_inplace", "bc_pull",
+        for name in ("bu_group", "heavy_degree", "relax_grid", "sssp_heavy", "pr_blocks", "pr_inplace", "bc_pull",
                      "bc_pull_ratio", "msbfs_group"):
             if name in tuned:
                 self.ctx.set_param(name, tuned[name])

@@ -13,7 +13,7 @@ import pytest
 
 import oracle as O
 from graphbrew_amd import gbk
-from graphbrew_amd.autotune import Autotuner, DELTAS, BU_GROUPS, ALPHA_BETA
+from graphbrew_amd.autotune import Autotuner, DELTAS, BU_GROUPS, ALPHA_BETA, SSSP_HEAVY
 
 pytestmark = pytest.mark.gpu
 
@@ -139,8 +139,11 @@ def test_autotuner_sssp_and_bfs(ctx, tmp_path):
     cache = str(tmp_path / "tune.json")
     t = Autotuner(ctx, repeats=1, cache=cache)
     srcs = O.source_picker(g, 3)[1]
+    heavy = ctx.get_param("sssp_heavy")
     best = t.tune_sssp(dg, srcs)
     assert best["delta"] in DELTAS and best["ms"] > 0
+    assert best["sssp_heavy"] in SSSP_HEAVY + (heavy,)
+    assert ctx.get_param("sssp_heavy") == heavy  # the sweep leaves the context as it was
     assert min(r["ms"] for r in t.log if r["kernel"] == "sssp") == best["ms"]
     bb = t.tune_bfs(dg, srcs)
     assert bb["bu_group"] in BU_GROUPS and (bb["alpha"], bb["beta"]) in ALPHA_BETA
