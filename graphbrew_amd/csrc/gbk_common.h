@@ -301,6 +301,11 @@ struct gbk_ctx {
   const gbk_graph *sssp_hint_graph = nullptr;
   int64_t sssp_hint_delta = 0;
   int64_t sssp_hint_steps = 0;
+  // The same for SSSP source batches, keyed by the batch's shape.
+  struct BatchHint {
+    int64_t n, m, k, delta, steps;
+  };
+  BatchHint sssp_batch_hint{0, 0, 0, 0, 0};
 };
 
 namespace gbk {
