@@ -252,8 +252,8 @@ def launch_bytes(family, kt, work, rec):
 
 
 def load_traffic(workload, family, single, field="bytes_per_launch"):
-    """HBM bytes per launch of the followed kernel (per call for the BFS
-    batch), measured with rocprofv3 PMC counters for this workload and mode
+    """HBM bytes of the followed kernel, per launch or per call (field),
+    measured with rocprofv3 PMC counters for this workload and mode
     (profiles/traffic.json)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -656,9 +656,18 @@ def main():
         avg_s = kt["ms"] / kt["launches"] / 1e3
         achieved = per_launch / avg_s / 1e9
         dom = max(rec, key=lambda f: rec[f][0])
+        # counted traffic: per launch, or per call spread over this run's
+        # launches (the number of blind no-op launches per call varies; they
+        # read nothing)
+        traffic = load_traffic(workload, family, args.single_source)
+        per_call = load_traffic(workload, family, args.single_source, "bytes_per_call")
+        if traffic is None and per_call is not None and family == "sssp":
+            sources = args.sssp_sources
+            calls = args.steps * (sources if args.single_source else -(-sources // 64))
+            traffic = round(per_call * calls / kt["launches"])
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(achieved * 1e9 / HBM_PEAK, 4),
-                "traffic": load_traffic(workload, family, args.single_source),
+                "traffic": traffic,
                 "kernel": kts[family][1], "family": family, "dominant_family": dom,
                 "launches": kt["launches"], "avg_launch_us": round(kt["ms"] / kt["launches"] * 1e3, 2),
                 "bytes_per_launch": round(per_launch), "bytes_basis": how,
