@@ -48,6 +48,8 @@ struct Tuning {
   int sssp_wide = 1;            // SSSP batches of 17..32 sources on 64-lane rows
   int64_t sssp_heavy = 512;     // one-source SSSP: rows with more edges to walk go to the
                                 // workgroup-wide heavy kernel
+  int sssp_tail = 1;            // one-source SSSP: after a pulled heavy pass, the vertices left are
+                                // relaxed Bellman-Ford style to the end (0: buckets to the end)
   int sssp_pull = 1;            // one-source SSSP heavy pass: 0 pushed from the bucket's vertices,
                                 // 1 pulled into the unsettled vertices when that reads fewer
                                 // edges, 2 always pulled

@@ -485,6 +485,7 @@ int gbk_ctx_set_param(gbk_ctx *ctx, const char *name, int64_t value) {
   else if (k == "sssp_wide" && (value == 0 || value == 1)) t.sssp_wide = static_cast<int>(value);
   else if (k == "sssp_pull" && value >= 0 && value <= 2) t.sssp_pull = static_cast<int>(value);
   else if (k == "sssp_heavy" && value >= 64) t.sssp_heavy = value;
+  else if (k == "sssp_tail" && (value == 0 || value == 1)) t.sssp_tail = static_cast<int>(value);
   else if (k == "msbfs_group" && value >= 1 && value <= 64 && (value & (value - 1)) == 0)
     t.msbfs_group = static_cast<int>(value);
   else if (k == "pr_segment" && (value == -1 || value == 0 || (value >= 6 && value <= 26)))
@@ -524,6 +525,7 @@ int gbk_ctx_get_param(gbk_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "sssp_wide") *value = t.sssp_wide;
   else if (k == "sssp_pull") *value = t.sssp_pull;
   else if (k == "sssp_heavy") *value = t.sssp_heavy;
+  else if (k == "sssp_tail") *value = t.sssp_tail;
   else if (k == "msbfs_group") *value = t.msbfs_group;
   else if (k == "pr_segment") *value = t.pr_segment;
   else if (k == "ktime") *value = ctx->ktime;

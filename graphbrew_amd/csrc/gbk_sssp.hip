@@ -68,6 +68,9 @@ struct SsspState {
   long long relax_rounds;
   long long buckets;
   long long steps;  // steps that found the search still open
+  // 1: the buckets are over and the rest runs Bellman-Ford style (rounds
+  // over every edge of the vertices improved in the round before)
+  int tail;
   int cur, phase, done, overflow;
 };
 
@@ -117,6 +120,7 @@ __global__ void sssp_init(int64_t n, int32_t source, unsigned long long *word, i
     st->relax_rounds = 0;
     st->buckets = 0;
     st->steps = 0;
+    st->tail = 0;
     st->cur = 0;
     st->phase = 0;
     st->done = 0;
@@ -288,7 +292,8 @@ __device__ __forceinline__ void relax_setup(RelaxEdge &f, int32_t *seg, const in
   f.word = word;
   f.nearq.init(seg + (threadIdx.x / kWave) * kSeg, &st->near_cnt[1 - cur], cur ? near0 : near1, cap,
                &st->overflow);
-  f.upper = delta * (vs->bucket + 1);
+  // the tail has no bucket end: every improvement is queued
+  f.upper = vs->tail ? INT64_MAX : delta * (vs->bucket + 1);
   f.tag = static_cast<uint32_t>(vs->relax_rounds + 1);
 }
 
@@ -339,7 +344,8 @@ sssp_relax(const int64_t *off, const int2 *nw, const int32_t *lc, SsspIn in,
   relax_setup(f, seg, nw, word, near0, near1, cap, delta, st);
   WaveQueue sq;
   sq.init(sseg + (threadIdx.x / kWave) * kSeg, &st->settled_cnt, settled, cap, &st->overflow);
-  const int64_t lower = f.upper - delta;
+  const bool tail = vs->tail;
+  const int64_t lower = tail ? 0 : f.upper - delta;
   const int32_t bucket = static_cast<int32_t>(vs->bucket);
   unsigned long long push_edges = 0, settled_in = 0;
   for (int64_t tile = blockIdx.x; tile * kBlock < count; tile += gridDim.x) {
@@ -356,10 +362,10 @@ sssp_relax(const int64_t *off, const int2 *nw, const int32_t *lc, SsspIn in,
         len = deg - nl;
       } else if (drow >= lower) {
         start = off[row];
-        len = nl;
+        len = tail ? deg : nl;
         // a round's list holds a vertex at most once, so no other lane
         // touches its stamp in this launch
-        if (has_heavy && deg > nl && stamp[row] != bucket) {
+        if (!tail && has_heavy && deg > nl && stamp[row] != bucket) {
           stamp[row] = bucket;
           first = true;
         }
@@ -401,17 +407,19 @@ sssp_relax(const int64_t *off, const int2 *nw, const int32_t *lc, SsspIn in,
 }
 
 // Heavy rows of the round's list, spread over all workgroups.
-__device__ void relax_close(SsspState *st, SsspAux *aux, int phase, int cur, unsigned long long *heavy_count,
-                            int has_heavy, int pull_mode, int relax_blocks, int64_t n, int64_t m_in, int64_t delta);
+__device__ void relax_close(SsspState *st, SsspAux *aux, int phase, int cur, int tail,
+                            unsigned long long *heavy_count, int has_heavy, int pull_mode, int tail_mode,
+                            int relax_blocks, int64_t n, int64_t m_in, int64_t delta);
 
 // The last workgroup to finish closes the round (relax_close).
 __global__ void __launch_bounds__(kBlock)
 sssp_relax_heavy(const int2 *nw, SsspIn in, unsigned long long *word, int32_t *near0,
-                 int32_t *near1, int64_t cap, int64_t delta, int has_heavy, int pull_mode, int relax_blocks,
-                 int64_t m_in, SsspState *st, SsspAux *aux, HeavyQueue hq) {
+                 int32_t *near1, int64_t cap, int64_t delta, int has_heavy, int pull_mode, int tail_mode,
+                 int relax_blocks, int64_t m_in, SsspState *st, SsspAux *aux, HeavyQueue hq) {
   const volatile SsspState *vs = st;
   const int phase = vs->phase;
   const int cur = vs->cur;
+  const int tail = vs->tail;
   if (vs->done || (phase != 0 && phase != 3 && phase != 4)) return;
   if (phase == 4) {
     PullEdge pf{in.nbr, in.wgt, word, kDistInf};
@@ -425,7 +433,8 @@ sssp_relax_heavy(const int2 *nw, SsspIn in, unsigned long long *word, int32_t *n
     f.nearq.flush_block();
   }
   if (last_to_arrive(aux->arrive[0], aux->arrive_top[0]))
-    relax_close(st, aux, phase, cur, hq.count, has_heavy, pull_mode, relax_blocks, in.n, m_in, delta);
+    relax_close(st, aux, phase, cur, tail, hq.count, has_heavy, pull_mode, tail_mode, relax_blocks, in.n, m_in,
+                delta);
 }
 
 // Work of the coming relaxation round, for the kernel timer only: rows of
@@ -451,14 +460,15 @@ __global__ void sssp_count_round(const int64_t *off, const int32_t *lc, SsspIn i
     const bool heavy = vs->phase == 3;
     const int32_t *q = heavy ? settled : (cur ? near1 : near0);
     const int64_t count = static_cast<int64_t>(heavy ? vs->settled_cnt : vs->near_cnt[cur]);
-    const int64_t lower = heavy ? 0 : delta * vs->bucket;
+    const bool tail = vs->tail;
+    const int64_t lower = heavy || tail ? 0 : delta * vs->bucket;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
       const int32_t u = q[i];
       if (word_dist(word[u]) >= lower) {
         rows++;
         const int64_t deg = off[u + 1] - off[u];
-        edges += static_cast<unsigned long long>(heavy ? deg - lc[u] : lc[u]);
+        edges += static_cast<unsigned long long>(heavy ? deg - lc[u] : tail ? deg : lc[u]);
       }
     }
   }
@@ -474,20 +484,23 @@ __global__ void sssp_count_round(const int64_t *off, const int32_t *lc, SsspIn i
 // all vertices (n / 8 edges' worth) are fewer than the bucket's heavy edges.
 // After a push the search for the next bucket opens (phase 1); a pull has
 // found the next bucket itself (phase 2, or done when nothing is left).
-__device__ void relax_close(SsspState *st, SsspAux *aux, int phase, int cur, unsigned long long *heavy_count,
-                            int has_heavy, int pull_mode, int relax_blocks, int64_t n, int64_t m_in, int64_t delta) {
-  // Every thread of the closing workgroup runs this.  phase and cur are the
-  // launch's own (read by every workgroup at its start); values other
+__device__ void relax_close(SsspState *st, SsspAux *aux, int phase, int cur, int tail,
+                            unsigned long long *heavy_count, int has_heavy, int pull_mode, int tail_mode,
+                            int relax_blocks, int64_t n, int64_t m_in, int64_t delta) {
+  // Every thread of the closing workgroup runs this.  phase, cur and tail
+  // are the launch's own (read by every workgroup at its start); values other
   // workgroups updated by atomics are read with agent loads, counters are
   // bumped by atomics, and the state is written by thread 0.
   if (phase == 4) {
+    // after a pull the vertices left have few edges: with tail_mode they are
+    // all queued (phase 5) and relaxed Bellman-Ford style to the end
     const unsigned long long nm = spread_take_min(aux->next_min);
     if (threadIdx.x == 0) {
       if (nm == ULLONG_MAX) {
         st->done = 1;
       } else {
         st->next_bucket = nm / static_cast<unsigned long long>(delta);
-        st->phase = 2;
+        st->phase = tail_mode ? 5 : 2;
       }
     }
   }
@@ -511,6 +524,10 @@ __device__ void relax_close(SsspState *st, SsspAux *aux, int phase, int cur, uns
   }
   __syncthreads();
   if (open) return;
+  if (tail) {
+    if (threadIdx.x == 0) st->done = 1;
+    return;
+  }
   // the bucket's light rounds are over: push or pull its heavy edges
   if (has_heavy && pull_mode == 1) {
     __shared__ unsigned long long red[2][kWavesPerBlock];
@@ -788,15 +805,19 @@ __global__ void sssp_min_bucket(const unsigned long long *word, int64_t n, int64
 }
 
 // Queues every vertex of the chosen bucket (each vertex once); the last
-// workgroup makes it the current bucket.
+// workgroup makes it the current bucket.  Opening the tail (phase 5), it
+// queues every reached vertex above the last bucket instead.
 __global__ void __launch_bounds__(kBlock)
 sssp_split(const unsigned long long *word, int64_t n, int32_t *near0, int32_t *near1, int64_t cap,
            int64_t delta, SsspState *st, SsspAux *aux) {
   const volatile SsspState *vs = st;
-  if (vs->done || vs->phase != 2) return;
+  const int phase = vs->phase;
+  if (vs->done || (phase != 2 && phase != 5)) return;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   const int cur = vs->cur;
   const int64_t nb = static_cast<int64_t>(vs->next_bucket);
+  // phase 5 opens the tail: every reached vertex above the last bucket
+  const int64_t above = delta * (vs->bucket + 1);
   WaveQueue q;
   q.init(seg + (threadIdx.x / kWave) * kSeg, &st->near_cnt[cur], cur ? near1 : near0, cap,
          &st->overflow);
@@ -806,7 +827,7 @@ sssp_split(const unsigned long long *word, int64_t n, int32_t *near0, int32_t *n
     bool take = false;
     if (v < n) {
       const int64_t d = word_dist(word[v]);
-      take = d < kDistInf && d / delta == nb;
+      take = d < kDistInf && (phase == 5 ? d >= above : d / delta == nb);
     }
     q.push(take, static_cast<int32_t>(v));
   }
@@ -814,6 +835,7 @@ sssp_split(const unsigned long long *word, int64_t n, int32_t *near0, int32_t *n
   if (last_to_arrive(aux->arrive[2], aux->arrive_top[2]) && threadIdx.x == 0) {
     st->bucket = nb;
     __hip_atomic_fetch_add(&st->buckets, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (phase == 5) st->tail = 1;
     st->phase = 0;
   }
 }
@@ -894,6 +916,7 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
   // the pull reads the in-CSR's weights (the out-CSR's on undirected graphs)
   const SsspIn in{g->in_off, g->in_nbr, g->in_wgt, n};
   const int pull_mode = in.off && in.nbr && in.wgt ? ctx->tune.sssp_pull : 0;
+  const int tail_mode = ctx->tune.sssp_tail;
   const bool count = ctx->ktime == GBK_KT_SSSP_RELAX && ctx->kt_dev;
   const unsigned count_grid = static_cast<unsigned>(kKtPartials < 256 ? kKtPartials : 256);
   const unsigned scan_grid = expand_grid(n);
@@ -912,8 +935,8 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
       sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->pnw, g->plc, in, word, near0, near1,
                                                settled, stamp, cap, delta, H, has_heavy, pull_mode, st, aux, hq);
       sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->pnw, in, word, near0, near1, cap, delta,
-                                                     has_heavy, pull_mode, static_cast<int>(relax_grid), g->m,
-                                                     st, aux, hq);
+                                                     has_heavy, pull_mode, tail_mode,
+                                                     static_cast<int>(relax_grid), g->m, st, aux, hq);
       kt_end(ctx, kt);
       sssp_min_bucket<<<scan_grid, kBlock, 0, s>>>(word, n, delta, st, aux);
       sssp_split<<<scan_grid, kBlock, 0, s>>>(word, n, near0, near1, cap, delta, st, aux);

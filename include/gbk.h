@@ -122,7 +122,9 @@ int gbk_ctx_sync(gbk_ctx *ctx);
  * (one-source SSSP's pass over a settled bucket's heavy edges: 0 pushed from the bucket's
  * vertices, 1 (default) pulled into the unsettled vertices when their edges are fewer, 2 always
  * pulled); "sssp_heavy" (one-source SSSP: rows with more edges to walk than this, default 512, go to
- * a kernel that spreads them over all workgroups); "msbfs_group"
+ * a kernel that spreads them over all workgroups); "sssp_tail" (1, default: after a pulled heavy pass
+ * the vertices left are relaxed Bellman-Ford style, every edge of each improved vertex per round,
+ * to the end; 0: buckets to the end); "msbfs_group"
  * (lanes per vertex in the BFS batch's bottom-up step: 1, 2, 4, ..., 64; default 32); and
  * "ktime" (a gbk_ktime_kernel) selecting the kernel whose
  * launches are timed one by one with HIP events (loops then run eagerly).  Returns

@@ -142,7 +142,8 @@ def test_sssp_parity(ctx, name):
 @pytest.mark.parametrize("sssp_heavy", [512, 64])
 def test_sssp_heavy_pass_push_and_pull(ctx, name, sssp_heavy):
     """The heavy pass pushed from the bucket's vertices (sssp_pull 0), pulled
-    into the unsettled ones (2) and chosen by edge counts (1) give the same
+    into the unsettled ones (2) and chosen by edge counts (1), each with and
+    without the Bellman-Ford tail after a pull (sssp_tail), give the same
     exact distances; a row threshold of 64 sends the pull's big rows through
     the heavy-row kernel."""
     g = SSSP_GRAPHS[name]()
@@ -155,12 +156,15 @@ def test_sssp_heavy_pass_push_and_pull(ctx, name, sssp_heavy):
             ref = O.dijkstra(g, s)
             for mode in (0, 2, 1):
                 ctx.set_param("sssp_pull", mode)
-                for delta in (4, 32, 64):
-                    dist, st = ctx.sssp(dg, s, delta)
-                    assert np.array_equal(dist, ref), (name, s, mode, delta)
-                    assert st.reached == int((ref != O.K_DIST_INF).sum())
+                for tail in (0, 1):
+                    ctx.set_param("sssp_tail", tail)
+                    for delta in (4, 32, 64):
+                        dist, st = ctx.sssp(dg, s, delta)
+                        assert np.array_equal(dist, ref), (name, s, mode, tail, delta)
+                        assert st.reached == int((ref != O.K_DIST_INF).sum())
     finally:
         ctx.set_param("sssp_pull", 1)
+        ctx.set_param("sssp_tail", 1)
         ctx.set_param("sssp_heavy", 512)
 
 
