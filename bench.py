@@ -43,7 +43,7 @@ def parse():
     p.add_argument("--bfs-sources", type=int, default=64)
     p.add_argument("--sssp-sources", type=int, default=64)
     p.add_argument("--bc-sources", type=int, default=4)
-    p.add_argument("--delta", type=int, default=48, help="delta of one-source SSSP runs")
+    p.add_argument("--delta", type=int, default=40, help="delta of one-source SSSP runs")
     p.add_argument("--batch-delta", type=int, default=1024,
                    help="bucket width of the 64-source SSSP batch (distances do not depend on it)")
     p.add_argument("--no-cpu-baseline", action="store_true")

@@ -5,7 +5,7 @@ with the device time the library reports (hipEvents on its stream), and
 keeps the fastest candidate whose output equals the default's (integer
 results: bit-exact; PR: 1e-4 relative).  Swept, per kernel:
 
-  sssp   delta in {1, 8, 32, 48, 64, 128, 256, 512}       (CLDelta -d)
+  sssp   delta in {1, 8, 32, 40, 48, 64, 128, 256, 512}   (CLDelta -d)
          then relax_grid in {256, 512, 1024, 2048} workgroups, then the
          rows handed to the all-workgroup kernel, sssp_heavy in {256, 512,
          1024, 2048} edges
@@ -33,7 +33,7 @@ import numpy as np
 
 from . import gbk
 
-DELTAS = (1, 8, 32, 48, 64, 128, 256, 512)
+DELTAS = (1, 8, 32, 40, 48, 64, 128, 256, 512)
 BU_GROUPS = (1, 2, 4, 8, 16)
 MSBFS_GROUPS = (4, 8, 16, 32, 64)
 ALPHA_BETA = ((15, 18), (8, 18), (30, 18), (15, 9), (15, 36))

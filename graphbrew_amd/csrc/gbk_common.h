@@ -46,7 +46,7 @@ struct Tuning {
                                 // frontier has more out-edges than the unlisted vertices, 2 bottom-up
   int bc_pull_ratio = 1;        // a frontier edge weighs this many open edges in that choice
   int sssp_wide = 1;            // SSSP batches of 17..32 sources on 64-lane rows
-  int64_t sssp_heavy = 512;     // one-source SSSP: rows with more edges to walk go to the
+  int64_t sssp_heavy = 256;     // one-source SSSP: rows with more edges to walk go to the
                                 // workgroup-wide heavy kernel
   int sssp_tail = 1;            // one-source SSSP: after a pulled heavy pass, the vertices left are
                                 // relaxed Bellman-Ford style to the end (0: buckets to the end)

@@ -121,7 +121,7 @@ int gbk_ctx_sync(gbk_ctx *ctx);
  * (1, default: SSSP batches of 17..32 sources run on 64-lane rows; 0: 32-lane rows); "sssp_pull"
  * (one-source SSSP's pass over a settled bucket's heavy edges: 0 pushed from the bucket's
  * vertices, 1 (default) pulled into the unsettled vertices when their edges are fewer, 2 always
- * pulled); "sssp_heavy" (one-source SSSP: rows with more edges to walk than this, default 512, go to
+ * pulled); "sssp_heavy" (one-source SSSP: rows with more edges to walk than this, default 256, go to
  * a kernel that spreads them over all workgroups); "sssp_tail" (1, default: after a pulled heavy pass
  * the vertices left are relaxed Bellman-Ford style, every edge of each improved vertex per round,
  * to the end; 0: buckets to the end); "msbfs_group"

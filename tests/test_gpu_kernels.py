@@ -139,7 +139,7 @@ def test_sssp_parity(ctx, name):
 
 
 @pytest.mark.parametrize("name", ["kron13w", "uniform12w", "star9000w", "directed_w"])
-@pytest.mark.parametrize("sssp_heavy", [512, 64])
+@pytest.mark.parametrize("sssp_heavy", [256, 64])
 def test_sssp_heavy_pass_push_and_pull(ctx, name, sssp_heavy):
     """The heavy pass pushed from the bucket's vertices (sssp_pull 0), pulled
     into the unsettled ones (2) and chosen by edge counts (1), each with and
@@ -165,7 +165,7 @@ def test_sssp_heavy_pass_push_and_pull(ctx, name, sssp_heavy):
     finally:
         ctx.set_param("sssp_pull", 1)
         ctx.set_param("sssp_tail", 1)
-        ctx.set_param("sssp_heavy", 512)
+        ctx.set_param("sssp_heavy", 256)
 
 
 def test_sssp_weighted_path_closed_form(ctx):
