@@ -130,6 +130,35 @@ def test_sssp_batch_equals_single_source_counts(ctx):
     dg.free()
 
 
+def test_sssp_step_counts_reused_across_graphs_of_one_shape(ctx):
+    """A call queues as many steps as the last one of its shape took before
+    it first reads the state: a relabelled copy has the same shape but other
+    step counts, and a path needs many more steps than the Kronecker graph
+    before it; every call stays exact."""
+    g = O.kron(12, weighted=True)
+    ids = O.mapping_random(g.n)
+    rg = O.relabel(g, ids)
+    dg = ctx.upload_graph(g)
+    drg, _ = ctx.relabel(dg, ids)
+    srcs = _sources(g, 16, seed=7)
+    for graph, dev, ss in ((g, dg, srcs), (rg, drg, ids[srcs]), (g, dg, srcs[::-1].copy())):
+        refs = [O.dijkstra(graph, int(s)) for s in ss]
+        dist, _, _, _ = ctx.sssp_batch(dev, ss, 1024)
+        for j, s in enumerate(ss):
+            assert np.array_equal(dist[j], refs[j])
+        for s in ss[:3]:
+            d1, _ = ctx.sssp(dev, int(s), 40)
+            assert np.array_equal(d1, O.dijkstra(graph, int(s)))
+    path = _fixture("weighted_path.wel", weighted=True, symmetrize=False)
+    dp = ctx.upload_graph(path)
+    for delta in (1, 1, 2):
+        d1, _ = ctx.sssp(dp, 0, delta)
+        assert np.array_equal(d1, O.dijkstra(path, 0))
+    drg.free()
+    dg.free()
+    dp.free()
+
+
 def test_sssp_batch_long_weighted_path(ctx):
     """64 sources on a weighted path: distances far above 65535, hundreds
     of buckets and rounds; every source equals Dijkstra."""
