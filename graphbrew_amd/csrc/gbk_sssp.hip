@@ -39,10 +39,15 @@
 // accesses, not by bandwidth.
 // Kernels of one step: relax and relax-heavy (a light round in phase 0, the
 // heavy pass pushed in phase 3 or pulled in phase 4), then min-bucket and
-// split (each acting only in its phase); the last workgroup of each kernel
-// advances the device state block.  The host queues steps without waiting and checks the
-// 'done' flag after 2, 4, ... 64 steps.  Distances are integers, so the
-// result is exact whatever the order of relaxations.
+// split (each acting only in its phase; split also opens the tail, phase 5);
+// the last workgroup of each kernel advances the device state block.  After
+// a pulled heavy pass the vertices left have few edges, so the buckets end:
+// those vertices are queued once and relaxed Bellman-Ford style (every edge
+// of each vertex improved in the round before) until a round improves
+// nothing.  The host queues steps without waiting -- first as many as the
+// previous call on the graph and delta took, then 4, 8, ... 64 -- and checks
+// the 'done' flag after each batch.  Distances are integers, so the result is
+// exact whatever the order of relaxations.
 #include <algorithm>
 #include <climits>
 #include <cstddef>
