@@ -58,7 +58,11 @@
 namespace gbk {
 
 constexpr int32_t kDistInf = INT32_MAX / 2;  // sssp.cc:67
-constexpr int kRelaxBatch = 4;               // edges per lane per trip
+constexpr int kRelaxBatch = 4;               // edges per lane per trip (2 and 8: the same or slower)
+// Workgroups of the relax-heavy launch: its chunks of hub rows are most of a
+// first bucket's light rounds (Kronecker scale 22, delta 40: 256 -> 1.81 ms
+// per source, 512 -> 1.62, 1024 -> 1.54, 1536 -> 1.52, 2048 -> 1.78).
+constexpr unsigned kSsspHeavyGrid = 1024;
 
 __device__ __forceinline__ int64_t word_dist(unsigned long long w) { return static_cast<int64_t>(w >> 32); }
 __device__ __forceinline__ unsigned long long make_word(int64_t d, uint32_t tag) {
@@ -939,7 +943,7 @@ int gbk_sssp_device(gbk_ctx *ctx, const gbk_graph *g, int32_t source, int32_t de
       const int kt = kt_begin(ctx, GBK_KT_SSSP_RELAX);
       sssp_relax<<<relax_grid, kBlock, 0, s>>>(g->off, g->pnw, g->plc, in, word, near0, near1,
                                                settled, stamp, cap, delta, H, has_heavy, pull_mode, st, aux, hq);
-      sssp_relax_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->pnw, in, word, near0, near1, cap, delta,
+      sssp_relax_heavy<<<kSsspHeavyGrid, kBlock, 0, s>>>(g->pnw, in, word, near0, near1, cap, delta,
                                                      has_heavy, pull_mode, tail_mode,
                                                      static_cast<int>(relax_grid), g->m, st, aux, hq);
       kt_end(ctx, kt);
