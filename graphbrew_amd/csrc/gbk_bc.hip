@@ -270,6 +270,11 @@ __global__ void bc_reach_edges(const int64_t *off, const int32_t *depth, int64_t
 
 using namespace gbk;
 
+// Workgroups of the heavy-row launches (chunks of hub rows spread over the
+// grid): 1024 instead of the frontier kernels' 512 took 4 sources on
+// Kronecker scale 22 from 25.0 to 23.0 ms.
+constexpr unsigned kBcHeavyGrid = 1024;
+
 int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int num_sources,
                   float *scores, gbk_bc_stats *stats) {
   const int64_t n = g->n;
@@ -312,7 +317,7 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
       const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
       bc_forward_level<<<expand_grid(le - lb), kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ,
                                                               queue, lb, le, cap, d, H, c, hq);
-      bc_forward_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ, queue, cap,
+      bc_forward_heavy<<<kBcHeavyGrid, kBlock, 0, s>>>(g->off, g->nbr, depth, sigma, succ, queue, cap,
                                                      d, c, hq);
       kt_end(ctx, kt);
       GBK_LAUNCH_CHECK(ctx, "bc_forward_level");
@@ -330,7 +335,7 @@ int gbk_bc_device(gbk_ctx *ctx, const gbk_graph *g, const int32_t *sources, int 
       const int kt = kt_begin(ctx, GBK_KT_BC_LEVEL);
       bc_backward_level<<<expand_grid(cnt), kBlock, 0, s>>>(g->off, g->nbr, queue + lb, cnt, sigma,
                                                            succ, coef, scores, H, hq);
-      bc_backward_heavy<<<kHeavyGrid, kBlock, 0, s>>>(g->off, g->nbr, sigma, succ, coef, hq, hpart);
+      bc_backward_heavy<<<kBcHeavyGrid, kBlock, 0, s>>>(g->off, g->nbr, sigma, succ, coef, hq, hpart);
       bc_backward_heavy_finish<<<4, kBlock, 0, s>>>(hq, sigma, hpart, coef, scores);
       kt_end(ctx, kt);
     }
